@@ -1,0 +1,248 @@
+/*
+ * bo_capi.h -- C ABI of the MI355X (gfx950) PDHG engine, libbo_hip.so.
+ *
+ * Plain C: pointers, sizes and POD structs, no C++ types, no exceptions.
+ * Every entry point names the reference interface it replaces (paths are
+ * relative to the BharatOpt source tree).  bo_status ordinals equal
+ * bharatopt::GPUStatus (include/bharatopt/gpu_backend.hpp:15-22).
+ *
+ * Memory: "d_" pointers are device pointers obtained from bo_buf_alloc;
+ * every other pointer is host memory owned by the caller.  Handles
+ * (bo_pdlp*) own their device memory until destroyed.  Calls on one handle
+ * must not overlap; distinct handles may be used from distinct host threads.
+ */
+#ifndef BO_CAPI_H
+#define BO_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bo_status {
+    BO_SUCCESS = 0,
+    BO_GPU_UNAVAILABLE = 1,
+    BO_GPU_UNINITIALIZED = 2,
+    BO_OUT_OF_MEMORY = 3,
+    BO_INVALID_ARGUMENT = 4,
+    BO_EXECUTION_FAILED = 5
+} bo_status;
+
+/* GPUDeviceInfo (include/bharatopt/gpu_backend.hpp:24-32) plus CU count. */
+typedef struct bo_device_info {
+    int is_available;
+    char device_name[256];
+    size_t total_memory_bytes;
+    size_t free_memory_bytes;
+    int compute_capability_major; /* gfx9.5.0 -> 9 / 50 */
+    int compute_capability_minor;
+    int multiprocessor_count;
+    char status_message[256];
+} bo_device_info;
+
+/* GpuTimingResult (include/bharatopt/gpu_backend.hpp:34-41); kernel time from HIP events. */
+typedef struct bo_timing {
+    double h2d_time_ms;
+    double kernel_time_ms;
+    double d2h_time_ms;
+    double sync_time_ms;
+    double total_gpu_time_ms;
+} bo_timing;
+
+/* Last error text of the calling thread ("" if none). */
+const char* bo_last_error(void);
+
+/* ---- device and buffers --------------------------------------------------- */
+/* GpuBackend::initialize (gpu_backend.hpp:175, src/gpu_backend.cpp:14-32). */
+bo_status bo_device_init(bo_device_info* info);
+/* GpuBuffer<T>::allocate (gpu_backend.hpp:78-93).  bytes==0 -> *d_ptr=NULL, SUCCESS. */
+bo_status bo_buf_alloc(size_t bytes, void** d_ptr);
+/* GpuBuffer<T>::deallocate (gpu_backend.hpp:95-99). */
+void bo_buf_free(void* d_ptr);
+/* GpuBuffer<T>::copy_host_to_device / copy_device_to_host (gpu_backend.hpp:101-120). */
+bo_status bo_buf_h2d(void* d_dst, const void* src, size_t bytes);
+bo_status bo_buf_d2h(void* dst, const void* d_src, size_t bytes);
+
+/* ---- FP64 primitives of GpuBackend (gpu_backend.hpp:179-227) -------------- */
+/* GpuBackend::spmv: y = A x, CSR, each row summed in column order (bit-equal
+ * to CSRMatrix::multiply, src/sparse_matrix.cpp:153-163). */
+bo_status bo_spmv_csr_f64(size_t rows, size_t cols, size_t nnz, const int* d_row_ptr,
+                          const int* d_col_idx, const double* d_val, const double* d_x,
+                          double* d_y, bo_timing* timing);
+/* GpuBackend::vector_add / vector_sub / vector_scale / axpy (src/gpu_backend.cpp:135-269). */
+bo_status bo_vec_add_f64(const double* d_a, const double* d_b, double* d_out, size_t n, bo_timing* t);
+bo_status bo_vec_sub_f64(const double* d_a, const double* d_b, double* d_out, size_t n, bo_timing* t);
+bo_status bo_vec_scale_f64(const double* d_in, double alpha, double* d_out, size_t n, bo_timing* t);
+bo_status bo_axpy_f64(double alpha, const double* d_x, const double* d_y_in, double* d_y_out,
+                      size_t n, bo_timing* t);
+/* GpuBackend::dot_product / norm2 (src/gpu_backend.cpp:271-327); fixed-order tree. */
+bo_status bo_dot_f64(const double* d_a, const double* d_b, double* result, size_t n, bo_timing* t);
+bo_status bo_norm2_f64(const double* d_v, double* result, size_t n, bo_timing* t);
+
+/* ---- standard-form LP ----------------------------------------------------- */
+/* min c'x s.t. row i: a_i x <= b_i (a_i x == b_i when is_eq[i]), lb <= x <= ub.
+ * This is the form CPUFirstOrderSolver::solve builds (src/first_order_solver.cpp:68-101).
+ * CSR with strictly ascending columns per row; all arrays are host memory. */
+typedef struct bo_lp {
+    int m;
+    int n;
+    long long nnz;
+    const int* row_ptr;
+    const int* col_idx;
+    const double* val;
+    const double* b;
+    const double* c;
+    const double* lb;
+    const double* ub;
+    const unsigned char* is_eq;
+} bo_lp;
+
+/* ---- reference-semantics PDHG (FP64) -------------------------------------- */
+/* Options of FirstOrderSolverOptions used by the loop (first_order_solver.hpp:29-38). */
+typedef struct bo_ref_options {
+    long long max_iterations;
+    double feasibility_tolerance;
+    double primal_step_size;
+    double dual_step_size;
+    int adaptive_step_size;
+    long long check_frequency;
+} bo_ref_options;
+
+typedef struct bo_ref_result {
+    int status; /* FirstOrderSolverStatus: 0 OPTIMAL, 1 MAX_ITERATIONS, 2 NUMERICAL_FAILURE */
+    long long iterations;
+    long long loop_iterations;
+    double constraint_violation;
+    double bound_violation;
+    int used_average;
+    double kernel_time_ms;
+    double transfer_time_ms;
+} bo_ref_result;
+
+/* The PDHG loop of CPUFirstOrderSolver::solve (src/first_order_solver.cpp:103-279)
+ * on the GPU in FP64 with the reference's summation orders and no fused
+ * multiply-adds: results are bit-equal to the reference.  x_out[n], y_out[m]. */
+bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* opts, double* x_out,
+                            double* y_out, bo_ref_result* result);
+
+/* ---- PDLP engine (FP32 iterates, FP64 checks) ----------------------------- */
+typedef struct bo_pdlp_params {
+    long long max_iterations;  /* default 100000 */
+    double eps_rel_kkt;        /* stop when FP64 rel-KKT <= this (default 1e-6) */
+    int ruiz_iterations;       /* default 10 */
+    int pock_chambolle;        /* 1: l1 (alpha=1) rescaling after Ruiz */
+    int check_every;           /* iterations between FP64 KKT / restart checks (64) */
+    int lanes_rows;            /* lanes per CSR row segment (0 = autotune) */
+    int lanes_cols;            /* lanes per CSC column segment (0 = autotune) */
+    int fp64_iterates;         /* 1: run the iteration in FP64 (error comparison run) */
+    int use_graph;             /* 1: replay each check interval as a hipGraph */
+    double time_limit_s;       /* 0 = none */
+    int verbose;
+} bo_pdlp_params;
+
+typedef struct bo_pdlp_stats {
+    int status; /* 0 OPTIMAL (rel-KKT met), 1 MAX_ITERATIONS, 2 NUMERICAL_FAILURE, 3 TIME_LIMIT */
+    long long iterations;
+    int restarts;
+    int used_average;
+    double rel_kkt;
+    double rel_primal;
+    double rel_dual;
+    double rel_gap;
+    double primal_obj;
+    double dual_obj;
+    double primal_weight;
+    double step_size;
+    double setup_ms;
+    double solve_ms;
+    double spmv_csr_bytes; /* algorithmic bytes of one A x product    */
+    double spmv_csc_bytes; /* algorithmic bytes of one A' y product   */
+} bo_pdlp_stats;
+
+/* Default parameters. */
+void bo_pdlp_default_params(bo_pdlp_params* p);
+/* Upload + Ruiz/Pock-Chambolle scaling + step size.  Replaces the set-up part of
+ * CPUFirstOrderSolver::solve (src/first_order_solver.cpp:101-156). */
+bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, struct bo_pdlp** out);
+/* Optional starting point in the original (unscaled) space; NULL = reference
+ * initialisation (box midpoint / finite bound, y = 0). */
+bo_status bo_pdlp_set_warm_start(struct bo_pdlp* h, const double* x0, const double* y0);
+/* Iterate to rel-KKT <= eps (src/first_order_solver.cpp:158-247 replaced by PDLP). */
+bo_status bo_pdlp_solve(struct bo_pdlp* h, bo_pdlp_stats* stats);
+/* Solution in the original space (x[n], y[m], reference sign convention y_I >= 0). */
+bo_status bo_pdlp_get_solution(struct bo_pdlp* h, double* x, double* y);
+void bo_pdlp_destroy(struct bo_pdlp* h);
+/* Time `reps` products of each SpMV kernel of the handle (HIP events, on the
+ * handle's stream).  ms are per-launch averages. */
+bo_status bo_pdlp_time_spmv(struct bo_pdlp* h, int reps, double* csr_ms, double* csc_ms);
+
+/* ---- full drop-in solve on a flattened LPModel ----------------------------- */
+/* LPModel (include/bharatopt/lp_model.hpp:60-127) as arrays: row-wise terms in
+ * insertion order. con_sense: 0 LE, 1 EQ, 2 GE, 3 RANGED; var_type: 0 continuous,
+ * 1 integer, 2 binary (NULL = continuous); sense: 0 minimize, 1 maximize. */
+typedef struct bo_model {
+    int num_vars;
+    int num_cons;
+    int sense;
+    double obj_offset;
+    const double* lb;
+    const double* ub;
+    const double* obj;
+    const int* var_type;
+    const int* con_sense;
+    const double* rhs;
+    const double* range_upper;
+    const long long* term_ptr;
+    const int* term_var;
+    const double* term_val;
+} bo_model;
+
+/* FirstOrderSolverOptions (first_order_solver.hpp:29-38) + engine choice. */
+typedef struct bo_fo_options {
+    long long max_iterations;
+    double feasibility_tolerance;
+    double optimality_tolerance;
+    double primal_step_size;
+    double dual_step_size;
+    int adaptive_step_size;
+    long long check_frequency;
+    int engine; /* 0: reference PDHG semantics (FP64, bit-equal); 1: PDLP to rel-KKT */
+    double eps_rel_kkt;
+} bo_fo_options;
+
+/* FirstOrderSolverResult / Stats (first_order_solver.hpp:41-60). */
+typedef struct bo_fo_result {
+    int status;
+    long long iterations;
+    double objective_value;
+    double primal_objective;
+    double dual_objective;
+    double primal_residual;
+    double dual_residual;
+    double constraint_violation;
+    double bound_violation;
+    double solve_time_ms;
+    double transfer_time_ms;
+    double kernel_time_ms;
+    long long primal_len;
+    long long dual_len;
+    char message[256];
+} bo_fo_result;
+
+/* PresolveEngine::presolve (include/bharatopt/presolve.hpp:96-102), host only:
+ * status (PresolveStatus ordinal), reduced sizes, objective offset and the
+ * reduced model's nonzero count.  Any pointer may be NULL. */
+bo_status bo_presolve_stats(const bo_model* model, int* status, int* reduced_vars,
+                            int* reduced_cons, double* obj_offset, long long* reduced_nnz);
+
+/* GPUFirstOrderSolver::solve (first_order_solver.hpp:89-98, src/first_order_solver.cpp:295-306):
+ * validate -> presolve -> standard form -> GPU PDHG -> postsolve -> objective.
+ * primal_out holds num_vars, dual_out num_cons entries. */
+bo_status bo_first_order_solve(const bo_model* model, const bo_fo_options* opts,
+                               bo_fo_result* result, double* primal_out, double* dual_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,366 @@
+"""MI355X PDHG engine for BharatOpt's first-order LP path (ctypes bindings).
+
+The compute lives in ``libbo_hip.so`` (HIP kernels for gfx950 + the C ABI of
+``include/bo_capi.h`` + the C++ drop-in classes).  This module only marshals
+numpy arrays across that ABI.  There is no CPU fallback: if the library is
+missing, importing ``lib()`` raises, and on a machine without a GPU every
+compute entry point returns ``BO_GPU_UNAVAILABLE``, which is raised here.
+
+The package directory name is not a Python identifier; load it with
+``importlib`` (see ``__graft_entry__.load_package``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lpdata  # noqa: F401  (re-exported for callers)
+from .lpdata import FlatModel, StdLP
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbo_hip.so")
+
+STATUS_NAMES = ["SUCCESS", "GPU_UNAVAILABLE", "GPU_UNINITIALIZED", "OUT_OF_MEMORY",
+                "INVALID_ARGUMENT", "EXECUTION_FAILED"]
+FO_STATUS = ["OPTIMAL", "MAX_ITERATIONS", "NUMERICAL_FAILURE", "INFEASIBLE_DETECTED",
+             "UNBOUNDED_DETECTED", "INTERRUPTED"]
+
+P = C.c_void_p
+_LIB = None
+
+
+class BoError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        msg = lib().bo_last_error().decode(errors="replace") if _LIB is not None else ""
+        name = STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else str(status)
+        super().__init__(f"{where}: {name} {msg}".strip())
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("is_available", C.c_int), ("device_name", C.c_char * 256),
+                ("total_memory_bytes", C.c_size_t), ("free_memory_bytes", C.c_size_t),
+                ("compute_capability_major", C.c_int), ("compute_capability_minor", C.c_int),
+                ("multiprocessor_count", C.c_int), ("status_message", C.c_char * 256)]
+
+
+class Timing(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("h2d_time_ms", "kernel_time_ms", "d2h_time_ms",
+                                          "sync_time_ms", "total_gpu_time_ms")]
+
+
+class BoLp(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("nnz", C.c_longlong), ("row_ptr", P),
+                ("col_idx", P), ("val", P), ("b", P), ("c", P), ("lb", P), ("ub", P), ("is_eq", P)]
+
+
+class RefOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_longlong), ("feasibility_tolerance", C.c_double),
+                ("primal_step_size", C.c_double), ("dual_step_size", C.c_double),
+                ("adaptive_step_size", C.c_int), ("check_frequency", C.c_longlong)]
+
+
+class RefResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("loop_iterations", C.c_longlong),
+                ("constraint_violation", C.c_double), ("bound_violation", C.c_double),
+                ("used_average", C.c_int), ("kernel_time_ms", C.c_double),
+                ("transfer_time_ms", C.c_double)]
+
+
+class PdlpParams(C.Structure):
+    _fields_ = [("max_iterations", C.c_longlong), ("eps_rel_kkt", C.c_double),
+                ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("check_every", C.c_int),
+                ("lanes_rows", C.c_int), ("lanes_cols", C.c_int), ("fp64_iterates", C.c_int),
+                ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int)]
+
+
+class PdlpStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("restarts", C.c_int),
+                ("used_average", C.c_int)] + [(k, C.c_double) for k in (
+                    "rel_kkt", "rel_primal", "rel_dual", "rel_gap", "primal_obj", "dual_obj",
+                    "primal_weight", "step_size", "setup_ms", "solve_ms", "spmv_csr_bytes",
+                    "spmv_csc_bytes")]
+
+
+class BoModel(C.Structure):
+    _fields_ = [("num_vars", C.c_int), ("num_cons", C.c_int), ("sense", C.c_int),
+                ("obj_offset", C.c_double), ("lb", P), ("ub", P), ("obj", P), ("var_type", P),
+                ("con_sense", P), ("rhs", P), ("range_upper", P), ("term_ptr", P),
+                ("term_var", P), ("term_val", P)]
+
+
+class FoOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_longlong), ("feasibility_tolerance", C.c_double),
+                ("optimality_tolerance", C.c_double), ("primal_step_size", C.c_double),
+                ("dual_step_size", C.c_double), ("adaptive_step_size", C.c_int),
+                ("check_frequency", C.c_longlong), ("engine", C.c_int), ("eps_rel_kkt", C.c_double)]
+
+
+class FoResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong)] + [(k, C.c_double) for k in (
+        "objective_value", "primal_objective", "dual_objective", "primal_residual", "dual_residual",
+        "constraint_violation", "bound_violation", "solve_time_ms", "transfer_time_ms",
+        "kernel_time_ms")] + [("primal_len", C.c_longlong), ("dual_len", C.c_longlong),
+                              ("message", C.c_char * 256)]
+
+
+# every symbol include/bo_capi.h declares
+EXPORTS = [
+    "bo_last_error", "bo_device_init", "bo_buf_alloc", "bo_buf_free", "bo_buf_h2d", "bo_buf_d2h",
+    "bo_spmv_csr_f64", "bo_vec_add_f64", "bo_vec_sub_f64", "bo_vec_scale_f64", "bo_axpy_f64",
+    "bo_dot_f64", "bo_norm2_f64", "bo_pdhg_ref_solve", "bo_pdlp_default_params", "bo_pdlp_create",
+    "bo_pdlp_set_warm_start", "bo_pdlp_solve", "bo_pdlp_get_solution", "bo_pdlp_destroy",
+    "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve",
+]
+
+
+def lib():
+    """Load libbo_hip.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.bo_last_error.restype = C.c_char_p
+        L.bo_device_init.argtypes = [P]
+        L.bo_buf_alloc.argtypes = [C.c_size_t, C.POINTER(P)]
+        L.bo_buf_free.argtypes = [P]
+        L.bo_buf_free.restype = None
+        L.bo_buf_h2d.argtypes = [P, P, C.c_size_t]
+        L.bo_buf_d2h.argtypes = [P, P, C.c_size_t]
+        L.bo_spmv_csr_f64.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, P, P, P, P, P, P]
+        for f in ("bo_vec_add_f64", "bo_vec_sub_f64"):
+            getattr(L, f).argtypes = [P, P, P, C.c_size_t, P]
+        L.bo_vec_scale_f64.argtypes = [P, C.c_double, P, C.c_size_t, P]
+        L.bo_axpy_f64.argtypes = [C.c_double, P, P, P, C.c_size_t, P]
+        L.bo_dot_f64.argtypes = [P, P, P, C.c_size_t, P]
+        L.bo_norm2_f64.argtypes = [P, P, C.c_size_t, P]
+        L.bo_pdhg_ref_solve.argtypes = [P, P, P, P, P]
+        L.bo_pdlp_default_params.argtypes = [P]
+        L.bo_pdlp_default_params.restype = None
+        L.bo_pdlp_create.argtypes = [P, P, C.POINTER(P)]
+        L.bo_pdlp_set_warm_start.argtypes = [P, P, P]
+        L.bo_pdlp_solve.argtypes = [P, P]
+        L.bo_pdlp_get_solution.argtypes = [P, P, P]
+        L.bo_pdlp_destroy.argtypes = [P]
+        L.bo_pdlp_destroy.restype = None
+        L.bo_pdlp_time_spmv.argtypes = [P, C.c_int, P, P]
+        L.bo_first_order_solve.argtypes = [P, P, P, P, P]
+        L.bo_presolve_stats.argtypes = [P, P, P, P, P, P]
+        for name in EXPORTS:
+            if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params"):
+                getattr(L, name).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def _check(status, where):
+    if status != 0:
+        raise BoError(status, where)
+
+
+def device_info():
+    d = DeviceInfo()
+    st = lib().bo_device_init(C.byref(d))
+    return st, d
+
+
+# ---------------------------------------------------------------- buffers
+class DeviceBuffer:
+    """Owning device allocation (float64 unless dtype says otherwise)."""
+
+    def __init__(self, count, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.ptr = P()
+        _check(lib().bo_buf_alloc(self.count * self.dtype.itemsize, C.byref(self.ptr)), "bo_buf_alloc")
+
+    @classmethod
+    def from_host(cls, arr, dtype=None):
+        arr = np.ascontiguousarray(arr, dtype=dtype or np.asarray(arr).dtype)
+        b = cls(arr.size, arr.dtype)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        _check(lib().bo_buf_h2d(self.ptr, _p(arr), arr.nbytes), "bo_buf_h2d")
+
+    def download(self, count=None):
+        out = np.empty(self.count if count is None else count, self.dtype)
+        _check(lib().bo_buf_d2h(_p(out), self.ptr, out.nbytes), "bo_buf_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().bo_buf_free(self.ptr)
+            self.ptr = P()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def spmv_f64(lp: StdLP, x):
+    """GpuBackend::spmv on a standard-form matrix (FP64, reference order)."""
+    rp = DeviceBuffer.from_host(lp.row_ptr, np.int32)
+    ci = DeviceBuffer.from_host(lp.col_idx if lp.nnz else np.zeros(1, np.int32), np.int32)
+    v = DeviceBuffer.from_host(lp.val if lp.nnz else np.zeros(1), np.float64)
+    dx = DeviceBuffer.from_host(np.asarray(x, np.float64))
+    dy = DeviceBuffer(max(lp.m, 1))
+    t = Timing()
+    _check(lib().bo_spmv_csr_f64(lp.m, lp.n, lp.nnz, rp.ptr, ci.ptr, v.ptr, dx.ptr, dy.ptr,
+                                 C.byref(t)), "bo_spmv_csr_f64")
+    return dy.download(lp.m), t
+
+
+# ---------------------------------------------------------------- LP marshalling
+class _LpHolder:
+    def __init__(self, lp: StdLP):
+        self.a = dict(rp=np.ascontiguousarray(lp.row_ptr, np.int32),
+                      ci=np.ascontiguousarray(lp.col_idx, np.int32),
+                      v=np.ascontiguousarray(lp.val, np.float64),
+                      b=np.ascontiguousarray(lp.b, np.float64),
+                      c=np.ascontiguousarray(lp.c, np.float64),
+                      lb=np.ascontiguousarray(lp.lb, np.float64),
+                      ub=np.ascontiguousarray(lp.ub, np.float64),
+                      eq=np.ascontiguousarray(lp.is_eq, np.uint8))
+        a = self.a
+        self.s = BoLp(lp.m, lp.n, lp.nnz, _p(a["rp"]), _p(a["ci"]), _p(a["v"]), _p(a["b"]),
+                      _p(a["c"]), _p(a["lb"]), _p(a["ub"]), _p(a["eq"]))
+
+
+@dataclass
+class RefSolve:
+    status: int
+    iterations: int
+    loop_iterations: int
+    constraint_violation: float
+    bound_violation: float
+    used_average: bool
+    kernel_time_ms: float
+    x: np.ndarray
+    y: np.ndarray
+
+
+def pdhg_ref_solve(lp: StdLP, max_iterations=50000, feasibility_tolerance=1e-7,
+                   primal_step_size=0.01, dual_step_size=0.01, adaptive_step_size=True,
+                   check_frequency=100) -> RefSolve:
+    """The reference's PDHG loop on the GPU (FP64, bit-equal)."""
+    h = _LpHolder(lp)
+    o = RefOptions(max_iterations, feasibility_tolerance, primal_step_size, dual_step_size,
+                   1 if adaptive_step_size else 0, check_frequency)
+    r = RefResult()
+    x = np.zeros(max(lp.n, 1))
+    y = np.zeros(max(lp.m, 1))
+    _check(lib().bo_pdhg_ref_solve(C.byref(h.s), C.byref(o), _p(x), _p(y), C.byref(r)),
+           "bo_pdhg_ref_solve")
+    return RefSolve(r.status, r.iterations, r.loop_iterations, r.constraint_violation,
+                    r.bound_violation, bool(r.used_average), r.kernel_time_ms, x[:lp.n], y[:lp.m])
+
+
+def default_pdlp_params(**kw) -> PdlpParams:
+    p = PdlpParams()
+    lib().bo_pdlp_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Pdlp:
+    """Handle of one LP resident on the GPU (bo_pdlp_*)."""
+
+    def __init__(self, lp: StdLP, **params):
+        self.lp = lp
+        self._h = _LpHolder(lp)
+        self.params = default_pdlp_params(**params)
+        self.handle = P()
+        _check(lib().bo_pdlp_create(C.byref(self._h.s), C.byref(self.params), C.byref(self.handle)),
+               "bo_pdlp_create")
+
+    def set_warm_start(self, x0=None, y0=None):
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        y0 = None if y0 is None else np.ascontiguousarray(y0, np.float64)
+        _check(lib().bo_pdlp_set_warm_start(self.handle, _p(x0), _p(y0)), "bo_pdlp_set_warm_start")
+
+    def solve(self) -> dict:
+        s = PdlpStats()
+        _check(lib().bo_pdlp_solve(self.handle, C.byref(s)), "bo_pdlp_solve")
+        return {k: getattr(s, k) for k, _ in PdlpStats._fields_}
+
+    def solution(self):
+        x = np.zeros(max(self.lp.n, 1))
+        y = np.zeros(max(self.lp.m, 1))
+        _check(lib().bo_pdlp_get_solution(self.handle, _p(x), _p(y)), "bo_pdlp_get_solution")
+        return x[:self.lp.n], y[:self.lp.m]
+
+    def time_kernels(self, reps=20):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().bo_pdlp_time_spmv(self.handle, reps, C.byref(a), C.byref(b)), "bo_pdlp_time_spmv")
+        return a.value, b.value
+
+    def close(self):
+        if self.handle:
+            lib().bo_pdlp_destroy(self.handle)
+            self.handle = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ModelHolder:
+    def __init__(self, fm: FlatModel):
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+        self.a = dict(lb=f64(fm.lb), ub=f64(fm.ub), obj=f64(fm.obj),
+                      vt=None if fm.var_type is None else np.ascontiguousarray(fm.var_type, np.int32),
+                      cs=np.ascontiguousarray(fm.con_sense, np.int32), rhs=f64(fm.rhs),
+                      ru=None if fm.range_upper is None else f64(fm.range_upper),
+                      tp=np.ascontiguousarray(fm.term_ptr, np.int64),
+                      tv=np.ascontiguousarray(fm.term_var, np.int32), tw=f64(fm.term_val))
+        a = self.a
+        self.s = BoModel(fm.num_vars, fm.num_cons, int(fm.sense), float(fm.obj_offset),
+                         _p(a["lb"]), _p(a["ub"]), _p(a["obj"]), _p(a["vt"]), _p(a["cs"]),
+                         _p(a["rhs"]), _p(a["ru"]), _p(a["tp"]), _p(a["tv"]), _p(a["tw"]))
+
+
+def presolve_stats(fm: FlatModel):
+    """PresolveEngine::presolve on the host (no GPU needed)."""
+    h = _ModelHolder(fm)
+    st, nv, nc, off, nz = C.c_int(), C.c_int(), C.c_int(), C.c_double(), C.c_longlong()
+    _check(lib().bo_presolve_stats(C.byref(h.s), C.byref(st), C.byref(nv), C.byref(nc), C.byref(off),
+                                   C.byref(nz)), "bo_presolve_stats")
+    return dict(status=st.value, vars=nv.value, cons=nc.value, offset=off.value, nnz=nz.value)
+
+
+def first_order_solve(fm: FlatModel, engine=0, max_iterations=50000, feasibility_tolerance=1e-7,
+                      optimality_tolerance=1e-7, primal_step_size=0.01, dual_step_size=0.01,
+                      adaptive_step_size=True, check_frequency=100, eps_rel_kkt=1e-6):
+    """GPUFirstOrderSolver::solve on a flattened LPModel (engine 0 = reference
+    PDHG semantics, 1 = PDLP).  Returns a dict of the result fields."""
+    h = _ModelHolder(fm)
+    o = FoOptions(max_iterations, feasibility_tolerance, optimality_tolerance, primal_step_size,
+                  dual_step_size, 1 if adaptive_step_size else 0, check_frequency, engine, eps_rel_kkt)
+    r = FoResult()
+    x = np.zeros(max(fm.num_vars, 1))
+    y = np.zeros(max(fm.num_cons * 2, 1))
+    st = lib().bo_first_order_solve(C.byref(h.s), C.byref(o), C.byref(r), _p(x), _p(y))
+    out = {k: getattr(r, k) for k, _ in FoResult._fields_ if k != "message"}
+    out["message"] = r.message.decode()
+    out["x"] = x[:r.primal_len].copy()
+    out["y"] = y[:r.dual_len].copy()
+    out["capi_status"] = st
+    if st not in (0,):
+        raise BoError(st, "bo_first_order_solve: " + out["message"])
+    return out

@@ -1,0 +1,80 @@
+// common.hpp -- shared HIP helpers for the MI355X (gfx950) PDHG engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/bo_capi.h"
+
+namespace bo {
+
+constexpr int kWave = 64;      // CDNA wavefront width
+constexpr int kBlock = 256;    // 4 waves per workgroup
+
+// hipError_t -> bo_status (the GPUStatus ordinals of gpu_backend.hpp:15-22).
+inline bo_status map_error(hipError_t e) {
+    switch (e) {
+        case hipSuccess: return BO_SUCCESS;
+        case hipErrorOutOfMemory: return BO_OUT_OF_MEMORY;
+        case hipErrorNoDevice:
+        case hipErrorInsufficientDriver: return BO_GPU_UNAVAILABLE;
+        case hipErrorInvalidValue: return BO_INVALID_ARGUMENT;
+        default: return BO_EXECUTION_FAILED;
+    }
+}
+
+void set_last_error(const std::string& msg);
+
+#define BO_TRY(expr)                                                              \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            ::bo::set_last_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+            return ::bo::map_error(_e);                                           \
+        }                                                                         \
+    } while (0)
+
+inline unsigned grid_for(size_t n, int block = kBlock) {
+    size_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+// ---- wave / block reductions (fixed tree order => deterministic) ----------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// std::max with the reference's NaN behaviour: (a < b) ? b : a drops a NaN b.
+__device__ inline double ref_max(double a, double b) { return (a < b) ? b : a; }
+
+__device__ inline double wave_max_ref(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = ref_max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Block-wide sum; every thread gets the result.  `scratch` holds kBlock/kWave doubles.
+__device__ inline double block_sum(double v, double* scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) t += scratch[w];
+    return t;
+}
+
+// Non-negative doubles (including +inf) order like their bit patterns, so a
+// 64-bit unsigned atomic max is an exact, order-independent max.
+__device__ inline void atomic_max_nonneg(unsigned long long* dst, double v) {
+    atomicMax(dst, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+}  // namespace bo

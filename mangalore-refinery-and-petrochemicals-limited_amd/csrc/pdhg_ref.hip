@@ -1,0 +1,346 @@
+// pdhg_ref.hip -- the reference PDHG loop on the GPU in FP64, bit-equal to
+// CPUFirstOrderSolver::solve (src/first_order_solver.cpp:103-279).
+//
+// Bit-equality comes from three rules: (1) every dot product is summed by one
+// thread in the reference's order -- rows in column order for A x
+// (src/sparse_matrix.cpp:153-163), columns in ascending row order for A'y,
+// which is the order the reference's row-order scatter (:175-185) adds into
+// each y[col]; (2) no fused multiply-adds (__dmul_rn/__dadd_rn, and this file
+// is built with -ffp-contract=off); (3) max reductions use the reference's
+// std::max NaN behaviour and an exact 64-bit atomic max, so they are
+// order-independent.  This engine is the drop-in's default: it reproduces
+// the reference's iterates, iteration counts and statuses.  Throughput work
+// lives in pdlp.hip.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace bo {
+namespace {
+
+__device__ inline double dclamp(double v, double lo, double hi) {
+    return (v < lo) ? lo : ((hi < v) ? hi : v);
+}
+
+// Steps (src/first_order_solver.cpp:103-128): row sums in column order, column
+// sums in ascending row order.
+__global__ void k_ref_steps(int m, int n, const int* __restrict__ rp, const double* __restrict__ rv,
+                            const int* __restrict__ cp, const double* __restrict__ cv, int adaptive,
+                            double primal_step, double dual_step, double* __restrict__ tau,
+                            double* __restrict__ sigma) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        double s = 0.0;
+        for (int k = rp[t]; k < rp[t + 1]; ++k) s = __dadd_rn(s, fabs(rv[k]));
+        sigma[t] = adaptive ? ((s > 1e-12) ? (0.95 / s) : 1.0) : dual_step;
+    }
+    if (t < n) {
+        double s = 0.0;
+        for (int k = cp[t]; k < cp[t + 1]; ++k) s = __dadd_rn(s, fabs(cv[k]));
+        tau[t] = adaptive ? ((s > 1e-12) ? (0.95 / s) : 1.0) : primal_step;
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int count) {
+    if (threadIdx.x < count) p[threadIdx.x] = 0ull;
+}
+
+// Wave-level max then one atomic per wave (values are >= 0 and never NaN).
+__device__ inline void reduce_max_to(unsigned long long* dst, double v) {
+    v = wave_max_ref(v);
+    if ((threadIdx.x % kWave) == 0) atomic_max_nonneg(dst, v);
+}
+
+// A'y (ordered) + primal step + ergodic sum + extrapolation
+// (src/first_order_solver.cpp:163-180).  x_prev equals x on entry.
+__global__ void k_ref_primal(int n, const int* __restrict__ cp, const int* __restrict__ ri,
+                             const double* __restrict__ cv, const double* __restrict__ y,
+                             const double* __restrict__ c, const double* __restrict__ tau,
+                             const double* __restrict__ lb, const double* __restrict__ ub,
+                             double* __restrict__ x, double* __restrict__ x_sum,
+                             double* __restrict__ x_hat, unsigned long long* __restrict__ pmax,
+                             int track) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double delta = 0.0;
+    if (j < n) {
+        double aty = 0.0;
+        for (int k = cp[j]; k < cp[j + 1]; ++k) aty = __dadd_rn(aty, __dmul_rn(cv[k], y[ri[k]]));
+        double xo = x[j];
+        double grad = __dadd_rn(c[j], aty);
+        double next_x = __dsub_rn(xo, __dmul_rn(tau[j], grad));
+        double new_x = dclamp(next_x, lb[j], ub[j]);
+        delta = fabs(__dsub_rn(new_x, xo));
+        x[j] = new_x;
+        x_sum[j] = __dadd_rn(x_sum[j], new_x);
+        x_hat[j] = __dsub_rn(__dmul_rn(2.0, new_x), xo);
+    }
+    if (track) reduce_max_to(pmax, ref_max(0.0, delta));
+}
+
+// A x_hat (ordered) + dual step (src/first_order_solver.cpp:183-192).
+__global__ void k_ref_dual(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ rv, const double* __restrict__ x_hat,
+                           const double* __restrict__ b, const double* __restrict__ sigma,
+                           const unsigned char* __restrict__ is_eq, double* __restrict__ y,
+                           unsigned long long* __restrict__ dmax, int track) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double delta = 0.0;
+    if (i < m) {
+        double ax = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) ax = __dadd_rn(ax, __dmul_rn(rv[k], x_hat[ci[k]]));
+        double yo = y[i];
+        double next_y = __dadd_rn(yo, __dmul_rn(sigma[i], __dsub_rn(ax, b[i])));
+        double new_y = (!is_eq[i]) ? ref_max(0.0, next_y) : next_y;
+        delta = fabs(__dsub_rn(new_y, yo));
+        y[i] = new_y;
+    }
+    if (track) reduce_max_to(dmax, ref_max(0.0, delta));
+}
+
+// x_avg = clamp(x_sum / count) and the bound violation of x (:208-218).
+__global__ void k_ref_avg(int n, const double* __restrict__ x, const double* __restrict__ x_sum,
+                          double count, const double* __restrict__ lb, const double* __restrict__ ub,
+                          double* __restrict__ x_avg, unsigned long long* __restrict__ bmax,
+                          int copy_x) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double bv = 0.0;
+    if (j < n) {
+        double xj = x[j];
+        if (xj < lb[j] - 1e-12) bv = ref_max(bv, lb[j] - xj);
+        if (xj > ub[j] + 1e-12) bv = ref_max(bv, xj - ub[j]);
+        x_avg[j] = copy_x ? xj : dclamp(x_sum[j] / count, lb[j], ub[j]);
+    }
+    reduce_max_to(bmax, bv);
+}
+
+__device__ inline double row_viol(double ax, double b, unsigned char eq) {
+    double r = __dsub_rn(ax, b);
+    return (!eq) ? ref_max(0.0, r) : fabs(r);
+}
+
+// Row violations of x and x_avg (:199-225).
+__global__ void k_ref_viol(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ rv, const double* __restrict__ x,
+                           const double* __restrict__ x_avg, const double* __restrict__ b,
+                           const unsigned char* __restrict__ is_eq,
+                           unsigned long long* __restrict__ vmax, unsigned long long* __restrict__ amax,
+                           int with_x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v1 = 0.0, v2 = 0.0;
+    if (i < m) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            double a = rv[k];
+            int j = ci[k];
+            if (with_x) s1 = __dadd_rn(s1, __dmul_rn(a, x[j]));
+            s2 = __dadd_rn(s2, __dmul_rn(a, x_avg[j]));
+        }
+        if (with_x) v1 = row_viol(s1, b[i], is_eq[i]);
+        v2 = row_viol(s2, b[i], is_eq[i]);
+    }
+    if (with_x) reduce_max_to(vmax, v1);
+    reduce_max_to(amax, v2);
+}
+
+struct DevBufs {
+    std::vector<void*> ptrs;
+    ~DevBufs() {
+        for (void* p : ptrs) hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        *p = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+inline double u64_to_double(unsigned long long u) {
+    double d;
+    std::memcpy(&d, &u, sizeof(d));
+    return d;
+}
+
+}  // namespace
+
+bo_status ensure_device_public();
+
+}  // namespace bo
+
+using namespace bo;
+
+extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o, double* x_out,
+                                       double* y_out, bo_ref_result* res) {
+    bo_status st = ensure_device_public();
+    if (st != BO_SUCCESS) return st;
+    if (!lp || !o || !res || lp->m < 0 || lp->n < 0 || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
+        return BO_INVALID_ARGUMENT;
+    const int m = lp->m, n = lp->n;
+    const long long nnz = lp->nnz;
+    if ((m > 0 && !lp->row_ptr) || (nnz > 0 && (!lp->col_idx || !lp->val)) ||
+        (m > 0 && (!lp->b || !lp->is_eq)) || (n > 0 && (!lp->c || !lp->lb || !lp->ub)))
+        return BO_INVALID_ARGUMENT;
+    if (m > 0 && lp->row_ptr[m] != nnz) return BO_INVALID_ARGUMENT;
+    for (long long k = 0; k < nnz; ++k)
+        if (lp->col_idx[k] < 0 || lp->col_idx[k] >= n) return BO_INVALID_ARGUMENT;
+    std::memset(res, 0, sizeof(*res));
+
+    auto t_xfer0 = std::chrono::high_resolution_clock::now();
+    // CSC from CSR: scanning rows in order keeps each column's rows ascending.
+    std::vector<int> cp(static_cast<size_t>(n) + 1, 0), ri(static_cast<size_t>(nnz));
+    std::vector<double> cv(static_cast<size_t>(nnz));
+    for (long long k = 0; k < nnz; ++k) cp[lp->col_idx[k] + 1]++;
+    for (int j = 0; j < n; ++j) cp[j + 1] += cp[j];
+    {
+        std::vector<int> pos(cp.begin(), cp.end() - 1);
+        for (int i = 0; i < m; ++i)
+            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
+                int p = pos[lp->col_idx[k]]++;
+                ri[p] = i;
+                cv[p] = lp->val[k];
+            }
+    }
+    // Initial point (src/first_order_solver.cpp:131-142).
+    std::vector<double> x0(static_cast<size_t>(n));
+    for (int j = 0; j < n; ++j) {
+        double l = lp->lb[j], u = lp->ub[j], target = 0.0;
+        if (l > -INFINITY && u < INFINITY) target = 0.5 * (l + u);
+        else if (l > -INFINITY) target = l;
+        else if (u < INFINITY) target = u;
+        x0[j] = (target < l) ? l : ((u < target) ? u : target);
+    }
+
+    DevBufs mem;
+    int *d_rp, *d_ci, *d_cp, *d_ri;
+    double *d_rv, *d_cv, *d_b, *d_c, *d_lb, *d_ub, *d_tau, *d_sigma, *d_x, *d_y, *d_xsum, *d_xhat, *d_xavg;
+    unsigned char* d_eq;
+    unsigned long long* d_red;  // [0] pmax [1] dmax [2] viol [3] avg viol [4] bound viol
+    BO_TRY(mem.alloc(&d_rp, m + 1));
+    BO_TRY(mem.alloc(&d_ci, nnz));
+    BO_TRY(mem.alloc(&d_rv, nnz));
+    BO_TRY(mem.alloc(&d_cp, n + 1));
+    BO_TRY(mem.alloc(&d_ri, nnz));
+    BO_TRY(mem.alloc(&d_cv, nnz));
+    BO_TRY(mem.alloc(&d_b, m));
+    BO_TRY(mem.alloc(&d_eq, m));
+    BO_TRY(mem.alloc(&d_sigma, m));
+    BO_TRY(mem.alloc(&d_y, m));
+    BO_TRY(mem.alloc(&d_c, n));
+    BO_TRY(mem.alloc(&d_lb, n));
+    BO_TRY(mem.alloc(&d_ub, n));
+    BO_TRY(mem.alloc(&d_tau, n));
+    BO_TRY(mem.alloc(&d_x, n));
+    BO_TRY(mem.alloc(&d_xsum, n));
+    BO_TRY(mem.alloc(&d_xhat, n));
+    BO_TRY(mem.alloc(&d_xavg, n));
+    BO_TRY(mem.alloc(&d_red, 8));
+    auto h2d = [](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    if (m > 0) BO_TRY(h2d(d_rp, lp->row_ptr, sizeof(int) * (m + 1)));
+    BO_TRY(h2d(d_ci, lp->col_idx, sizeof(int) * nnz));
+    BO_TRY(h2d(d_rv, lp->val, sizeof(double) * nnz));
+    BO_TRY(h2d(d_cp, cp.data(), sizeof(int) * (n + 1)));
+    BO_TRY(h2d(d_ri, ri.data(), sizeof(int) * nnz));
+    BO_TRY(h2d(d_cv, cv.data(), sizeof(double) * nnz));
+    BO_TRY(h2d(d_b, lp->b, sizeof(double) * m));
+    BO_TRY(h2d(d_eq, lp->is_eq, m));
+    BO_TRY(h2d(d_c, lp->c, sizeof(double) * n));
+    BO_TRY(h2d(d_lb, lp->lb, sizeof(double) * n));
+    BO_TRY(h2d(d_ub, lp->ub, sizeof(double) * n));
+    BO_TRY(h2d(d_x, x0.data(), sizeof(double) * n));
+    BO_TRY(hipMemset(d_y, 0, sizeof(double) * (m ? m : 1)));
+    BO_TRY(hipMemset(d_xsum, 0, sizeof(double) * (n ? n : 1)));
+    BO_TRY(hipMemset(d_xhat, 0, sizeof(double) * (n ? n : 1)));
+    BO_TRY(hipDeviceSynchronize());
+    auto t_xfer1 = std::chrono::high_resolution_clock::now();
+
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, 0);
+
+    const unsigned gm = grid_for(m), gn = grid_for(n), gmax = grid_for(m > n ? m : n);
+    hipLaunchKernelGGL(k_ref_steps, dim3(gmax), dim3(kBlock), 0, 0, m, n, d_rp, d_rv, d_cp, d_cv,
+                       o->adaptive_step_size ? 1 : 0, o->primal_step_size, o->dual_step_size, d_tau,
+                       d_sigma);
+
+    long long check_freq = o->check_frequency < 100 ? o->check_frequency : 100;
+    if (check_freq < 1) check_freq = 1;  // the reference divides by zero here
+    long long iter = 0, avg_count = 0;
+    bool converged = false, failed = false;
+    unsigned long long red[8];
+
+    while (iter < o->max_iterations) {
+        iter++;
+        const int track = (iter % check_freq == 0 || iter == o->max_iterations) ? 1 : 0;
+        if (track) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d_red, 8);
+        hipLaunchKernelGGL(k_ref_primal, dim3(gn), dim3(kBlock), 0, 0, n, d_cp, d_ri, d_cv, d_y, d_c,
+                           d_tau, d_lb, d_ub, d_x, d_xsum, d_xhat, d_red + 0, track);
+        avg_count++;
+        hipLaunchKernelGGL(k_ref_dual, dim3(gm), dim3(kBlock), 0, 0, m, d_rp, d_ci, d_rv, d_xhat, d_b,
+                           d_sigma, d_eq, d_y, d_red + 1, track);
+        if (track) {
+            hipLaunchKernelGGL(k_ref_avg, dim3(gn), dim3(kBlock), 0, 0, n, d_x, d_xsum,
+                               static_cast<double>(avg_count), d_lb, d_ub, d_xavg, d_red + 4, 0);
+            hipLaunchKernelGGL(k_ref_viol, dim3(gm), dim3(kBlock), 0, 0, m, d_rp, d_ci, d_rv, d_x, d_xavg,
+                               d_b, d_eq, d_red + 2, d_red + 3, 1);
+            BO_TRY(hipGetLastError());
+            BO_TRY(hipMemcpy(red, d_red, sizeof(red), hipMemcpyDeviceToHost));
+            const double pmax = u64_to_double(red[0]), dmax = u64_to_double(red[1]);
+            const double max_viol = u64_to_double(red[2]), avg_viol = u64_to_double(red[3]);
+            const double bviol = u64_to_double(red[4]);
+            const double best = (avg_viol < max_viol) ? avg_viol : max_viol;  // std::min
+            res->constraint_violation = best;
+            res->bound_violation = bviol;
+            if (std::isnan(best) || best > 1e15) {
+                failed = true;
+                break;
+            }
+            const bool feasible = best <= o->feasibility_tolerance && bviol <= o->feasibility_tolerance;
+            const bool stationary = pmax <= 1e-7 && dmax <= 1e-7;
+            if (feasible && stationary && iter >= 100) {
+                converged = true;
+                break;
+            }
+        }
+    }
+    res->loop_iterations = iter;
+    if (failed) {
+        res->status = 2;
+        res->iterations = 0;  // the reference returns before recording the count
+    } else {
+        res->status = converged ? 0 : 1;
+        res->iterations = iter;
+        // Iterate selection (src/first_order_solver.cpp:261-279).
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d_red, 8);
+        hipLaunchKernelGGL(k_ref_avg, dim3(gn), dim3(kBlock), 0, 0, n, d_x, d_xsum,
+                           static_cast<double>(avg_count), d_lb, d_ub, d_xavg, d_red + 4,
+                           avg_count > 0 ? 0 : 1);
+        hipLaunchKernelGGL(k_ref_viol, dim3(gm), dim3(kBlock), 0, 0, m, d_rp, d_ci, d_rv, d_x, d_xavg,
+                           d_b, d_eq, d_red + 2, d_red + 3, 0);
+        BO_TRY(hipGetLastError());
+        BO_TRY(hipMemcpy(red, d_red, sizeof(red), hipMemcpyDeviceToHost));
+        const double avg_viol = u64_to_double(red[3]);
+        const bool use_avg = !converged && (avg_viol <= res->constraint_violation);
+        res->used_average = use_avg ? 1 : 0;
+        if (n > 0) BO_TRY(hipMemcpy(x_out, use_avg ? d_xavg : d_x, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (m > 0) BO_TRY(hipMemcpy(y_out, d_y, sizeof(double) * m, hipMemcpyDeviceToHost));
+    }
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    BO_TRY(hipGetLastError());
+    res->kernel_time_ms = ms;
+    res->transfer_time_ms = std::chrono::duration<double, std::milli>(t_xfer1 - t_xfer0).count();
+    return BO_SUCCESS;
+}

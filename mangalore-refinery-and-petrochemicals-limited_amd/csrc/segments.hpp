@@ -1,0 +1,176 @@
+// segments.hpp -- length-binned segment kernels: one launch covers every row
+// (or column) of a CSR (or CSC) matrix whose segments were sorted by length.
+//
+// Segments are grouped into bins; a bin with L lanes per segment puts 256/L
+// segments in a 256-thread workgroup (L <= 64, one wavefront holds 64/L
+// segments) or spreads one segment over the whole workgroup (L == 256).
+// Lanes of a segment read its nonzeros with stride L (coalesced), accumulate
+// in double, and reduce with a fixed xor tree, so results are deterministic.
+// The bin a workgroup serves is read from a small table passed by value;
+// workgroup ids are remapped so each XCD receives a contiguous range of
+// segments (consecutive workgroup ids are dispatched round-robin over XCDs).
+#pragma once
+
+#include "common.hpp"
+
+namespace bo {
+
+constexpr int kMaxBins = 8;
+constexpr int kNumXcd = 8;
+
+struct BinTable {
+    int nbins;
+    int total_blocks;
+    int block_start[kMaxBins + 1];
+    int seg_start[kMaxBins + 1];
+    int lanes[kMaxBins];
+};
+
+// Accumulation modes of a segment pass.
+enum SegMode : int {
+    kSumProd = 0,  // sum_k val[k] * vec[idx[k]]           (SpMV)
+    kMaxAbs = 1,   // max_k |val[k] * vec[idx[k]]|        (Ruiz inf-norms)
+    kSumAbs = 2,   // sum_k |val[k] * vec[idx[k]]|        (Pock-Chambolle l1-norms)
+    kWrite = 3,    // epi.write(seg, k, val[k], idx[k]) per nonzero, no reduction
+};
+
+__device__ inline int xcd_remap(int bid, int nblocks) {
+    const int q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+    const int xcd = bid % kNumXcd, local = bid / kNumXcd;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+template <int MODE>
+__device__ inline double seg_combine(double a, double b) {
+    if constexpr (MODE == kMaxAbs) return a > b ? a : b;
+    else return a + b;
+}
+
+template <int MODE, class TV, class TX>
+__device__ inline double seg_term(TV v, TX x) {
+    double p = static_cast<double>(v) * static_cast<double>(x);
+    if constexpr (MODE == kSumProd) return p;
+    else return fabs(p);
+}
+
+template <int MODE, int L>
+__device__ inline double group_reduce(double v) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v = seg_combine<MODE>(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Epilogue concept:
+//   static constexpr int NV;        number of gathered vectors (1 or 2)
+//   static constexpr int Q;         number of per-block partial sums (0..8)
+//   const TX* vec(int v) const;     gathered vector v
+//   __device__ void operator()(int seg, const double* acc, double* part) const;
+template <int MODE, int L, class TV, class TX, class Epi>
+__device__ inline void seg_process(const BinTable& bt, int bin, int vblock, const int* __restrict__ ptr,
+                                   const int* __restrict__ idx, const TV* __restrict__ val,
+                                   const Epi& epi, double* part) {
+    constexpr int NV = Epi::NV;
+    const int seg_end = bt.seg_start[bin + 1];
+    if constexpr (L <= kWave) {
+        constexpr int per_block = kBlock / L;
+        const int seg = bt.seg_start[bin] + (vblock - bt.block_start[bin]) * per_block +
+                        static_cast<int>(threadIdx.x) / L;
+        const int lane = static_cast<int>(threadIdx.x) % L;
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        const bool live = seg < seg_end;
+        if (live) {
+            const int k0 = ptr[seg], k1 = ptr[seg + 1];
+            for (int k = k0 + lane; k < k1; k += L) {
+                const TV a = val[k];
+                const int j = idx[k];
+                if constexpr (MODE == kWrite) {
+                    epi.write(seg, k, a, j);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a, epi.vec(v)[j]));
+                }
+            }
+        }
+        if constexpr (MODE != kWrite) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = group_reduce<MODE, L>(acc[v]);
+            if (live && lane == 0) epi(seg, acc, part);
+        }
+    } else {
+        // one segment per workgroup
+        __shared__ double red[NV][kBlock / kWave];
+        const int seg = bt.seg_start[bin] + (vblock - bt.block_start[bin]);
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        const int k0 = ptr[seg], k1 = ptr[seg + 1];
+        for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kBlock) {
+            const TV a = val[k];
+            const int j = idx[k];
+            if constexpr (MODE == kWrite) {
+                epi.write(seg, k, a, j);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a, epi.vec(v)[j]));
+            }
+        }
+        if constexpr (MODE == kWrite) return;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = group_reduce<MODE, kWave>(acc[v]);
+        const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) red[v][wid] = acc[v];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                double t = red[v][0];
+                for (int w = 1; w < kBlock / kWave; ++w) t = seg_combine<MODE>(t, red[v][w]);
+                acc[v] = t;
+            }
+            (void)seg_end;
+            epi(seg, acc, part);
+        }
+    }
+}
+
+template <int MODE, class TV, class TX, class Epi>
+__global__ void __launch_bounds__(kBlock)
+k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx,
+           const TV* __restrict__ val, Epi epi, double* __restrict__ partials,
+           const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int Q = Epi::Q;
+    const int vblock = xcd_remap(static_cast<int>(blockIdx.x), bt.total_blocks);
+    int bin = 0;
+    while (bin + 1 < bt.nbins && vblock >= bt.block_start[bin + 1]) ++bin;
+    double part[Q > 0 ? Q : 1];
+#pragma unroll
+    for (int q = 0; q < (Q > 0 ? Q : 1); ++q) part[q] = 0.0;
+    switch (bt.lanes[bin]) {
+        case 1: seg_process<MODE, 1, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 2: seg_process<MODE, 2, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 4: seg_process<MODE, 4, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 8: seg_process<MODE, 8, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 16: seg_process<MODE, 16, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 32: seg_process<MODE, 32, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 64: seg_process<MODE, 64, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        default: seg_process<MODE, 256, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+    }
+    if constexpr (Q > 0) {
+        __shared__ double s[kBlock / kWave];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            double t = block_sum(part[q], s);
+            if (threadIdx.x == 0) partials[static_cast<size_t>(blockIdx.x) * Q + q] = t;
+        }
+    }
+}
+
+}  // namespace bo

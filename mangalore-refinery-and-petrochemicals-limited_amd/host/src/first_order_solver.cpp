@@ -1,0 +1,276 @@
+// first_order_solver.cpp -- GPUFirstOrderSolver: the host pipeline around the
+// GPU PDHG engines.  Stage order, messages and result fields follow the
+// reference's CPUFirstOrderSolver::solve (src/first_order_solver.cpp:14-290):
+// validation gate, presolve, bound-only closed form when no rows or columns
+// remain, standard form (<= rows, GE rows negated, EQ rows flagged), the
+// PDHG loop (on the GPU), postsolve, objective on the original model.
+//
+// Divergences, all deliberate:
+//   * no GPU -> NUMERICAL_FAILURE with a message, never a CPU fallback;
+//   * RANGED rows become two <= rows (the reference keeps only "<= rhs");
+//     their dual is reported as y_upper - y_lower;
+//   * engine PDLP stops on the FP64 relative KKT error instead of the
+//     reference's stagnation test.
+#include <bharatopt/first_order_solver.hpp>
+#include <bharatopt/model_validator.hpp>
+#include <bharatopt/presolve.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+namespace bharatopt {
+
+namespace {
+
+using clk = std::chrono::high_resolution_clock;
+
+real_t original_objective(const LPModel& model, const std::vector<real_t>& x) {
+    real_t obj = model.obj_offset();
+    for (size_t j = 0; j < model.num_variables(); ++j) obj += model.variables()[j].obj_coeff * x[j];
+    return obj;
+}
+
+// min c'x, rows <= b (or == b), l <= x <= u; ranged rows split in two.
+struct StandardForm {
+    int m_model = 0;  // rows of the (reduced) model
+    int m = 0;        // rows after splitting ranged rows
+    int n = 0;
+    CSRMatrix A;
+    std::vector<real_t> b, c, lb, ub;
+    std::vector<unsigned char> is_eq;
+    std::vector<int> lower_of;  // extra row -> model row (ranged lower sides)
+};
+
+StandardForm build_standard_form(const LPModel& model) {
+    StandardForm sf;
+    const real_t sense_mult = model.sense() == ObjectiveSense::MINIMIZE ? 1.0 : -1.0;
+    sf.n = static_cast<int>(model.num_variables());
+    sf.m_model = static_cast<int>(model.num_constraints());
+    int extra = 0;
+    for (const auto& c : model.constraints()) extra += c.sense == ConstraintSense::RANGED;
+    sf.m = sf.m_model + extra;
+    sf.c.resize(sf.n);
+    sf.lb.resize(sf.n);
+    sf.ub.resize(sf.n);
+    for (int j = 0; j < sf.n; ++j) {
+        const Variable& v = model.variables()[static_cast<size_t>(j)];
+        sf.c[j] = v.obj_coeff * sense_mult;
+        sf.lb[j] = v.lower_bound;
+        sf.ub[j] = v.upper_bound;
+    }
+    sf.b.assign(sf.m, 0.0);
+    sf.is_eq.assign(sf.m, 0);
+    COOMatrix coo(static_cast<size_t>(sf.m), static_cast<size_t>(sf.n));
+    int next_extra = sf.m_model;
+    for (int i = 0; i < sf.m_model; ++i) {
+        const Constraint& con = model.constraints()[static_cast<size_t>(i)];
+        real_t mult = 1.0;
+        real_t rhs = con.rhs;
+        if (con.sense == ConstraintSense::GREATER_EQUAL) {
+            mult = -1.0;
+            rhs = -rhs;
+        } else if (con.sense == ConstraintSense::EQUAL) {
+            sf.is_eq[i] = 1;
+        } else if (con.sense == ConstraintSense::RANGED) {
+            rhs = con.range_upper;  // a'x <= upper here, -a'x <= -lower in the extra row
+            const int e = next_extra++;
+            sf.b[e] = -con.rhs;
+            sf.lower_of.push_back(i);
+            for (const auto& t : con.terms) coo.add_entry(e, t.first, -t.second);
+        }
+        sf.b[i] = rhs;
+        for (const auto& t : con.terms) coo.add_entry(i, t.first, t.second * mult);
+    }
+    sf.A = CSRMatrix::from_coo(coo);
+    return sf;
+}
+
+std::vector<real_t> fold_duals(const StandardForm& sf, const std::vector<real_t>& y) {
+    std::vector<real_t> out(y.begin(), y.begin() + sf.m_model);
+    for (size_t k = 0; k < sf.lower_of.size(); ++k) out[sf.lower_of[k]] -= y[sf.m_model + k];
+    return out;
+}
+
+void max_violation(const StandardForm& sf, const std::vector<real_t>& x, real_t& row_viol, real_t& bound_viol) {
+    std::vector<real_t> ax = sf.A.multiply(x);
+    row_viol = 0.0;
+    for (int i = 0; i < sf.m; ++i) {
+        const real_t r = ax[i] - sf.b[i];
+        const real_t v = sf.is_eq[i] ? std::abs(r) : std::max(0.0, r);
+        row_viol = std::max(row_viol, v);
+    }
+    bound_viol = 0.0;
+    for (int j = 0; j < sf.n; ++j) {
+        if (x[j] < sf.lb[j] - 1e-12) bound_viol = std::max(bound_viol, sf.lb[j] - x[j]);
+        if (x[j] > sf.ub[j] + 1e-12) bound_viol = std::max(bound_viol, x[j] - sf.ub[j]);
+    }
+}
+
+bo_lp as_bo_lp(const StandardForm& sf) {
+    bo_lp lp{};
+    lp.m = sf.m;
+    lp.n = sf.n;
+    lp.nnz = static_cast<long long>(sf.A.nnz());
+    lp.row_ptr = sf.A.row_offsets().data();
+    lp.col_idx = sf.A.col_indices().data();
+    lp.val = sf.A.values().data();
+    lp.b = sf.b.data();
+    lp.c = sf.c.data();
+    lp.lb = sf.lb.data();
+    lp.ub = sf.ub.data();
+    lp.is_eq = sf.is_eq.data();
+    return lp;
+}
+
+std::string device_error(bo_status s) {
+    const char* e = bo_last_error();
+    return std::string("GPU execution failed (status ") + std::to_string(static_cast<int>(s)) + "): " + (e ? e : "");
+}
+
+}  // namespace
+
+CPUFirstOrderSolver::CPUFirstOrderSolver(FirstOrderSolverOptions options) : options_(options) {}
+
+FirstOrderSolverResult CPUFirstOrderSolver::solve(const LPModel&) {
+    FirstOrderSolverResult r;
+    r.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+    r.message = "CPUFirstOrderSolver is not part of the MI355X library; use backend GPU_FIRST_ORDER.";
+    return r;
+}
+
+GPUFirstOrderSolver::GPUFirstOrderSolver(FirstOrderSolverOptions options) : options_(options) {}
+
+FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
+    FirstOrderSolverResult result;
+    const auto t0 = clk::now();
+
+    GpuBackend& gpu = GpuBackend::instance();
+    gpu.initialize();
+    if (!gpu.is_available()) {
+        result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+        result.message = "GPU unavailable: " + gpu.device_info().status_message;
+        return result;
+    }
+
+    if (!ModelValidator().validate(model).is_valid()) {
+        result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+        result.message = "Model validation failed before First-Order LP Solver.";
+        return result;
+    }
+
+    PresolveResult pre = PresolveEngine().presolve(model);
+    if (pre.status == PresolveStatus::INFEASIBLE) {
+        result.status = FirstOrderSolverStatus::INFEASIBLE_DETECTED;
+        result.message = "Presolve detected infeasibility.";
+        return result;
+    }
+    const LPModel& work = pre.reduced_model;
+    const size_t m0 = work.num_constraints(), n0 = work.num_variables();
+    const real_t sense_mult = work.sense() == ObjectiveSense::MINIMIZE ? 1.0 : -1.0;
+
+    if (m0 == 0 || n0 == 0) {  // bound-only problem: each variable at its cheaper bound
+        std::vector<real_t> x(n0, 0.0);
+        for (size_t j = 0; j < n0; ++j) {
+            const Variable& v = work.variables()[j];
+            const real_t cj = v.obj_coeff * sense_mult;
+            const real_t l = v.lower_bound, u = v.upper_bound;
+            if (cj > 1e-12) x[j] = (l > -BHARATOPT_INFINITY) ? l : 0.0;
+            else if (cj < -1e-12) x[j] = (u < BHARATOPT_INFINITY) ? u : 0.0;
+            else x[j] = clamp(0.0, (l > -BHARATOPT_INFINITY) ? l : -1e9, (u < BHARATOPT_INFINITY) ? u : 1e9);
+        }
+        result.primal_solution = pre.postsolve.recover_solution(x);
+        result.status = FirstOrderSolverStatus::OPTIMAL;
+        result.objective_value = original_objective(model, result.primal_solution);
+        result.stats.primal_objective = result.objective_value;
+        result.message = "LP model solved via bound optimization & presolve postsolve.";
+        return result;
+    }
+
+    const StandardForm sf = build_standard_form(work);
+    const bo_lp lp = as_bo_lp(sf);
+    std::vector<real_t> x(static_cast<size_t>(sf.n)), y(static_cast<size_t>(sf.m));
+    bool converged = false;
+
+    if (options_.engine == FirstOrderEngine::REFERENCE_PDHG) {
+        bo_ref_options o{};
+        o.max_iterations = static_cast<long long>(options_.max_iterations);
+        o.feasibility_tolerance = options_.feasibility_tolerance;
+        o.primal_step_size = options_.primal_step_size;
+        o.dual_step_size = options_.dual_step_size;
+        o.adaptive_step_size = options_.adaptive_step_size ? 1 : 0;
+        o.check_frequency = static_cast<long long>(options_.check_frequency);
+        bo_ref_result r{};
+        const bo_status s = bo_pdhg_ref_solve(&lp, &o, x.data(), y.data(), &r);
+        if (s != BO_SUCCESS) {
+            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+            result.message = device_error(s);
+            return result;
+        }
+        result.stats.constraint_violation = r.constraint_violation;
+        result.stats.bound_violation = r.bound_violation;
+        result.stats.kernel_time_ms = r.kernel_time_ms;
+        result.stats.transfer_time_ms = r.transfer_time_ms;
+        if (r.status == 2) {
+            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+            result.message = "Numerical failure: exploded iterate detected during PDHG iterations.";
+            return result;
+        }
+        converged = r.status == 0;
+        result.stats.iterations = static_cast<size_t>(r.iterations);
+    } else {
+        bo_pdlp_params p;
+        bo_pdlp_default_params(&p);
+        p.max_iterations = static_cast<long long>(options_.max_iterations);
+        p.eps_rel_kkt = options_.relative_kkt_tolerance;
+        bo_pdlp* h = nullptr;
+        bo_status s = bo_pdlp_create(&lp, &p, &h);
+        bo_pdlp_stats st{};
+        if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
+        if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
+        bo_pdlp_destroy(h);
+        if (s != BO_SUCCESS) {
+            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+            result.message = device_error(s);
+            return result;
+        }
+        if (st.status == 2) {
+            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+            result.message = "Numerical failure: non-finite KKT error during PDLP iterations.";
+            return result;
+        }
+        converged = st.status == 0;
+        result.stats.iterations = static_cast<size_t>(st.iterations);
+        result.stats.relative_kkt = st.rel_kkt;
+        result.stats.restarts = static_cast<size_t>(st.restarts);
+        result.stats.dual_objective = st.dual_obj * sense_mult + work.obj_offset();
+        result.stats.primal_residual = st.rel_primal;
+        result.stats.dual_residual = st.rel_dual;
+        result.stats.kernel_time_ms = st.solve_ms;
+        real_t rv = 0.0, bv = 0.0;
+        max_violation(sf, x, rv, bv);
+        result.stats.constraint_violation = rv;
+        result.stats.bound_violation = bv;
+    }
+    result.stats.solve_time_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    if (converged) {
+        result.status = FirstOrderSolverStatus::OPTIMAL;
+        result.message = "PDHG/PDLP-style first-order solver converged to OPTIMAL solution.";
+    } else {
+        result.status = FirstOrderSolverStatus::MAX_ITERATIONS;
+        result.message = "PDHG/PDLP-style first-order solver reached maximum iteration limit.";
+    }
+    result.primal_solution = pre.postsolve.recover_solution(x);
+    result.dual_solution = fold_duals(sf, y);
+    result.objective_value = original_objective(model, result.primal_solution);
+    result.stats.primal_objective = result.objective_value;
+    return result;
+}
+
+FirstOrderLPSolver::FirstOrderLPSolver(FirstOrderSolverOptions options) : options_(options) {}
+
+FirstOrderSolverResult FirstOrderLPSolver::solve(const LPModel& model) {
+    if (options_.backend_type == FirstOrderBackendType::GPU_FIRST_ORDER) return GPUFirstOrderSolver(options_).solve(model);
+    return CPUFirstOrderSolver(options_).solve(model);
+}
+
+}  // namespace bharatopt

@@ -1,0 +1,152 @@
+"""Writes the golden fixtures of tests/golden/ (run in the build container,
+where oracle/_ref -- the reference's own sources compiled by oracle/Makefile --
+and scipy's HiGHS are available):
+
+* ref_pdhg_kat.json   CPUFirstOrderSolver results of the reference build on
+                      seeded models: status, iteration count, objective,
+                      constraint violation and SHA-256 of the raw FP64 bytes of
+                      the primal and dual vectors (bit-exact pins), plus a
+                      SHA-256 of the model arrays so a changed generator is
+                      detected instead of silently compared.
+* lp_optima.json      optimal objectives of the PDLP test instances from two
+                      independent exact solvers: the reference's simplex
+                      chain (dual -> primal Dantzig -> primal Bland) where it
+                      is tractable, and HiGHS (scipy.optimize.linprog).
+
+Usage:  python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+pkg = __graft_entry__.load_package()
+lpd = pkg.lpdata
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        if a is None:
+            h.update(b"none")
+        else:
+            h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def model_sha(fm):
+    return sha(fm.lb, fm.ub, fm.obj, fm.con_sense, fm.rhs, fm.range_upper, fm.term_ptr,
+               fm.term_var, fm.term_val, np.array([fm.sense, fm.num_vars, fm.num_cons]))
+
+
+def kat_models():
+    """name -> (FlatModel, solver kwargs)"""
+    out = {}
+    out["benchmark10"] = (lpd.benchmark10_model(), {})
+    for s in (42, 43, 44):
+        out[f"random_300x1000_s{s}_it2000"] = (lpd.random_lp(300, 1000, 5, s).to_flat(),
+                                              dict(max_iterations=2000))
+    out["random_60x200_s7_converge"] = (lpd.random_lp(60, 200, 5, 7).to_flat(), {})
+    out["staircase_2x60x150_s5"] = (lpd.staircase_lp(2, 60, 150, 10, seed=5).to_flat(),
+                                    dict(max_iterations=3000))
+    mb = lpd.ModelBuilder(sense=1)  # max 3x + 5y, 2x + y <= 8, x + 2y <= 8
+    x = mb.add_variable(0.0, np.inf, 3.0)
+    y = mb.add_variable(0.0, np.inf, 5.0)
+    mb.add_constraint([(x, 2.0), (y, 1.0)], lpd.LE, 8.0)
+    mb.add_constraint([(x, 1.0), (y, 2.0)], lpd.LE, 8.0)
+    out["hand_lp_64_3"] = (mb.build(), {})
+    mb = lpd.ModelBuilder(sense=1)
+    x = mb.add_variable(0.0, 100.0, 5.0)
+    y = mb.add_variable(0.0, 100.0, 6.0)
+    mb.add_constraint([(x, 1.0), (y, 1.0)], lpd.LE, 50.0)
+    out["iteration_limit_5"] = (mb.build(), dict(max_iterations=5))
+    out["presolve_mix"] = (presolve_mix_model(), {})
+    out["nonadaptive_steps"] = (lpd.random_lp(40, 120, 4, 11).to_flat(),
+                                dict(adaptive_step_size=False, max_iterations=1500))
+    out["check_frequency_7"] = (lpd.random_lp(40, 120, 4, 12).to_flat(),
+                                dict(check_frequency=7, max_iterations=1500))
+    return out
+
+
+def presolve_mix_model():
+    """Exercises every presolve pass before PDHG: a fixed variable, an empty
+    column, a singleton row, GE and EQ rows, a free variable, maximisation."""
+    mb = lpd.ModelBuilder(sense=1)
+    v = [mb.add_variable(0.0, 10.0, 1.0 + 0.5 * j) for j in range(6)]
+    fixed = mb.add_variable(2.0, 2.0, 3.0)
+    empty = mb.add_variable(-1.0, 4.0, -2.0)
+    free = mb.add_variable(-np.inf, np.inf, 0.5)
+    mb.add_constraint([(v[0], 1.0), (v[1], 2.0), (fixed, 1.0)], lpd.LE, 12.0)
+    mb.add_constraint([(v[1], 1.0), (v[2], 1.0), (v[3], 1.0)], lpd.LE, 9.0)
+    mb.add_constraint([(v[2], 1.0), (v[4], -1.0)], lpd.GE, -3.0)
+    mb.add_constraint([(v[3], 1.0), (v[4], 1.0), (v[5], 1.0), (free, 1.0)], lpd.EQ, 10.0)
+    mb.add_constraint([(v[5], 2.0)], lpd.LE, 7.0)          # singleton row -> bound
+    mb.add_constraint([(free, 1.0), (v[0], 1.0)], lpd.LE, 6.0)
+    mb.add_constraint([(free, -1.0), (v[1], 1.0)], lpd.LE, 5.0)
+    _ = empty
+    return mb.build()
+
+
+def pdlp_instances():
+    out = {}
+    for s in (42, 43, 44, 45, 46):
+        out[f"random_300x1000_s{s}"] = lpd.random_lp(300, 1000, 5, s)
+    out["random_60x200_s7"] = lpd.random_lp(60, 200, 5, 7)
+    out["staircase_3x200x500_s3"] = lpd.staircase_lp(3, 200, 500, 20, seed=3)
+    out["zipf_2000x5000_s9"] = lpd.zipf_lp(2000, 5000, 1.5, 200, seed=9)
+    return out
+
+
+def highs(lp):
+    from scipy.optimize import linprog
+    from scipy.sparse import csr_matrix
+    A = csr_matrix((lp.val, lp.col_idx, lp.row_ptr), shape=(lp.m, lp.n))
+    eq = lp.is_eq.astype(bool)
+    res = linprog(lp.c, A_ub=A[~eq] if (~eq).any() else None, b_ub=lp.b[~eq] if (~eq).any() else None,
+                  A_eq=A[eq] if eq.any() else None, b_eq=lp.b[eq] if eq.any() else None,
+                  bounds=list(zip(lp.lb, lp.ub)), method="highs",
+                  options=dict(primal_feasibility_tolerance=1e-10, dual_feasibility_tolerance=1e-10))
+    assert res.status == 0, res.message
+    return float(res.fun)
+
+
+def main():
+    if po.ref_lib() is None:
+        raise SystemExit("oracle/_ref is not built; run make -C oracle first")
+    kat = {}
+    for name, (fm, kw) in kat_models().items():
+        r = po.ref_fo_solve(fm, **kw)
+        kat[name] = dict(kwargs=kw, model_sha=model_sha(fm), status=r["status"],
+                         iterations=r["iterations"], objective=float.hex(r["objective"]),
+                         constraint_violation=float.hex(r["constraint_violation"]),
+                         x_sha=sha(r["x"]), y_sha=sha(r["y"]), x_len=len(r["x"]), y_len=len(r["y"]))
+        print(name, r["status"], r["iterations"], r["objective"], flush=True)
+    with open(os.path.join(HERE, "ref_pdhg_kat.json"), "w") as f:
+        json.dump(kat, f, indent=1, sort_keys=True)
+
+    opt = {}
+    for name, lp in pdlp_instances().items():
+        entry = dict(model_sha=sha(lp.row_ptr, lp.col_idx, lp.val, lp.b, lp.c, lp.lb, lp.ub, lp.is_eq),
+                     highs=highs(lp))
+        if lp.m * lp.n <= 400_000:
+            s = po.ref_simplex(lp.to_flat())
+            if s["status"] == 0:
+                entry["reference_simplex"] = s["objective"]
+                entry["reference_simplex_stage"] = s["stage"]
+        opt[name] = entry
+        print(name, entry, flush=True)
+    with open(os.path.join(HERE, "lp_optima.json"), "w") as f:
+        json.dump(opt, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
