@@ -47,14 +47,16 @@ def log(*a):
 
 def fused_kernel_bytes(lp):
     """Algorithmic bytes of one launch of each fused iteration kernel.
-    CSR pass (A x + dual step): 8 B/nnz (fp32 value + int32 index), row
-    pointers, x gathered once (4n), and per row ax r/w, y r/w, b, is_eq,
-    the FP64 running sum r/w: 37 B/row.
-    CSC pass (A'y + primal step): 8 B/nnz, column pointers, y gathered once
-    (4m), and per column x r/w, c, l, u, FP64 running sum r/w: 36 B/column."""
+    CSR pass (A x+ and the dual Halpern step): 8 B/nnz (fp32 value + int32
+    index), row pointers, x+ gathered once (4n), and per row: carried A z
+    r/w (FP64, 16), dual iterate r/w (FP64, 16), b (4), is_eq (1), dual
+    anchor (8), A z0 (8), y+ and the fp32 copy of y written (8): 61 B/row.
+    CSC pass (A'y and the primal Halpern step): 8 B/nnz, column pointers,
+    y copy gathered once (4m), and per column: primal iterate r/w (FP64, 16),
+    c, l, u (12), primal anchor (8), x+ written (4): 40 B/column."""
     m, n, z = lp.m, lp.n, lp.nnz
-    csr = 8.0 * z + 4.0 * (m + 1) + 4.0 * n + 37.0 * m
-    csc = 8.0 * z + 4.0 * (n + 1) + 4.0 * m + 36.0 * n
+    csr = 8.0 * z + 4.0 * (m + 1) + 4.0 * n + 61.0 * m
+    csc = 8.0 * z + 4.0 * (n + 1) + 4.0 * m + 40.0 * n
     return csr, csc
 
 
@@ -127,6 +129,12 @@ def main():
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="replay only the two fused iteration kernels (and a 256 MiB copy "
+                         "kernel for counter calibration); for hardware-counter runs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each kernel directly instead of replaying hipGraphs (same kernels; "
+                         "used under rocprofv3, whose kernel tracing does not survive graph capture here)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -144,9 +152,19 @@ def main():
     log(f"workload {args.workload}: {desc}; m={lp.m} n={lp.n} nnz={lp.nnz}; "
         f"generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    h = pkg.Pdlp(lp, eps_rel_kkt=args.eps, max_iterations=200000)
+    h = pkg.Pdlp(lp, eps_rel_kkt=args.eps, max_iterations=200000, use_graph=0 if args.no_graph else 1)
     setup_s = time.perf_counter() - t0
-    log(f"upload+autotune {setup_s:.2f}s lanes_rows={h.params.lanes_rows} lanes_cols={h.params.lanes_cols}")
+    log(f"upload+autotune {setup_s:.2f}s")
+
+    if args.kernels_only:
+        csr_ms, csc_ms = h.time_kernels(reps=20)
+        copy_bytes = 256 << 20
+        copy_ms = pkg.copy_kernel_ms(copy_bytes, reps=5)
+        csr_b, csc_b = fused_kernel_bytes(lp)
+        print(json.dumps({"mode": "kernels-only", "workload": desc, "csr_pass_dual_step_ms": csr_ms,
+                          "csc_pass_primal_step_ms": csc_ms, "csr_pass_bytes": csr_b, "csc_pass_bytes": csc_b,
+                          "copy_kernel_bytes_read": copy_bytes, "copy_kernel_ms": copy_ms}), flush=True)
+        return
 
     for w in range(args.warmup):
         s = h.solve()
@@ -192,7 +210,7 @@ def main():
         "config": {
             "workload": desc, "m": lp.m, "n": lp.n, "nnz": lp.nnz, "eps_rel_kkt": args.eps,
             "iterates": "fp32", "checks": "fp64", "start": "cold",
-            "lanes_rows": h.params.lanes_rows, "lanes_cols": h.params.lanes_cols,
+            "lanes_rows": int(last["lanes_rows"]), "lanes_cols": int(last["lanes_cols"]),
             "parallelism": "single-gpu",
         },
         "solved": solved,

@@ -80,6 +80,12 @@ bo_status bo_axpy_f64(double alpha, const double* d_x, const double* d_y_in, dou
 bo_status bo_dot_f64(const double* d_a, const double* d_b, double* result, size_t n, bo_timing* t);
 bo_status bo_norm2_f64(const double* d_v, double* result, size_t n, bo_timing* t);
 
+/* Measurement utility (no reference counterpart): `reps` launches of a plain
+ * 16-byte-vector copy kernel over `bytes` bytes (read once, written once) on
+ * the default stream; *ms is the mean launch time from HIP events.  Used to
+ * calibrate hardware byte counters against a kernel of known traffic. */
+bo_status bo_copy_kernel_calibrate(size_t bytes, int reps, double* ms);
+
 /* ---- standard-form LP ----------------------------------------------------- */
 /* min c'x s.t. row i: a_i x <= b_i (a_i x == b_i when is_eq[i]), lb <= x <= ub.
  * This is the form CPUFirstOrderSolver::solve builds (src/first_order_solver.cpp:68-101).
@@ -158,6 +164,8 @@ typedef struct bo_pdlp_stats {
     double solve_ms;
     double spmv_csr_bytes; /* algorithmic bytes of one A x product    */
     double spmv_csc_bytes; /* algorithmic bytes of one A' y product   */
+    int lanes_rows;        /* nonzeros-per-lane target the tuner chose for the CSR pass */
+    int lanes_cols;        /* ... and for the CSC pass */
 } bo_pdlp_stats;
 
 /* Default parameters. */

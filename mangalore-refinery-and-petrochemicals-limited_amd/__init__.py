@@ -82,7 +82,7 @@ class PdlpStats(C.Structure):
                 ("used_average", C.c_int)] + [(k, C.c_double) for k in (
                     "rel_kkt", "rel_primal", "rel_dual", "rel_gap", "primal_obj", "dual_obj",
                     "primal_weight", "step_size", "setup_ms", "solve_ms", "spmv_csr_bytes",
-                    "spmv_csc_bytes")]
+                    "spmv_csc_bytes")] + [("lanes_rows", C.c_int), ("lanes_cols", C.c_int)]
 
 
 class BoModel(C.Structure):
@@ -113,7 +113,7 @@ EXPORTS = [
     "bo_spmv_csr_f64", "bo_vec_add_f64", "bo_vec_sub_f64", "bo_vec_scale_f64", "bo_axpy_f64",
     "bo_dot_f64", "bo_norm2_f64", "bo_pdhg_ref_solve", "bo_pdlp_default_params", "bo_pdlp_create",
     "bo_pdlp_set_warm_start", "bo_pdlp_solve", "bo_pdlp_get_solution", "bo_pdlp_destroy",
-    "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve",
+    "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
 ]
 
 
@@ -150,6 +150,7 @@ def lib():
         L.bo_pdlp_time_spmv.argtypes = [P, C.c_int, P, P]
         L.bo_first_order_solve.argtypes = [P, P, P, P, P]
         L.bo_presolve_stats.argtypes = [P, P, P, P, P, P]
+        L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
         for name in EXPORTS:
             if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params"):
                 getattr(L, name).restype = C.c_int
@@ -333,6 +334,13 @@ class _ModelHolder:
         self.s = BoModel(fm.num_vars, fm.num_cons, int(fm.sense), float(fm.obj_offset),
                          _p(a["lb"]), _p(a["ub"]), _p(a["obj"]), _p(a["vt"]), _p(a["cs"]),
                          _p(a["rhs"]), _p(a["ru"]), _p(a["tp"]), _p(a["tv"]), _p(a["tw"]))
+
+
+def copy_kernel_ms(nbytes, reps=20):
+    """Mean time of a plain copy kernel over nbytes (read + write)."""
+    ms = C.c_double()
+    _check(lib().bo_copy_kernel_calibrate(int(nbytes), int(reps), C.byref(ms)), "bo_copy_kernel_calibrate")
+    return ms.value
 
 
 def presolve_stats(fm: FlatModel):

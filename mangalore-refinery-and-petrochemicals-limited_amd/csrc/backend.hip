@@ -109,6 +109,13 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int count, do
     if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(t) : t;
 }
 
+// Plain streaming copy, 16 B per lane per step (calibration of byte counters).
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 struct EventTimer {
     hipEvent_t a{}, b{};
     bool on;
@@ -290,6 +297,40 @@ bo_status bo_dot_f64(const double* a, const double* b, double* result, size_t n,
 }
 bo_status bo_norm2_f64(const double* v, double* result, size_t n, bo_timing* t) {
     return reduce_op(v, v, result, n, 1, t);
+}
+
+bo_status bo_copy_kernel_calibrate(size_t bytes, int reps, double* ms) {
+    bo_status st = ensure_device();
+    if (st != BO_SUCCESS) return st;
+    if (bytes < 16 || reps < 1 || !ms) return BO_INVALID_ARGUMENT;
+    const size_t n16 = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    BO_TRY(hipMalloc(&a, n16 * 16));
+    hipError_t e = hipMalloc(&b, n16 * 16);
+    if (e != hipSuccess) {
+        hipFree(a);
+        BO_TRY(e);
+    }
+    hipMemset(a, 1, n16 * 16);
+    const unsigned g = 256u * 64u;  // 64 workgroups per CU on a 256-CU part
+    hipLaunchKernelGGL(k_copy16, dim3(g), dim3(kBlock), 0, 0, a, b, n16);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(g), dim3(kBlock), 0, 0, a, b, n16);
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipError_t le = hipGetLastError();
+    hipFree(a);
+    hipFree(b);
+    BO_TRY(le);
+    *ms = static_cast<double>(t) / reps;
+    return BO_SUCCESS;
 }
 
 }  // extern "C"

@@ -1,27 +1,33 @@
-// pdlp.hip -- restarted PDHG (PDLP-style) engine for one LP on one MI355X.
+// pdlp.hip -- restarted, reflected Halpern PDHG engine for one LP on one MI355X.
 //
 // Replaces the loop of CPUFirstOrderSolver::solve (src/first_order_solver.cpp:158-247)
-// when the caller asks for a rel-KKT target.  Algorithm (from the PDLP papers,
-// Applegate et al. 2021/2023, not from the reference, which has none of it):
+// when the caller asks for a relative-KKT target.  The algorithm comes from
+// the PDLP line of work (Applegate et al. 2021; Lu & Yang, cuPDLP 2023 and
+// the restarted Halpern PDHG of cuPDLPx 2025), not from the reference, which
+// has none of it:
 //   * Ruiz equilibration (inf-norm, 10 sweeps) + Pock-Chambolle (alpha = 1),
-//     all on the device from the resident FP64 matrix;
-//   * constant step eta = 0.95 / ||A_hat||_2 (power iteration on the device),
+//     computed on the device from the resident FP64 matrix on every solve;
+//   * constant step eta = 0.95 / ||A_hat||_2 (device power iteration),
 //     tau = eta / omega, sigma = eta * omega;
-//   * iteration (reference sign convention: min c'x, Ax <= b / = b, y_I >= 0):
-//         x+ = proj_[l,u](x - tau (c + A'y))            CSC pass, fused epilogue
-//         y+ = proj_Y(y + sigma (2 A x+ - A x - b))     CSR pass, fused epilogue
-//     two launches per iteration; A x is carried so the extrapolation needs no
-//     third product;
-//   * every `check_every` iterations an FP64 check on the *unscaled* problem
-//     for the current and the average iterate (rel-KKT, weighted KKT), then a
+//   * one PDHG step T(x, y) = (x+, y+) in the reference sign convention
+//     (min c'x, Ax <= b or = b, y_I >= 0):
+//         x+ = proj_[l,u](x - tau (c + A'y))             CSC pass, fused epilogue
+//         y+ = proj_Y(y + sigma (A (2 x+ - x) - b))       CSR pass, fused epilogue
+//     followed by the reflected Halpern update towards the restart anchor z0:
+//         z <- w (2 T(z) - z) + (1 - w) z0,   w = (k+1)/(k+2), k = steps since restart.
+//     A x is carried (A z is linear in the update), so each step costs
+//     exactly one CSC and one CSR pass, two launches;
+//   * every `check_every` steps an FP64 check on the *unscaled* problem for
+//     the current iterate and for T(z) (rel-KKT and weighted KKT), then a
 //     one-workgroup decision kernel applies the KKT restart rule
 //     (beta 0.2 / 0.8 / 0.36), the primal-weight update (theta 0.5) and the
 //     stopping test -- no host round trip inside an interval, which is
 //     captured once as a hipGraph and replayed.
-// Iterates and matrix values are FP32 (or FP64 for the error-comparison run);
-// all accumulations, checks, averages and the returned solution are FP64.
-// Rows and columns are stored sorted by length so the length-binned segment
-// kernels (segments.hpp) cover a whole product in one launch.
+// Precision: matrix values and the vectors the SpMVs gather (x+ and a copy of
+// y) are FP32 (FP64 in the error-comparison mode); accumulations, the
+// Halpern state (iterate, anchor, carried A x), checks and the returned
+// solution are FP64.  Rows and columns are stored sorted by descending
+// length so one length-binned segment launch (segments.hpp) covers a product.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,14 +52,17 @@ struct PdlpState {
     double eta, omega, tau, sigma;
     double b_norm, c_norm, eps;
     double kkt_last_restart, kkt_prev_candidate;
+    // [0] current iterate z, [1] candidate T(z)
     double rel_kkt[2], rel_primal[2], rel_dual[2], rel_gap[2], pobj[2], dobj[2], kkt_w[2];
-    double sigma_max, pw_u2, pw_w2;
+    double sigma_max;
     long long iter, iter_last_restart, max_iter;
-    int done, status, restart, restart_to_avg, best, restarts, checks, pad;
+    int done, status, restart, restart_to_cand, best, restarts, checks, pad;
 };
 
-__device__ inline double avg_count(const PdlpState* st) {
-    return static_cast<double>(st->iter - st->iter_last_restart);
+// Halpern weight of step t of the current interval.
+__device__ inline double halpern_w(const PdlpState* st, int t) {
+    const double k = static_cast<double>(st->iter - st->iter_last_restart + t);
+    return (k + 1.0) / (k + 2.0);
 }
 
 template <class T>
@@ -62,7 +71,7 @@ __device__ inline T clampT(T v, T lo, T hi) {
 }
 
 // ---------------------------------------------------------------- epilogues
-// y = A v (optionally scaled by seg_scale), partial sum of squares of y.
+// out = A v, partial sum of squares of out.
 template <class TX, class TO>
 struct EpiStore {
     using TXv = TX;
@@ -107,75 +116,81 @@ struct EpiWrite {
     __device__ void operator()(int, const double*, double*) const {}
 };
 
-// Primal step on column j (CSC pass gathers y).
+// Column j: primal step from A'y (gathered y copy), Halpern update of x.
 template <class T>
 struct EpiPrimal {
     using TXv = T;
     static constexpr int NV = 1;
     static constexpr int Q = 0;
-    const T* y;
+    const T* yg;       // gathered: the current dual iterate, rounded to T
     const PdlpState* st;
     const T* c;
     const T* lb;
     const T* ub;
-    T* x;
-    double* xsum;
-    __device__ const T* vec(int) const { return y; }
+    double* xk;        // Halpern iterate (in/out)
+    const double* x0;  // restart anchor
+    T* xp;             // x+ (gathered by the CSR pass)
+    int t;             // step index within the interval
+    __device__ const T* vec(int) const { return yg; }
     __device__ void operator()(int j, const double* acc, double*) const {
-        const double tau = st->tau;
-        const double xo = static_cast<double>(x[j]);
-        const double xn = xo - tau * (static_cast<double>(c[j]) + acc[0]);
-        const T xt = clampT(static_cast<T>(xn), lb[j], ub[j]);
-        x[j] = xt;
-        xsum[j] += static_cast<double>(xt);
+        const double xo = xk[j];
+        const T xt = clampT(static_cast<T>(xo - st->tau * (static_cast<double>(c[j]) + acc[0])), lb[j], ub[j]);
+        xp[j] = xt;
+        const double w = halpern_w(st, t);
+        xk[j] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[j];
     }
 };
 
-// Dual step on row i (CSR pass gathers the new x); carries A x.
+// Row i: dual step from A x+ (gathered x+), Halpern update of y and of A x.
 template <class T>
 struct EpiDual {
     using TXv = T;
     static constexpr int NV = 1;
     static constexpr int Q = 0;
-    const T* x;
+    const T* xp;       // gathered: x+
     const PdlpState* st;
     const T* b;
     const unsigned char* eq;
-    T* ax;
-    T* y;
-    double* ysum;
-    __device__ const T* vec(int) const { return x; }
+    double* ax;        // A z (carried, in/out)
+    const double* ax0; // A z0
+    double* yk;        // Halpern iterate (in/out)
+    const double* y0;  // restart anchor
+    T* yp;             // y+ (restart candidate)
+    T* yg;             // yk rounded to T (gathered by the next CSC pass)
+    int t;
+    __device__ const T* vec(int) const { return xp; }
     __device__ void operator()(int i, const double* acc, double*) const {
-        const double sig = st->sigma;
-        const double axo = static_cast<double>(ax[i]);
-        double yn = static_cast<double>(y[i]) + sig * (2.0 * acc[0] - axo - static_cast<double>(b[i]));
+        const double axo = ax[i], yo = yk[i];
+        double yn = yo + st->sigma * (2.0 * acc[0] - axo - static_cast<double>(b[i]));
         if (!eq[i] && yn < 0.0) yn = 0.0;
         const T yt = static_cast<T>(yn);
-        y[i] = yt;
-        ax[i] = static_cast<T>(acc[0]);
-        ysum[i] += static_cast<double>(yt);
+        yp[i] = yt;
+        const double w = halpern_w(st, t);
+        const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[i];
+        yk[i] = ynew;
+        yg[i] = static_cast<T>(ynew);
+        ax[i] = w * (2.0 * acc[0] - axo) + (1.0 - w) * ax0[i];
     }
 };
 
-// FP64 check, row side: gathers unscaled x (current, average).
-// partials: [0,1] ||viol||^2 (cur, avg)  [2,3] scaled  [4,5] b'y (cur, avg)
+// FP64 check, row side: gathers unscaled x (current, candidate).
+// partials: [0,1] ||viol||^2 (cur, cand)  [2,3] scaled  [4,5] b'y (cur, cand)
 template <class T>
 struct EpiCheckRows {
     using TXv = double;
     static constexpr int NV = 2;
     static constexpr int Q = 6;
     const double* xo_cur;
-    const double* xo_avg;
-    const PdlpState* st;
+    const double* xo_cand;
     const double* b;
     const unsigned char* eq;
     const double* rs;
-    const T* y;
-    const double* ysum;
+    const double* yk;
+    const T* yp;
     double* yo_cur;
-    double* yo_avg;
-    T* ax_avg;  // scaled A x_avg, used when restarting to the average
-    __device__ const double* vec(int v) const { return v == 0 ? xo_cur : xo_avg; }
+    double* yo_cand;
+    double* ax_cand;  // scaled A x_cand, taken over when restarting to the candidate
+    __device__ const double* vec(int v) const { return v == 0 ? xo_cur : xo_cand; }
     __device__ void operator()(int i, const double* acc, double* part) const {
         const double bi = b[i], ri = rs[i];
         const bool e = eq[i] != 0;
@@ -186,30 +201,30 @@ struct EpiCheckRows {
             part[p] += v * v;
             part[2 + p] += (ri * v) * (ri * v);
         }
-        ax_avg[i] = static_cast<T>(ri * acc[1]);
-        const double cnt = avg_count(st);
-        const double yc = ri * static_cast<double>(y[i]);
-        const double ya = cnt > 0.0 ? ri * (ysum[i] / cnt) : yc;
+        ax_cand[i] = ri * acc[1];
+        double yc = ri * yk[i];
+        if (!e && yc < 0.0) yc = 0.0;  // the reported dual is sign-feasible
+        const double ya = ri * static_cast<double>(yp[i]);
         yo_cur[i] = yc;
-        yo_avg[i] = ya;
+        yo_cand[i] = ya;
         part[4] += bi * yc;
         part[5] += bi * ya;
     }
 };
 
-// FP64 check, column side: gathers unscaled y (current, average).
+// FP64 check, column side: gathers unscaled y (current, candidate).
 // partials: [0,1] ||dual res||^2  [2,3] scaled  [4,5] sum l*plam^+ + u*plam^-
 struct EpiCheckCols {
     using TXv = double;
     static constexpr int NV = 2;
     static constexpr int Q = 6;
     const double* yo_cur;
-    const double* yo_avg;
+    const double* yo_cand;
     const double* c;
     const double* lb;
     const double* ub;
     const double* cs;
-    __device__ const double* vec(int v) const { return v == 0 ? yo_cur : yo_avg; }
+    __device__ const double* vec(int v) const { return v == 0 ? yo_cur : yo_cand; }
     __device__ void operator()(int j, const double* acc, double* part) const {
         const double l = lb[j], u = ub[j], sj = cs[j];
         const bool lf = isfinite(l), uf = isfinite(u);
@@ -244,37 +259,43 @@ __global__ void k_apply_norm(double* scale, const double* norm, size_t n) {
     }
 }
 
-// Scaled vectors: c_hat = s c, l_hat = l / s, u_hat = u / s  (columns);
-// b_hat = r b (rows).  Also the initial point.
+// Scaled column data c_hat = s c, l_hat = l / s, u_hat = u / s and the
+// starting point (projection of 0 or of the warm start) as iterate + anchor.
 template <class T>
 __global__ void k_scale_cols(int n, const double* s, const double* c, const double* lb, const double* ub,
-                             const double* wx, int has_warm, T* c_hat, T* l_hat, T* u_hat, T* x,
-                             T* x_last, double* xsum) {
+                             const double* wx, int has_warm, T* c_hat, T* l_hat, T* u_hat, double* xk,
+                             double* x0, T* xp) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const double sj = s[j];
     c_hat[j] = static_cast<T>(c[j] * sj);
-    const T lo = static_cast<T>(lb[j] / sj), hi = static_cast<T>(ub[j] / sj);
-    l_hat[j] = lo;
-    u_hat[j] = hi;
-    const T x0 = clampT(static_cast<T>(has_warm ? wx[j] / sj : 0.0), lo, hi);
-    x[j] = x0;
-    x_last[j] = x0;
-    xsum[j] = 0.0;
+    const double lo = lb[j] / sj, hi = ub[j] / sj;
+    l_hat[j] = static_cast<T>(lo);
+    u_hat[j] = static_cast<T>(hi);
+    const double xs = clampT(has_warm ? wx[j] / sj : 0.0, lo, hi);  // the FP64 state keeps a warm start exact
+    xk[j] = xs;
+    x0[j] = xs;
+    xp[j] = static_cast<T>(xs);
 }
 
 template <class T>
 __global__ void k_scale_rows(int m, const double* r, const double* b, const unsigned char* eq,
-                             const double* wy, int has_warm, T* b_hat, T* y, T* y_last, double* ysum) {
+                             const double* wy, int has_warm, T* b_hat, double* yk, double* y0, T* yg, T* yp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const double ri = r[i];
     b_hat[i] = static_cast<T>(b[i] * ri);
-    double y0 = has_warm ? wy[i] / ri : 0.0;
-    if (!eq[i] && y0 < 0.0) y0 = 0.0;
-    y[i] = static_cast<T>(y0);
-    y_last[i] = static_cast<T>(y0);
-    ysum[i] = 0.0;
+    double ys = has_warm ? wy[i] / ri : 0.0;
+    if (!eq[i] && ys < 0.0) ys = 0.0;
+    yk[i] = ys;
+    y0[i] = ys;
+    yg[i] = static_cast<T>(ys);
+    yp[i] = static_cast<T>(ys);
+}
+
+__global__ void k_copy_f64(double* dst, const double* src, size_t n) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // Power iteration helpers.
@@ -320,7 +341,7 @@ __global__ void k_init_state(PdlpState* st, const double* pw_u2, const double* c
     st->done = 0;
     st->status = 1;
     st->restart = 0;
-    st->restart_to_avg = 0;
+    st->restart_to_cand = 0;
     st->best = 0;
     st->restarts = 0;
     st->checks = 0;
@@ -344,24 +365,23 @@ __global__ void k_advance(PdlpState* st, int k) {
     if (threadIdx.x == 0 && !st->done) st->iter += k;
 }
 
-// Unscaled x for the check: xo = clamp(s x_hat, l, u) (current and average);
+// Unscaled x for the check: xo = clamp(s x, l, u) for the iterate and for x+;
 // partials [0,1] = c'xo.
 template <class T>
-__global__ void k_check_x(int n, const PdlpState* st, const T* x, const double* xsum, const double* s,
+__global__ void k_check_x(int n, const PdlpState* st, const double* xk, const T* xp, const double* s,
                           const double* c, const double* lb, const double* ub, double* xo_cur,
-                          double* xo_avg, double* partials) {
+                          double* xo_cand, double* partials) {
     if (st->done) return;
     __shared__ double sh[kBlock / kWave];
-    const double cnt = avg_count(st);
     double p0 = 0.0, p1 = 0.0;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const double sj = s[j], l = lb[j], u = ub[j];
-        double xc = sj * static_cast<double>(x[j]);
+        double xc = sj * xk[j];
         xc = xc < l ? l : (xc > u ? u : xc);
-        double xa = cnt > 0.0 ? sj * (xsum[j] / cnt) : xc;
+        double xa = sj * static_cast<double>(xp[j]);
         xa = xa < l ? l : (xa > u ? u : xa);
         xo_cur[j] = xc;
-        xo_avg[j] = xa;
+        xo_cand[j] = xa;
         p0 += c[j] * xc;
         p1 += c[j] * xa;
     }
@@ -379,25 +399,32 @@ struct CheckDims {
 
 // One workgroup: reduce the check partials in a fixed order, then the
 // stopping test, the KKT restart rule and the bookkeeping.
-__global__ void k_decide(PdlpState* st, CheckDims d, const double* px, const double* pr, const double* pc) {
+constexpr int kDecideThreads = 1024;
+
+__global__ void __launch_bounds__(kDecideThreads)
+k_decide(PdlpState* st, CheckDims d, const double* px, const double* pr, const double* pc) {
     if (st->done) return;
-    __shared__ double sh[kBlock / kWave];
+    __shared__ double sh[kDecideThreads / kWave];
+    double a[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) a[k] = 0.0;
+    for (int b = threadIdx.x; b < d.nb_x; b += blockDim.x) {
+        a[0] += px[b * 2 + 0];
+        a[1] += px[b * 2 + 1];
+    }
+    for (int b = threadIdx.x; b < d.nb_rows; b += blockDim.x) {
+        const double* p = pr + static_cast<size_t>(b) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a[2 + k] += p[k];
+    }
+    for (int b = threadIdx.x; b < d.nb_cols; b += blockDim.x) {
+        const double* p = pc + static_cast<size_t>(b) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a[8 + k] += p[k];
+    }
     double q[14];
-    for (int k = 0; k < 2; ++k) {
-        double a = 0.0;
-        for (int b = threadIdx.x; b < d.nb_x; b += blockDim.x) a += px[b * 2 + k];
-        q[k] = block_sum(a, sh);
-    }
-    for (int k = 0; k < 6; ++k) {
-        double a = 0.0;
-        for (int b = threadIdx.x; b < d.nb_rows; b += blockDim.x) a += pr[static_cast<size_t>(b) * 6 + k];
-        q[2 + k] = block_sum(a, sh);
-    }
-    for (int k = 0; k < 6; ++k) {
-        double a = 0.0;
-        for (int b = threadIdx.x; b < d.nb_cols; b += blockDim.x) a += pc[static_cast<size_t>(b) * 6 + k];
-        q[8 + k] = block_sum(a, sh);
-    }
+#pragma unroll
+    for (int k = 0; k < 14; ++k) q[k] = block_sum(a[k], sh);
     if (threadIdx.x != 0) return;
     // q: [0,1] c'x  [2,3] ||pviol||^2 [4,5] scaled [6,7] b'y  [8,9] ||dres||^2 [10,11] scaled [12,13] dterm
     const double w = st->omega;
@@ -437,7 +464,7 @@ __global__ void k_decide(PdlpState* st, CheckDims d, const double* px, const dou
         st->status = 1;
         return;
     }
-    if (st->iter == 0) {  // initial point
+    if (st->iter == 0) {  // starting point
         st->kkt_last_restart = st->kkt_w[0];
         st->kkt_prev_candidate = st->kkt_w[0];
         return;
@@ -451,51 +478,48 @@ __global__ void k_decide(PdlpState* st, CheckDims d, const double* px, const dou
     st->kkt_prev_candidate = kc;
     if (restart) {
         st->restart = 1;
-        st->restart_to_avg = cand;
+        st->restart_to_cand = cand;
         st->restarts += 1;
         st->kkt_last_restart = kc;
     }
 }
 
-// Restart: move to the candidate, measure the move since the last restart,
-// reset the averages.  partials [0] = ||dx||^2 (n side) or ||dy||^2 (m side).
+// Restart: move to the chosen point, make it the new anchor, measure the
+// move of the anchor.  partials [0] = ||dx||^2 (n side) or ||dy||^2 (m side).
 template <class T>
-__global__ void k_restart_x(int n, const PdlpState* st, T* x, T* x_last, double* xsum, double* partials) {
+__global__ void k_restart_x(int n, const PdlpState* st, double* xk, double* x0, const T* xp, double* partials) {
     if (st->done || !st->restart) return;
     __shared__ double sh[kBlock / kWave];
-    const double cnt = avg_count(st);
     double acc = 0.0;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        T xn = x[j];
-        if (st->restart_to_avg && cnt > 0.0) xn = static_cast<T>(xsum[j] / cnt);
-        x[j] = xn;
-        const double d = static_cast<double>(xn) - static_cast<double>(x_last[j]);
+        const double xn = st->restart_to_cand ? static_cast<double>(xp[j]) : xk[j];
+        const double d = xn - x0[j];
         acc += d * d;
-        x_last[j] = xn;
-        xsum[j] = 0.0;
+        xk[j] = xn;
+        x0[j] = xn;
     }
     double t = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 template <class T>
-__global__ void k_restart_y(int m, const PdlpState* st, T* y, T* y_last, double* ysum, T* ax,
-                            const T* ax_avg, double* partials) {
+__global__ void k_restart_y(int m, const PdlpState* st, double* yk, double* y0, const T* yp, T* yg, double* ax,
+                            double* ax0, const double* ax_cand, double* partials) {
     if (st->done || !st->restart) return;
     __shared__ double sh[kBlock / kWave];
-    const double cnt = avg_count(st);
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
-        T yn = y[i];
-        if (st->restart_to_avg && cnt > 0.0) {
-            yn = static_cast<T>(ysum[i] / cnt);
-            ax[i] = ax_avg[i];
+        double yn = yk[i];
+        if (st->restart_to_cand) {
+            yn = static_cast<double>(yp[i]);
+            ax[i] = ax_cand[i];
         }
-        y[i] = yn;
-        const double d = static_cast<double>(yn) - static_cast<double>(y_last[i]);
+        const double d = yn - y0[i];
         acc += d * d;
-        y_last[i] = yn;
-        ysum[i] = 0.0;
+        yk[i] = yn;
+        y0[i] = yn;
+        yg[i] = static_cast<T>(yn);
+        ax0[i] = ax[i];
     }
     double t = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -535,7 +559,8 @@ struct DevMem {
     }
 };
 
-// Bin table for segments sorted by ascending length.
+// Bin table for segments sorted by length (descending, so lanes per segment
+// fall from bin to bin and every lane count forms one contiguous bin).
 BinTable make_bins(const std::vector<int>& sorted_len, int per_lane) {
     BinTable bt{};
     static const int kLanes[kMaxBins] = {1, 2, 4, 8, 16, 32, 64, 256};
@@ -591,24 +616,25 @@ struct Engine final : EngineBase {
     std::vector<int> row_of;  // permuted position -> original row
     std::vector<int> col_of;  // permuted position -> original column
     std::vector<int> row_len_sorted, col_len_sorted;
-    // device: FP64 original (permuted) matrix in CSR and CSC
+    // FP64 original (permuted) matrix in CSR and CSC; scaled working values
     int *rp = nullptr, *ci = nullptr, *cp = nullptr, *ri = nullptr;
     double *rv64 = nullptr, *cv64 = nullptr;
-    T *rv = nullptr, *cv = nullptr;  // scaled working values
-    // FP64 original vectors (permuted)
+    T *rv = nullptr, *cv = nullptr;
+    // FP64 original vectors (permuted) and scaling
     double *b64 = nullptr, *c64 = nullptr, *lb64 = nullptr, *ub64 = nullptr;
     unsigned char* eq = nullptr;
     double *rs = nullptr, *cs = nullptr, *rnorm = nullptr, *cnorm = nullptr;
-    // working vectors
+    // scaled problem data and iteration state
     T *b_hat = nullptr, *c_hat = nullptr, *l_hat = nullptr, *u_hat = nullptr;
-    T *x = nullptr, *y = nullptr, *ax = nullptr, *ax_avg = nullptr, *x_last = nullptr, *y_last = nullptr;
+    double *xk = nullptr, *x0 = nullptr, *yk = nullptr, *y0 = nullptr, *ax = nullptr, *ax0 = nullptr;
+    T *xp = nullptr, *yp = nullptr, *yg = nullptr;
     T *pw_v = nullptr, *pw_u = nullptr;
-    double *xsum = nullptr, *ysum = nullptr, *xo_cur = nullptr, *xo_avg = nullptr, *yo_cur = nullptr,
-           *yo_avg = nullptr;
+    // check buffers
+    double *xo_cur = nullptr, *xo_cand = nullptr, *yo_cur = nullptr, *yo_cand = nullptr, *ax_cand = nullptr;
     double *wx = nullptr, *wy = nullptr;
     int has_warm = 0;
     double *part_x = nullptr, *part_r = nullptr, *part_c = nullptr, *part_dx = nullptr, *part_dy = nullptr;
-    double* scal = nullptr;  // small scalars: [0] ||u||^2 [1] ||w||^2 [2] ||c_hat||^2 [3] ||b_hat||^2
+    double* scal = nullptr;  // [0] ||u||^2 [1] ||w||^2 [2] ||c_hat||^2 [3] ||b_hat||^2
     PdlpState* st = nullptr;
     PdlpState* st_host = nullptr;  // pinned, read back after each interval
     PdlpState* st_init = nullptr;  // pinned, staged for the upload in prepare()
@@ -635,7 +661,8 @@ struct Engine final : EngineBase {
         BO_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PdlpState)));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_init), sizeof(PdlpState)));
-        // ---- permutations: rows and columns by ascending length (stable)
+        std::memset(st_host, 0, sizeof(PdlpState));
+        // ---- permutations: rows and columns by descending length (stable)
         std::vector<int> rlen(m), clen(n, 0);
         for (int i = 0; i < m; ++i) rlen[i] = lp->row_ptr[i + 1] - lp->row_ptr[i];
         for (long long k = 0; k < nnz; ++k) clen[lp->col_idx[k]]++;
@@ -643,8 +670,8 @@ struct Engine final : EngineBase {
         col_of.resize(n);
         std::iota(row_of.begin(), row_of.end(), 0);
         std::iota(col_of.begin(), col_of.end(), 0);
-        std::stable_sort(row_of.begin(), row_of.end(), [&](int a, int b) { return rlen[a] < rlen[b]; });
-        std::stable_sort(col_of.begin(), col_of.end(), [&](int a, int b) { return clen[a] < clen[b]; });
+        std::stable_sort(row_of.begin(), row_of.end(), [&](int a, int b) { return rlen[a] > rlen[b]; });
+        std::stable_sort(col_of.begin(), col_of.end(), [&](int a, int b) { return clen[a] > clen[b]; });
         std::vector<int> col_pos(n);
         for (int p2 = 0; p2 < n; ++p2) col_pos[col_of[p2]] = p2;
         row_len_sorted.resize(m);
@@ -709,14 +736,16 @@ struct Engine final : EngineBase {
         BO_TRY(mem.alloc(&rs, m));
         BO_TRY(mem.alloc(&rnorm, m));
         BO_TRY(mem.alloc(&b_hat, m));
-        BO_TRY(mem.alloc(&y, m));
+        BO_TRY(mem.alloc(&yk, m));
+        BO_TRY(mem.alloc(&y0, m));
+        BO_TRY(mem.alloc(&yp, m));
+        BO_TRY(mem.alloc(&yg, m));
         BO_TRY(mem.alloc(&ax, m));
-        BO_TRY(mem.alloc(&ax_avg, m));
-        BO_TRY(mem.alloc(&y_last, m));
+        BO_TRY(mem.alloc(&ax0, m));
+        BO_TRY(mem.alloc(&ax_cand, m));
         BO_TRY(mem.alloc(&pw_u, m));
-        BO_TRY(mem.alloc(&ysum, m));
         BO_TRY(mem.alloc(&yo_cur, m));
-        BO_TRY(mem.alloc(&yo_avg, m));
+        BO_TRY(mem.alloc(&yo_cand, m));
         BO_TRY(mem.alloc(&wy, m));
         BO_TRY(mem.alloc(&c64, n));
         BO_TRY(mem.alloc(&lb64, n));
@@ -726,12 +755,12 @@ struct Engine final : EngineBase {
         BO_TRY(mem.alloc(&c_hat, n));
         BO_TRY(mem.alloc(&l_hat, n));
         BO_TRY(mem.alloc(&u_hat, n));
-        BO_TRY(mem.alloc(&x, n));
-        BO_TRY(mem.alloc(&x_last, n));
+        BO_TRY(mem.alloc(&xk, n));
+        BO_TRY(mem.alloc(&x0, n));
+        BO_TRY(mem.alloc(&xp, n));
         BO_TRY(mem.alloc(&pw_v, n));
-        BO_TRY(mem.alloc(&xsum, n));
         BO_TRY(mem.alloc(&xo_cur, n));
-        BO_TRY(mem.alloc(&xo_avg, n));
+        BO_TRY(mem.alloc(&xo_cand, n));
         BO_TRY(mem.alloc(&wx, n));
         BO_TRY(mem.alloc(&scal, 8));
         BO_TRY(mem.alloc(&st, 1));
@@ -791,90 +820,106 @@ struct Engine final : EngineBase {
 
     const int* done_flag() const { return &st->done; }
 
-    void launch_primal(const BinTable& bt, const int* done) {
-        EpiPrimal<T> e{y, st, c_hat, l_hat, u_hat, x, xsum};
+    void launch_primal(const BinTable& bt, const int* done, int t) {
+        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t};
         launch_cols<kSumProd>(bt, cv, e, nullptr, done);
     }
-    void launch_dual(const BinTable& bt, const int* done) {
-        EpiDual<T> e{x, st, b_hat, eq, ax, y, ysum};
+    void launch_dual(const BinTable& bt, const int* done, int t) {
+        EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t};
         launch_rows<kSumProd>(bt, rv, e, nullptr, done);
     }
 
     void launch_check() {
-        hipLaunchKernelGGL(k_check_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, x, xsum, cs, c64, lb64,
-                           ub64, xo_cur, xo_avg, part_x);
-        EpiCheckRows<T> er{xo_cur, xo_avg, st, b64, eq, rs, y, ysum, yo_cur, yo_avg, ax_avg};
+        hipLaunchKernelGGL(k_check_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, xk, xp, cs, c64, lb64, ub64,
+                           xo_cur, xo_cand, part_x);
+        EpiCheckRows<T> er{xo_cur, xo_cand, b64, eq, rs, yk, yp, yo_cur, yo_cand, ax_cand};
         launch_rows<kSumProd>(bt_rows, rv64, er, part_r, done_flag());
-        EpiCheckCols ec{yo_cur, yo_avg, c64, lb64, ub64, cs};
+        EpiCheckCols ec{yo_cur, yo_cand, c64, lb64, ub64, cs};
         launch_cols<kSumProd>(bt_cols, cv64, ec, part_c, done_flag());
         CheckDims d{nb_x, bt_rows.total_blocks, bt_cols.total_blocks};
-        hipLaunchKernelGGL(k_decide, dim3(1), dim3(kBlock), 0, stream, st, d, part_x, part_r, part_c);
-        hipLaunchKernelGGL(k_restart_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, x, x_last, xsum, part_dx);
-        hipLaunchKernelGGL(k_restart_y<T>, dim3(nb_y), dim3(kBlock), 0, stream, m, st, y, y_last, ysum, ax,
-                           ax_avg, part_dy);
+        hipLaunchKernelGGL(k_decide, dim3(1), dim3(kDecideThreads), 0, stream, st, d, part_x, part_r, part_c);
+        hipLaunchKernelGGL(k_restart_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, xk, x0, xp, part_dx);
+        hipLaunchKernelGGL(k_restart_y<T>, dim3(nb_y), dim3(kBlock), 0, stream, m, st, yk, y0, yp, yg, ax, ax0,
+                           ax_cand, part_dy);
         hipLaunchKernelGGL(k_restart_finish, dim3(1), dim3(kBlock), 0, stream, st, part_dx, nb_x, part_dy, nb_y);
     }
 
-    // K iterations followed by the check.
+    // k steps followed by the check.
     void launch_interval(int k) {
-        for (int it = 0; it < k; ++it) {
-            launch_primal(bt_cols, done_flag());
-            launch_dual(bt_rows, done_flag());
+        for (int t = 0; t < k; ++t) {
+            launch_primal(bt_cols, done_flag(), t);
+            launch_dual(bt_rows, done_flag(), t);
         }
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, st, k);
         launch_check();
     }
 
+    // Alternating launches, as inside a solve (primal, dual, primal, ...).
+    bo_status time_pair(const BinTable& br, const BinTable& bc, int reps, double* csr_ms, double* csc_ms) {
+        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps) + 1);
+        for (auto& e : ev) BO_TRY(hipEventCreate(&e));
+        launch_primal(bc, nullptr, 0);
+        launch_dual(br, nullptr, 0);
+        for (int r = 0; r < reps; ++r) {
+            hipEventRecord(ev[2 * r], stream);
+            launch_primal(bc, nullptr, r);
+            hipEventRecord(ev[2 * r + 1], stream);
+            launch_dual(br, nullptr, r);
+        }
+        hipEventRecord(ev[2 * reps], stream);
+        hipEventSynchronize(ev[2 * reps]);
+        double sp = 0.0, sd = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            float a = 0.f, b = 0.f;
+            hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]);
+            hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]);
+            sp += a;
+            sd += b;
+        }
+        for (auto& e : ev) hipEventDestroy(e);
+        BO_TRY(hipGetLastError());
+        if (csc_ms) *csc_ms = sp / reps;
+        if (csr_ms) *csr_ms = sd / reps;
+        return BO_SUCCESS;
+    }
+
+    // Nonzeros-per-lane target of each pass, chosen by timing the fused
+    // iteration kernels in alternation (the cache state of a real solve).
     bo_status autotune() {
-        // time the two iteration kernels at candidate nonzeros-per-lane targets
-        const int cands[] = {1, 2, 4, 8, 16};
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
-        hipEvent_t e0, e1;
-        BO_TRY(hipEventCreate(&e0));
-        BO_TRY(hipEventCreate(&e1));
-        auto time_it = [&](auto&& fn) {
-            fn();
-            hipEventRecord(e0, stream);
-            for (int r = 0; r < 5; ++r) fn();
-            hipEventRecord(e1, stream);
-            hipEventSynchronize(e1);
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
-            return ms;
-        };
-        float best_r = 1e30f, best_c = 1e30f;
+        const int cands[] = {2, 4, 8, 16};
+        double best_r = 1e30, best_c = 1e30;
         int kr = 4, kc = 4;
+        const BinTable base_r = make_bins(row_len_sorted, 4), base_c = make_bins(col_len_sorted, 4);
         for (int kp : cands) {
+            double r_ms = 0.0, c_ms = 0.0;
             if (prm.lanes_rows <= 0) {
-                BinTable bt = make_bins(row_len_sorted, kp);
-                float t = time_it([&] { launch_dual(bt, nullptr); });
-                if (t < best_r) { best_r = t; kr = kp; }
+                bo_status s = time_pair(make_bins(row_len_sorted, kp), base_c, 4, &r_ms, nullptr);
+                if (s != BO_SUCCESS) return s;
+                if (r_ms < best_r) { best_r = r_ms; kr = kp; }
             }
             if (prm.lanes_cols <= 0) {
-                BinTable bt = make_bins(col_len_sorted, kp);
-                float t = time_it([&] { launch_primal(bt, nullptr); });
-                if (t < best_c) { best_c = t; kc = kp; }
+                bo_status s = time_pair(base_r, make_bins(col_len_sorted, kp), 4, nullptr, &c_ms);
+                if (s != BO_SUCCESS) return s;
+                if (c_ms < best_c) { best_c = c_ms; kc = kp; }
             }
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        BO_TRY(hipGetLastError());
         if (prm.lanes_rows <= 0) { bt_rows = make_bins(row_len_sorted, kr); prm.lanes_rows = kr; }
         if (prm.lanes_cols <= 0) { bt_cols = make_bins(col_len_sorted, kc); prm.lanes_cols = kc; }
         return BO_SUCCESS;
     }
 
-    bo_status set_warm_start(const double* x0, const double* y0) override {
-        if (!x0 && !y0) {
+    bo_status set_warm_start(const double* wx_h, const double* wy_h) override {
+        if (!wx_h && !wy_h) {
             has_warm = 0;
             return BO_SUCCESS;
         }
         std::vector<double> hx(n, 0.0), hy(m, 0.0);
-        if (x0)
-            for (int p2 = 0; p2 < n; ++p2) hx[p2] = x0[col_of[p2]];
-        if (y0)
-            for (int p2 = 0; p2 < m; ++p2) hy[p2] = y0[row_of[p2]];
+        if (wx_h)
+            for (int p2 = 0; p2 < n; ++p2) hx[p2] = wx_h[col_of[p2]];
+        if (wy_h)
+            for (int p2 = 0; p2 < m; ++p2) hy[p2] = wy_h[row_of[p2]];
         if (n) BO_TRY(hipMemcpyAsync(wx, hx.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
         if (m) BO_TRY(hipMemcpyAsync(wy, hy.data(), sizeof(double) * m, hipMemcpyHostToDevice, stream));
         BO_TRY(hipStreamSynchronize(stream));
@@ -882,7 +927,7 @@ struct Engine final : EngineBase {
         return BO_SUCCESS;
     }
 
-    // Scaling, step size, initial point, initial Ax: everything device-side.
+    // Scaling, step size, starting point, A x0: everything device-side.
     bo_status prepare() {
         const unsigned gm = grid_for(m), gn = grid_for(n);
         hipLaunchKernelGGL(k_fill_f64, dim3(gm), dim3(kBlock), 0, stream, rs, 1.0, static_cast<size_t>(m));
@@ -909,9 +954,9 @@ struct Engine final : EngineBase {
         EpiWrite<T> wc{cs, rs, cv};
         launch_cols<kWrite>(bc, cv64, wc, nullptr, nullptr);
         hipLaunchKernelGGL(k_scale_cols<T>, dim3(gn), dim3(kBlock), 0, stream, n, cs, c64, lb64, ub64, wx,
-                           has_warm, c_hat, l_hat, u_hat, x, x_last, xsum);
+                           has_warm, c_hat, l_hat, u_hat, xk, x0, xp);
         hipLaunchKernelGGL(k_scale_rows<T>, dim3(gm), dim3(kBlock), 0, stream, m, rs, b64, eq, wy, has_warm,
-                           b_hat, y, y_last, ysum);
+                           b_hat, yk, y0, yg, yp);
         // power iteration for ||A_hat||_2
         hipLaunchKernelGGL(k_pw_init<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n);
         hipLaunchKernelGGL(k_sq_partial<T>, dim3(nb_x), dim3(kBlock), 0, stream, pw_v, n, part_dx);
@@ -941,9 +986,10 @@ struct Engine final : EngineBase {
         st_init->max_iter = prm.max_iterations;
         BO_TRY(hipMemcpyAsync(st, st_init, sizeof(PdlpState), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, stream, st, scal + 0, scal + 2, scal + 3);
-        // initial A x
-        EpiStore<T, T> ea{x, ax};
+        // A x0 (the starting point is also the first anchor)
+        EpiStore<T, double> ea{xp, ax};
         launch_rows<kSumProd>(br, rv, ea, part_r, nullptr);
+        hipLaunchKernelGGL(k_copy_f64, dim3(gm), dim3(kBlock), 0, stream, ax0, ax, static_cast<size_t>(m));
         BO_TRY(hipGetLastError());
         return BO_SUCCESS;
     }
@@ -952,7 +998,7 @@ struct Engine final : EngineBase {
         auto t0 = std::chrono::high_resolution_clock::now();
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
-        launch_check();  // iteration 0: KKT of the starting point
+        launch_check();  // step 0: KKT of the starting point
         BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
         BO_TRY(hipStreamSynchronize(stream));
         auto t1 = std::chrono::high_resolution_clock::now();
@@ -1010,6 +1056,8 @@ struct Engine final : EngineBase {
             out->solve_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
             out->spmv_csr_bytes = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 4.0 * m;
             out->spmv_csc_bytes = 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 4.0 * n;
+            out->lanes_rows = prm.lanes_rows;
+            out->lanes_cols = prm.lanes_cols;
         }
         return BO_SUCCESS;
     }
@@ -1017,8 +1065,8 @@ struct Engine final : EngineBase {
     bo_status get_solution(double* hx, double* hy) override {
         const int b = st_host->best;
         std::vector<double> tx(n), ty(m);
-        if (n) BO_TRY(hipMemcpyAsync(tx.data(), b ? xo_avg : xo_cur, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
-        if (m) BO_TRY(hipMemcpyAsync(ty.data(), b ? yo_avg : yo_cur, sizeof(double) * m, hipMemcpyDeviceToHost, stream));
+        if (n) BO_TRY(hipMemcpyAsync(tx.data(), b ? xo_cand : xo_cur, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+        if (m) BO_TRY(hipMemcpyAsync(ty.data(), b ? yo_cand : yo_cur, sizeof(double) * m, hipMemcpyDeviceToHost, stream));
         BO_TRY(hipStreamSynchronize(stream));
         if (hx)
             for (int p2 = 0; p2 < n; ++p2) hx[col_of[p2]] = tx[p2];
@@ -1027,33 +1075,11 @@ struct Engine final : EngineBase {
         return BO_SUCCESS;
     }
 
-    // Times the two fused iteration kernels (CSR pass + dual step, CSC pass +
-    // primal step) on the handle's stream.  Leaves the iterate buffers
-    // modified: call after get_solution (solve() re-initialises everything).
+    // Times the two fused iteration kernels as they alternate in a solve.
+    // Leaves the iterate buffers modified: call after get_solution (solve()
+    // re-initialises everything).
     bo_status time_spmv(int reps, double* csr_ms, double* csc_ms) override {
-        if (reps < 1) reps = 1;
-        hipEvent_t e0, e1;
-        BO_TRY(hipEventCreate(&e0));
-        BO_TRY(hipEventCreate(&e1));
-        float ms = 0.f;
-        launch_dual(bt_rows, nullptr);
-        hipEventRecord(e0, stream);
-        for (int r = 0; r < reps; ++r) launch_dual(bt_rows, nullptr);
-        hipEventRecord(e1, stream);
-        hipEventSynchronize(e1);
-        hipEventElapsedTime(&ms, e0, e1);
-        if (csr_ms) *csr_ms = ms / reps;
-        launch_primal(bt_cols, nullptr);
-        hipEventRecord(e0, stream);
-        for (int r = 0; r < reps; ++r) launch_primal(bt_cols, nullptr);
-        hipEventRecord(e1, stream);
-        hipEventSynchronize(e1);
-        hipEventElapsedTime(&ms, e0, e1);
-        if (csc_ms) *csc_ms = ms / reps;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        BO_TRY(hipGetLastError());
-        return BO_SUCCESS;
+        return time_pair(bt_rows, bt_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms);
     }
 };
 
@@ -1093,8 +1119,9 @@ bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp*
     if ((lp->m > 0 && (!lp->row_ptr || !lp->b || !lp->is_eq)) || (lp->n > 0 && (!lp->c || !lp->lb || !lp->ub)) ||
         (lp->nnz > 0 && (!lp->col_idx || !lp->val)))
         return BO_INVALID_ARGUMENT;
-    if (lp->nnz >= (1LL << 31)) return BO_INVALID_ARGUMENT;
+    if (lp->nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;  // int32 offsets with unrolled strides
     if (lp->m > 0 && (lp->row_ptr[0] != 0 || lp->row_ptr[lp->m] != lp->nnz)) return BO_INVALID_ARGUMENT;
+    if (lp->m == 0 && lp->nnz != 0) return BO_INVALID_ARGUMENT;
     for (int i = 0; i < lp->m; ++i)
         if (lp->row_ptr[i + 1] < lp->row_ptr[i]) return BO_INVALID_ARGUMENT;
     for (long long k = 0; k < lp->nnz; ++k)

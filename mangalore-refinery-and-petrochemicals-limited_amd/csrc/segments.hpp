@@ -6,9 +6,11 @@
 // segments) or spreads one segment over the whole workgroup (L == 256).
 // Lanes of a segment read its nonzeros with stride L (coalesced), accumulate
 // in double, and reduce with a fixed xor tree, so results are deterministic.
-// The bin a workgroup serves is read from a small table passed by value;
-// workgroup ids are remapped so each XCD receives a contiguous range of
-// segments (consecutive workgroup ids are dispatched round-robin over XCDs).
+// The bin a workgroup serves is read from a small table passed by value.
+// Segments are sorted by descending length, so the heaviest workgroups are
+// dispatched first (longest-processing-time-first), and consecutive
+// workgroup ids -- which the hardware deals round-robin to the 8 XCDs -- carry
+// similar work, which keeps the XCDs balanced.
 #pragma once
 
 #include "common.hpp"
@@ -34,12 +36,6 @@ enum SegMode : int {
     kWrite = 3,    // epi.write(seg, k, val[k], idx[k]) per nonzero, no reduction
 };
 
-__device__ inline int xcd_remap(int bid, int nblocks) {
-    const int q = nblocks / kNumXcd, r = nblocks % kNumXcd;
-    const int xcd = bid % kNumXcd, local = bid / kNumXcd;
-    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
-
 template <int MODE>
 __device__ inline double seg_combine(double a, double b) {
     if constexpr (MODE == kMaxAbs) return a > b ? a : b;
@@ -51,6 +47,34 @@ __device__ inline double seg_term(TV v, TX x) {
     double p = static_cast<double>(v) * static_cast<double>(x);
     if constexpr (MODE == kSumProd) return p;
     else return fabs(p);
+}
+
+// Lanes of a segment walk its nonzeros with stride L, kUnroll nonzeros per
+// trip: all index/value loads of a trip are issued before any gather, so a
+// lane keeps kUnroll independent loads in flight instead of one.
+constexpr int kUnroll = 4;
+
+template <int MODE, int L, int NV, class TV, class Epi>
+__device__ inline void seg_accumulate(int k0, int k1, int lane, const int* __restrict__ idx,
+                                      const TV* __restrict__ val, const Epi& epi, double* acc) {
+    for (int k = k0 + lane; k < k1; k += kUnroll * L) {
+        TV a[kUnroll];
+        int j[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int kk = k + u * L;
+            a[u] = kk < k1 ? val[kk] : TV(0);
+            j[u] = kk < k1 ? idx[kk] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (j[u] >= 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], epi.vec(v)[j[u]]));
+            }
+        }
+    }
 }
 
 template <int MODE, int L>
@@ -82,16 +106,10 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
         const bool live = seg < seg_end;
         if (live) {
             const int k0 = ptr[seg], k1 = ptr[seg + 1];
-            for (int k = k0 + lane; k < k1; k += L) {
-                const TV a = val[k];
-                const int j = idx[k];
-                if constexpr (MODE == kWrite) {
-                    epi.write(seg, k, a, j);
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a, epi.vec(v)[j]));
-                }
+            if constexpr (MODE == kWrite) {
+                for (int k = k0 + lane; k < k1; k += L) epi.write(seg, k, val[k], idx[k]);
+            } else {
+                seg_accumulate<MODE, L, NV>(k0, k1, lane, idx, val, epi, acc);
             }
         }
         if constexpr (MODE != kWrite) {
@@ -107,18 +125,12 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
         const int k0 = ptr[seg], k1 = ptr[seg + 1];
-        for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kBlock) {
-            const TV a = val[k];
-            const int j = idx[k];
-            if constexpr (MODE == kWrite) {
-                epi.write(seg, k, a, j);
-            } else {
-#pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a, epi.vec(v)[j]));
-            }
+        if constexpr (MODE == kWrite) {
+            for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kBlock) epi.write(seg, k, val[k], idx[k]);
+            return;
+        } else {
+            seg_accumulate<MODE, kBlock, NV>(k0, k1, static_cast<int>(threadIdx.x), idx, val, epi, acc);
         }
-        if constexpr (MODE == kWrite) return;
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = group_reduce<MODE, kWave>(acc[v]);
         const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -147,7 +159,7 @@ k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx
            const int* __restrict__ done) {
     if (done && *done) return;
     constexpr int Q = Epi::Q;
-    const int vblock = xcd_remap(static_cast<int>(blockIdx.x), bt.total_blocks);
+    const int vblock = static_cast<int>(blockIdx.x);
     int bin = 0;
     while (bin + 1 < bt.nbins && vblock >= bt.block_start[bin + 1]) ++bin;
     double part[Q > 0 ? Q : 1];
