@@ -139,12 +139,15 @@ typedef struct bo_pdlp_params {
     int ruiz_iterations;       /* default 10 */
     int pock_chambolle;        /* 1: l1 (alpha=1) rescaling after Ruiz */
     int check_every;           /* iterations between FP64 KKT / restart checks (64) */
-    int lanes_rows;            /* lanes per CSR row segment (0 = autotune) */
-    int lanes_cols;            /* lanes per CSC column segment (0 = autotune) */
+    int lanes_rows;            /* CSR pass schedule: k > 0 segment kernel with k nonzeros per lane,
+                                  -c stream kernel with an LDS budget of c (1024/2048/4096)
+                                  nonzeros per workgroup, 0 = autotune */
+    int lanes_cols;            /* CSC pass schedule, same encoding */
     int fp64_iterates;         /* 1: run the iteration in FP64 (error comparison run) */
     int use_graph;             /* 1: replay each check interval as a hipGraph */
     double time_limit_s;       /* 0 = none */
     int verbose;
+    int hot_columns;           /* entries of x+ kept in LDS by the CSR pass: -1 tune, 0 off */
 } bo_pdlp_params;
 
 typedef struct bo_pdlp_stats {
@@ -164,8 +167,9 @@ typedef struct bo_pdlp_stats {
     double solve_ms;
     double spmv_csr_bytes; /* algorithmic bytes of one A x product    */
     double spmv_csc_bytes; /* algorithmic bytes of one A' y product   */
-    int lanes_rows;        /* nonzeros-per-lane target the tuner chose for the CSR pass */
-    int lanes_cols;        /* ... and for the CSC pass */
+    int lanes_rows;        /* schedule used by the CSR pass (encoding of bo_pdlp_params) */
+    int lanes_cols;        /* ... and by the CSC pass */
+    int hot_columns;       /* LDS-cached entries of x+ in the CSR pass (0 = none) */
 } bo_pdlp_stats;
 
 /* Default parameters. */

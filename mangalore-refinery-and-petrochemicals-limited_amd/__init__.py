@@ -74,7 +74,8 @@ class PdlpParams(C.Structure):
     _fields_ = [("max_iterations", C.c_longlong), ("eps_rel_kkt", C.c_double),
                 ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("check_every", C.c_int),
                 ("lanes_rows", C.c_int), ("lanes_cols", C.c_int), ("fp64_iterates", C.c_int),
-                ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int)]
+                ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int),
+                ("hot_columns", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -82,7 +83,8 @@ class PdlpStats(C.Structure):
                 ("used_average", C.c_int)] + [(k, C.c_double) for k in (
                     "rel_kkt", "rel_primal", "rel_dual", "rel_gap", "primal_obj", "dual_obj",
                     "primal_weight", "step_size", "setup_ms", "solve_ms", "spmv_csr_bytes",
-                    "spmv_csc_bytes")] + [("lanes_rows", C.c_int), ("lanes_cols", C.c_int)]
+                    "spmv_csc_bytes")] + [("lanes_rows", C.c_int), ("lanes_cols", C.c_int),
+                                          ("hot_columns", C.c_int)]
 
 
 class BoModel(C.Structure):

@@ -40,6 +40,7 @@
 
 #include "common.hpp"
 #include "segments.hpp"
+#include "stream.hpp"
 
 namespace bo {
 
@@ -598,6 +599,48 @@ BinTable make_bins(const std::vector<int>& sorted_len, int per_lane) {
     return bt;
 }
 
+// Stream-kernel blocks over segments of the given lengths: contiguous runs
+// with at most `cap` nonzeros and at most 256 segments; a longer segment is a
+// block of its own.  Returns the block start offsets plus the end sentinel.
+std::vector<int> stream_blocks(const std::vector<int>& len, int cap) {
+    std::vector<int> blk{0};
+    const int nseg = static_cast<int>(len.size());
+    long long cur = 0;
+    int cnt = 0;
+    for (int s = 0; s < nseg; ++s) {
+        const int l = len[s];
+        if (l > cap) {
+            if (cnt > 0) blk.push_back(s);
+            blk.push_back(s + 1);
+            cur = 0;
+            cnt = 0;
+            continue;
+        }
+        if (cnt > 0 && (cur + l > cap || cnt == kBlock)) {
+            blk.push_back(s);
+            cur = 0;
+            cnt = 0;
+        }
+        cur += l;
+        ++cnt;
+    }
+    if (cnt > 0 || blk.size() == 1) blk.push_back(nseg);
+    return blk;
+}
+
+constexpr int kStreamCaps[3] = {1024, 2048, 4096};
+
+// How one fused iteration pass is launched: the length-binned segment kernel
+// (bt, optional LDS copy of the hottest gathered entries) or, when cap > 0,
+// the stream kernel with that LDS budget.
+struct PassPlan {
+    BinTable bt{};
+    int kp = 4;
+    int cap = 0;
+    int hot = 0;
+    int hot_grid = 0;
+};
+
 struct EngineBase {
     virtual ~EngineBase() = default;
     virtual bo_status set_warm_start(const double* x0, const double* y0) = 0;
@@ -638,12 +681,15 @@ struct Engine final : EngineBase {
     PdlpState* st = nullptr;
     PdlpState* st_host = nullptr;  // pinned, read back after each interval
     PdlpState* st_init = nullptr;  // pinned, staged for the upload in prepare()
-    BinTable bt_rows{}, bt_cols{};
+    PassPlan plan_rows{}, plan_cols{};  // CSR (dual) pass and CSC (primal) pass
+    int* sblk[2][3] = {};               // stream blocks per pass (rows, cols) and LDS budget
+    int nsblk[2][3] = {};
     int nb_x = 1, nb_y = 1;
     double b_norm = 0.0, c_norm = 0.0;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_k = 0;
+    int num_cus = 1;
 
     ~Engine() override {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
@@ -782,29 +828,67 @@ struct Engine final : EngineBase {
         // ---- launch geometry
         nb_x = static_cast<int>(std::min<unsigned>(grid_for(n), 1024));
         nb_y = static_cast<int>(std::min<unsigned>(grid_for(m), 1024));
-        if (prm.lanes_rows < 0) prm.lanes_rows = 0;
-        if (prm.lanes_cols < 0) prm.lanes_cols = 0;
-        const int pr_rows = prm.lanes_rows > 0 ? prm.lanes_rows : 4;
-        const int pr_cols = prm.lanes_cols > 0 ? prm.lanes_cols : 4;
-        bt_rows = make_bins(row_len_sorted, pr_rows);
-        bt_cols = make_bins(col_len_sorted, pr_cols);
         BO_TRY(mem.alloc(&part_x, static_cast<size_t>(nb_x) * 2));
         BO_TRY(mem.alloc(&part_dx, nb_x));
         BO_TRY(mem.alloc(&part_dy, nb_y));
-        // partial buffers sized for the largest bin tables the tuner may try
+        // stream-kernel blocks for every LDS budget the plans may use
+        for (int pass = 0; pass < 2; ++pass) {
+            const std::vector<int>& len = pass == 0 ? row_len_sorted : col_len_sorted;
+            for (int c = 0; c < 3; ++c) {
+                const std::vector<int> blk = stream_blocks(len, kStreamCaps[c]);
+                nsblk[pass][c] = static_cast<int>(blk.size()) - 1;
+                BO_TRY(mem.alloc(&sblk[pass][c], blk.size()));
+                BO_TRY(up(sblk[pass][c], blk.data(), sizeof(int) * blk.size()));
+            }
+        }
+        BO_TRY(hipStreamSynchronize(stream));
+        int dev = 0;
+        BO_TRY(hipGetDevice(&dev));
+        BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // launch plans: lanes > 0 fixes the segment kernel's nonzeros-per-lane
+        // target, lanes < 0 selects the stream kernel with LDS budget -lanes,
+        // 0 lets the tuner choose
+        const bool tune = prm.lanes_rows == 0 || prm.lanes_cols == 0 || prm.hot_columns < 0;
+        plan_rows = fixed_plan(row_len_sorted, prm.lanes_rows);
+        plan_cols = fixed_plan(col_len_sorted, prm.lanes_cols);
+        if (prm.hot_columns > 0 && plan_rows.cap == 0) {
+            plan_rows.hot = std::min(prm.hot_columns, n);
+            plan_rows.hot_grid = grid_for_hot(plan_rows.hot);
+            if (plan_rows.hot_grid <= 0) plan_rows.hot = 0;  // does not fit in LDS: plain pass
+        }
+        // partial buffers (check passes use the plans' bin tables)
         size_t max_r = 0, max_c = 0;
-        for (int kp : {1, 2, 4, 8, 16, pr_rows, pr_cols}) {
+        for (int kp : {1, 2, 4, 8, 16, plan_rows.kp, plan_cols.kp}) {
             max_r = std::max<size_t>(max_r, make_bins(row_len_sorted, kp).total_blocks);
             max_c = std::max<size_t>(max_c, make_bins(col_len_sorted, kp).total_blocks);
         }
         BO_TRY(mem.alloc(&part_r, max_r * 6));
         BO_TRY(mem.alloc(&part_c, max_c * 6));
-        if (prm.lanes_rows <= 0 || prm.lanes_cols <= 0) {
+        if (tune) {
             bo_status s2 = autotune();
             if (s2 != BO_SUCCESS) return s2;
         }
+        prm.lanes_rows = plan_code(plan_rows);
+        prm.lanes_cols = plan_code(plan_cols);
+        prm.hot_columns = plan_rows.hot;
         return BO_SUCCESS;
     }
+
+    static int cap_index(int cap) { return cap <= 1024 ? 0 : (cap <= 2048 ? 1 : 2); }
+
+    PassPlan fixed_plan(const std::vector<int>& len, int code) const {
+        PassPlan p;
+        if (code < 0) {
+            p.cap = kStreamCaps[cap_index(-code)];
+            p.kp = 4;
+        } else {
+            p.kp = code > 0 ? code : 4;
+        }
+        p.bt = make_bins(len, p.kp);
+        return p;
+    }
+
+    static int plan_code(const PassPlan& p) { return p.cap > 0 ? -p.cap : p.kp; }
 
     // ---- launch helpers
     template <int MODE, class TV, class Epi>
@@ -820,23 +904,63 @@ struct Engine final : EngineBase {
 
     const int* done_flag() const { return &st->done; }
 
-    void launch_primal(const BinTable& bt, const int* done, int t) {
-        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t};
-        launch_cols<kSumProd>(bt, cv, e, nullptr, done);
+    template <class Epi>
+    void launch_stream(int pass, int cap, const int* ptr, const int* idx, const T* vals, const Epi& e,
+                       const int* done) {
+        const int c = cap_index(cap);
+        const int nb = nsblk[pass][c];
+        switch (c) {
+            case 0:
+                hipLaunchKernelGGL((k_stream<1024, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c], nb,
+                                   ptr, idx, vals, e, done);
+                break;
+            case 1:
+                hipLaunchKernelGGL((k_stream<2048, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c], nb,
+                                   ptr, idx, vals, e, done);
+                break;
+            default:
+                hipLaunchKernelGGL((k_stream<4096, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c], nb,
+                                   ptr, idx, vals, e, done);
+                break;
+        }
     }
-    void launch_dual(const BinTable& bt, const int* done, int t) {
+
+    void launch_primal(const PassPlan& p, const int* done, int t) {
+        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t};
+        if (p.cap > 0) launch_stream(1, p.cap, cp, ri, cv, e, done);
+        else launch_cols<kSumProd>(p.bt, cv, e, nullptr, done);
+    }
+    void launch_dual(const PassPlan& p, const int* done, int t) {
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t};
-        launch_rows<kSumProd>(bt, rv, e, nullptr, done);
+        if (p.cap > 0) {
+            launch_stream(0, p.cap, rp, ci, rv, e, done);
+        } else if (p.hot > 0) {
+            hipLaunchKernelGGL((k_segments_hot<kSumProd, T, T, EpiDual<T>>),
+                               dim3(std::min(p.hot_grid, p.bt.total_blocks)), dim3(kBlock),
+                               static_cast<unsigned>(p.hot * sizeof(T)), stream, p.bt, rp, ci, rv, e, p.hot, done);
+        } else {
+            launch_rows<kSumProd>(p.bt, rv, e, nullptr, done);
+        }
+    }
+
+    // Resident workgroups of the persistent CSR pass with `hot` LDS entries.
+    int grid_for_hot(int hot) {
+        int per_cu = 0;
+        hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_segments_hot<kSumProd, T, T, EpiDual<T>>), kBlock,
+            hot * sizeof(T));
+        if (err != hipSuccess || per_cu < 1) return 0;
+        return per_cu * num_cus;
     }
 
     void launch_check() {
         hipLaunchKernelGGL(k_check_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, xk, xp, cs, c64, lb64, ub64,
                            xo_cur, xo_cand, part_x);
         EpiCheckRows<T> er{xo_cur, xo_cand, b64, eq, rs, yk, yp, yo_cur, yo_cand, ax_cand};
-        launch_rows<kSumProd>(bt_rows, rv64, er, part_r, done_flag());
+        launch_rows<kSumProd>(plan_rows.bt, rv64, er, part_r, done_flag());
         EpiCheckCols ec{yo_cur, yo_cand, c64, lb64, ub64, cs};
-        launch_cols<kSumProd>(bt_cols, cv64, ec, part_c, done_flag());
-        CheckDims d{nb_x, bt_rows.total_blocks, bt_cols.total_blocks};
+        launch_cols<kSumProd>(plan_cols.bt, cv64, ec, part_c, done_flag());
+        CheckDims d{nb_x, plan_rows.bt.total_blocks, plan_cols.bt.total_blocks};
         hipLaunchKernelGGL(k_decide, dim3(1), dim3(kDecideThreads), 0, stream, st, d, part_x, part_r, part_c);
         hipLaunchKernelGGL(k_restart_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, xk, x0, xp, part_dx);
         hipLaunchKernelGGL(k_restart_y<T>, dim3(nb_y), dim3(kBlock), 0, stream, m, st, yk, y0, yp, yg, ax, ax0,
@@ -847,24 +971,24 @@ struct Engine final : EngineBase {
     // k steps followed by the check.
     void launch_interval(int k) {
         for (int t = 0; t < k; ++t) {
-            launch_primal(bt_cols, done_flag(), t);
-            launch_dual(bt_rows, done_flag(), t);
+            launch_primal(plan_cols, done_flag(), t);
+            launch_dual(plan_rows, done_flag(), t);
         }
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, st, k);
         launch_check();
     }
 
     // Alternating launches, as inside a solve (primal, dual, primal, ...).
-    bo_status time_pair(const BinTable& br, const BinTable& bc, int reps, double* csr_ms, double* csc_ms) {
+    bo_status time_pair(const PassPlan& pr, const PassPlan& pc, int reps, double* csr_ms, double* csc_ms) {
         std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps) + 1);
         for (auto& e : ev) BO_TRY(hipEventCreate(&e));
-        launch_primal(bc, nullptr, 0);
-        launch_dual(br, nullptr, 0);
+        launch_primal(pc, nullptr, 0);
+        launch_dual(pr, nullptr, 0);
         for (int r = 0; r < reps; ++r) {
             hipEventRecord(ev[2 * r], stream);
-            launch_primal(bc, nullptr, r);
+            launch_primal(pc, nullptr, r);
             hipEventRecord(ev[2 * r + 1], stream);
-            launch_dual(br, nullptr, r);
+            launch_dual(pr, nullptr, r);
         }
         hipEventRecord(ev[2 * reps], stream);
         hipEventSynchronize(ev[2 * reps]);
@@ -883,30 +1007,58 @@ struct Engine final : EngineBase {
         return BO_SUCCESS;
     }
 
-    // Nonzeros-per-lane target of each pass, chosen by timing the fused
-    // iteration kernels in alternation (the cache state of a real solve).
+    // The schedule of each pass -- segment kernel with a nonzeros-per-lane
+    // target (for the CSR pass optionally with an LDS copy of the hottest x+
+    // entries) or stream kernel with an LDS budget -- chosen by timing the
+    // fused iteration kernels in alternation, the cache state of a real solve.
     bo_status autotune() {
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
-        const int cands[] = {2, 4, 8, 16};
+        const int codes[] = {2, 4, 8, 16, -1024, -2048, -4096};
+        std::vector<PassPlan> cr, cc;
+        if (prm.lanes_rows == 0) {
+            for (int code : codes) cr.push_back(fixed_plan(row_len_sorted, code));
+        } else {
+            cr.push_back(plan_rows);
+        }
+        if (prm.lanes_cols == 0) {
+            for (int code : codes) cc.push_back(fixed_plan(col_len_sorted, code));
+        } else {
+            cc.push_back(plan_cols);
+        }
+        const PassPlan base_r = fixed_plan(row_len_sorted, 4), base_c = fixed_plan(col_len_sorted, 4);
         double best_r = 1e30, best_c = 1e30;
-        int kr = 4, kc = 4;
-        const BinTable base_r = make_bins(row_len_sorted, 4), base_c = make_bins(col_len_sorted, 4);
-        for (int kp : cands) {
-            double r_ms = 0.0, c_ms = 0.0;
-            if (prm.lanes_rows <= 0) {
-                bo_status s = time_pair(make_bins(row_len_sorted, kp), base_c, 4, &r_ms, nullptr);
+        PassPlan pr = cr[0], pc = cc[0];
+        for (const PassPlan& cand : cr) {
+            double t_ms = 0.0;
+            bo_status s = time_pair(cand, base_c, 4, &t_ms, nullptr);
+            if (s != BO_SUCCESS) return s;
+            if (t_ms < best_r) { best_r = t_ms; pr = cand; }
+        }
+        for (const PassPlan& cand : cc) {
+            double t_ms = 0.0;
+            bo_status s = time_pair(base_r, cand, 4, nullptr, &t_ms);
+            if (s != BO_SUCCESS) return s;
+            if (t_ms < best_c) { best_c = t_ms; pc = cand; }
+        }
+        if (prm.hot_columns > 0 && pr.cap == 0) {  // fixed by the caller
+            pr.hot = plan_rows.hot;
+            pr.hot_grid = plan_rows.hot_grid;
+        } else if (prm.hot_columns < 0 && pr.cap == 0) {  // LDS copy of the hottest x+ entries
+            for (int hot : {4096, 8192, 16384}) {
+                if (hot > n) break;
+                PassPlan h = pr;
+                h.hot = hot;
+                h.hot_grid = grid_for_hot(hot);
+                if (h.hot_grid <= 0) continue;
+                double t_ms = 0.0;
+                bo_status s = time_pair(h, pc, 4, &t_ms, nullptr);
                 if (s != BO_SUCCESS) return s;
-                if (r_ms < best_r) { best_r = r_ms; kr = kp; }
-            }
-            if (prm.lanes_cols <= 0) {
-                bo_status s = time_pair(base_r, make_bins(col_len_sorted, kp), 4, nullptr, &c_ms);
-                if (s != BO_SUCCESS) return s;
-                if (c_ms < best_c) { best_c = c_ms; kc = kp; }
+                if (t_ms < 0.97 * best_r) { best_r = t_ms; pr = h; }
             }
         }
-        if (prm.lanes_rows <= 0) { bt_rows = make_bins(row_len_sorted, kr); prm.lanes_rows = kr; }
-        if (prm.lanes_cols <= 0) { bt_cols = make_bins(col_len_sorted, kc); prm.lanes_cols = kc; }
+        plan_rows = pr;
+        plan_cols = pc;
         return BO_SUCCESS;
     }
 
@@ -1058,6 +1210,7 @@ struct Engine final : EngineBase {
             out->spmv_csc_bytes = 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 4.0 * n;
             out->lanes_rows = prm.lanes_rows;
             out->lanes_cols = prm.lanes_cols;
+            out->hot_columns = plan_rows.hot;
         }
         return BO_SUCCESS;
     }
@@ -1079,7 +1232,7 @@ struct Engine final : EngineBase {
     // Leaves the iterate buffers modified: call after get_solution (solve()
     // re-initialises everything).
     bo_status time_spmv(int reps, double* csr_ms, double* csc_ms) override {
-        return time_pair(bt_rows, bt_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms);
+        return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms);
     }
 };
 
@@ -1108,6 +1261,7 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
     p->use_graph = 1;
     p->time_limit_s = 0.0;
     p->verbose = 0;
+    p->hot_columns = -1;
 }
 
 bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp** out) {

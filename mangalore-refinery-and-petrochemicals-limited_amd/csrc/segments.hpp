@@ -49,29 +49,48 @@ __device__ inline double seg_term(TV v, TX x) {
     else return fabs(p);
 }
 
-// Lanes of a segment walk its nonzeros with stride L, kUnroll nonzeros per
-// trip: all index/value loads of a trip are issued before any gather, so a
-// lane keeps kUnroll independent loads in flight instead of one.
-constexpr int kUnroll = 4;
+// Lanes of a segment walk its nonzeros with stride L, unroll<L>() nonzeros
+// per trip: all index/value loads of a trip are issued before any gather, so
+// a lane keeps that many independent loads in flight.  Narrow groups unroll
+// deeper so one trip covers about 128 nonzeros of a segment.
+template <int L>
+constexpr int unroll() {
+    return L >= 32 ? 4 : (L == 16 ? 8 : 16);
+}
 
-template <int MODE, int L, int NV, class TV, class Epi>
+// Optional LDS copy of the first `count` entries of the gathered vector (the
+// hottest ones: segments of the other dimension are stored longest first).
+template <class TX>
+struct HotCache {
+    const TX* lds;
+    int count;
+};
+
+template <int MODE, int L, int NV, bool HOT, class TV, class Epi, class TX>
 __device__ inline void seg_accumulate(int k0, int k1, int lane, const int* __restrict__ idx,
-                                      const TV* __restrict__ val, const Epi& epi, double* acc) {
-    for (int k = k0 + lane; k < k1; k += kUnroll * L) {
-        TV a[kUnroll];
-        int j[kUnroll];
+                                      const TV* __restrict__ val, const Epi& epi, double* acc,
+                                      HotCache<TX> hot) {
+    constexpr int U = unroll<L>();
+    for (int k = k0 + lane; k < k1; k += U * L) {
+        TV a[U];
+        int j[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             const int kk = k + u * L;
             a[u] = kk < k1 ? val[kk] : TV(0);
             j[u] = kk < k1 ? idx[kk] : -1;
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (j[u] >= 0) {
+                if constexpr (HOT) {
+                    const TX xv = j[u] < hot.count ? hot.lds[j[u]] : epi.vec(0)[j[u]];
+                    acc[0] = seg_combine<MODE>(acc[0], seg_term<MODE>(a[u], xv));
+                } else {
 #pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], epi.vec(v)[j[u]]));
+                    for (int v = 0; v < NV; ++v)
+                        acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], epi.vec(v)[j[u]]));
+                }
             }
         }
     }
@@ -89,10 +108,10 @@ __device__ inline double group_reduce(double v) {
 //   static constexpr int Q;         number of per-block partial sums (0..8)
 //   const TX* vec(int v) const;     gathered vector v
 //   __device__ void operator()(int seg, const double* acc, double* part) const;
-template <int MODE, int L, class TV, class TX, class Epi>
+template <int MODE, int L, class TV, class TX, class Epi, bool HOT = false>
 __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, const int* __restrict__ ptr,
                                    const int* __restrict__ idx, const TV* __restrict__ val,
-                                   const Epi& epi, double* part) {
+                                   const Epi& epi, double* part, HotCache<TX> hot = {nullptr, 0}) {
     constexpr int NV = Epi::NV;
     const int seg_end = bt.seg_start[bin + 1];
     if constexpr (L <= kWave) {
@@ -109,7 +128,7 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
             if constexpr (MODE == kWrite) {
                 for (int k = k0 + lane; k < k1; k += L) epi.write(seg, k, val[k], idx[k]);
             } else {
-                seg_accumulate<MODE, L, NV>(k0, k1, lane, idx, val, epi, acc);
+                seg_accumulate<MODE, L, NV, HOT>(k0, k1, lane, idx, val, epi, acc, hot);
             }
         }
         if constexpr (MODE != kWrite) {
@@ -129,7 +148,7 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
             for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kBlock) epi.write(seg, k, val[k], idx[k]);
             return;
         } else {
-            seg_accumulate<MODE, kBlock, NV>(k0, k1, static_cast<int>(threadIdx.x), idx, val, epi, acc);
+            seg_accumulate<MODE, kBlock, NV, HOT>(k0, k1, static_cast<int>(threadIdx.x), idx, val, epi, acc, hot);
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = group_reduce<MODE, kWave>(acc[v]);
@@ -181,6 +200,43 @@ k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx
         for (int q = 0; q < Q; ++q) {
             double t = block_sum(part[q], s);
             if (threadIdx.x == 0) partials[static_cast<size_t>(blockIdx.x) * Q + q] = t;
+        }
+    }
+}
+
+// Persistent variant with an LDS copy of the hottest `hot_count` entries of
+// the (single) gathered vector: a grid of resident workgroups walks the
+// segment groups g = blockIdx.x, blockIdx.x + gridDim.x, ... in bin order,
+// after each workgroup has filled its LDS copy once.  Dynamic LDS holds
+// hot_count elements of TX.  No per-block partials.
+template <int MODE, class TV, class TX, class Epi>
+__global__ void __launch_bounds__(kBlock)
+k_segments_hot(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx,
+               const TV* __restrict__ val, Epi epi, int hot_count, const int* __restrict__ done) {
+    static_assert(Epi::Q == 0 && Epi::NV == 1, "hot-cache passes gather one vector and keep no partials");
+    if (done && *done) return;
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    TX* lds = reinterpret_cast<TX*>(dyn_lds);
+    const TX* src = epi.vec(0);
+    for (int i = threadIdx.x; i < hot_count; i += kBlock) lds[i] = src[i];
+    __syncthreads();
+    const HotCache<TX> hot{lds, hot_count};
+    double part[1] = {0.0};
+    int bin = 0;
+    for (int g = blockIdx.x; g < bt.total_blocks; g += gridDim.x) {
+        while (bin + 1 < bt.nbins && g >= bt.block_start[bin + 1]) ++bin;
+        switch (bt.lanes[bin]) {
+            case 1: seg_process<MODE, 1, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 2: seg_process<MODE, 2, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 4: seg_process<MODE, 4, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 8: seg_process<MODE, 8, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 16: seg_process<MODE, 16, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 32: seg_process<MODE, 32, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 64: seg_process<MODE, 64, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            default:
+                seg_process<MODE, 256, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot);
+                __syncthreads();  // the one-segment path reuses its LDS reduction slots next round
+                break;
         }
     }
 }

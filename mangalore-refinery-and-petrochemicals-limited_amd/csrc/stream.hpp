@@ -1,0 +1,111 @@
+// stream.hpp -- "stream" segment kernel: every workgroup owns a contiguous
+// range of segments whose nonzeros fit a fixed LDS budget (CSR-stream, after
+// Greathouse & Daga, SC'14).  Phase 1 streams the range's nonzeros with all
+// 256 lanes busy and consecutive lanes on consecutive nonzeros, gathers the
+// vector and stores the FP64 products in LDS; phase 2 reduces each segment
+// from LDS with a power-of-two group of lanes and runs the epilogue.  A
+// segment longer than the budget gets a workgroup of its own and is reduced
+// in registers.  Block boundaries are built on the host (stream_blocks).
+#pragma once
+
+#include "segments.hpp"
+
+namespace bo {
+
+constexpr int kStreamMaxCap = 4096;  // largest LDS budget (nonzeros) of a stream block
+
+// lanes per segment in the reduce phase of a block holding nseg segments
+__device__ inline int stream_group(int nseg) {
+    int L = 64;
+    while (L > 1 && L * nseg > kBlock) L >>= 1;
+    return L;
+}
+
+template <int L>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+template <int CAP, class TV, class TX, class Epi>
+__global__ void __launch_bounds__(kBlock)
+k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, const int* __restrict__ idx,
+         const TV* __restrict__ val, Epi epi, const int* __restrict__ done) {
+    static_assert(Epi::NV == 1 && Epi::Q == 0, "stream passes gather one vector and keep no partials");
+    if (done && *done) return;
+    __shared__ double prod[CAP];
+    __shared__ double red[kBlock / kWave];
+    const int b = blockIdx.x;
+    const int s0 = blk[b], s1 = blk[b + 1];
+    const int k0 = ptr[s0], k1 = ptr[s1];
+    const int tid = threadIdx.x;
+    const TX* xv = epi.vec(0);
+    if (k1 - k0 > CAP) {  // one long segment: register accumulation, block reduction
+        double acc = 0.0;
+        for (int k = k0 + tid; k < k1; k += 4 * kBlock) {
+            TV a[4];
+            int j[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k + u * kBlock;
+                a[u] = kk < k1 ? val[kk] : TV(0);
+                j[u] = kk < k1 ? idx[kk] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j[u] >= 0) acc += static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
+        }
+        acc = group_sum<kWave>(acc);
+        if ((tid % kWave) == 0) red[tid / kWave] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+            double r[1] = {t};
+            epi(s0, r, nullptr);
+        }
+        return;
+    }
+    // phase 1: products of the block's nonzeros into LDS
+    constexpr int U = CAP / kBlock;
+    {
+        TV a[U];
+        int j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k0 + tid + u * kBlock;
+            a[u] = kk < k1 ? val[kk] : TV(0);
+            j[u] = kk < k1 ? idx[kk] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
+    }
+    __syncthreads();
+    // phase 2: segment sums from LDS, L lanes per segment
+    const int nseg = s1 - s0;
+    const int L = stream_group(nseg);
+    const int seg = s0 + tid / L;
+    const int lane = tid % L;
+    double acc = 0.0;
+    if (seg < s1) {
+        const int a0 = ptr[seg] - k0, a1 = ptr[seg + 1] - k0;
+        for (int q = a0 + lane; q < a1; q += L) acc += prod[q];
+    }
+    switch (L) {  // uniform across the block
+        case 64: acc = group_sum<64>(acc); break;
+        case 32: acc = group_sum<32>(acc); break;
+        case 16: acc = group_sum<16>(acc); break;
+        case 8: acc = group_sum<8>(acc); break;
+        case 4: acc = group_sum<4>(acc); break;
+        case 2: acc = group_sum<2>(acc); break;
+        default: break;
+    }
+    if (seg < s1 && lane == 0) {
+        double r[1] = {acc};
+        epi(seg, r, nullptr);
+    }
+}
+
+}  // namespace bo

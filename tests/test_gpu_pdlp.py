@@ -95,6 +95,76 @@ def test_pdlp_no_graph_same_answer(pkg, instances):
     assert np.array_equal(xa, xb) and np.array_equal(ya, yb)  # deterministic reductions
 
 
+@pytest.mark.parametrize("hot", [1000, 4096])
+def test_pdlp_lds_hot_columns_bitequal(pkg, instances, hot):
+    # the persistent CSR pass reading the hottest x+ entries from LDS gathers the
+    # same values in the same order as the plain pass: identical iterates
+    lp = instances["zipf_2000x5000_s9"]
+    a = pkg.Pdlp(lp, lanes_rows=4, lanes_cols=4, hot_columns=0)
+    b = pkg.Pdlp(lp, lanes_rows=4, lanes_cols=4, hot_columns=hot)
+    sa, sb = a.solve(), b.solve()
+    assert sb["hot_columns"] == hot and sa["hot_columns"] == 0
+    assert sa["iterations"] == sb["iterations"]
+    xa, ya = a.solution()
+    xb, yb = b.solution()
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+
+
+@pytest.mark.parametrize("cap", [1024, 2048, 4096])
+def test_pdlp_stream_schedule(pkg, po, instances, cap):
+    # both passes on the stream kernel (LDS-staged products); the zipf case has
+    # columns longer than every budget, so the one-segment blocks run too
+    name = "zipf_2000x5000_s9"
+    lp = instances[name]
+    cl = np.bincount(lp.col_idx, minlength=lp.n)
+    rl = np.diff(lp.row_ptr)
+    print("case: stream cap", cap, "max col", cl.max(), "max row", rl.max(), flush=True)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=-cap, lanes_cols=-cap, hot_columns=0)
+    s = h.solve()
+    assert s["lanes_rows"] == -cap and s["lanes_cols"] == -cap
+    x, y = h.solution()
+    check_solution(po, lp, x, y, s, name)
+    s2 = h.solve()  # deterministic
+    x2, y2 = h.solution()
+    assert s2["iterations"] == s["iterations"] and np.array_equal(x, x2) and np.array_equal(y, y2)
+
+
+def dense_line_lp(lpdata, m=1500, n=5000, k=4, seed=11):
+    """random_lp entries plus one dense row (n > 4096 nonzeros) and one dense
+    column (m nonzeros, between 1024 and 2048): segments longer than every
+    stream budget, reduced by a workgroup of their own."""
+    rng = np.random.default_rng(seed)
+    rows = np.argsort(rng.random((n, m)), axis=1)[:, :k]
+    cols = np.repeat(np.arange(n), k)
+    rows = rows.reshape(-1)
+    keep = (rows != 0) & (cols != 0)
+    rows = np.concatenate([rows[keep], np.zeros(n, np.int64), np.arange(1, m)])
+    cols = np.concatenate([cols[keep], np.arange(n), np.zeros(m - 1, np.int64)])
+    vals = rng.standard_normal(rows.size)
+    rp, ci, v = lpdata._csr_from_cols(m, n, cols, rows, vals)
+    return lpdata._finish(m, n, rp, ci, v, rng, rng.uniform(0.0, 10.0, n))
+
+
+@pytest.mark.parametrize("cap", [1024, 4096])
+def test_pdlp_stream_long_segments(pkg, po, lpdata, cap):
+    lp = dense_line_lp(lpdata)
+    assert np.diff(lp.row_ptr).max() > 4096 and 1024 < np.bincount(lp.col_idx).max() < 2048
+    print("case: dense-line LP, stream cap", cap, flush=True)
+    seg = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=4, lanes_cols=4, hot_columns=0)
+    sa = seg.solve()
+    xa, ya = seg.solution()
+    stm = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=-cap, lanes_cols=-cap, hot_columns=0)
+    sb = stm.solve()
+    xb, yb = stm.solution()
+    ka, kb = po.kkt(lp, xa, ya), po.kkt(lp, xb, yb)
+    print("segment iters", sa["iterations"], "rel-KKT %.3e" % ka["rel_kkt"], "stream iters", sb["iterations"],
+          "rel-KKT %.3e" % kb["rel_kkt"], flush=True)
+    assert sa["status"] == 0 and sb["status"] == 0
+    assert ka["rel_kkt"] <= TOL and kb["rel_kkt"] <= TOL
+    fa, fb = ka["primal_obj"], kb["primal_obj"]
+    assert abs(fa - fb) / (1 + abs(fa)) <= 1e-5
+
+
 def test_pdlp_deterministic_repeat(pkg, instances):
     lp = instances["zipf_2000x5000_s9"]
     h = pkg.Pdlp(lp)
