@@ -140,6 +140,8 @@ typedef struct bo_pdlp_params {
     int pock_chambolle;        /* 1: l1 (alpha=1) rescaling after Ruiz */
     int check_every;           /* iterations between FP64 KKT / restart checks (64) */
     int lanes_rows;            /* CSR pass schedule: k > 0 segment kernel with k nonzeros per lane,
+                                  -k (1..64) tiled gather (LDS-resident vector block, SELL-64
+                                  slices) + reduce kernels, cut for k tiles per CU,
                                   -c stream kernel with an LDS budget of c (1024/2048/4096)
                                   nonzeros per workgroup, 0 = autotune */
     int lanes_cols;            /* CSC pass schedule, same encoding */
@@ -188,6 +190,27 @@ void bo_pdlp_destroy(struct bo_pdlp* h);
 /* Time `reps` products of each SpMV kernel of the handle (HIP events, on the
  * handle's stream).  ms are per-launch averages. */
 bo_status bo_pdlp_time_spmv(struct bo_pdlp* h, int reps, double* csr_ms, double* csc_ms);
+
+/* Diagnostics without a counterpart in the reference: the schedule a pass
+ * runs with and the algorithmic bytes of its kernels (pass 0: CSR pass,
+ * A x+ and the dual step; pass 1: CSC pass, A'y and the primal step). */
+typedef struct bo_pdlp_pass_info {
+    int schedule;        /* encoding of bo_pdlp_params.lanes_rows */
+    int kernels;         /* launches per pass: 1 fused, 2 tiled (gather, reduce) */
+    int blocks;          /* tiled: blocks of the gathered vector */
+    int block_entries;   /* tiled: entries per block, held in LDS */
+    int tiles;           /* tiled: workgroups of the gather kernel */
+    int slices;          /* tiled: SELL-64 slices */
+    long long pairs;     /* tiled: (block, segment) pairs holding nonzeros */
+    long long padded;    /* tiled: stored entries including slice padding */
+    double gather_bytes; /* algorithmic bytes of one launch of the (gather) kernel */
+    double reduce_bytes; /* ... of the reduce + epilogue kernel; 0 when fused */
+} bo_pdlp_pass_info;
+bo_status bo_pdlp_get_pass_info(struct bo_pdlp* h, int pass, bo_pdlp_pass_info* out);
+/* Per-launch averages over `reps` alternating iterations, HIP events on the
+ * handle's stream: ms4 = [CSC gather, CSC reduce, CSR gather, CSR reduce]
+ * (a fused pass reports its kernel as gather and 0 as reduce). */
+bo_status bo_pdlp_time_kernels(struct bo_pdlp* h, int reps, double* ms4);
 
 /* ---- full drop-in solve on a flattened LPModel ----------------------------- */
 /* LPModel (include/bharatopt/lp_model.hpp:60-127) as arrays: row-wise terms in

@@ -87,6 +87,12 @@ class PdlpStats(C.Structure):
                                           ("hot_columns", C.c_int)]
 
 
+class PdlpPassInfo(C.Structure):
+    _fields_ = [("schedule", C.c_int), ("kernels", C.c_int), ("blocks", C.c_int), ("block_entries", C.c_int),
+                ("tiles", C.c_int), ("slices", C.c_int), ("pairs", C.c_longlong), ("padded", C.c_longlong),
+                ("gather_bytes", C.c_double), ("reduce_bytes", C.c_double)]
+
+
 class BoModel(C.Structure):
     _fields_ = [("num_vars", C.c_int), ("num_cons", C.c_int), ("sense", C.c_int),
                 ("obj_offset", C.c_double), ("lb", P), ("ub", P), ("obj", P), ("var_type", P),
@@ -116,6 +122,7 @@ EXPORTS = [
     "bo_dot_f64", "bo_norm2_f64", "bo_pdhg_ref_solve", "bo_pdlp_default_params", "bo_pdlp_create",
     "bo_pdlp_set_warm_start", "bo_pdlp_solve", "bo_pdlp_get_solution", "bo_pdlp_destroy",
     "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
+    "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels",
 ]
 
 
@@ -305,6 +312,18 @@ class Pdlp:
         y = np.zeros(max(self.lp.m, 1))
         _check(lib().bo_pdlp_get_solution(self.handle, _p(x), _p(y)), "bo_pdlp_get_solution")
         return x[:self.lp.n], y[:self.lp.m]
+
+    def pass_info(self, pass_):
+        """Schedule and algorithmic bytes of pass 0 (CSR, dual step) or 1 (CSC, primal step)."""
+        info = PdlpPassInfo()
+        _check(lib().bo_pdlp_get_pass_info(self.handle, int(pass_), C.byref(info)), "bo_pdlp_get_pass_info")
+        return {k: getattr(info, k) for k, _ in PdlpPassInfo._fields_}
+
+    def time_kernels4(self, reps=20):
+        """Per-launch ms of [CSC gather, CSC reduce, CSR gather, CSR reduce]."""
+        ms = (C.c_double * 4)()
+        _check(lib().bo_pdlp_time_kernels(self.handle, int(reps), ms), "bo_pdlp_time_kernels")
+        return list(ms)
 
     def time_kernels(self, reps=20):
         a, b = C.c_double(), C.c_double()

@@ -41,6 +41,7 @@
 #include "common.hpp"
 #include "segments.hpp"
 #include "stream.hpp"
+#include "tiles.hpp"
 
 namespace bo {
 
@@ -631,15 +632,20 @@ std::vector<int> stream_blocks(const std::vector<int>& len, int cap) {
 constexpr int kStreamCaps[3] = {1024, 2048, 4096};
 
 // How one fused iteration pass is launched: the length-binned segment kernel
-// (bt, optional LDS copy of the hottest gathered entries) or, when cap > 0,
-// the stream kernel with that LDS budget.
+// (bt, optional LDS copy of the hottest gathered entries), the stream kernel
+// with LDS budget cap (cap > 0), or the tiled gather + reduction kernels
+// (tiled > 0: the layout cut for that many tiles per CU).  bt also serves the
+// FP64 check passes.
 struct PassPlan {
     BinTable bt{};
     int kp = 4;
     int cap = 0;
     int hot = 0;
     int hot_grid = 0;
+    int tiled = 0;
 };
+
+constexpr int kMaxTilesPerCu = 64;
 
 struct EngineBase {
     virtual ~EngineBase() = default;
@@ -647,6 +653,8 @@ struct EngineBase {
     virtual bo_status solve(bo_pdlp_stats* stats) = 0;
     virtual bo_status get_solution(double* x, double* y) = 0;
     virtual bo_status time_spmv(int reps, double* csr_ms, double* csc_ms) = 0;
+    virtual bo_status time_kernels(int reps, double* ms4) = 0;
+    virtual bo_status pass_info(int pass, bo_pdlp_pass_info* out) = 0;
 };
 
 template <class T>
@@ -684,6 +692,22 @@ struct Engine final : EngineBase {
     PassPlan plan_rows{}, plan_cols{};  // CSR (dual) pass and CSC (primal) pass
     int* sblk[2][3] = {};               // stream blocks per pass (rows, cols) and LDS budget
     int nsblk[2][3] = {};
+    // tiled layout per pass (0: CSR pass gathering x+, 1: CSC pass gathering y)
+    struct TilePass {
+        int k = 0;  // tiles per CU the layout was cut for; 0 = not built
+        int B = 0, W = 0, ntiles = 0, nslices = 0;
+        int ch = 8;  // SELL entries loaded up front per slice
+        long long pairs = 0;
+        size_t padded = 0;
+        TileDesc* td = nullptr;
+        uint2* sl = nullptr;
+        unsigned short* perm = nullptr;
+        unsigned short* idx = nullptr;
+        int* src = nullptr;
+        T* val = nullptr;
+        double* part = nullptr;
+    } tp[2];
+    int lds_max = 0;
     int nb_x = 1, nb_y = 1;
     double b_norm = 0.0, c_norm = 0.0;
     hipGraphExec_t graph_exec = nullptr;
@@ -845,13 +869,20 @@ struct Engine final : EngineBase {
         int dev = 0;
         BO_TRY(hipGetDevice(&dev));
         BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+        BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        // tiled layouts: for an explicit request, or for the tuner (2 tiles per CU)
+        auto tiles_for = [](int code) {
+            return code == 0 ? 2 : (code < 0 && code >= -kMaxTilesPerCu ? -code : 0);
+        };
+        if (int k = tiles_for(prm.lanes_rows)) BO_TRY(build_tile_pass(0, k, h_rp, h_ci));
+        if (int k = tiles_for(prm.lanes_cols)) BO_TRY(build_tile_pass(1, k, h_cp, h_ri));
         // launch plans: lanes > 0 fixes the segment kernel's nonzeros-per-lane
         // target, lanes < 0 selects the stream kernel with LDS budget -lanes,
         // 0 lets the tuner choose
         const bool tune = prm.lanes_rows == 0 || prm.lanes_cols == 0 || prm.hot_columns < 0;
-        plan_rows = fixed_plan(row_len_sorted, prm.lanes_rows);
-        plan_cols = fixed_plan(col_len_sorted, prm.lanes_cols);
-        if (prm.hot_columns > 0 && plan_rows.cap == 0) {
+        plan_rows = fixed_plan(0, prm.lanes_rows);
+        plan_cols = fixed_plan(1, prm.lanes_cols);
+        if (prm.hot_columns > 0 && plan_rows.cap == 0 && plan_rows.tiled == 0) {
             plan_rows.hot = std::min(prm.hot_columns, n);
             plan_rows.hot_grid = grid_for_hot(plan_rows.hot);
             if (plan_rows.hot_grid <= 0) plan_rows.hot = 0;  // does not fit in LDS: plain pass
@@ -874,13 +905,72 @@ struct Engine final : EngineBase {
         return BO_SUCCESS;
     }
 
+    // Cuts and uploads the tiled layout of one pass; leaves it unbuilt when the
+    // LDS is too small or the padded layout would not fit 32-bit offsets.
+    hipError_t build_tile_pass(int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx) {
+        TilePass& t = tp[pass];
+        const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+        const int wmax = std::min(65536, lds_max / static_cast<int>(sizeof(T))) / 64 * 64;
+        if (wmax < 64 || nseg == 0) return hipSuccess;
+        HostTiles h;
+        if (!build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, k * num_cus, h)) return hipSuccess;
+        hipError_t e = hipSuccess;
+        for (const void* f : {reinterpret_cast<const void*>(&k_tile<T, 8, tile_slices_per_wave<8>()>),
+                              reinterpret_cast<const void*>(&k_tile<T, 16, tile_slices_per_wave<16>()>)})
+            if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         static_cast<int>(h.W * sizeof(T)))) != hipSuccess)
+                return e;
+        // up-front chunk: 16 entries when the SELL slices are long on average
+        double narrow_entries = 0.0, narrow_slices = 0.0;
+        for (const uint2& d : h.slices)
+            if (!(d.y & kWideFlag)) {
+                narrow_entries += d.y;
+                narrow_slices += 1.0;
+            }
+        t.ch = narrow_slices > 0 && narrow_entries / narrow_slices > 10.0 ? 16 : 8;
+        t.B = h.B;
+        t.W = h.W;
+        t.ntiles = static_cast<int>(h.tiles.size());
+        t.nslices = static_cast<int>(h.slices.size());
+        t.padded = h.padded;
+        t.pairs = 0;
+        for (unsigned short r : h.perm) t.pairs += r != kNoSeg;
+        if ((e = mem.alloc(&t.td, h.tiles.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.sl, h.slices.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.perm, h.perm.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.idx, h.padded)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.src, h.padded)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.val, h.padded)) != hipSuccess) return e;
+        const size_t np = static_cast<size_t>(h.B) * nseg;
+        if ((e = mem.alloc(&t.part, np)) != hipSuccess) return e;
+        auto up = [&](void* d, const void* src, size_t bytes) {
+            return bytes ? hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
+        };
+        if ((e = up(t.td, h.tiles.data(), sizeof(TileDesc) * h.tiles.size())) != hipSuccess) return e;
+        if ((e = up(t.sl, h.slices.data(), sizeof(uint2) * h.slices.size())) != hipSuccess) return e;
+        if ((e = up(t.perm, h.perm.data(), sizeof(unsigned short) * h.perm.size())) != hipSuccess) return e;
+        if ((e = up(t.idx, h.idx.data(), sizeof(unsigned short) * h.padded)) != hipSuccess) return e;
+        if ((e = up(t.src, h.src.data(), sizeof(int) * h.padded)) != hipSuccess) return e;
+        // slots of (block, segment) pairs without nonzeros are read as 0 forever
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(np)), dim3(kBlock), 0, stream, t.part, 0.0, np);
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;  // host staging goes out of scope
+        t.k = k;
+        return hipGetLastError();
+    }
+
     static int cap_index(int cap) { return cap <= 1024 ? 0 : (cap <= 2048 ? 1 : 2); }
 
-    PassPlan fixed_plan(const std::vector<int>& len, int code) const {
+    // Schedule code: k > 0 segment kernel with k nonzeros per lane; -1..-64
+    // tiled kernels (layout cut for that many tiles per CU, used when built);
+    // <= -1024 stream kernel with LDS budget -code; 0 = segment kernel, k = 4
+    // (the tuner's starting point).
+    PassPlan fixed_plan(int pass, int code) const {
+        const std::vector<int>& len = pass == 0 ? row_len_sorted : col_len_sorted;
         PassPlan p;
-        if (code < 0) {
+        if (code <= -1024) {
             p.cap = kStreamCaps[cap_index(-code)];
-            p.kp = 4;
+        } else if (code < 0) {
+            if (tp[pass].k == -code) p.tiled = -code;
         } else {
             p.kp = code > 0 ? code : 4;
         }
@@ -888,7 +978,7 @@ struct Engine final : EngineBase {
         return p;
     }
 
-    static int plan_code(const PassPlan& p) { return p.cap > 0 ? -p.cap : p.kp; }
+    static int plan_code(const PassPlan& p) { return p.tiled > 0 ? -p.tiled : (p.cap > 0 ? -p.cap : p.kp); }
 
     // ---- launch helpers
     template <int MODE, class TV, class Epi>
@@ -925,13 +1015,40 @@ struct Engine final : EngineBase {
         }
     }
 
-    void launch_primal(const PassPlan& p, const int* done, int t) {
+    // Tiled pass: gather kernel into the partials, `mid` (optional event),
+    // reduction + epilogue kernel.
+    template <class Epi>
+    void launch_tiled(int pass, const T* gvec, const Epi& e, const int* done, hipEvent_t mid) {
+        const TilePass& t = tp[pass];
+        const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+        const unsigned lds = static_cast<unsigned>(t.W * sizeof(T));
+        if (t.ntiles > 0 && t.ch > 8)
+            hipLaunchKernelGGL((k_tile<T, 16, tile_slices_per_wave<16>()>), dim3(t.ntiles), dim3(kTileThreads), lds, stream,
+                               t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
+        else if (t.ntiles > 0)
+            hipLaunchKernelGGL((k_tile<T, 8, tile_slices_per_wave<8>()>), dim3(t.ntiles), dim3(kTileThreads), lds, stream,
+                               t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
+        if (mid) hipEventRecord(mid, stream);
+        hipLaunchKernelGGL(k_tile_reduce<Epi>, dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.B, t.part, e,
+                           done);
+    }
+
+    void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
         EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t};
+        if (p.tiled > 0) {
+            launch_tiled(1, yg, e, done, mid);
+            return;
+        }
         if (p.cap > 0) launch_stream(1, p.cap, cp, ri, cv, e, done);
         else launch_cols<kSumProd>(p.bt, cv, e, nullptr, done);
+        if (mid) hipEventRecord(mid, stream);
     }
-    void launch_dual(const PassPlan& p, const int* done, int t) {
+    void launch_dual(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t};
+        if (p.tiled > 0) {
+            launch_tiled(0, xp, e, done, mid);
+            return;
+        }
         if (p.cap > 0) {
             launch_stream(0, p.cap, rp, ci, rv, e, done);
         } else if (p.hot > 0) {
@@ -941,6 +1058,7 @@ struct Engine final : EngineBase {
         } else {
             launch_rows<kSumProd>(p.bt, rv, e, nullptr, done);
         }
+        if (mid) hipEventRecord(mid, stream);
     }
 
     // Resident workgroups of the persistent CSR pass with `hot` LDS entries.
@@ -979,31 +1097,35 @@ struct Engine final : EngineBase {
     }
 
     // Alternating launches, as inside a solve (primal, dual, primal, ...).
-    bo_status time_pair(const PassPlan& pr, const PassPlan& pc, int reps, double* csr_ms, double* csc_ms) {
-        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps) + 1);
+    // ms4 (optional): average of [CSC gather, CSC reduce, CSR gather, CSR
+    // reduce]; a single-kernel pass reports its time as gather, 0 as reduce.
+    bo_status time_pair(const PassPlan& pr, const PassPlan& pc, int reps, double* csr_ms, double* csc_ms,
+                        double* ms4 = nullptr) {
+        std::vector<hipEvent_t> ev(4 * static_cast<size_t>(reps) + 1);
         for (auto& e : ev) BO_TRY(hipEventCreate(&e));
         launch_primal(pc, nullptr, 0);
         launch_dual(pr, nullptr, 0);
         for (int r = 0; r < reps; ++r) {
-            hipEventRecord(ev[2 * r], stream);
-            launch_primal(pc, nullptr, r);
-            hipEventRecord(ev[2 * r + 1], stream);
-            launch_dual(pr, nullptr, r);
+            hipEventRecord(ev[4 * r], stream);
+            launch_primal(pc, nullptr, r, ev[4 * r + 1]);
+            hipEventRecord(ev[4 * r + 2], stream);
+            launch_dual(pr, nullptr, r, ev[4 * r + 3]);
         }
-        hipEventRecord(ev[2 * reps], stream);
-        hipEventSynchronize(ev[2 * reps]);
-        double sp = 0.0, sd = 0.0;
-        for (int r = 0; r < reps; ++r) {
-            float a = 0.f, b = 0.f;
-            hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]);
-            hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]);
-            sp += a;
-            sd += b;
-        }
+        hipEventRecord(ev[4 * reps], stream);
+        hipEventSynchronize(ev[4 * reps]);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int r = 0; r < reps; ++r)
+            for (int q = 0; q < 4; ++q) {
+                float a = 0.f;
+                hipEventElapsedTime(&a, ev[4 * r + q], ev[4 * r + q + 1]);
+                acc[q] += a;
+            }
         for (auto& e : ev) hipEventDestroy(e);
         BO_TRY(hipGetLastError());
-        if (csc_ms) *csc_ms = sp / reps;
-        if (csr_ms) *csr_ms = sd / reps;
+        if (csc_ms) *csc_ms = (acc[0] + acc[1]) / reps;
+        if (csr_ms) *csr_ms = (acc[2] + acc[3]) / reps;
+        if (ms4)
+            for (int q = 0; q < 4; ++q) ms4[q] = acc[q] / reps;
         return BO_SUCCESS;
     }
 
@@ -1014,19 +1136,19 @@ struct Engine final : EngineBase {
     bo_status autotune() {
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
-        const int codes[] = {2, 4, 8, 16, -1024, -2048, -4096};
+        const int codes[] = {2, 4, 8, 16, -1024, -2048, -4096, -2};
         std::vector<PassPlan> cr, cc;
         if (prm.lanes_rows == 0) {
-            for (int code : codes) cr.push_back(fixed_plan(row_len_sorted, code));
+            for (int code : codes) cr.push_back(fixed_plan(0, code));
         } else {
             cr.push_back(plan_rows);
         }
         if (prm.lanes_cols == 0) {
-            for (int code : codes) cc.push_back(fixed_plan(col_len_sorted, code));
+            for (int code : codes) cc.push_back(fixed_plan(1, code));
         } else {
             cc.push_back(plan_cols);
         }
-        const PassPlan base_r = fixed_plan(row_len_sorted, 4), base_c = fixed_plan(col_len_sorted, 4);
+        const PassPlan base_r = fixed_plan(0, 4), base_c = fixed_plan(1, 4);
         double best_r = 1e30, best_c = 1e30;
         PassPlan pr = cr[0], pc = cc[0];
         for (const PassPlan& cand : cr) {
@@ -1041,10 +1163,12 @@ struct Engine final : EngineBase {
             if (s != BO_SUCCESS) return s;
             if (t_ms < best_c) { best_c = t_ms; pc = cand; }
         }
-        if (prm.hot_columns > 0 && pr.cap == 0) {  // fixed by the caller
+        if (pr.cap > 0 || pr.tiled > 0) {
+            // no hot-column variant
+        } else if (prm.hot_columns > 0) {  // fixed by the caller
             pr.hot = plan_rows.hot;
             pr.hot_grid = plan_rows.hot_grid;
-        } else if (prm.hot_columns < 0 && pr.cap == 0) {  // LDS copy of the hottest x+ entries
+        } else if (prm.hot_columns < 0) {  // LDS copy of the hottest x+ entries
             for (int hot : {4096, 8192, 16384}) {
                 if (hot > n) break;
                 PassPlan h = pr;
@@ -1105,6 +1229,12 @@ struct Engine final : EngineBase {
         launch_rows<kWrite>(br, rv64, wr, nullptr, nullptr);
         EpiWrite<T> wc{cs, rs, cv};
         launch_cols<kWrite>(bc, cv64, wc, nullptr, nullptr);
+        for (int pass = 0; pass < 2; ++pass) {
+            const TilePass& t = tp[pass];
+            if (t.k > 0 && t.padded > 0)
+                hipLaunchKernelGGL(k_tile_values<T>, dim3(grid_for(t.padded)), dim3(kBlock), 0, stream, t.src, t.padded,
+                                   pass == 0 ? rv : cv, t.val);
+        }
         hipLaunchKernelGGL(k_scale_cols<T>, dim3(gn), dim3(kBlock), 0, stream, n, cs, c64, lb64, ub64, wx,
                            has_warm, c_hat, l_hat, u_hat, xk, x0, xp);
         hipLaunchKernelGGL(k_scale_rows<T>, dim3(gm), dim3(kBlock), 0, stream, m, rs, b64, eq, wy, has_warm,
@@ -1234,6 +1364,39 @@ struct Engine final : EngineBase {
     bo_status time_spmv(int reps, double* csr_ms, double* csc_ms) override {
         return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms);
     }
+
+    bo_status time_kernels(int reps, double* ms4) override {
+        return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, nullptr, nullptr, ms4);
+    }
+
+    // Schedule of a pass and the algorithmic bytes of its kernels: what each
+    // launch must move at least -- matrix entries in the stored format, the
+    // gathered vector once, pointers or slice descriptors, partials, and the
+    // epilogue's per-segment reads and writes.
+    bo_status pass_info(int pass, bo_pdlp_pass_info* out) override {
+        const PassPlan& p = pass == 0 ? plan_rows : plan_cols;
+        const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+        const double tb = sizeof(T);
+        const double epi = pass == 0 ? 49.0 + 3.0 * tb : 24.0 + 4.0 * tb;
+        std::memset(out, 0, sizeof(*out));
+        out->schedule = plan_code(p);
+        if (p.tiled > 0) {
+            const TilePass& t = tp[pass];
+            out->kernels = 2;
+            out->blocks = t.B;
+            out->block_entries = t.W;
+            out->tiles = t.ntiles;
+            out->slices = t.nslices;
+            out->pairs = t.pairs;
+            out->padded = static_cast<long long>(t.padded);
+            out->gather_bytes = (tb + 2.0) * nnz + tb * ng + 8.0 * t.nslices + 2.0 * t.pairs + 8.0 * t.pairs;
+            out->reduce_bytes = 8.0 * t.B * nseg + epi * nseg;
+        } else {
+            out->kernels = 1;
+            out->gather_bytes = (tb + 4.0) * nnz + 4.0 * (nseg + 1) + tb * ng + epi * nseg;
+        }
+        return BO_SUCCESS;
+    }
 };
 
 }  // namespace
@@ -1315,6 +1478,16 @@ bo_status bo_pdlp_get_solution(bo_pdlp* h, double* x, double* y) {
 }
 
 void bo_pdlp_destroy(bo_pdlp* h) { delete h; }
+
+bo_status bo_pdlp_time_kernels(bo_pdlp* h, int reps, double* ms4) {
+    if (!h || !h->eng || !ms4) return BO_INVALID_ARGUMENT;
+    return h->eng->time_kernels(reps, ms4);
+}
+
+bo_status bo_pdlp_get_pass_info(bo_pdlp* h, int pass, bo_pdlp_pass_info* out) {
+    if (!h || !h->eng || !out || pass < 0 || pass > 1) return BO_INVALID_ARGUMENT;
+    return h->eng->pass_info(pass, out);
+}
 
 bo_status bo_pdlp_time_spmv(bo_pdlp* h, int reps, double* csr_ms, double* csc_ms) {
     if (!h) return BO_INVALID_ARGUMENT;

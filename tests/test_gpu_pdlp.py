@@ -165,6 +165,58 @@ def test_pdlp_stream_long_segments(pkg, po, lpdata, cap):
     assert abs(fa - fb) / (1 + abs(fa)) <= 1e-5
 
 
+@pytest.mark.parametrize("k", [1, 2, 4])
+def test_pdlp_tiled_schedule(pkg, po, instances, k):
+    # both passes on the tiled gather (vector block in LDS, SELL-64 slices,
+    # FP64 partials) + reduction kernels
+    name = "zipf_2000x5000_s9"
+    lp = instances[name]
+    print("case: tiled", k, "per CU", flush=True)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=-k, lanes_cols=-k, hot_columns=0)
+    s = h.solve()
+    assert s["lanes_rows"] == -k and s["lanes_cols"] == -k
+    for p in (0, 1):
+        info = h.pass_info(p)
+        assert info["kernels"] == 2 and info["tiles"] >= 1 and info["padded"] >= lp.nnz, info
+    x, y = h.solution()
+    check_solution(po, lp, x, y, s, name)
+    s2 = h.solve()
+    x2, y2 = h.solution()
+    assert s2["iterations"] == s["iterations"] and np.array_equal(x, x2) and np.array_equal(y, y2)
+
+
+@pytest.mark.parametrize("fp64", [0, 1])
+def test_pdlp_tiled_many_blocks(pkg, po, lpdata, fp64):
+    # n = 90000 gathered entries do not fit one LDS block: several blocks,
+    # partials added by the reduction kernel; compared with the segment schedule
+    lp = lpdata.random_lp(3000, 90000, 3, 5)
+    print("case: tiled many blocks, fp64", fp64, flush=True)
+    seg = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=4, lanes_cols=4, hot_columns=0, fp64_iterates=fp64)
+    sa = seg.solve()
+    xa, ya = seg.solution()
+    til = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=-2, lanes_cols=-2, hot_columns=0, fp64_iterates=fp64)
+    sb = til.solve()
+    xb, yb = til.solution()
+    info = til.pass_info(0)
+    print("segment iters", sa["iterations"], "tiled iters", sb["iterations"], "csr pass", info, flush=True)
+    assert info["kernels"] == 2 and info["blocks"] >= 3
+    ka, kb = po.kkt(lp, xa, ya), po.kkt(lp, xb, yb)
+    assert sa["status"] == 0 and sb["status"] == 0
+    assert ka["rel_kkt"] <= TOL and kb["rel_kkt"] <= TOL
+    assert abs(ka["primal_obj"] - kb["primal_obj"]) / (1 + abs(ka["primal_obj"])) <= 1e-5
+
+
+def test_pdlp_tiled_long_segments(pkg, po, lpdata):
+    # a dense row and a dense column: pairs of 128+ nonzeros take wide slices
+    lp = dense_line_lp(lpdata)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, lanes_rows=-1, lanes_cols=-1, hot_columns=0)
+    s = h.solve()
+    x, y = h.solution()
+    k = po.kkt(lp, x, y)
+    print("tiled dense-line iters", s["iterations"], "rel-KKT %.3e" % k["rel_kkt"], flush=True)
+    assert s["status"] == 0 and k["rel_kkt"] <= TOL and s["lanes_rows"] == -1
+
+
 def test_pdlp_deterministic_repeat(pkg, instances):
     lp = instances["zipf_2000x5000_s9"]
     h = pkg.Pdlp(lp)
