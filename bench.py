@@ -45,19 +45,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fused_kernel_bytes(lp):
-    """Algorithmic bytes of one launch of each fused iteration kernel.
-    CSR pass (A x+ and the dual Halpern step): 8 B/nnz (fp32 value + int32
-    index), row pointers, x+ gathered once (4n), and per row: carried A z
-    r/w (FP64, 16), dual iterate r/w (FP64, 16), b (4), is_eq (1), dual
-    anchor (8), A z0 (8), y+ and the fp32 copy of y written (8): 61 B/row.
-    CSC pass (A'y and the primal Halpern step): 8 B/nnz, column pointers,
-    y copy gathered once (4m), and per column: primal iterate r/w (FP64, 16),
-    c, l, u (12), primal anchor (8), x+ written (4): 40 B/column."""
-    m, n, z = lp.m, lp.n, lp.nnz
-    csr = 8.0 * z + 4.0 * (m + 1) + 4.0 * n + 61.0 * m
-    csc = 8.0 * z + 4.0 * (n + 1) + 4.0 * m + 40.0 * n
-    return csr, csc
+KERNEL_ROLES = ("csc_gather", "csc_reduce", "csr_gather", "csr_reduce")
+
+
+def kernel_table(h, reps):
+    """Per-launch time (HIP events on the engine's stream, alternating passes as
+    in a solve) and algorithmic bytes of every kernel of one PDHG iteration.
+    A pass runs either one fused kernel (its "gather" entry; reduce = 0) or
+    the tiled gather kernel plus the reduction + epilogue kernel.  The engine
+    reports the bytes: matrix entries in the stored format, the gathered
+    vector once, slice descriptors, perm entries and partials (tiled), and
+    the epilogue's per-row/column reads and writes."""
+    ms4 = h.time_kernels4(reps)
+    info = {0: h.pass_info(0), 1: h.pass_info(1)}
+    table = {}
+    for k, role in enumerate(KERNEL_ROLES):
+        p = 1 if role.startswith("csc") else 0
+        b = info[p]["gather_bytes"] if role.endswith("gather") else info[p]["reduce_bytes"]
+        if ms4[k] > 0 and b > 0:
+            table[role] = {"ms": ms4[k], "bytes": b, "gbps": b / (ms4[k] * 1e-3) / 1e9}
+    return table, info
 
 
 def spmv_bytes(r, c, z):
@@ -132,6 +139,10 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="replay only the two fused iteration kernels (and a 256 MiB copy "
                          "kernel for counter calibration); for hardware-counter runs")
+    ap.add_argument("--lanes-rows", type=int, default=0,
+                    help="CSR pass schedule (engine encoding; 0 = tune)")
+    ap.add_argument("--lanes-cols", type=int, default=0,
+                    help="CSC pass schedule (engine encoding; 0 = tune)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each kernel directly instead of replaying hipGraphs (same kernels; "
                          "used under rocprofv3, whose kernel tracing does not survive graph capture here)")
@@ -152,17 +163,17 @@ def main():
     log(f"workload {args.workload}: {desc}; m={lp.m} n={lp.n} nnz={lp.nnz}; "
         f"generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    h = pkg.Pdlp(lp, eps_rel_kkt=args.eps, max_iterations=200000, use_graph=0 if args.no_graph else 1)
+    h = pkg.Pdlp(lp, eps_rel_kkt=args.eps, max_iterations=200000, use_graph=0 if args.no_graph else 1,
+                 lanes_rows=args.lanes_rows, lanes_cols=args.lanes_cols)
     setup_s = time.perf_counter() - t0
     log(f"upload+autotune {setup_s:.2f}s")
 
     if args.kernels_only:
-        csr_ms, csc_ms = h.time_kernels(reps=20)
+        table, info = kernel_table(h, 20)
         copy_bytes = 256 << 20
         copy_ms = pkg.copy_kernel_ms(copy_bytes, reps=5)
-        csr_b, csc_b = fused_kernel_bytes(lp)
-        print(json.dumps({"mode": "kernels-only", "workload": desc, "csr_pass_dual_step_ms": csr_ms,
-                          "csc_pass_primal_step_ms": csc_ms, "csr_pass_bytes": csr_b, "csc_pass_bytes": csc_b,
+        print(json.dumps({"mode": "kernels-only", "workload": desc, "kernels": table,
+                          "schedules": {"csr": info[0]["schedule"], "csc": info[1]["schedule"]},
                           "copy_kernel_bytes_read": copy_bytes, "copy_kernel_ms": copy_ms}), flush=True)
         return
 
@@ -184,13 +195,9 @@ def main():
     last = stats[-1]
 
     # dominant-kernel roofline: HIP events on the engine's stream, same kernels
-    csr_ms, csc_ms = h.time_kernels(reps=50)
-    csr_b, csc_b = fused_kernel_bytes(lp)
-    if csc_ms >= csr_ms:
-        kname, kms, kbytes = "csc_pass_primal_step", csc_ms, csc_b
-    else:
-        kname, kms, kbytes = "csr_pass_dual_step", csr_ms, csr_b
-    achieved = kbytes / (kms * 1e-3) / 1e9
+    table, info = kernel_table(h, 50)
+    kname = max(table, key=lambda k: table[k]["ms"])
+    dom = table[kname]
     traffic = _traffic_from_profiles(args.workload)
     iter_ms = np.mean([t * 1e3 / max(s["iterations"], 1) for t, s in zip(times, stats)])
 
@@ -219,15 +226,19 @@ def main():
         "rel_kkt": last["rel_kkt"],
         "primal_objective": last["primal_obj"],
         "ms_per_iteration": iter_ms,
-        "kernels_ms": {"csr_pass_dual_step": csr_ms, "csc_pass_primal_step": csc_ms},
-        "spmv_gbps": {
-            "csr_survey_bytes": spmv_bytes(lp.m, lp.n, lp.nnz) / (csr_ms * 1e-3) / 1e9,
-            "csc_survey_bytes": spmv_bytes(lp.n, lp.m, lp.nnz) / (csc_ms * 1e-3) / 1e9,
+        "kernels": table,
+        "pass_layout": {"csr": info[0], "csc": info[1]},
+        "spmv_gbps": {  # SpMV bytes of the reference formula over each whole pass
+            "csr_pass": spmv_bytes(lp.m, lp.n, lp.nnz)
+            / (sum(table[k]["ms"] for k in table if k.startswith("csr")) * 1e-3) / 1e9,
+            "csc_pass": spmv_bytes(lp.n, lp.m, lp.nnz)
+            / (sum(table[k]["ms"] for k in table if k.startswith("csc")) * 1e-3) / 1e9,
         },
         "roofline": {
-            "bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if traffic is None else traffic.get(kname),
+            "bound": "hbm", "kernel": kname, "achieved": dom["gbps"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dom["gbps"] / HBM_PEAK_GBS,
+            "traffic": None if traffic is None or kname not in traffic else traffic[kname]["bytes"],
+            "algorithmic_bytes": dom["bytes"], "launch_ms": dom["ms"],
         },
         "setup_s": setup_s,
     }

@@ -914,12 +914,9 @@ struct Engine final : EngineBase {
         if (wmax < 64 || nseg == 0) return hipSuccess;
         HostTiles h;
         if (!build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, k * num_cus, h)) return hipSuccess;
-        hipError_t e = hipSuccess;
-        for (const void* f : {reinterpret_cast<const void*>(&k_tile<T, 8, tile_slices_per_wave<8>()>),
-                              reinterpret_cast<const void*>(&k_tile<T, 16, tile_slices_per_wave<16>()>)})
-            if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         static_cast<int>(h.W * sizeof(T)))) != hipSuccess)
-                return e;
+        hipError_t e = pass == 0 ? set_tile_lds<0>(static_cast<int>(h.W * sizeof(T)))
+                                 : set_tile_lds<1>(static_cast<int>(h.W * sizeof(T)));
+        if (e != hipSuccess) return e;
         // up-front chunk: 16 entries when the SELL slices are long on average
         double narrow_entries = 0.0, narrow_slices = 0.0;
         for (const uint2& d : h.slices)
@@ -1015,19 +1012,29 @@ struct Engine final : EngineBase {
         }
     }
 
+    template <int PASS>
+    static hipError_t set_tile_lds(int bytes) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&k_tile<T, 8, tile_slices_per_wave<8>(), PASS>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<T, 16, tile_slices_per_wave<16>(), PASS>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    }
+
     // Tiled pass: gather kernel into the partials, `mid` (optional event),
     // reduction + epilogue kernel.
-    template <class Epi>
-    void launch_tiled(int pass, const T* gvec, const Epi& e, const int* done, hipEvent_t mid) {
-        const TilePass& t = tp[pass];
-        const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+    template <int PASS, class Epi>
+    void launch_tiled(const T* gvec, const Epi& e, const int* done, hipEvent_t mid) {
+        const TilePass& t = tp[PASS];
+        const int nseg = PASS == 0 ? m : n, ng = PASS == 0 ? n : m;
         const unsigned lds = static_cast<unsigned>(t.W * sizeof(T));
         if (t.ntiles > 0 && t.ch > 8)
-            hipLaunchKernelGGL((k_tile<T, 16, tile_slices_per_wave<16>()>), dim3(t.ntiles), dim3(kTileThreads), lds, stream,
-                               t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
+            hipLaunchKernelGGL((k_tile<T, 16, tile_slices_per_wave<16>(), PASS>), dim3(t.ntiles), dim3(kTileThreads),
+                               lds, stream, t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
         else if (t.ntiles > 0)
-            hipLaunchKernelGGL((k_tile<T, 8, tile_slices_per_wave<8>()>), dim3(t.ntiles), dim3(kTileThreads), lds, stream,
-                               t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
+            hipLaunchKernelGGL((k_tile<T, 8, tile_slices_per_wave<8>(), PASS>), dim3(t.ntiles), dim3(kTileThreads),
+                               lds, stream, t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
         if (mid) hipEventRecord(mid, stream);
         hipLaunchKernelGGL(k_tile_reduce<Epi>, dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.B, t.part, e,
                            done);
@@ -1036,7 +1043,7 @@ struct Engine final : EngineBase {
     void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
         EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t};
         if (p.tiled > 0) {
-            launch_tiled(1, yg, e, done, mid);
+            launch_tiled<1>(yg, e, done, mid);
             return;
         }
         if (p.cap > 0) launch_stream(1, p.cap, cp, ri, cv, e, done);
@@ -1046,7 +1053,7 @@ struct Engine final : EngineBase {
     void launch_dual(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t};
         if (p.tiled > 0) {
-            launch_tiled(0, xp, e, done, mid);
+            launch_tiled<0>(xp, e, done, mid);
             return;
         }
         if (p.cap > 0) {

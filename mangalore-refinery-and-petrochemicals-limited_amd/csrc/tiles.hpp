@@ -308,7 +308,9 @@ __device__ inline void batch_finish(const SliceBatch<T, CH, NS>& b, const T* lds
 
 // One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of 64
 // entries of a slice loaded up front; NS: slices a wave loads at once.
-template <class T, int CH, int NS>
+// PASS only names the instantiation (0: CSR pass, 1: CSC pass), so that
+// profiles tell the two passes apart.
+template <class T, int CH, int NS, int PASS>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(const TileDesc* __restrict__ tiles, const uint2* __restrict__ slices, const unsigned short* __restrict__ perm,
        const T* __restrict__ val, const unsigned short* __restrict__ idx, const T* __restrict__ gvec, int ng, int W,
