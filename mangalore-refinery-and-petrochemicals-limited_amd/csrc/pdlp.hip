@@ -696,16 +696,17 @@ struct Engine final : EngineBase {
     struct TilePass {
         int k = 0;  // tiles per CU the layout was cut for; 0 = not built
         int B = 0, W = 0, ntiles = 0, nslices = 0;
-        int ch = 8;  // SELL entries loaded up front per slice
+        unsigned lds_bytes = 0;  // dynamic LDS of the gather kernel
         long long pairs = 0;
         size_t padded = 0;
         TileDesc* td = nullptr;
-        uint2* sl = nullptr;
-        unsigned short* perm = nullptr;
+        unsigned* words = nullptr;  // slice words
         unsigned short* idx = nullptr;
         int* src = nullptr;
         T* val = nullptr;
-        double* part = nullptr;
+        int* seg_ptr = nullptr;     // segment -> partial slots
+        unsigned* slot = nullptr;
+        double* part = nullptr;     // one row of 64 partials per slice
     } tp[2];
     int lds_max = 0;
     int nb_x = 1, nb_y = 1;
@@ -910,46 +911,43 @@ struct Engine final : EngineBase {
     hipError_t build_tile_pass(int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx) {
         TilePass& t = tp[pass];
         const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+        if (nseg == 0) return hipSuccess;
+        // LDS holds the vector block: the largest block that fits
         const int wmax = std::min(65536, lds_max / static_cast<int>(sizeof(T))) / 64 * 64;
-        if (wmax < 64 || nseg == 0) return hipSuccess;
+        if (wmax < 64) return hipSuccess;
         HostTiles h;
         if (!build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, k * num_cus, h)) return hipSuccess;
-        hipError_t e = pass == 0 ? set_tile_lds<0>(static_cast<int>(h.W * sizeof(T)))
-                                 : set_tile_lds<1>(static_cast<int>(h.W * sizeof(T)));
+        // buffer ranges of the kernels are 32-bit byte counts below 2^31
+        if (h.pairs * 4 >= (1LL << 31) - 64 || static_cast<long long>(h.slice_word.size()) * kWave * 8 >= (1LL << 31) - 64)
+            return hipSuccess;
+        const size_t lds_need = static_cast<size_t>(h.W) * sizeof(T);
+        hipError_t e = pass == 0 ? set_tile_lds<0>(static_cast<int>(lds_need))
+                                 : set_tile_lds<1>(static_cast<int>(lds_need));
         if (e != hipSuccess) return e;
-        // up-front chunk: 16 entries when the SELL slices are long on average
-        double narrow_entries = 0.0, narrow_slices = 0.0;
-        for (const uint2& d : h.slices)
-            if (!(d.y & kWideFlag)) {
-                narrow_entries += d.y;
-                narrow_slices += 1.0;
-            }
-        t.ch = narrow_slices > 0 && narrow_entries / narrow_slices > 10.0 ? 16 : 8;
+        t.lds_bytes = static_cast<unsigned>(lds_need);
         t.B = h.B;
         t.W = h.W;
         t.ntiles = static_cast<int>(h.tiles.size());
-        t.nslices = static_cast<int>(h.slices.size());
+        t.nslices = static_cast<int>(h.slice_word.size());
         t.padded = h.padded;
-        t.pairs = 0;
-        for (unsigned short r : h.perm) t.pairs += r != kNoSeg;
+        t.pairs = h.pairs;
         if ((e = mem.alloc(&t.td, h.tiles.size())) != hipSuccess) return e;
-        if ((e = mem.alloc(&t.sl, h.slices.size())) != hipSuccess) return e;
-        if ((e = mem.alloc(&t.perm, h.perm.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.words, h.slice_word.size())) != hipSuccess) return e;
         if ((e = mem.alloc(&t.idx, h.padded)) != hipSuccess) return e;
         if ((e = mem.alloc(&t.src, h.padded)) != hipSuccess) return e;
         if ((e = mem.alloc(&t.val, h.padded)) != hipSuccess) return e;
-        const size_t np = static_cast<size_t>(h.B) * nseg;
-        if ((e = mem.alloc(&t.part, np)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.seg_ptr, h.seg_ptr.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.slot, h.slot.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.part, h.slice_word.size() * kWave)) != hipSuccess) return e;
         auto up = [&](void* d, const void* src, size_t bytes) {
             return bytes ? hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
         };
         if ((e = up(t.td, h.tiles.data(), sizeof(TileDesc) * h.tiles.size())) != hipSuccess) return e;
-        if ((e = up(t.sl, h.slices.data(), sizeof(uint2) * h.slices.size())) != hipSuccess) return e;
-        if ((e = up(t.perm, h.perm.data(), sizeof(unsigned short) * h.perm.size())) != hipSuccess) return e;
+        if ((e = up(t.words, h.slice_word.data(), sizeof(unsigned) * h.slice_word.size())) != hipSuccess) return e;
         if ((e = up(t.idx, h.idx.data(), sizeof(unsigned short) * h.padded)) != hipSuccess) return e;
         if ((e = up(t.src, h.src.data(), sizeof(int) * h.padded)) != hipSuccess) return e;
-        // slots of (block, segment) pairs without nonzeros are read as 0 forever
-        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(np)), dim3(kBlock), 0, stream, t.part, 0.0, np);
+        if ((e = up(t.seg_ptr, h.seg_ptr.data(), sizeof(int) * h.seg_ptr.size())) != hipSuccess) return e;
+        if ((e = up(t.slot, h.slot.data(), sizeof(unsigned) * h.slot.size())) != hipSuccess) return e;
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;  // host staging goes out of scope
         t.k = k;
         return hipGetLastError();
@@ -1014,30 +1012,31 @@ struct Engine final : EngineBase {
 
     template <int PASS>
     static hipError_t set_tile_lds(int bytes) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&k_tile<T, 8, tile_slices_per_wave<8>(), PASS>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e != hipSuccess) return e;
-        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<T, 16, tile_slices_per_wave<16>(), PASS>),
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<T, kTileRows, PASS>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     }
 
     // Tiled pass: gather kernel into the partials, `mid` (optional event),
-    // reduction + epilogue kernel.
+    // reduction + epilogue kernel.  Only the reduction reads the done flag:
+    // partials computed after convergence are never used.
     template <int PASS, class Epi>
     void launch_tiled(const T* gvec, const Epi& e, const int* done, hipEvent_t mid) {
         const TilePass& t = tp[PASS];
         const int nseg = PASS == 0 ? m : n, ng = PASS == 0 ? n : m;
-        const unsigned lds = static_cast<unsigned>(t.W * sizeof(T));
-        if (t.ntiles > 0 && t.ch > 8)
-            hipLaunchKernelGGL((k_tile<T, 16, tile_slices_per_wave<16>(), PASS>), dim3(t.ntiles), dim3(kTileThreads),
-                               lds, stream, t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
-        else if (t.ntiles > 0)
-            hipLaunchKernelGGL((k_tile<T, 8, tile_slices_per_wave<8>(), PASS>), dim3(t.ntiles), dim3(kTileThreads),
-                               lds, stream, t.td, t.sl, t.perm, t.val, t.idx, gvec, ng, t.W, nseg, t.part, done);
+        if (t.ntiles > 0)
+            hipLaunchKernelGGL((k_tile<T, kTileRows, PASS>), dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream,
+                               t.td, t.words, t.val, t.idx, gvec, ng, t.W, t.part);
         if (mid) hipEventRecord(mid, stream);
-        hipLaunchKernelGGL(k_tile_reduce<Epi>, dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.B, t.part, e,
-                           done);
+        const int nslot = static_cast<int>(t.pairs), npart = t.nslices * kWave;
+        if (t.B <= 4)
+            hipLaunchKernelGGL((k_tile_reduce<4, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.seg_ptr,
+                               t.slot, nslot, t.part, npart, e, done);
+        else if (t.B <= 8)
+            hipLaunchKernelGGL((k_tile_reduce<8, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.seg_ptr,
+                               t.slot, nslot, t.part, npart, e, done);
+        else
+            hipLaunchKernelGGL((k_tile_reduce<16, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
+                               t.seg_ptr, t.slot, nslot, t.part, npart, e, done);
     }
 
     void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
@@ -1396,8 +1395,9 @@ struct Engine final : EngineBase {
             out->slices = t.nslices;
             out->pairs = t.pairs;
             out->padded = static_cast<long long>(t.padded);
-            out->gather_bytes = (tb + 2.0) * nnz + tb * ng + 8.0 * t.nslices + 2.0 * t.pairs + 8.0 * t.pairs;
-            out->reduce_bytes = 8.0 * t.B * nseg + epi * nseg;
+            out->gather_bytes = (tb + 2.0) * nnz + tb * ng + 4.0 * t.nslices +
+                                static_cast<double>(sizeof(TileDesc)) * t.ntiles + 8.0 * t.pairs;
+            out->reduce_bytes = 4.0 * (nseg + 1) + 12.0 * t.pairs + epi * nseg;
         } else {
             out->kernels = 1;
             out->gather_bytes = (tb + 4.0) * nnz + 4.0 * (nseg + 1) + tb * ng + epi * nseg;

@@ -1,21 +1,27 @@
-// tiles.hpp -- two-level ("tiled") schedule of a fused iteration pass:
-// the gathered vector is cut into blocks of W consecutive entries that fit in
-// LDS, and every workgroup (a tile) owns one block and a contiguous range of
-// segments.  It copies its block into LDS with coalesced loads once, then
-// walks the nonzeros of its segments that fall into the block: they are
-// stored as SELL-64 slices (64 segments per slice, sorted by length inside
-// the tile, entry j of lane l at slice_offset + 64 j + l, padded to the
-// slice's longest segment) with 16-bit block-local indices, so every global
-// load is a fully coalesced stream (4 + 2 bytes per nonzero for FP32) and
-// every gather is an LDS read.  A (block, segment) pair with at least
-// kWideMin nonzeros gets a "wide" slice of its own instead: its entries are
-// contiguous, the 64 lanes of a wave stride over them and a fixed xor tree
-// adds the lanes, so one long segment never pads 63 short ones.  Each tile
-// writes one FP64 partial per (block, segment) pair it holds; a reduction
-// kernel adds the B partials of a segment in block order and runs the pass's
-// epilogue.  Pairs without nonzeros are never written: their slots stay at
-// the zero they were initialised with.  Results are deterministic (fixed
-// slice walk, fixed block order).
+// tiles.hpp -- two-level ("tiled") schedule of a fused iteration pass.
+//
+// The gathered vector is cut into B blocks of W consecutive entries that fit
+// in LDS.  A tile is one 1024-thread workgroup: it owns one block and a
+// contiguous range of segments (rows or columns), copies its block into LDS
+// once with 16-byte loads, then walks the nonzeros of its segments that fall
+// into the block -- its (block, segment) pairs -- gathering from LDS.
+//
+// Layout of a tile's pairs (built on the host):
+//   * pairs sorted by length; 64 of them form a SELL slice (entry j of lane l
+//     at slice start + 64 j + l, padded to the slice's longest pair), and a
+//     long pair forms a "wide" slice of its own instead (entry j at slice
+//     start + j: the 64 lanes stride over it and add with a fixed xor tree);
+//   * the slices are dealt to the 16 wavefronts longest-first (LPT) and each
+//     wavefront's slices are stored back to back, so a wavefront streams ONE
+//     contiguous run of 64-entry rows: FP32 values and 16-bit block-local
+//     indices, fully coalesced, read with raw buffer loads bounded to the run
+//     (rows past the end read 0 without touching memory) and double-buffered
+//     in registers so the next CH rows are always in flight;
+//   * at the end of each slice the wavefront stores its 64 partial sums
+//     (FP64) as one coalesced row of the partial array, in slice order.
+// A reduction kernel adds, for every segment, its partials in block order
+// (through a segment -> partial-slot index built on the host) and runs the
+// pass epilogue.  Results are deterministic: fixed walk, fixed block order.
 #pragma once
 
 #include <algorithm>
@@ -26,22 +32,25 @@
 
 namespace bo {
 
-constexpr int kTileThreads = 1024;                 // one workgroup per CU: the block fills most of the LDS
+constexpr int kTileThreads = 1024;           // one workgroup per CU: the block fills most of the LDS
 constexpr int kTileWaves = kTileThreads / kWave;
-template <int CH>
-constexpr int tile_slices_per_wave() { return CH > 8 ? 1 : 2; }  // SELL slices a wave loads at once
-constexpr unsigned short kNoSeg = 0xFFFF;          // perm entry of an unused lane
-constexpr int kTileMaxSpan = 65535;                // local segment ids stay below this (16-bit, kNoSeg excluded)
-constexpr int kWideMin = 128;                      // pair length from which a pair gets a wide slice
-constexpr unsigned kWideFlag = 0x80000000u;        // slice length word: wide slice
+constexpr int kWideMin = 128;                // pair length from which a pair gets a wide slice
+constexpr unsigned kWideFlag = 0x80000000u;  // slice word: wide slice (low bits: rows)
+constexpr int kFillTrips = 10;               // 16-byte loads per thread covering a 160 KB block
+constexpr int kTileRows = 8;                 // rows of a wavefront's run in flight per buffer
+
+struct WaveDesc {
+    unsigned off;  // first entry of the wavefront's rows (64 entries per row)
+    int nrows;     // rows of the run
+    int slice0;    // first slice (global index; the partial row of slice q is part[64 q ...])
+    int nslice;    // slices of the run
+};
 
 struct TileDesc {
-    int block;   // gathered block: entries [block * W, block * W + W)
-    int seg0;    // first segment of the tile's range
-    int slice0;  // first SELL slice (the wide slices come first)
-    int nslice;  // slices of the tile
-    int nwide;   // wide slices among them
-    int pad[3];
+    int block;  // gathered block: entries [block * W, block * W + W)
+    int nslice;
+    int pad[2];
+    WaveDesc wave[kTileWaves];
 };
 
 // Host-side layout of one pass.
@@ -49,19 +58,28 @@ struct HostTiles {
     int B = 0;  // blocks
     int W = 0;  // entries per block
     std::vector<TileDesc> tiles;
-    std::vector<uint2> slices;          // x: first padded entry, y: length (| kWideFlag: wide slice)
-    std::vector<unsigned short> perm;   // 64 per slice: segment - seg0, or kNoSeg
+    std::vector<unsigned> slice_word;   // per slice: rows | kWideFlag for a wide slice
     std::vector<unsigned short> idx;    // padded entries: block-local gathered index
     std::vector<int> src;               // padded entries: source nonzero, -1 for padding
+    std::vector<int> seg_ptr;           // per segment: range of its partial slots in `slot`
+    std::vector<unsigned> slot;         // partial slots (64 q + lane), block order within a segment
     size_t padded = 0;
+    long long pairs = 0;
+    int max_wave_slices = 0;
 };
 
 // Builds the tiled layout of a segment matrix (ptr over nseg segments, gidx
 // the gathered index of every nonzero, ng gathered entries).  wmax: largest
-// block (a multiple of 64); target_tiles: about how many tiles to cut.
-inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int wmax, int target_tiles, HostTiles& h) {
+// block (a multiple of 64); target_tiles: about how many tiles to cut;
+// wide_min: pair length from which a pair gets a wide slice.
+// Returns false when the padded layout does not fit 32-bit offsets.
+inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int wmax, int target_tiles, HostTiles& h,
+                        int wide_min = kWideMin) {
     h = HostTiles{};
-    if (nseg <= 0 || ng <= 0) return true;
+    if (nseg <= 0 || ng <= 0) {
+        h.seg_ptr.assign(static_cast<size_t>(std::max(nseg, 0)) + 1, 0);
+        return true;
+    }
     const int B = std::max(1, (ng + wmax - 1) / wmax);
     int W = (ng + B - 1) / B;
     W = (W + 63) / 64 * 64;
@@ -95,11 +113,10 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
             }
         }
     }
-    // tiles: consecutive pairs of one block, equal work (nonzeros plus a
-    // per-pair cost for the partial and its index).  Blocks get tile counts
-    // in proportion to their work (largest remainders), so the layout has
-    // target_tiles tiles -- a whole number of rounds of resident workgroups --
-    // unless a block needs more to keep local segment ids in 16 bits.
+    // Tiles: consecutive pairs of one block with equal work (nonzeros plus a
+    // per-pair cost for its partial).  Blocks get tile counts in proportion
+    // to their work (largest remainders), so the layout has target_tiles
+    // tiles -- a whole number of rounds of resident workgroups.
     std::vector<long long> bwork(B, 0);
     long long total = 0;
     for (int c = 0; c < B; ++c) {
@@ -134,8 +151,7 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
         long long w = 0, cum = 0;
         int cut = 1;  // next equal-work boundary: cut * bwork / ntile
         for (size_t q = 0; q < pairs[c].size(); ++q) {
-            const bool full = cum * ntile[c] >= cut * bwork[c];
-            if (q > p0 && (full || pairs[c][q].s - pairs[c][p0].s >= kTileMaxSpan)) {
+            if (q > p0 && cum * ntile[c] >= cut * bwork[c]) {
                 spans.push_back(Span{c, p0, q, w});
                 p0 = q;
                 w = 0;
@@ -148,61 +164,94 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
     }
     // heaviest tiles first (longest-processing-time-first dispatch)
     std::stable_sort(spans.begin(), spans.end(), [](const Span& a, const Span& b) { return a.work > b.work; });
+
+    // (segment, block, slot) of every pair, gathered into seg_ptr / slot below
+    std::vector<int> pair_seg, pair_blk;
+    std::vector<unsigned> pair_slot;
     std::vector<size_t> order;
+    struct Slice {
+        std::vector<size_t> members;  // pair indices (into pairs[c]); one for a wide slice
+        int rows;
+        bool wide;
+    };
     for (const Span& sp : spans) {
         const std::vector<Pair>& P = pairs[sp.c];
         order.resize(sp.p1 - sp.p0);
         std::iota(order.begin(), order.end(), sp.p0);
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return P[a].len > P[b].len; });
-        TileDesc td{sp.c, P[sp.p0].s, static_cast<int>(h.slices.size()), 0, 0, {0, 0, 0}};
-        size_t nwide = 0;
-        while (nwide < order.size() && P[order[nwide]].len >= kWideMin) ++nwide;
-        for (size_t q = 0; q < nwide; ++q) {  // wide slices: one pair, contiguous entries
-            const Pair& p = P[order[q]];
-            const size_t off = h.padded;
-            if (off + static_cast<size_t>(p.len) > 0xFFFFFFFFull || static_cast<unsigned>(p.len) >= kWideFlag) return false;
-            h.slices.push_back(make_uint2(static_cast<unsigned>(off), static_cast<unsigned>(p.len) | kWideFlag));
-            h.padded += static_cast<size_t>(p.len);
-            h.idx.resize(h.padded, 0);
-            h.src.resize(h.padded, -1);
-            for (int j = 0; j < p.len; ++j) {
-                const int k = ent[p.start + j];
-                h.idx[off + j] = static_cast<unsigned short>(gidx[k] - sp.c * W);
-                h.src[off + j] = k;
-            }
-            h.perm.push_back(static_cast<unsigned short>(p.s - td.seg0));
-            for (int l = 1; l < kWave; ++l) h.perm.push_back(kNoSeg);
-            td.nslice++;
-            td.nwide++;
+        // slices, longest pairs first: a wide slice per pair of wide_min or
+        // more nonzeros, then SELL slices of 64 pairs
+        std::vector<Slice> sl;
+        size_t q = 0;
+        for (; q < order.size() && P[order[q]].len >= wide_min; ++q)
+            sl.push_back(Slice{{order[q]}, (P[order[q]].len + kWave - 1) / kWave, true});
+        for (; q < order.size(); q += kWave) {
+            Slice s{{}, P[order[q]].len, false};
+            for (size_t r = q; r < std::min(order.size(), q + kWave); ++r) s.members.push_back(order[r]);
+            sl.push_back(std::move(s));
         }
-        for (size_t q0 = nwide; q0 < order.size(); q0 += kWave) {
-            const size_t q1 = std::min(order.size(), q0 + kWave);
-            const int L = P[order[q0]].len;
-            const size_t off = h.padded;
-            if (off + static_cast<size_t>(L) * kWave > 0xFFFFFFFFull) return false;
-            h.slices.push_back(make_uint2(static_cast<unsigned>(off), static_cast<unsigned>(L)));
-            h.padded += static_cast<size_t>(L) * kWave;
-            h.idx.resize(h.padded, 0);
-            h.src.resize(h.padded, -1);
-            for (size_t q = q0; q < q0 + kWave; ++q) {
-                if (q >= q1) {
-                    h.perm.push_back(kNoSeg);
-                    continue;
+        // deal the slices to the wavefronts, longest first, to the least loaded
+        std::vector<int> owner(sl.size());
+        std::vector<long long> load(kTileWaves, 0);
+        std::vector<size_t> by_rows(sl.size());
+        std::iota(by_rows.begin(), by_rows.end(), 0);
+        std::stable_sort(by_rows.begin(), by_rows.end(), [&](size_t a, size_t b) { return sl[a].rows > sl[b].rows; });
+        for (size_t i : by_rows) {
+            const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            owner[i] = w;
+            load[w] += sl[i].rows + 1;
+        }
+        TileDesc td{};
+        td.block = sp.c;
+        for (int w = 0; w < kTileWaves; ++w) {
+            WaveDesc& wd = td.wave[w];
+            if (h.padded > 0xFFFFFFFFull) return false;
+            wd.off = static_cast<unsigned>(h.padded);
+            wd.slice0 = static_cast<int>(h.slice_word.size());
+            for (size_t i : by_rows) {
+                if (owner[i] != w) continue;
+                const Slice& s = sl[i];
+                const size_t off = h.padded;
+                const size_t ent_n = static_cast<size_t>(s.rows) * kWave;
+                if (off + ent_n > 0xFFFFFFFFull) return false;
+                const unsigned gslice = static_cast<unsigned>(h.slice_word.size());
+                if (gslice >= (1u << 26)) return false;
+                h.slice_word.push_back(static_cast<unsigned>(s.rows) | (s.wide ? kWideFlag : 0u));
+                h.padded += ent_n;
+                h.idx.resize(h.padded, 0);
+                h.src.resize(h.padded, -1);
+                for (size_t lane = 0; lane < s.members.size(); ++lane) {
+                    const Pair& p = P[s.members[lane]];
+                    for (int j = 0; j < p.len; ++j) {
+                        const int k = ent[p.start + j];
+                        const size_t e = s.wide ? off + j : off + static_cast<size_t>(j) * kWave + lane;
+                        h.idx[e] = static_cast<unsigned short>(gidx[k] - sp.c * W);
+                        h.src[e] = k;
+                    }
+                    pair_seg.push_back(p.s);
+                    pair_blk.push_back(sp.c);
+                    pair_slot.push_back(gslice * kWave + static_cast<unsigned>(s.wide ? 0 : lane));
                 }
-                const Pair& p = P[order[q]];
-                h.perm.push_back(static_cast<unsigned short>(p.s - td.seg0));
-                const size_t lane = q - q0;
-                for (int j = 0; j < p.len; ++j) {
-                    const int k = ent[p.start + j];
-                    const size_t e = off + static_cast<size_t>(j) * kWave + lane;
-                    h.idx[e] = static_cast<unsigned short>(gidx[k] - sp.c * W);
-                    h.src[e] = k;
-                }
+                wd.nrows += s.rows;
+                wd.nslice++;
             }
-            td.nslice++;
+            td.nslice += wd.nslice;
+            h.max_wave_slices = std::max(h.max_wave_slices, wd.nslice);
         }
         h.tiles.push_back(td);
     }
+    // segment -> partial slots, in block order
+    h.pairs = static_cast<long long>(pair_seg.size());
+    h.seg_ptr.assign(static_cast<size_t>(nseg) + 1, 0);
+    for (int s : pair_seg) h.seg_ptr[s + 1]++;
+    for (int s = 0; s < nseg; ++s) h.seg_ptr[s + 1] += h.seg_ptr[s];
+    std::vector<size_t> pidx(pair_seg.size());
+    std::iota(pidx.begin(), pidx.end(), 0);
+    std::stable_sort(pidx.begin(), pidx.end(), [&](size_t a, size_t b) {
+        return pair_seg[a] != pair_seg[b] ? pair_seg[a] < pair_seg[b] : pair_blk[a] < pair_blk[b];
+    });
+    h.slot.resize(pidx.size());
+    for (size_t i = 0; i < pidx.size(); ++i) h.slot[i] = pair_slot[pidx[i]];
     return true;
 }
 
@@ -235,147 +284,162 @@ __device__ inline unsigned short buf_idx(__amdgpu_buffer_rsrc_t r, int byte_off)
     return __builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0);
 }
 
-// NS slices in flight for one wave: buffer resources limited to each
-// slice's entries, the lane's perm entries and the first CH rows of 64
-// entries (a wide slice is the same layout: entry j at offset j).
-template <class T, int CH, int NS>
-struct SliceBatch {
-    __amdgpu_buffer_rsrc_t rv[NS], ri[NS];
-    int rows[NS];
-    int wide[NS];
-    unsigned short r[NS];
-    T a[NS][CH];
-    unsigned short g[NS][CH];
+// CH rows of a wavefront's run, starting at row r (rows past the run read 0).
+template <class T, int CH>
+struct RowChunk {
+    T a[CH];
+    unsigned short g[CH];
 };
 
-// Issues every load of the batch starting at slice s (slices s, s + 16, ...).
-// Slices at or past `end` become empty ones: their descriptor and perm reads
-// go to slice end - 1 (the tile has at least one slice) and their buffer
-// range is 0 bytes, so their loads read 0 and touch no memory.
-template <class T, int CH, int NS>
-__device__ inline void batch_load(SliceBatch<T, CH, NS>& b, int s, int end, const uint2* __restrict__ slices,
-                                  const unsigned short* __restrict__ perm, const T* __restrict__ val,
-                                  const unsigned short* __restrict__ idx, int lane) {
+template <class T, int CH>
+__device__ inline void chunk_load(RowChunk<T, CH>& c, __amdgpu_buffer_rsrc_t rv, __amdgpu_buffer_rsrc_t ri, int r,
+                                  int lane) {
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-        const int sq = s + q * kTileWaves;
-        const bool live = sq < end;
-        const int sr = live ? sq : end - 1;
-        const uint2 d = slices[sr];
-        const unsigned x = __builtin_amdgcn_readfirstlane(d.x);
-        const unsigned y = __builtin_amdgcn_readfirstlane(d.y);
-        const int w = (y & kWideFlag) != 0;
-        const int len = live ? (w ? static_cast<int>(y & ~kWideFlag) : static_cast<int>(y) * kWave) : 0;
-        b.rows[q] = (len + kWave - 1) / kWave;
-        b.wide[q] = w;
-        b.rv[q] = slice_rsrc(val + x, len * static_cast<int>(sizeof(T)));
-        b.ri[q] = slice_rsrc(idx + x, len * 2);
-        const unsigned short pr = perm[static_cast<size_t>(sr) * kWave + lane];
-        b.r[q] = live ? pr : kNoSeg;
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            b.a[q][u] = buf_val<T>(b.rv[q], (u * kWave + lane) * static_cast<int>(sizeof(T)));
-            b.g[q][u] = buf_idx(b.ri[q], (u * kWave + lane) * 2);
-        }
+    for (int u = 0; u < CH; ++u) {
+        c.a[u] = buf_val<T>(rv, ((r + u) * kWave + lane) * static_cast<int>(sizeof(T)));
+        c.g[u] = buf_idx(ri, ((r + u) * kWave + lane) * 2);
     }
 }
 
-// Gathers from the LDS block, finishes slices longer than CH rows, and
-// stores one partial per segment (a wide slice adds its 64 lanes first).
-template <class T, int CH, int NS>
-__device__ inline void batch_finish(const SliceBatch<T, CH, NS>& b, const T* lds, double* __restrict__ out,
+// Walk state of a wavefront: the running sum of the current slice, the row
+// after its last row, and the slice words of the run (lane q holds the word
+// of slice q of the current window of 64 slices).
+struct RunState {
+    double acc;
+    int sl;     // current slice within the run
+    int bound;  // row after the current slice's last row
+    int wide;
+    unsigned words;
+};
+
+__device__ inline unsigned run_word(const unsigned* __restrict__ slice_word, int first, int nslice, int base,
                                     int lane) {
+    const int q = base + lane;
+    return q < nslice ? slice_word[first + q] : 0u;
+}
+
+template <class T, int CH>
+__device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, int nrows, const T* lds, RunState& st,
+                                    const WaveDesc& wd, const unsigned* __restrict__ slice_word,
+                                    double* __restrict__ part, int lane) {
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-        double acc = 0.0;
-#pragma unroll
-        for (int u = 0; u < CH; ++u) acc += static_cast<double>(b.a[q][u]) * static_cast<double>(lds[b.g[q][u]]);
-        for (int j0 = CH; j0 < b.rows[q]; j0 += CH) {
-            T c[CH];
-            unsigned short h[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                c[u] = buf_val<T>(b.rv[q], ((j0 + u) * kWave + lane) * static_cast<int>(sizeof(T)));
-                h[u] = buf_idx(b.ri[q], ((j0 + u) * kWave + lane) * 2);
+    for (int u = 0; u < CH; ++u) {
+        if (r + u >= nrows) return;
+        st.acc += static_cast<double>(c.a[u]) * static_cast<double>(lds[c.g[u]]);
+        if (r + u + 1 == st.bound) {  // last row of the current slice (uniform)
+            double* row = part + static_cast<size_t>(wd.slice0 + st.sl) * kWave;
+            if (st.wide) {
+                const double t = wave_sum(st.acc);
+                if (lane == 0) row[0] = t;
+            } else {
+                row[lane] = st.acc;
             }
-#pragma unroll
-            for (int u = 0; u < CH; ++u) acc += static_cast<double>(c[u]) * static_cast<double>(lds[h[u]]);
+            st.acc = 0.0;
+            if (++st.sl < wd.nslice) {
+                if ((st.sl & (kWave - 1)) == 0)
+                    st.words = run_word(slice_word, wd.slice0, wd.nslice, st.sl, lane);
+                const unsigned w =
+                    static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(st.words), st.sl & (kWave - 1)));
+                st.bound += static_cast<int>(w & ~kWideFlag);
+                st.wide = (w & kWideFlag) != 0;
+            }
         }
-        if (b.wide[q]) acc = wave_sum(acc);
-        if (b.r[q] != kNoSeg) out[b.r[q]] = acc;
     }
 }
 
-// One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of 64
-// entries of a slice loaded up front; NS: slices a wave loads at once.
-// PASS only names the instantiation (0: CSR pass, 1: CSC pass), so that
-// profiles tell the two passes apart.
-template <class T, int CH, int NS, int PASS>
+// One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of the
+// wavefront's run kept in flight per buffer.  PASS only names the
+// instantiation (0: CSR pass, 1: CSC pass), so that profiles tell the two
+// passes apart.
+template <class T, int CH, int PASS>
 __global__ void __launch_bounds__(kTileThreads)
-k_tile(const TileDesc* __restrict__ tiles, const uint2* __restrict__ slices, const unsigned short* __restrict__ perm,
-       const T* __restrict__ val, const unsigned short* __restrict__ idx, const T* __restrict__ gvec, int ng, int W,
-       int nseg, double* __restrict__ part, const int* __restrict__ done) {
-    if (done && *done) return;
+k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_word, const T* __restrict__ val,
+       const unsigned short* __restrict__ idx, const T* __restrict__ gvec, int ng, int W,
+       double* __restrict__ part) {
     extern __shared__ __align__(16) unsigned char tile_lds[];
     T* lds = reinterpret_cast<T*>(tile_lds);
-    const TileDesc td = tiles[blockIdx.x];
     const int tid = threadIdx.x;
-    {
-        const int g0 = td.block * W;
-        const int gn = min(W, ng - g0);
-        constexpr int V = 16 / sizeof(T);  // entries per 16-byte load; g0 is a multiple of 64
-        using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
-        const Vec* src = reinterpret_cast<const Vec*>(gvec + g0);
-        Vec* dst = reinterpret_cast<Vec*>(lds);
-        const int nv = gn / V;
-        for (int i = tid; i < nv; i += kTileThreads) dst[i] = src[i];
-        for (int i = nv * V + tid; i < gn; i += kTileThreads) lds[i] = gvec[g0 + i];
-    }
-    __syncthreads();
-    // wave-uniform values live in scalar registers (slice bounds, buffer
-    // resources), so nothing below branches per lane
+    // wave-uniform values live in scalar registers, so nothing below
+    // branches per lane
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = tid % kWave;
-    double* out = part + static_cast<size_t>(td.block) * nseg + td.seg0;
-    const int end = td.slice0 + td.nslice;
-    // Software pipeline over batches of NS slices (wave w takes slices w,
-    // w + 16, ...): the loads of the next batch are issued before the
-    // current batch is reduced and its partials stored, so each wave always
-    // has a batch in flight and no store sits between a load and its use.
-    constexpr int step = NS * kTileWaves;
-    int s = td.slice0 + wave;
-    if (s >= end) return;
-    SliceBatch<T, CH, NS> A, Bt;
-    batch_load(A, s, end, slices, perm, val, idx, lane);
+    const int block = tiles[blockIdx.x].block;
+    const WaveDesc wd = tiles[blockIdx.x].wave[wave];
+    // Start-up: the block's 16-byte loads (into registers), the first row
+    // chunk and the run's slice words are all issued before anything waits,
+    // so the LDS fill overlaps the first matrix fetch.
+    const int g0 = block * W;
+    const int gn = min(W, ng - g0);
+    constexpr int V = 16 / sizeof(T);
+    const int nv = gn / V;  // whole 16-byte groups of the block (g0 is a multiple of 64)
+    const __amdgpu_buffer_rsrc_t rx = slice_rsrc(gvec + g0, nv * 16);
+    uint4 f[kFillTrips];
+#pragma unroll
+    for (int u = 0; u < kFillTrips; ++u)
+        f[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rx, (u * kTileThreads + tid) * 16, 0, 0));
+    const int nrows = wd.nrows;
+    const __amdgpu_buffer_rsrc_t rv = slice_rsrc(val + wd.off, nrows * kWave * static_cast<int>(sizeof(T)));
+    const __amdgpu_buffer_rsrc_t ri = slice_rsrc(idx + wd.off, nrows * kWave * 2);
+    RowChunk<T, CH> A, Bc;
+    chunk_load(A, rv, ri, 0, lane);
+    RunState st{0.0, 0, 0, 0, run_word(slice_word, wd.slice0, wd.nslice, 0, lane)};
+    uint4* dst = reinterpret_cast<uint4*>(tile_lds);
+#pragma unroll
+    for (int u = 0; u < kFillTrips; ++u) {
+        const int i = u * kTileThreads + tid;
+        if (i < nv) dst[i] = f[u];
+    }
+    for (int i = nv * V + tid; i < gn; i += kTileThreads) lds[i] = gvec[g0 + i];  // tail of the last block
+    __syncthreads();
+    if (nrows == 0) return;
+    {
+        const unsigned w0 = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(st.words), 0));
+        st.bound = static_cast<int>(w0 & ~kWideFlag);
+        st.wide = (w0 & kWideFlag) != 0;
+    }
+    // Double-buffered walk: the next CH rows are loaded before the current
+    // ones are gathered and their finished slices stored, so no store sits
+    // between a load and its use.
+    int r = 0;
     for (;;) {
-        batch_load(Bt, s + step, end, slices, perm, val, idx, lane);
-        batch_finish(A, lds, out, lane);
-        s += step;
-        if (s >= end) break;
-        batch_load(A, s + step, end, slices, perm, val, idx, lane);
-        batch_finish(Bt, lds, out, lane);
-        s += step;
-        if (s >= end) break;
+        chunk_load(Bc, rv, ri, r + CH, lane);
+        chunk_finish(A, r, nrows, lds, st, wd, slice_word, part, lane);
+        r += CH;
+        if (r >= nrows) break;
+        chunk_load(A, rv, ri, r + CH, lane);
+        chunk_finish(Bc, r, nrows, lds, st, wd, slice_word, part, lane);
+        r += CH;
+        if (r >= nrows) break;
     }
 }
 
-// Segment s: the B partials in block order, then the pass epilogue.
-template <class Epi>
+// Segment s: its partials in block order, then the pass epilogue.  The slot
+// and partial loads of up to U pairs are issued at once: raw buffer loads at
+// an out-of-range offset read 0, so no load sits under a per-lane branch.
+template <int U, class Epi>
 __global__ void __launch_bounds__(kBlock)
-k_tile_reduce(int nseg, int B, const double* __restrict__ part, Epi epi, const int* __restrict__ done) {
+k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restrict__ slot, int nslot,
+              const double* __restrict__ part, int npart, Epi epi, const int* __restrict__ done) {
     if (done && *done) return;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= nseg) return;
-    // runs of 8 partial loads in flight; added in block order
+    constexpr unsigned kOut = 0x7FFFFFF0u;  // beyond every range below (ranges are < 2^31 bytes)
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(slot, nslot * 4);
+    const __amdgpu_buffer_rsrc_t rp = slice_rsrc(part, npart * 8);
+    const int q0 = seg_ptr[s], q1 = seg_ptr[s + 1];
     double acc = 0.0;
-    int c = 0;
-    for (; c + 8 <= B; c += 8) {
-        double q[8];
+    for (int q = q0; q < q1; q += U) {
+        unsigned p[U];
+        double v[U];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) q[u] = part[static_cast<size_t>(c + u) * nseg + s];
+        for (int u = 0; u < U; ++u)
+            p[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(q + u < q1 ? (q + u) * 4u : kOut), 0, 0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += q[u];
+        for (int u = 0; u < U; ++u)
+            v[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                  rp, static_cast<int>(q + u < q1 ? p[u] * 8u : kOut), 0, 0));
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u];
     }
-    for (; c < B; ++c) acc += part[static_cast<size_t>(c) * nseg + s];
     double r[1] = {acc};
     epi(s, r, nullptr);
 }
