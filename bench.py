@@ -39,6 +39,11 @@ WORKLOADS = {
     "c2": ("staircase_lp periods=24 block=2000x5000 inv=200 seed=42",
            lambda lpd: lpd.staircase_lp(24, 2000, 5000, 200, seed=42)),
 }
+# batched workloads: one step = one solve of the whole batch
+BATCH_WORKLOADS = {
+    "c3": ("batch_lps count=512 m~U[200,20000] n=3m 5 nnz/col seed=42 (seeds 43..554)",
+           lambda lpd: lpd.batch_lps(512, 200, 20000, 42)),
+}
 
 
 def log(*a):
@@ -128,12 +133,75 @@ def _traffic_from_profiles(workload):
     return None
 
 
+def bench_batch(pkg, args):
+    """Batched workload: LPs solved per second by one bo_pdlp_batch solve of
+    the whole batch (inputs resident in HBM, every LP to FP64 rel-KKT <= eps,
+    each LP re-checked by the oracle's KKT evaluator when the baseline leg
+    runs)."""
+    desc, build = BATCH_WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    lps = build(pkg.lpdata)
+    log(f"workload {args.workload}: {desc}; {len(lps)} LPs, sum m={sum(l.m for l in lps)} "
+        f"sum n={sum(l.n for l in lps)} sum nnz={sum(l.nnz for l in lps)}; generated in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=args.eps, max_iterations=args.max_iterations,
+                      use_graph=0 if args.no_graph else 1)
+    setup_s = time.perf_counter() - t0
+    log(f"upload+layout {setup_s:.2f}s")
+    for w in range(args.warmup):
+        per, tot = b.solve()
+        log(f"warmup {w}: solved {tot['solved']}/{tot['count']} max_iters {tot['max_iterations']} "
+            f"{tot['solve_ms']:.0f} ms")
+    times, tots = [], []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        per, tot = b.solve()
+        times.append(time.perf_counter() - t0)
+        tots.append(tot)
+        log(f"step {k}: solved {tot['solved']}/{tot['count']} max_iters {tot['max_iterations']} "
+            f"{times[-1] * 1e3:.0f} ms")
+    total = sum(times)
+    solved = sum(t["solved"] for t in tots)
+    its = np.array([p["iterations"] for p in per])
+    out = {
+        "metric": METRIC,
+        "value": solved / total,
+        "unit": "LPs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": desc, "lps": len(lps), "eps_rel_kkt": args.eps,
+                   "max_iterations": args.max_iterations, "iterates": "fp32", "checks": "fp64",
+                   "start": "cold", "parallelism": "single-gpu"},
+        "solved_per_step": tots[-1]["solved"],
+        "iterations_p50_p90_max": [float(np.percentile(its, 50)), float(np.percentile(its, 90)), int(its.max())],
+        "roofline": None,
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        worst = 0.0
+        for k, lp in enumerate(lps):
+            if per[k]["status"] == 0:
+                x, y = b.solution(k)
+                worst = max(worst, po.kkt(lp, x, y)["rel_kkt"])
+        out["checker_worst_rel_kkt_of_solved"] = worst
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS))
+    ap.add_argument("--max-iterations", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--kernels-only", action="store_true",
@@ -157,13 +225,15 @@ def main():
     st, info = pkg.device_info()
     if st != 0:
         raise SystemExit("no GPU visible: " + info.status_message.decode())
+    if args.workload in BATCH_WORKLOADS:
+        return bench_batch(pkg, args)
     desc, build = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     lp = build(pkg.lpdata)
     log(f"workload {args.workload}: {desc}; m={lp.m} n={lp.n} nnz={lp.nnz}; "
         f"generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    h = pkg.Pdlp(lp, eps_rel_kkt=args.eps, max_iterations=200000, use_graph=0 if args.no_graph else 1,
+    h = pkg.Pdlp(lp, eps_rel_kkt=args.eps, max_iterations=args.max_iterations, use_graph=0 if args.no_graph else 1,
                  lanes_rows=args.lanes_rows, lanes_cols=args.lanes_cols)
     setup_s = time.perf_counter() - t0
     log(f"upload+autotune {setup_s:.2f}s")

@@ -212,6 +212,36 @@ bo_status bo_pdlp_get_pass_info(struct bo_pdlp* h, int pass, bo_pdlp_pass_info* 
  * (a fused pass reports its kernel as gather and 0 as reduce). */
 bo_status bo_pdlp_time_kernels(struct bo_pdlp* h, int reps, double* ms4);
 
+/* ---- batched PDLP: many LPs of uneven size solved together ---------------- */
+/* No single counterpart in the reference, which solves one model per
+ * FirstOrderLPSolver::solve call (src/first_order_solver.cpp:311-319): this is
+ * the pre-built-CSR batch entry point for callers with many independent LPs.
+ * The LPs are laid out block-diagonally on the device and every launch covers
+ * all of them; each LP keeps its own step sizes, primal weight, restarts and
+ * FP64 stopping test (bo_pdlp_params applies to every LP) and stops on its
+ * own.  Per-LP results equal a bo_pdlp solve of that LP up to summation order. */
+typedef struct bo_pdlp_batch_stats {
+    int count;                 /* LPs in the batch */
+    int solved;                /* LPs that met eps_rel_kkt */
+    long long max_iterations;  /* iterations of the slowest LP */
+    long long sum_iterations;  /* sum over LPs */
+    long long loop_steps;      /* steps the batch loop ran */
+    double solve_ms;           /* host wall time of bo_pdlp_batch_solve */
+    int tiled;                 /* 1: both passes on the tiled schedule */
+} bo_pdlp_batch_stats;
+struct bo_pdlp_batch;
+bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params* params,
+                               struct bo_pdlp_batch** out);
+/* Warm start for every LP: x and y concatenated in batch order (sum n, sum m
+ * entries, original space); NULL, NULL clears it. */
+bo_status bo_pdlp_batch_set_warm_start(struct bo_pdlp_batch* h, const double* x, const double* y);
+/* per_lp (count entries, may be NULL): the stats of every LP, as a single solve reports
+ * them; their solve_ms is the batch's wall time. */
+bo_status bo_pdlp_batch_solve(struct bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total);
+/* Solution of LP k in its original space (x[n_k], y[m_k]). */
+bo_status bo_pdlp_batch_get_solution(struct bo_pdlp_batch* h, int k, double* x, double* y);
+void bo_pdlp_batch_destroy(struct bo_pdlp_batch* h);
+
 /* ---- full drop-in solve on a flattened LPModel ----------------------------- */
 /* LPModel (include/bharatopt/lp_model.hpp:60-127) as arrays: row-wise terms in
  * insertion order. con_sense: 0 LE, 1 EQ, 2 GE, 3 RANGED; var_type: 0 continuous,

@@ -93,6 +93,12 @@ class PdlpPassInfo(C.Structure):
                 ("gather_bytes", C.c_double), ("reduce_bytes", C.c_double)]
 
 
+class PdlpBatchStats(C.Structure):
+    _fields_ = [("count", C.c_int), ("solved", C.c_int), ("max_iterations", C.c_longlong),
+                ("sum_iterations", C.c_longlong), ("loop_steps", C.c_longlong), ("solve_ms", C.c_double),
+                ("tiled", C.c_int)]
+
+
 class BoModel(C.Structure):
     _fields_ = [("num_vars", C.c_int), ("num_cons", C.c_int), ("sense", C.c_int),
                 ("obj_offset", C.c_double), ("lb", P), ("ub", P), ("obj", P), ("var_type", P),
@@ -122,7 +128,8 @@ EXPORTS = [
     "bo_dot_f64", "bo_norm2_f64", "bo_pdhg_ref_solve", "bo_pdlp_default_params", "bo_pdlp_create",
     "bo_pdlp_set_warm_start", "bo_pdlp_solve", "bo_pdlp_get_solution", "bo_pdlp_destroy",
     "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
-    "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels",
+    "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels", "bo_pdlp_batch_create", "bo_pdlp_batch_set_warm_start",
+    "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution", "bo_pdlp_batch_destroy",
 ]
 
 
@@ -161,7 +168,8 @@ def lib():
         L.bo_presolve_stats.argtypes = [P, P, P, P, P, P]
         L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
         for name in EXPORTS:
-            if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params"):
+            if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params",
+                            "bo_pdlp_batch_destroy"):
                 getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -333,6 +341,54 @@ class Pdlp:
     def close(self):
         if self.handle:
             lib().bo_pdlp_destroy(self.handle)
+            self.handle = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PdlpBatch:
+    """Handle of many LPs solved together on the GPU (bo_pdlp_batch_*)."""
+
+    def __init__(self, lps, **params):
+        self.lps = list(lps)
+        self._h = [_LpHolder(lp) for lp in self.lps]
+        arr = (BoLp * len(self.lps))(*[h.s for h in self._h])
+        self._arr = arr
+        self.params = default_pdlp_params(**params)
+        self.handle = P()
+        _check(lib().bo_pdlp_batch_create(arr, len(self.lps), C.byref(self.params), C.byref(self.handle)),
+               "bo_pdlp_batch_create")
+
+    def set_warm_start(self, xs=None, ys=None):
+        """xs, ys: lists of per-LP vectors (original space), or None to clear."""
+        if xs is None and ys is None:
+            _check(lib().bo_pdlp_batch_set_warm_start(self.handle, None, None), "bo_pdlp_batch_set_warm_start")
+            return
+        x = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in xs]))
+        y = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in ys]))
+        _check(lib().bo_pdlp_batch_set_warm_start(self.handle, _p(x), _p(y)), "bo_pdlp_batch_set_warm_start")
+
+    def solve(self):
+        per = (PdlpStats * len(self.lps))()
+        tot = PdlpBatchStats()
+        _check(lib().bo_pdlp_batch_solve(self.handle, per, C.byref(tot)), "bo_pdlp_batch_solve")
+        return ([{k: getattr(p, k) for k, _ in PdlpStats._fields_} for p in per],
+                {k: getattr(tot, k) for k, _ in PdlpBatchStats._fields_})
+
+    def solution(self, k):
+        lp = self.lps[k]
+        x = np.zeros(max(lp.n, 1))
+        y = np.zeros(max(lp.m, 1))
+        _check(lib().bo_pdlp_batch_get_solution(self.handle, int(k), _p(x), _p(y)), "bo_pdlp_batch_get_solution")
+        return x[:lp.n], y[:lp.m]
+
+    def close(self):
+        if self.handle:
+            lib().bo_pdlp_batch_destroy(self.handle)
             self.handle = P()
 
     def __del__(self):

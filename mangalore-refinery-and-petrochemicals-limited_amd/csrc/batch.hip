@@ -1,0 +1,1057 @@
+// batch.hip -- many LPs of uneven size solved together by the restarted,
+// reflected Halpern PDHG of pdlp.hip.
+//
+// The LPs are laid out as one block-diagonal LP: rows of LP k follow those of
+// LP k-1, and so do its columns.  Every launch covers all LPs at once.
+//  * The two iteration passes run on the tiled layout (tiles.hpp).  A tile
+//    that holds rows or columns of finished LPs only is skipped.
+//  * The epilogues read the state of the row's or column's own LP (inst maps).
+//  * Scaling is row- and column-local, so it is the same as scaling each LP
+//    on its own.
+//  * Power iteration, norms and the FP64 checks reduce over "chunks": up to
+//    256 consecutive rows or columns of one LP.  Each LP sums its chunk
+//    partials in a fixed order, in one wavefront, and then runs the same
+//    stopping test, restart rule and primal-weight update as the single-LP
+//    engine (pdlp_core.hpp).
+// One interval of steps, its check and its restarts are captured once as a
+// hipGraph and replayed until every LP has stopped.  The replacement for the
+// reference's one-model-at-a-time FirstOrderLPSolver::solve
+// (src/first_order_solver.cpp:311-319) when many independent LPs are solved.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "pdlp_core.hpp"
+#include "segments.hpp"
+#include "tiles.hpp"
+
+namespace bo {
+
+bo_status ensure_device_public();
+
+namespace {
+
+// Up to kBlock consecutive rows or columns of one LP.
+struct Chunk {
+    int inst;
+    int begin;
+    int end;
+};
+
+// ---------------------------------------------------------------- epilogues
+// norm_out[seg] = own_scale[seg] * acc   (row/column norms of the scaled matrix)
+struct EpiNormB {
+    using TXv = double;
+    static constexpr int NV = 1;
+    static constexpr int Q = 0;
+    const double* other_scale;
+    const double* own_scale;
+    double* norm_out;
+    __device__ const double* vec(int) const { return other_scale; }
+    __device__ void operator()(int seg, const double* acc, double*) const { norm_out[seg] = own_scale[seg] * acc[0]; }
+};
+
+// out[k] = own_scale[seg] * val[k] * other_scale[j]
+template <class TO>
+struct EpiWriteB {
+    using TXv = double;
+    static constexpr int NV = 1;
+    static constexpr int Q = 0;
+    const double* own_scale;
+    const double* other_scale;
+    TO* out;
+    __device__ const double* vec(int) const { return other_scale; }
+    __device__ void write(int seg, int k, double a, int j) const {
+        out[k] = static_cast<TO>(own_scale[seg] * a * other_scale[j]);
+    }
+    __device__ void operator()(int, const double*, double*) const {}
+};
+
+// out = A v
+template <class TX, class TO>
+struct EpiStoreB {
+    using TXv = TX;
+    static constexpr int NV = 1;
+    static constexpr int Q = 0;
+    const TX* v;
+    TO* out;
+    __device__ const TX* vec(int) const { return v; }
+    __device__ void operator()(int seg, const double* acc, double*) const { out[seg] = static_cast<TO>(acc[0]); }
+};
+
+// ---------------------------------------------------------------- elementwise
+__global__ void kb_fill_f64(double* p, double v, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void kb_apply_norm(double* scale, const double* norm, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double v = norm[i];
+        if (v > 0.0 && isfinite(v)) scale[i] /= sqrt(v);
+    }
+}
+
+template <class T>
+__global__ void kb_scale_cols(int n, const double* s, const double* c, const double* lb, const double* ub,
+                              const double* wx, int has_warm, T* c_hat, T* l_hat, T* u_hat, double* xk, double* x0,
+                              T* xp) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double sj = s[j];
+    c_hat[j] = static_cast<T>(c[j] * sj);
+    const double lo = lb[j] / sj, hi = ub[j] / sj;
+    l_hat[j] = static_cast<T>(lo);
+    u_hat[j] = static_cast<T>(hi);
+    const double xs = clampT(has_warm ? wx[j] / sj : 0.0, lo, hi);
+    xk[j] = xs;
+    x0[j] = xs;
+    xp[j] = static_cast<T>(xs);
+}
+
+template <class T>
+__global__ void kb_scale_rows(int m, const double* r, const double* b, const unsigned char* eq, const double* wy,
+                              int has_warm, T* b_hat, double* yk, double* y0, T* yg, T* yp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const double ri = r[i];
+    b_hat[i] = static_cast<T>(b[i] * ri);
+    double ys = has_warm ? wy[i] / ri : 0.0;
+    if (!eq[i] && ys < 0.0) ys = 0.0;
+    yk[i] = ys;
+    y0[i] = ys;
+    yg[i] = static_cast<T>(ys);
+    yp[i] = static_cast<T>(ys);
+}
+
+__global__ void kb_copy_f64(double* dst, const double* src, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+template <class T>
+__global__ void kb_pw_init(T* v, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    unsigned h = static_cast<unsigned>(j) * 2654435761u + 12345u;
+    h ^= h >> 13;
+    h *= 0x5bd1e995u;
+    h ^= h >> 15;
+    v[j] = static_cast<T>(0.5 + static_cast<double>(h & 0xffffu) / 65536.0);
+}
+
+// part[c] = sum of v^2 over chunk c
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_sq_chunks(const Chunk* chunks, const T* v, double* part) {
+    __shared__ double sh[kBlock / kWave];
+    const Chunk ch = chunks[blockIdx.x];
+    const int i = ch.begin + static_cast<int>(threadIdx.x);
+    double t = 0.0;
+    if (i < ch.end) {
+        const double a = static_cast<double>(v[i]);
+        t = a * a;
+    }
+    const double s = block_sum(t, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[k] = sum of the Q-wide chunk partials of LP k (slot q), chunks in order;
+// one wavefront per LP.
+__global__ void kb_inst_sum(int K, const int* chunk_ptr, const double* part, int Q, int q, double* out) {
+    const int k = blockIdx.x * (blockDim.x / kWave) + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (k >= K) return;
+    double a = 0.0;
+    for (int c = chunk_ptr[k] + lane; c < chunk_ptr[k + 1]; c += kWave) a += part[static_cast<size_t>(c) * Q + q];
+    a = wave_sum(a);
+    if (lane == 0) out[k] = a;
+}
+
+template <class T>
+__global__ void kb_pw_normalize(T* v, int n, const int* inst, const double* norm2) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double nv = sqrt(norm2[inst[j]]);
+    if (nv > 0.0) v[j] = static_cast<T>(static_cast<double>(v[j]) / nv);
+}
+
+__global__ void kb_init_state(int K, PdlpState* st, const PdlpState* init, const double* pw_u2, const double* c2,
+                              const double* b2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    st[k] = init[k];
+    init_state_core(st + k, pw_u2[k], c2[k], b2[k]);
+}
+
+__global__ void kb_advance(int K, PdlpState* st, int steps) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K && !st[k].done) st[k].iter += steps;
+}
+
+// ---------------------------------------------------------------- FP64 check
+// Unscaled x: xo = clamp(s x, l, u) for the iterate and for x+; partials
+// [2c, 2c+1] = c'xo over chunk c (0 for a finished LP).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+kb_check_x(const Chunk* chunks, const PdlpState* st, const double* xk, const T* xp, const double* s, const double* c,
+           const double* lb, const double* ub, double* xo_cur, double* xo_cand, double* part) {
+    __shared__ double sh[kBlock / kWave];
+    const Chunk ch = chunks[blockIdx.x];
+    const bool live = !st[ch.inst].done;
+    const int j = ch.begin + static_cast<int>(threadIdx.x);
+    double p0 = 0.0, p1 = 0.0;
+    if (live && j < ch.end) {
+        const double sj = s[j], l = lb[j], u = ub[j];
+        const double xc = clampT(sj * xk[j], l, u);
+        const double xa = clampT(sj * static_cast<double>(xp[j]), l, u);
+        xo_cur[j] = xc;
+        xo_cand[j] = xa;
+        p0 = c[j] * xc;
+        p1 = c[j] * xa;
+    }
+    const double t0 = block_sum(p0, sh);
+    const double t1 = block_sum(p1, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x + 0] = t0;
+        part[2 * blockIdx.x + 1] = t1;
+    }
+}
+
+// Row side, one thread per row (unscaled FP64 matrix, unscaled x).
+// partials per chunk: [0,1] ||viol||^2  [2,3] scaled  [4,5] b'y   (cur, cand)
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+kb_check_rows(const Chunk* chunks, const PdlpState* st, const int* rp, const int* ci, const double* rv,
+              const double* xo_cur, const double* xo_cand, const double* b, const unsigned char* eq, const double* rs,
+              const double* yk, const T* yp, double* yo_cur, double* yo_cand, double* ax_cand, double* part) {
+    __shared__ double sh[kBlock / kWave];
+    const Chunk ch = chunks[blockIdx.x];
+    const bool live = !st[ch.inst].done;
+    const int i = ch.begin + static_cast<int>(threadIdx.x);
+    double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (live && i < ch.end) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const double v = rv[k];
+            const int j = ci[k];
+            a0 += v * xo_cur[j];
+            a1 += v * xo_cand[j];
+        }
+        const double bi = b[i], ri = rs[i];
+        const bool e = eq[i] != 0;
+        const double acc[2] = {a0, a1};
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double r = acc[p] - bi;
+            const double v = e ? fabs(r) : (r > 0.0 ? r : 0.0);
+            q[p] = v * v;
+            q[2 + p] = (ri * v) * (ri * v);
+        }
+        ax_cand[i] = ri * a1;
+        double yc = ri * yk[i];
+        if (!e && yc < 0.0) yc = 0.0;
+        const double ya = ri * static_cast<double>(yp[i]);
+        yo_cur[i] = yc;
+        yo_cand[i] = ya;
+        q[4] = bi * yc;
+        q[5] = bi * ya;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double t = block_sum(q[k], sh);
+        if (threadIdx.x == 0) part[6 * blockIdx.x + k] = t;
+    }
+}
+
+// Column side, one thread per column.
+// partials per chunk: [0,1] ||dual res||^2  [2,3] scaled  [4,5] sum l plam+ + u plam-
+__global__ void __launch_bounds__(kBlock)
+kb_check_cols(const Chunk* chunks, const PdlpState* st, const int* cp, const int* ri, const double* cv,
+              const double* yo_cur, const double* yo_cand, const double* c, const double* lb, const double* ub,
+              const double* cs, double* part) {
+    __shared__ double sh[kBlock / kWave];
+    const Chunk ch = chunks[blockIdx.x];
+    const bool live = !st[ch.inst].done;
+    const int j = ch.begin + static_cast<int>(threadIdx.x);
+    double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (live && j < ch.end) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = cp[j]; k < cp[j + 1]; ++k) {
+            const double v = cv[k];
+            const int i = ri[k];
+            a0 += v * yo_cur[i];
+            a1 += v * yo_cand[i];
+        }
+        const double l = lb[j], u = ub[j], sj = cs[j];
+        const bool lf = isfinite(l), uf = isfinite(u);
+        const double acc[2] = {a0, a1};
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double lam = c[j] + acc[p];
+            double pl;
+            if (lf && uf) pl = lam;
+            else if (lf) pl = lam > 0.0 ? lam : 0.0;
+            else if (uf) pl = lam < 0.0 ? lam : 0.0;
+            else pl = 0.0;
+            const double d = lam - pl;
+            q[p] = d * d;
+            q[2 + p] = (sj * d) * (sj * d);
+            q[4 + p] = pl > 0.0 ? l * pl : (pl < 0.0 ? u * pl : 0.0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double t = block_sum(q[k], sh);
+        if (threadIdx.x == 0) part[6 * blockIdx.x + k] = t;
+    }
+}
+
+// One wavefront per LP: its check sums (chunks in order, fixed lane tree),
+// then the stopping test and restart rule of the single-LP engine.
+__global__ void kb_decide(int K, PdlpState* st, const int* col_ptr, const double* px, const double* pc,
+                          const int* row_ptr, const double* pr) {
+    const int k = blockIdx.x * (blockDim.x / kWave) + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (k >= K || st[k].done) return;
+    double a[14];
+#pragma unroll
+    for (int q = 0; q < 14; ++q) a[q] = 0.0;
+    for (int c = col_ptr[k] + lane; c < col_ptr[k + 1]; c += kWave) {
+        a[0] += px[2 * c + 0];
+        a[1] += px[2 * c + 1];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a[8 + q] += pc[6 * static_cast<size_t>(c) + q];
+    }
+    for (int c = row_ptr[k] + lane; c < row_ptr[k + 1]; c += kWave) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a[2 + q] += pr[6 * static_cast<size_t>(c) + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 14; ++q) a[q] = wave_sum(a[q]);
+    if (lane == 0) decide_core(st + k, a);
+}
+
+// Restart of the LPs that decided to: move to the chosen point, make it the
+// anchor; per-chunk ||dx||^2 / ||dy||^2 of the anchor move.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+kb_restart_x(const Chunk* chunks, const PdlpState* st, double* xk, double* x0, const T* xp, double* part) {
+    __shared__ double sh[kBlock / kWave];
+    const Chunk ch = chunks[blockIdx.x];
+    const PdlpState* s = st + ch.inst;
+    const bool go = !s->done && s->restart;
+    const int j = ch.begin + static_cast<int>(threadIdx.x);
+    double d2 = 0.0;
+    if (go && j < ch.end) {
+        const double xn = s->restart_to_cand ? static_cast<double>(xp[j]) : xk[j];
+        const double d = xn - x0[j];
+        d2 = d * d;
+        xk[j] = xn;
+        x0[j] = xn;
+    }
+    const double t = block_sum(d2, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+kb_restart_y(const Chunk* chunks, const PdlpState* st, double* yk, double* y0, const T* yp, T* yg, double* ax,
+             double* ax0, const double* ax_cand, double* part) {
+    __shared__ double sh[kBlock / kWave];
+    const Chunk ch = chunks[blockIdx.x];
+    const PdlpState* s = st + ch.inst;
+    const bool go = !s->done && s->restart;
+    const int i = ch.begin + static_cast<int>(threadIdx.x);
+    double d2 = 0.0;
+    if (go && i < ch.end) {
+        double yn = yk[i];
+        if (s->restart_to_cand) {
+            yn = static_cast<double>(yp[i]);
+            ax[i] = ax_cand[i];
+        }
+        const double d = yn - y0[i];
+        d2 = d * d;
+        yk[i] = yn;
+        y0[i] = yn;
+        yg[i] = static_cast<T>(yn);
+        ax0[i] = ax[i];
+    }
+    const double t = block_sum(d2, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void kb_restart_finish(int K, PdlpState* st, const int* col_ptr, const double* pdx, const int* row_ptr,
+                                  const double* pdy) {
+    const int k = blockIdx.x * (blockDim.x / kWave) + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (k >= K || st[k].done || !st[k].restart) return;
+    double a = 0.0, b = 0.0;
+    for (int c = col_ptr[k] + lane; c < col_ptr[k + 1]; c += kWave) a += pdx[c];
+    for (int c = row_ptr[k] + lane; c < row_ptr[k + 1]; c += kWave) b += pdy[c];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane == 0) restart_finish_core(st + k, a, b);
+}
+
+// Tile flags for the next interval (1 while any of its LPs runs) and the
+// number of LPs still running.
+__global__ void kb_tile_active(int ntiles, const TileDesc* tiles, const int* inst, const PdlpState* st,
+                               unsigned char* active) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int k0 = inst[tiles[t].seg_lo], k1 = inst[tiles[t].seg_hi];
+    unsigned char a = 0;
+    for (int k = k0; k <= k1; ++k) a |= st[k].done ? 0 : 1;
+    active[t] = a;
+}
+
+__global__ void kb_count_running(int K, const PdlpState* st, int* running) {
+    __shared__ double sh[kBlock / kWave];
+    double c = 0.0;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) c += st[k].done ? 0.0 : 1.0;
+    const double t = block_sum(c, sh);
+    if (threadIdx.x == 0) *running = static_cast<int>(t);
+}
+
+// ---------------------------------------------------------------- host side
+struct DevMemB {
+    std::vector<void*> ptrs;
+    ~DevMemB() {
+        for (void* p : ptrs) hipFree(p);
+    }
+    template <class U>
+    hipError_t alloc(U** p, size_t count) {
+        *p = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(U) * (count ? count : 1));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+// One bin holding every segment with L lanes each: segment order is free.
+BinTable one_bin(int nseg, int L) {
+    BinTable bt{};
+    bt.nbins = 1;
+    bt.lanes[0] = L;
+    bt.seg_start[0] = 0;
+    bt.seg_start[1] = nseg;
+    bt.block_start[0] = 0;
+    const int per = L <= kWave ? kBlock / L : 1;
+    bt.block_start[1] = std::max(1, (nseg + per - 1) / per);
+    bt.total_blocks = bt.block_start[1];
+    return bt;
+}
+
+std::vector<Chunk> make_chunks(const std::vector<int>& off, std::vector<int>& ptr) {
+    std::vector<Chunk> ch;
+    const int K = static_cast<int>(off.size()) - 1;
+    ptr.assign(K + 1, 0);
+    for (int k = 0; k < K; ++k) {
+        ptr[k] = static_cast<int>(ch.size());
+        for (int b = off[k]; b < off[k + 1]; b += kBlock) ch.push_back(Chunk{k, b, std::min(off[k + 1], b + kBlock)});
+    }
+    ptr[K] = static_cast<int>(ch.size());
+    if (ch.empty()) ch.push_back(Chunk{0, 0, 0});  // keeps launches valid
+    return ch;
+}
+
+struct BatchBase {
+    virtual ~BatchBase() = default;
+    virtual bo_status set_warm_start(const double* x, const double* y) = 0;
+    virtual bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) = 0;
+    virtual bo_status get_solution(int k, double* x, double* y) = 0;
+};
+
+template <class T>
+struct Batch final : BatchBase {
+    int K = 0, m = 0, n = 0;
+    long long nnz = 0;
+    bo_pdlp_params prm{};
+    hipStream_t stream = nullptr;
+    DevMemB mem;
+    std::vector<int> roff, coff;  // row / column offset of every LP (K + 1)
+    // FP64 matrix (CSR and CSC), scaled working values
+    int *rp = nullptr, *ci = nullptr, *cp = nullptr, *ri = nullptr;
+    double *rv64 = nullptr, *cv64 = nullptr;
+    T *rv = nullptr, *cv = nullptr;
+    double *b64 = nullptr, *c64 = nullptr, *lb64 = nullptr, *ub64 = nullptr;
+    unsigned char* eq = nullptr;
+    int *inst_r = nullptr, *inst_c = nullptr;
+    double *rs = nullptr, *cs = nullptr, *rnorm = nullptr, *cnorm = nullptr;
+    T *b_hat = nullptr, *c_hat = nullptr, *l_hat = nullptr, *u_hat = nullptr;
+    double *xk = nullptr, *x0 = nullptr, *yk = nullptr, *y0 = nullptr, *ax = nullptr, *ax0 = nullptr;
+    T *xp = nullptr, *yp = nullptr, *yg = nullptr, *pw_v = nullptr, *pw_u = nullptr;
+    double *xo_cur = nullptr, *xo_cand = nullptr, *yo_cur = nullptr, *yo_cand = nullptr, *ax_cand = nullptr;
+    double *wx = nullptr, *wy = nullptr;
+    int has_warm = 0;
+    // chunks
+    Chunk *rch = nullptr, *cch = nullptr;
+    int nrch = 0, ncch = 0;
+    int *rch_ptr = nullptr, *cch_ptr = nullptr;
+    double *part_x = nullptr, *part_r = nullptr, *part_c = nullptr, *part_dx = nullptr, *part_dy = nullptr;
+    double *part_sq = nullptr, *inst_a = nullptr, *inst_b = nullptr, *inst_c2 = nullptr;
+    PdlpState* st = nullptr;
+    PdlpState* st_host = nullptr;
+    PdlpState* st_init = nullptr;
+    int* running = nullptr;
+    int* running_host = nullptr;
+    BinTable bt_r{}, bt_c{};
+    // tiled passes
+    struct TilePass {
+        bool built = false;
+        int B = 0, W = 0, ntiles = 0, nslices = 0;
+        long long pairs = 0;
+        size_t padded = 0;
+        TileDesc* td = nullptr;
+        unsigned* words = nullptr;
+        unsigned short* idx = nullptr;
+        int* src = nullptr;
+        T* val = nullptr;
+        int* seg_ptr = nullptr;
+        unsigned* slot = nullptr;
+        double* part = nullptr;
+        unsigned char* active = nullptr;
+    } tp[2];
+    int num_cus = 1, lds_max = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_k = 0;
+
+    ~Batch() override {
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        if (graph) hipGraphDestroy(graph);
+        if (st_host) hipHostFree(st_host);
+        if (st_init) hipHostFree(st_init);
+        if (running_host) hipHostFree(running_host);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    hipError_t up(void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
+    }
+
+    hipError_t build_tiles_pass(int pass, const std::vector<int>& ptr, const std::vector<int>& gidx) {
+        TilePass& t = tp[pass];
+        const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+        if (nseg == 0 || ng == 0) return hipSuccess;
+        const int wmax = std::min(65536, lds_max / static_cast<int>(sizeof(T))) / 64 * 64;
+        if (wmax < 64) return hipSuccess;
+        HostTiles h;
+        if (!build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, 2 * num_cus, h)) return hipSuccess;
+        if (h.pairs * 4 >= (1LL << 31) - 64 || static_cast<long long>(h.slice_word.size()) * kWave * 8 >= (1LL << 31) - 64)
+            return hipSuccess;
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(
+                                               pass == 0 ? &k_tile<T, kTileRows, 2> : &k_tile<T, kTileRows, 3>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           static_cast<int>(h.W * sizeof(T)));
+        if (e != hipSuccess) return e;
+        t.B = h.B;
+        t.W = h.W;
+        t.ntiles = static_cast<int>(h.tiles.size());
+        t.nslices = static_cast<int>(h.slice_word.size());
+        t.padded = h.padded;
+        t.pairs = h.pairs;
+        if ((e = mem.alloc(&t.td, h.tiles.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.words, h.slice_word.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.idx, h.padded)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.src, h.padded)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.val, h.padded)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.seg_ptr, h.seg_ptr.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.slot, h.slot.size())) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.part, h.slice_word.size() * kWave)) != hipSuccess) return e;
+        if ((e = mem.alloc(&t.active, h.tiles.size())) != hipSuccess) return e;
+        if ((e = up(t.td, h.tiles.data(), sizeof(TileDesc) * h.tiles.size())) != hipSuccess) return e;
+        if ((e = up(t.words, h.slice_word.data(), sizeof(unsigned) * h.slice_word.size())) != hipSuccess) return e;
+        if ((e = up(t.idx, h.idx.data(), sizeof(unsigned short) * h.padded)) != hipSuccess) return e;
+        if ((e = up(t.src, h.src.data(), sizeof(int) * h.padded)) != hipSuccess) return e;
+        if ((e = up(t.seg_ptr, h.seg_ptr.data(), sizeof(int) * h.seg_ptr.size())) != hipSuccess) return e;
+        if ((e = up(t.slot, h.slot.data(), sizeof(unsigned) * h.slot.size())) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        t.built = true;
+        return hipSuccess;
+    }
+
+    bo_status create(const bo_lp* lps, int count, const bo_pdlp_params* p) {
+        prm = *p;
+        K = count;
+        roff.assign(K + 1, 0);
+        coff.assign(K + 1, 0);
+        for (int k = 0; k < K; ++k) {
+            roff[k + 1] = roff[k] + lps[k].m;
+            coff[k + 1] = coff[k] + lps[k].n;
+            nnz += lps[k].nnz;
+        }
+        m = roff[K];
+        n = coff[K];
+        BO_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PdlpState) * std::max(K, 1)));
+        BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_init), sizeof(PdlpState) * std::max(K, 1)));
+        BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&running_host), sizeof(int)));
+        std::memset(st_host, 0, sizeof(PdlpState) * std::max(K, 1));
+        // block-diagonal CSR (rows in LP order, columns shifted), its CSC
+        std::vector<int> h_rp(m + 1, 0), h_ci(nnz), h_ir(m), h_ic(n);
+        std::vector<double> h_rv(nnz), hb(m), hc(n), hl(n), hu(n);
+        std::vector<unsigned char> he(m);
+        std::vector<double> bn(K), cn(K);
+        long long o = 0;
+        for (int k = 0; k < K; ++k) {
+            const bo_lp& lp = lps[k];
+            double b2 = 0.0, c2 = 0.0;
+            for (int i = 0; i < lp.m; ++i) {
+                const int gi = roff[k] + i;
+                for (int q = lp.row_ptr[i]; q < lp.row_ptr[i + 1]; ++q, ++o) {
+                    h_ci[o] = coff[k] + lp.col_idx[q];
+                    h_rv[o] = lp.val[q];
+                }
+                h_rp[gi + 1] = static_cast<int>(o);
+                hb[gi] = lp.b[i];
+                he[gi] = lp.is_eq[i];
+                h_ir[gi] = k;
+                b2 += lp.b[i] * lp.b[i];
+            }
+            for (int j = 0; j < lp.n; ++j) {
+                const int gj = coff[k] + j;
+                hc[gj] = lp.c[j];
+                hl[gj] = lp.lb[j];
+                hu[gj] = lp.ub[j];
+                h_ic[gj] = k;
+                c2 += lp.c[j] * lp.c[j];
+            }
+            bn[k] = std::sqrt(b2);
+            cn[k] = std::sqrt(c2);
+        }
+        std::vector<int> h_cp(n + 1, 0), h_ri(nnz);
+        std::vector<double> h_cv(nnz);
+        for (long long q = 0; q < nnz; ++q) h_cp[h_ci[q] + 1]++;
+        for (int j = 0; j < n; ++j) h_cp[j + 1] += h_cp[j];
+        {
+            std::vector<int> pos(h_cp.begin(), h_cp.end() - 1);
+            for (int i = 0; i < m; ++i)
+                for (int q = h_rp[i]; q < h_rp[i + 1]; ++q) {
+                    const int d = pos[h_ci[q]]++;
+                    h_ri[d] = i;
+                    h_cv[d] = h_rv[q];
+                }
+        }
+        for (int k = 0; k < K; ++k) {
+            PdlpState& s = st_init[k];
+            std::memset(&s, 0, sizeof(s));
+            s.b_norm = bn[k];
+            s.c_norm = cn[k];
+            s.eps = prm.eps_rel_kkt;
+            s.max_iter = prm.max_iterations;
+        }
+        // device memory
+        BO_TRY(mem.alloc(&rp, m + 1));
+        BO_TRY(mem.alloc(&ci, nnz));
+        BO_TRY(mem.alloc(&rv64, nnz));
+        BO_TRY(mem.alloc(&rv, nnz));
+        BO_TRY(mem.alloc(&cp, n + 1));
+        BO_TRY(mem.alloc(&ri, nnz));
+        BO_TRY(mem.alloc(&cv64, nnz));
+        BO_TRY(mem.alloc(&cv, nnz));
+        BO_TRY(mem.alloc(&b64, m));
+        BO_TRY(mem.alloc(&eq, m));
+        BO_TRY(mem.alloc(&inst_r, m));
+        BO_TRY(mem.alloc(&rs, m));
+        BO_TRY(mem.alloc(&rnorm, m));
+        BO_TRY(mem.alloc(&b_hat, m));
+        for (double** v : {&yk, &y0, &ax, &ax0, &ax_cand, &yo_cur, &yo_cand, &wy}) BO_TRY(mem.alloc(v, m));
+        for (T** v : {&yp, &yg, &pw_u}) BO_TRY(mem.alloc(v, m));
+        BO_TRY(mem.alloc(&c64, n));
+        BO_TRY(mem.alloc(&lb64, n));
+        BO_TRY(mem.alloc(&ub64, n));
+        BO_TRY(mem.alloc(&inst_c, n));
+        BO_TRY(mem.alloc(&cs, n));
+        BO_TRY(mem.alloc(&cnorm, n));
+        for (T** v : {&c_hat, &l_hat, &u_hat, &xp, &pw_v}) BO_TRY(mem.alloc(v, n));
+        for (double** v : {&xk, &x0, &xo_cur, &xo_cand, &wx}) BO_TRY(mem.alloc(v, n));
+        BO_TRY(mem.alloc(&st, std::max(K, 1)));
+        BO_TRY(mem.alloc(&running, 1));
+        BO_TRY(up(rp, h_rp.data(), sizeof(int) * (m + 1)));
+        BO_TRY(up(ci, h_ci.data(), sizeof(int) * nnz));
+        BO_TRY(up(rv64, h_rv.data(), sizeof(double) * nnz));
+        BO_TRY(up(cp, h_cp.data(), sizeof(int) * (n + 1)));
+        BO_TRY(up(ri, h_ri.data(), sizeof(int) * nnz));
+        BO_TRY(up(cv64, h_cv.data(), sizeof(double) * nnz));
+        BO_TRY(up(b64, hb.data(), sizeof(double) * m));
+        BO_TRY(up(eq, he.data(), m));
+        BO_TRY(up(inst_r, h_ir.data(), sizeof(int) * m));
+        BO_TRY(up(c64, hc.data(), sizeof(double) * n));
+        BO_TRY(up(lb64, hl.data(), sizeof(double) * n));
+        BO_TRY(up(ub64, hu.data(), sizeof(double) * n));
+        BO_TRY(up(inst_c, h_ic.data(), sizeof(int) * n));
+        // chunks
+        std::vector<int> hrp, hcp;
+        const std::vector<Chunk> rc = make_chunks(roff, hrp), cc = make_chunks(coff, hcp);
+        nrch = static_cast<int>(rc.size());
+        ncch = static_cast<int>(cc.size());
+        BO_TRY(mem.alloc(&rch, rc.size()));
+        BO_TRY(mem.alloc(&cch, cc.size()));
+        BO_TRY(mem.alloc(&rch_ptr, K + 1));
+        BO_TRY(mem.alloc(&cch_ptr, K + 1));
+        BO_TRY(up(rch, rc.data(), sizeof(Chunk) * rc.size()));
+        BO_TRY(up(cch, cc.data(), sizeof(Chunk) * cc.size()));
+        BO_TRY(up(rch_ptr, hrp.data(), sizeof(int) * (K + 1)));
+        BO_TRY(up(cch_ptr, hcp.data(), sizeof(int) * (K + 1)));
+        BO_TRY(mem.alloc(&part_x, 2 * static_cast<size_t>(ncch)));
+        BO_TRY(mem.alloc(&part_c, 6 * static_cast<size_t>(ncch)));
+        BO_TRY(mem.alloc(&part_r, 6 * static_cast<size_t>(nrch)));
+        BO_TRY(mem.alloc(&part_dx, ncch));
+        BO_TRY(mem.alloc(&part_dy, nrch));
+        BO_TRY(mem.alloc(&part_sq, std::max(nrch, ncch)));
+        BO_TRY(mem.alloc(&inst_a, std::max(K, 1)));
+        BO_TRY(mem.alloc(&inst_b, std::max(K, 1)));
+        BO_TRY(mem.alloc(&inst_c2, std::max(K, 1)));
+        BO_TRY(hipStreamSynchronize(stream));
+        bt_r = one_bin(m, 4);
+        bt_c = one_bin(n, 4);
+        int dev = 0;
+        BO_TRY(hipGetDevice(&dev));
+        BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+        BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        BO_TRY(build_tiles_pass(0, h_rp, h_ci));
+        BO_TRY(build_tiles_pass(1, h_cp, h_ri));
+        return BO_SUCCESS;
+    }
+
+    template <int MODE, class TV, class Epi>
+    void rows(const TV* vals, const Epi& e) {
+        hipLaunchKernelGGL((k_segments<MODE, TV, typename Epi::TXv, Epi>), dim3(bt_r.total_blocks), dim3(kBlock), 0,
+                           stream, bt_r, rp, ci, vals, e, nullptr, nullptr);
+    }
+    template <int MODE, class TV, class Epi>
+    void cols(const TV* vals, const Epi& e) {
+        hipLaunchKernelGGL((k_segments<MODE, TV, typename Epi::TXv, Epi>), dim3(bt_c.total_blocks), dim3(kBlock), 0,
+                           stream, bt_c, cp, ri, vals, e, nullptr, nullptr);
+    }
+    // per-LP sum of squares of v (rows or columns side) into out[K]
+    template <class TV>
+    void inst_sq(bool row_side, const TV* v, double* out) {
+        hipLaunchKernelGGL(kb_sq_chunks<TV>, dim3(row_side ? nrch : ncch), dim3(kBlock), 0, stream,
+                           row_side ? rch : cch, v, part_sq);
+        hipLaunchKernelGGL(kb_inst_sum, dim3(grid_for(static_cast<size_t>(std::max(K, 1)) * kWave)), dim3(kBlock), 0,
+                           stream, K, row_side ? rch_ptr : cch_ptr, part_sq, 1, 0, out);
+    }
+
+    bo_status prepare() {
+        const unsigned gm = grid_for(m), gn = grid_for(n);
+        hipLaunchKernelGGL(kb_fill_f64, dim3(gm), dim3(kBlock), 0, stream, rs, 1.0, static_cast<size_t>(m));
+        hipLaunchKernelGGL(kb_fill_f64, dim3(gn), dim3(kBlock), 0, stream, cs, 1.0, static_cast<size_t>(n));
+        for (int it = 0; it < prm.ruiz_iterations; ++it) {
+            rows<kMaxAbs>(rv64, EpiNormB{cs, rs, rnorm});
+            cols<kMaxAbs>(cv64, EpiNormB{rs, cs, cnorm});
+            hipLaunchKernelGGL(kb_apply_norm, dim3(gm), dim3(kBlock), 0, stream, rs, rnorm, static_cast<size_t>(m));
+            hipLaunchKernelGGL(kb_apply_norm, dim3(gn), dim3(kBlock), 0, stream, cs, cnorm, static_cast<size_t>(n));
+        }
+        if (prm.pock_chambolle) {
+            rows<kSumAbs>(rv64, EpiNormB{cs, rs, rnorm});
+            cols<kSumAbs>(cv64, EpiNormB{rs, cs, cnorm});
+            hipLaunchKernelGGL(kb_apply_norm, dim3(gm), dim3(kBlock), 0, stream, rs, rnorm, static_cast<size_t>(m));
+            hipLaunchKernelGGL(kb_apply_norm, dim3(gn), dim3(kBlock), 0, stream, cs, cnorm, static_cast<size_t>(n));
+        }
+        rows<kWrite>(rv64, EpiWriteB<T>{rs, cs, rv});
+        cols<kWrite>(cv64, EpiWriteB<T>{cs, rs, cv});
+        for (int pass = 0; pass < 2; ++pass) {
+            const TilePass& t = tp[pass];
+            if (t.built && t.padded > 0)
+                hipLaunchKernelGGL(k_tile_values<T>, dim3(grid_for(t.padded)), dim3(kBlock), 0, stream, t.src,
+                                   t.padded, pass == 0 ? rv : cv, t.val);
+        }
+        hipLaunchKernelGGL(kb_scale_cols<T>, dim3(gn), dim3(kBlock), 0, stream, n, cs, c64, lb64, ub64, wx, has_warm,
+                           c_hat, l_hat, u_hat, xk, x0, xp);
+        hipLaunchKernelGGL(kb_scale_rows<T>, dim3(gm), dim3(kBlock), 0, stream, m, rs, b64, eq, wy, has_warm, b_hat,
+                           yk, y0, yg, yp);
+        // power iteration per LP
+        hipLaunchKernelGGL(kb_pw_init<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n);
+        inst_sq(false, pw_v, inst_b);
+        hipLaunchKernelGGL(kb_pw_normalize<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n, inst_c, inst_b);
+        for (int it = 0; it < 40; ++it) {
+            rows<kSumProd>(rv, EpiStoreB<T, T>{pw_v, pw_u});
+            cols<kSumProd>(cv, EpiStoreB<T, T>{pw_u, pw_v});
+            inst_sq(false, pw_v, inst_b);
+            hipLaunchKernelGGL(kb_pw_normalize<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n, inst_c, inst_b);
+        }
+        rows<kSumProd>(rv, EpiStoreB<T, T>{pw_v, pw_u});
+        inst_sq(true, pw_u, inst_a);    // ||A_hat v||^2 with ||v|| = 1
+        inst_sq(false, c_hat, inst_c2);  // ||c_hat||^2
+        inst_sq(true, b_hat, inst_b);   // ||b_hat||^2
+        BO_TRY(hipMemcpyAsync(st, st_init, sizeof(PdlpState) * K, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(kb_init_state, dim3(grid_for(K)), dim3(kBlock), 0, stream, K, st, st, inst_a, inst_c2,
+                           inst_b);
+        // A x0 (the starting point is also the first anchor)
+        rows<kSumProd>(rv, EpiStoreB<T, double>{xp, ax});
+        hipLaunchKernelGGL(kb_copy_f64, dim3(gm), dim3(kBlock), 0, stream, ax0, ax, static_cast<size_t>(m));
+        for (int pass = 0; pass < 2; ++pass)
+            if (tp[pass].built)
+                hipLaunchKernelGGL(kb_tile_active, dim3(grid_for(tp[pass].ntiles)), dim3(kBlock), 0, stream,
+                                   tp[pass].ntiles, tp[pass].td, pass == 0 ? inst_r : inst_c, st, tp[pass].active);
+        BO_TRY(hipGetLastError());
+        return BO_SUCCESS;
+    }
+
+    void launch_primal(int t) {
+        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, inst_c};
+        const TilePass& tl = tp[1];
+        if (tl.built) {
+            hipLaunchKernelGGL((k_tile<T, kTileRows, 3>), dim3(tl.ntiles), dim3(kTileThreads),
+                               static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, yg, m,
+                               tl.W, tl.part, tl.active);
+            launch_reduce(tl, n, e);
+        } else {
+            cols<kSumProd>(cv, e);
+        }
+    }
+    void launch_dual(int t) {
+        EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, inst_r};
+        const TilePass& tl = tp[0];
+        if (tl.built) {
+            hipLaunchKernelGGL((k_tile<T, kTileRows, 2>), dim3(tl.ntiles), dim3(kTileThreads),
+                               static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, xp, n,
+                               tl.W, tl.part, tl.active);
+            launch_reduce(tl, m, e);
+        } else {
+            rows<kSumProd>(rv, e);
+        }
+    }
+    template <class Epi>
+    void launch_reduce(const TilePass& tl, int nseg, const Epi& e) {
+        const int nslot = static_cast<int>(tl.pairs), npart = tl.nslices * kWave;
+        if (tl.B <= 4)
+            hipLaunchKernelGGL((k_tile_reduce<4, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
+                               tl.seg_ptr, tl.slot, nslot, tl.part, npart, e, nullptr);
+        else if (tl.B <= 8)
+            hipLaunchKernelGGL((k_tile_reduce<8, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
+                               tl.seg_ptr, tl.slot, nslot, tl.part, npart, e, nullptr);
+        else
+            hipLaunchKernelGGL((k_tile_reduce<16, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
+                               tl.seg_ptr, tl.slot, nslot, tl.part, npart, e, nullptr);
+    }
+
+    void launch_check() {
+        const unsigned gw = grid_for(static_cast<size_t>(std::max(K, 1)) * kWave);
+        hipLaunchKernelGGL(kb_check_x<T>, dim3(ncch), dim3(kBlock), 0, stream, cch, st, xk, xp, cs, c64, lb64, ub64,
+                           xo_cur, xo_cand, part_x);
+        hipLaunchKernelGGL(kb_check_rows<T>, dim3(nrch), dim3(kBlock), 0, stream, rch, st, rp, ci, rv64, xo_cur,
+                           xo_cand, b64, eq, rs, yk, yp, yo_cur, yo_cand, ax_cand, part_r);
+        hipLaunchKernelGGL(kb_check_cols, dim3(ncch), dim3(kBlock), 0, stream, cch, st, cp, ri, cv64, yo_cur, yo_cand,
+                           c64, lb64, ub64, cs, part_c);
+        hipLaunchKernelGGL(kb_decide, dim3(gw), dim3(kBlock), 0, stream, K, st, cch_ptr, part_x, part_c, rch_ptr,
+                           part_r);
+        hipLaunchKernelGGL(kb_restart_x<T>, dim3(ncch), dim3(kBlock), 0, stream, cch, st, xk, x0, xp, part_dx);
+        hipLaunchKernelGGL(kb_restart_y<T>, dim3(nrch), dim3(kBlock), 0, stream, rch, st, yk, y0, yp, yg, ax, ax0,
+                           ax_cand, part_dy);
+        hipLaunchKernelGGL(kb_restart_finish, dim3(gw), dim3(kBlock), 0, stream, K, st, cch_ptr, part_dx, rch_ptr,
+                           part_dy);
+        for (int pass = 0; pass < 2; ++pass)
+            if (tp[pass].built)
+                hipLaunchKernelGGL(kb_tile_active, dim3(grid_for(tp[pass].ntiles)), dim3(kBlock), 0, stream,
+                                   tp[pass].ntiles, tp[pass].td, pass == 0 ? inst_r : inst_c, st, tp[pass].active);
+        hipLaunchKernelGGL(kb_count_running, dim3(1), dim3(kBlock), 0, stream, K, st, running);
+    }
+
+    void launch_interval(int k) {
+        for (int t = 0; t < k; ++t) {
+            launch_primal(t);
+            launch_dual(t);
+        }
+        hipLaunchKernelGGL(kb_advance, dim3(grid_for(K)), dim3(kBlock), 0, stream, K, st, k);
+        launch_check();
+    }
+
+    bo_status set_warm_start(const double* x, const double* y) override {
+        if (!x && !y) {
+            has_warm = 0;
+            return BO_SUCCESS;
+        }
+        if (!x || !y) return BO_INVALID_ARGUMENT;
+        BO_TRY(hipMemcpyAsync(wx, x, sizeof(double) * n, hipMemcpyHostToDevice, stream));
+        BO_TRY(hipMemcpyAsync(wy, y, sizeof(double) * m, hipMemcpyHostToDevice, stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        has_warm = 1;
+        return BO_SUCCESS;
+    }
+
+    bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) override {
+        auto t0 = std::chrono::high_resolution_clock::now();
+        bo_status s0 = prepare();
+        if (s0 != BO_SUCCESS) return s0;
+        launch_check();  // step 0: KKT of the starting points
+        BO_TRY(hipMemcpyAsync(running_host, running, sizeof(int), hipMemcpyDeviceToHost, stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        const int Kc = prm.check_every > 0 ? prm.check_every : 64;
+        long long iters = 0;
+        long long intervals = 0;
+        while (*running_host > 0) {
+            const long long left = prm.max_iterations - iters;
+            const int k = static_cast<int>(std::min<long long>(Kc, left));
+            if (k <= 0) break;
+            if (prm.use_graph && k == Kc) {
+                if (!graph_exec || graph_k != Kc) {
+                    if (graph_exec) hipGraphExecDestroy(graph_exec);
+                    if (graph) hipGraphDestroy(graph);
+                    graph_exec = nullptr;
+                    graph = nullptr;
+                    BO_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                    launch_interval(Kc);
+                    BO_TRY(hipStreamEndCapture(stream, &graph));
+                    BO_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+                    graph_k = Kc;
+                }
+                BO_TRY(hipGraphLaunch(graph_exec, stream));
+            } else {
+                launch_interval(k);
+            }
+            BO_TRY(hipGetLastError());
+            iters += k;
+            ++intervals;
+            BO_TRY(hipMemcpyAsync(running_host, running, sizeof(int), hipMemcpyDeviceToHost, stream));
+            BO_TRY(hipStreamSynchronize(stream));
+            if (prm.time_limit_s > 0.0 &&
+                std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count() >
+                    prm.time_limit_s)
+                break;
+        }
+        BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState) * K, hipMemcpyDeviceToHost, stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        auto t2 = std::chrono::high_resolution_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+        int solved = 0;
+        long long max_it = 0, sum_it = 0;
+        for (int k = 0; k < K; ++k) {
+            const PdlpState& h = st_host[k];
+            const int b = h.best;
+            const int status = h.done ? h.status : 1;
+            solved += status == 0;
+            max_it = std::max(max_it, h.iter);
+            sum_it += h.iter;
+            if (per_lp) {
+                bo_pdlp_stats& o = per_lp[k];
+                std::memset(&o, 0, sizeof(o));
+                o.status = status;
+                o.iterations = h.iter;
+                o.restarts = h.restarts;
+                o.used_average = b;
+                o.rel_kkt = h.rel_kkt[b];
+                o.rel_primal = h.rel_primal[b];
+                o.rel_dual = h.rel_dual[b];
+                o.rel_gap = h.rel_gap[b];
+                o.primal_obj = h.pobj[b];
+                o.dual_obj = h.dobj[b];
+                o.primal_weight = h.omega;
+                o.step_size = h.eta;
+                o.solve_ms = ms;
+            }
+        }
+        if (total) {
+            std::memset(total, 0, sizeof(*total));
+            total->count = K;
+            total->solved = solved;
+            total->max_iterations = max_it;
+            total->sum_iterations = sum_it;
+            total->loop_steps = iters;
+            total->solve_ms = ms;
+            total->tiled = tp[0].built && tp[1].built;
+        }
+        return BO_SUCCESS;
+    }
+
+    bo_status get_solution(int k, double* x, double* y) override {
+        if (k < 0 || k >= K) return BO_INVALID_ARGUMENT;
+        const int b = st_host[k].best;
+        const int nk = coff[k + 1] - coff[k], mk = roff[k + 1] - roff[k];
+        if (x && nk)
+            BO_TRY(hipMemcpyAsync(x, (b ? xo_cand : xo_cur) + coff[k], sizeof(double) * nk, hipMemcpyDeviceToHost,
+                                  stream));
+        if (y && mk)
+            BO_TRY(hipMemcpyAsync(y, (b ? yo_cand : yo_cur) + roff[k], sizeof(double) * mk, hipMemcpyDeviceToHost,
+                                  stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        return BO_SUCCESS;
+    }
+};
+
+bool valid_lp(const bo_lp& lp) {
+    if (lp.m < 0 || lp.n < 0 || lp.nnz < 0) return false;
+    if ((lp.m > 0 && (!lp.row_ptr || !lp.b || !lp.is_eq)) || (lp.n > 0 && (!lp.c || !lp.lb || !lp.ub)) ||
+        (lp.nnz > 0 && (!lp.col_idx || !lp.val)))
+        return false;
+    if (lp.m > 0 && (lp.row_ptr[0] != 0 || lp.row_ptr[lp.m] != lp.nnz)) return false;
+    if (lp.m == 0 && lp.nnz != 0) return false;
+    for (int i = 0; i < lp.m; ++i)
+        if (lp.row_ptr[i + 1] < lp.row_ptr[i]) return false;
+    for (long long k = 0; k < lp.nnz; ++k)
+        if (lp.col_idx[k] < 0 || lp.col_idx[k] >= lp.n) return false;
+    return true;
+}
+
+}  // namespace
+}  // namespace bo
+
+struct bo_pdlp_batch {
+    std::unique_ptr<bo::BatchBase> eng;
+};
+
+using namespace bo;
+
+extern "C" {
+
+bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params* params, bo_pdlp_batch** out) {
+    if (!out) return BO_INVALID_ARGUMENT;
+    *out = nullptr;
+    bo_status s = ensure_device_public();
+    if (s != BO_SUCCESS) return s;
+    if (!lps || count <= 0) return BO_INVALID_ARGUMENT;
+    long long m = 0, n = 0, nnz = 0;
+    for (int k = 0; k < count; ++k) {
+        if (!valid_lp(lps[k])) return BO_INVALID_ARGUMENT;
+        m += lps[k].m;
+        n += lps[k].n;
+        nnz += lps[k].nnz;
+    }
+    if (m >= (1LL << 31) - 1 || n >= (1LL << 31) - 1 || nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;
+    bo_pdlp_params p;
+    if (params) p = *params;
+    else bo_pdlp_default_params(&p);
+    auto h = std::make_unique<bo_pdlp_batch>();
+    bo_status st;
+    if (p.fp64_iterates) {
+        auto e = std::make_unique<Batch<double>>();
+        st = e->create(lps, count, &p);
+        h->eng = std::move(e);
+    } else {
+        auto e = std::make_unique<Batch<float>>();
+        st = e->create(lps, count, &p);
+        h->eng = std::move(e);
+    }
+    if (st != BO_SUCCESS) return st;
+    *out = h.release();
+    return BO_SUCCESS;
+}
+
+bo_status bo_pdlp_batch_set_warm_start(bo_pdlp_batch* h, const double* x, const double* y) {
+    if (!h) return BO_INVALID_ARGUMENT;
+    return h->eng->set_warm_start(x, y);
+}
+
+bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
+    if (!h) return BO_INVALID_ARGUMENT;
+    return h->eng->solve(per_lp, total);
+}
+
+bo_status bo_pdlp_batch_get_solution(bo_pdlp_batch* h, int k, double* x, double* y) {
+    if (!h) return BO_INVALID_ARGUMENT;
+    return h->eng->get_solution(k, x, y);
+}
+
+void bo_pdlp_batch_destroy(bo_pdlp_batch* h) { delete h; }
+
+}  // extern "C"

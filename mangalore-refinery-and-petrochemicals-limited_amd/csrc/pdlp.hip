@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pdlp_core.hpp"
 #include "segments.hpp"
 #include "stream.hpp"
 #include "tiles.hpp"
@@ -48,29 +49,6 @@ namespace bo {
 bo_status ensure_device_public();
 
 namespace {
-
-// ---------------------------------------------------------------- state
-struct PdlpState {
-    double eta, omega, tau, sigma;
-    double b_norm, c_norm, eps;
-    double kkt_last_restart, kkt_prev_candidate;
-    // [0] current iterate z, [1] candidate T(z)
-    double rel_kkt[2], rel_primal[2], rel_dual[2], rel_gap[2], pobj[2], dobj[2], kkt_w[2];
-    double sigma_max;
-    long long iter, iter_last_restart, max_iter;
-    int done, status, restart, restart_to_cand, best, restarts, checks, pad;
-};
-
-// Halpern weight of step t of the current interval.
-__device__ inline double halpern_w(const PdlpState* st, int t) {
-    const double k = static_cast<double>(st->iter - st->iter_last_restart + t);
-    return (k + 1.0) / (k + 2.0);
-}
-
-template <class T>
-__device__ inline T clampT(T v, T lo, T hi) {
-    return v < lo ? lo : (v > hi ? hi : v);
-}
 
 // ---------------------------------------------------------------- epilogues
 // out = A v, partial sum of squares of out.
@@ -116,63 +94,6 @@ struct EpiWrite {
         out[k] = static_cast<TO>(own_scale[seg] * a * other_scale[j]);
     }
     __device__ void operator()(int, const double*, double*) const {}
-};
-
-// Column j: primal step from A'y (gathered y copy), Halpern update of x.
-template <class T>
-struct EpiPrimal {
-    using TXv = T;
-    static constexpr int NV = 1;
-    static constexpr int Q = 0;
-    const T* yg;       // gathered: the current dual iterate, rounded to T
-    const PdlpState* st;
-    const T* c;
-    const T* lb;
-    const T* ub;
-    double* xk;        // Halpern iterate (in/out)
-    const double* x0;  // restart anchor
-    T* xp;             // x+ (gathered by the CSR pass)
-    int t;             // step index within the interval
-    __device__ const T* vec(int) const { return yg; }
-    __device__ void operator()(int j, const double* acc, double*) const {
-        const double xo = xk[j];
-        const T xt = clampT(static_cast<T>(xo - st->tau * (static_cast<double>(c[j]) + acc[0])), lb[j], ub[j]);
-        xp[j] = xt;
-        const double w = halpern_w(st, t);
-        xk[j] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[j];
-    }
-};
-
-// Row i: dual step from A x+ (gathered x+), Halpern update of y and of A x.
-template <class T>
-struct EpiDual {
-    using TXv = T;
-    static constexpr int NV = 1;
-    static constexpr int Q = 0;
-    const T* xp;       // gathered: x+
-    const PdlpState* st;
-    const T* b;
-    const unsigned char* eq;
-    double* ax;        // A z (carried, in/out)
-    const double* ax0; // A z0
-    double* yk;        // Halpern iterate (in/out)
-    const double* y0;  // restart anchor
-    T* yp;             // y+ (restart candidate)
-    T* yg;             // yk rounded to T (gathered by the next CSC pass)
-    int t;
-    __device__ const T* vec(int) const { return xp; }
-    __device__ void operator()(int i, const double* acc, double*) const {
-        const double axo = ax[i], yo = yk[i];
-        double yn = yo + st->sigma * (2.0 * acc[0] - axo - static_cast<double>(b[i]));
-        if (!eq[i] && yn < 0.0) yn = 0.0;
-        const T yt = static_cast<T>(yn);
-        yp[i] = yt;
-        const double w = halpern_w(st, t);
-        const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[i];
-        yk[i] = ynew;
-        yg[i] = static_cast<T>(ynew);
-        ax[i] = w * (2.0 * acc[0] - axo) + (1.0 - w) * ax0[i];
-    }
 };
 
 // FP64 check, row side: gathers unscaled x (current, candidate).
@@ -331,24 +252,7 @@ __global__ void k_pw_normalize(T* v, int n, const double* norm2) {
 // eta, omega, tau, sigma from the power-iteration estimate and ||c_hat||, ||b_hat||.
 __global__ void k_init_state(PdlpState* st, const double* pw_u2, const double* c2, const double* b2) {
     if (threadIdx.x != 0) return;
-    const double smax = sqrt(*pw_u2);
-    st->sigma_max = smax;
-    st->eta = smax > 0.0 ? 0.95 / smax : 1.0;
-    const double cn = sqrt(*c2), bn = sqrt(*b2);
-    st->omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
-    st->tau = st->eta / st->omega;
-    st->sigma = st->eta * st->omega;
-    st->iter = 0;
-    st->iter_last_restart = 0;
-    st->done = 0;
-    st->status = 1;
-    st->restart = 0;
-    st->restart_to_cand = 0;
-    st->best = 0;
-    st->restarts = 0;
-    st->checks = 0;
-    st->kkt_last_restart = 0.0;
-    st->kkt_prev_candidate = INFINITY;
+    init_state_core(st, *pw_u2, *c2, *b2);
 }
 
 template <class T>
@@ -428,62 +332,7 @@ k_decide(PdlpState* st, CheckDims d, const double* px, const double* pr, const d
 #pragma unroll
     for (int k = 0; k < 14; ++k) q[k] = block_sum(a[k], sh);
     if (threadIdx.x != 0) return;
-    // q: [0,1] c'x  [2,3] ||pviol||^2 [4,5] scaled [6,7] b'y  [8,9] ||dres||^2 [10,11] scaled [12,13] dterm
-    const double w = st->omega;
-    bool bad = false;
-    for (int p = 0; p < 2; ++p) {
-        const double pobj = q[0 + p];
-        const double dobj = -q[6 + p] + q[12 + p];
-        const double rp = sqrt(q[2 + p]), rd = sqrt(q[8 + p]);
-        st->pobj[p] = pobj;
-        st->dobj[p] = dobj;
-        st->rel_primal[p] = rp / (1.0 + st->b_norm);
-        st->rel_dual[p] = rd / (1.0 + st->c_norm);
-        st->rel_gap[p] = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
-        double k = st->rel_primal[p];
-        if (st->rel_dual[p] > k) k = st->rel_dual[p];
-        if (st->rel_gap[p] > k) k = st->rel_gap[p];
-        st->rel_kkt[p] = k;
-        const double g = pobj - dobj;
-        st->kkt_w[p] = sqrt(w * q[4 + p] + q[10 + p] / w + g * g);
-        if (!isfinite(k)) bad = true;
-    }
-    st->checks += 1;
-    const int best = st->rel_kkt[1] < st->rel_kkt[0] ? 1 : 0;
-    st->best = best;
-    if (bad) {
-        st->done = 1;
-        st->status = 2;
-        return;
-    }
-    if (st->rel_kkt[best] <= st->eps) {
-        st->done = 1;
-        st->status = 0;
-        return;
-    }
-    if (st->iter >= st->max_iter) {
-        st->done = 1;
-        st->status = 1;
-        return;
-    }
-    if (st->iter == 0) {  // starting point
-        st->kkt_last_restart = st->kkt_w[0];
-        st->kkt_prev_candidate = st->kkt_w[0];
-        return;
-    }
-    const int cand = st->kkt_w[1] < st->kkt_w[0] ? 1 : 0;
-    const double kc = st->kkt_w[cand];
-    const long long since = st->iter - st->iter_last_restart;
-    const bool restart = (kc <= 0.2 * st->kkt_last_restart) ||
-                         (kc <= 0.8 * st->kkt_last_restart && kc > st->kkt_prev_candidate) ||
-                         (static_cast<double>(since) >= 0.36 * static_cast<double>(st->iter));
-    st->kkt_prev_candidate = kc;
-    if (restart) {
-        st->restart = 1;
-        st->restart_to_cand = cand;
-        st->restarts += 1;
-        st->kkt_last_restart = kc;
-    }
+    decide_core(st, q);
 }
 
 // Restart: move to the chosen point, make it the new anchor, measure the
@@ -536,14 +385,7 @@ __global__ void k_restart_finish(PdlpState* st, const double* pdx, int nbx, cons
     const double dx2 = block_sum(a, sh);
     const double dy2 = block_sum(b, sh);
     if (threadIdx.x != 0) return;
-    const double dx = sqrt(dx2), dy = sqrt(dy2);
-    if (dx > 1e-10 && dy > 1e-10) {
-        st->omega = exp(0.5 * log(dy / dx) + 0.5 * log(st->omega));
-        st->tau = st->eta / st->omega;
-        st->sigma = st->eta * st->omega;
-    }
-    st->iter_last_restart = st->iter;
-    st->restart = 0;
+    restart_finish_core(st, dx2, dy2);
 }
 
 // ---------------------------------------------------------------- host side
@@ -1025,7 +867,7 @@ struct Engine final : EngineBase {
         const int nseg = PASS == 0 ? m : n, ng = PASS == 0 ? n : m;
         if (t.ntiles > 0)
             hipLaunchKernelGGL((k_tile<T, kTileRows, PASS>), dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream,
-                               t.td, t.words, t.val, t.idx, gvec, ng, t.W, t.part);
+                               t.td, t.words, t.val, t.idx, gvec, ng, t.W, t.part, nullptr);
         if (mid) hipEventRecord(mid, stream);
         const int nslot = static_cast<int>(t.pairs), npart = t.nslices * kWave;
         if (t.B <= 4)
@@ -1040,7 +882,7 @@ struct Engine final : EngineBase {
     }
 
     void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
-        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t};
+        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, nullptr};
         if (p.tiled > 0) {
             launch_tiled<1>(yg, e, done, mid);
             return;
@@ -1050,7 +892,7 @@ struct Engine final : EngineBase {
         if (mid) hipEventRecord(mid, stream);
     }
     void launch_dual(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
-        EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t};
+        EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, nullptr};
         if (p.tiled > 0) {
             launch_tiled<0>(xp, e, done, mid);
             return;

@@ -1,0 +1,198 @@
+// pdlp_core.hpp -- per-LP state and step logic of the restarted, reflected
+// Halpern PDHG, shared by the single-LP engine (pdlp.hip) and the batched
+// engine (batch.hip).  Everything here acts on one LP's PdlpState; the
+// batched engine keeps one state per LP and maps every row and column to
+// its LP.
+#pragma once
+
+#include "common.hpp"
+
+namespace bo {
+
+struct PdlpState {
+    double eta, omega, tau, sigma;
+    double b_norm, c_norm, eps;
+    double kkt_last_restart, kkt_prev_candidate;
+    // [0] current iterate z, [1] candidate T(z)
+    double rel_kkt[2], rel_primal[2], rel_dual[2], rel_gap[2], pobj[2], dobj[2], kkt_w[2];
+    double sigma_max;
+    long long iter, iter_last_restart, max_iter;
+    int done, status, restart, restart_to_cand, best, restarts, checks, pad;
+};
+
+// Halpern weight of step t of the current interval.
+__device__ inline double halpern_w(const PdlpState* st, int t) {
+    const double k = static_cast<double>(st->iter - st->iter_last_restart + t);
+    return (k + 1.0) / (k + 2.0);
+}
+
+template <class T>
+__host__ __device__ inline T clampT(T v, T lo, T hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Column j: primal step from A'y (gathered y copy), Halpern update of x.
+// inst (batched engine): the LP of every column, whose state st[inst[j]]
+// drives the step; a finished LP is left untouched.
+template <class T>
+struct EpiPrimal {
+    using TXv = T;
+    static constexpr int NV = 1;
+    static constexpr int Q = 0;
+    const T* yg;       // gathered: the current dual iterate, rounded to T
+    const PdlpState* st;
+    const T* c;
+    const T* lb;
+    const T* ub;
+    double* xk;        // Halpern iterate (in/out)
+    const double* x0;  // restart anchor
+    T* xp;             // x+ (gathered by the CSR pass)
+    int t;             // step index within the interval
+    const int* inst;   // nullptr: one LP
+    __device__ const T* vec(int) const { return yg; }
+    __device__ void operator()(int j, const double* acc, double*) const {
+        const PdlpState* s = inst ? st + inst[j] : st;
+        if (inst && s->done) return;
+        const double xo = xk[j];
+        const T xt = clampT(static_cast<T>(xo - s->tau * (static_cast<double>(c[j]) + acc[0])), lb[j], ub[j]);
+        xp[j] = xt;
+        const double w = halpern_w(s, t);
+        xk[j] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[j];
+    }
+};
+
+// Row i: dual step from A x+ (gathered x+), Halpern update of y and of A x.
+template <class T>
+struct EpiDual {
+    using TXv = T;
+    static constexpr int NV = 1;
+    static constexpr int Q = 0;
+    const T* xp;       // gathered: x+
+    const PdlpState* st;
+    const T* b;
+    const unsigned char* eq;
+    double* ax;        // A z (carried, in/out)
+    const double* ax0; // A z0
+    double* yk;        // Halpern iterate (in/out)
+    const double* y0;  // restart anchor
+    T* yp;             // y+ (restart candidate)
+    T* yg;             // yk rounded to T (gathered by the next CSC pass)
+    int t;
+    const int* inst;   // nullptr: one LP
+    __device__ const T* vec(int) const { return xp; }
+    __device__ void operator()(int i, const double* acc, double*) const {
+        const PdlpState* s = inst ? st + inst[i] : st;
+        if (inst && s->done) return;
+        const double axo = ax[i], yo = yk[i];
+        double yn = yo + s->sigma * (2.0 * acc[0] - axo - static_cast<double>(b[i]));
+        if (!eq[i] && yn < 0.0) yn = 0.0;
+        const T yt = static_cast<T>(yn);
+        yp[i] = yt;
+        const double w = halpern_w(s, t);
+        const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[i];
+        yk[i] = ynew;
+        yg[i] = static_cast<T>(ynew);
+        ax[i] = w * (2.0 * acc[0] - axo) + (1.0 - w) * ax0[i];
+    }
+};
+
+// eta, omega, tau, sigma from the power-iteration estimate ||A_hat u||^2 and
+// ||c_hat||^2, ||b_hat||^2; the iteration counters start over.
+__device__ inline void init_state_core(PdlpState* st, double pw_u2, double c2, double b2) {
+    const double smax = sqrt(pw_u2);
+    st->sigma_max = smax;
+    st->eta = smax > 0.0 ? 0.95 / smax : 1.0;
+    const double cn = sqrt(c2), bn = sqrt(b2);
+    st->omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    st->tau = st->eta / st->omega;
+    st->sigma = st->eta * st->omega;
+    st->iter = 0;
+    st->iter_last_restart = 0;
+    st->done = 0;
+    st->status = 1;
+    st->restart = 0;
+    st->restart_to_cand = 0;
+    st->best = 0;
+    st->restarts = 0;
+    st->checks = 0;
+    st->kkt_last_restart = 0.0;
+    st->kkt_prev_candidate = INFINITY;
+}
+
+// The check's sums q (FP64, unscaled problem):
+//   [0,1] c'x  [2,3] ||primal violation||^2  [4,5] the same, scaled  [6,7] b'y
+//   [8,9] ||dual residual||^2  [10,11] the same, scaled  [12,13] sum l plam+ + u plam-
+// for the current iterate [even] and the candidate T(z) [odd]; then the
+// stopping test, the KKT restart rule (beta 0.2 / 0.8 / 0.36) and the
+// bookkeeping.
+__device__ inline void decide_core(PdlpState* st, const double* q) {
+    const double w = st->omega;
+    bool bad = false;
+    for (int p = 0; p < 2; ++p) {
+        const double pobj = q[0 + p];
+        const double dobj = -q[6 + p] + q[12 + p];
+        const double rp = sqrt(q[2 + p]), rd = sqrt(q[8 + p]);
+        st->pobj[p] = pobj;
+        st->dobj[p] = dobj;
+        st->rel_primal[p] = rp / (1.0 + st->b_norm);
+        st->rel_dual[p] = rd / (1.0 + st->c_norm);
+        st->rel_gap[p] = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+        double k = st->rel_primal[p];
+        if (st->rel_dual[p] > k) k = st->rel_dual[p];
+        if (st->rel_gap[p] > k) k = st->rel_gap[p];
+        st->rel_kkt[p] = k;
+        const double g = pobj - dobj;
+        st->kkt_w[p] = sqrt(w * q[4 + p] + q[10 + p] / w + g * g);
+        if (!isfinite(k)) bad = true;
+    }
+    st->checks += 1;
+    const int best = st->rel_kkt[1] < st->rel_kkt[0] ? 1 : 0;
+    st->best = best;
+    if (bad) {
+        st->done = 1;
+        st->status = 2;
+        return;
+    }
+    if (st->rel_kkt[best] <= st->eps) {
+        st->done = 1;
+        st->status = 0;
+        return;
+    }
+    if (st->iter >= st->max_iter) {
+        st->done = 1;
+        st->status = 1;
+        return;
+    }
+    if (st->iter == 0) {  // starting point
+        st->kkt_last_restart = st->kkt_w[0];
+        st->kkt_prev_candidate = st->kkt_w[0];
+        return;
+    }
+    const int cand = st->kkt_w[1] < st->kkt_w[0] ? 1 : 0;
+    const double kc = st->kkt_w[cand];
+    const long long since = st->iter - st->iter_last_restart;
+    const bool restart = (kc <= 0.2 * st->kkt_last_restart) ||
+                         (kc <= 0.8 * st->kkt_last_restart && kc > st->kkt_prev_candidate) ||
+                         (static_cast<double>(since) >= 0.36 * static_cast<double>(st->iter));
+    st->kkt_prev_candidate = kc;
+    if (restart) {
+        st->restart = 1;
+        st->restart_to_cand = cand;
+        st->restarts += 1;
+        st->kkt_last_restart = kc;
+    }
+}
+
+// After a restart: primal-weight update (theta = 0.5) from the anchor move.
+__device__ inline void restart_finish_core(PdlpState* st, double dx2, double dy2) {
+    const double dx = sqrt(dx2), dy = sqrt(dy2);
+    if (dx > 1e-10 && dy > 1e-10) {
+        st->omega = exp(0.5 * log(dy / dx) + 0.5 * log(st->omega));
+        st->tau = st->eta / st->omega;
+        st->sigma = st->eta * st->omega;
+    }
+    st->iter_last_restart = st->iter;
+    st->restart = 0;
+}
+
+}  // namespace bo

@@ -47,9 +47,10 @@ struct WaveDesc {
 };
 
 struct TileDesc {
-    int block;  // gathered block: entries [block * W, block * W + W)
+    int block;   // gathered block: entries [block * W, block * W + W)
     int nslice;
-    int pad[2];
+    int seg_lo;  // first and last segment the tile holds pairs of
+    int seg_hi;
     WaveDesc wave[kTileWaves];
 };
 
@@ -203,6 +204,8 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
         }
         TileDesc td{};
         td.block = sp.c;
+        td.seg_lo = P[sp.p0].s;
+        td.seg_hi = P[sp.p1 - 1].s;
         for (int w = 0; w < kTileWaves; ++w) {
             WaveDesc& wd = td.wave[w];
             if (h.padded > 0xFFFFFFFFull) return false;
@@ -349,13 +352,15 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, int nrows, 
 
 // One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of the
 // wavefront's run kept in flight per buffer.  PASS only names the
-// instantiation (0: CSR pass, 1: CSC pass), so that profiles tell the two
-// passes apart.
+// instantiation (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of the
+// batched engine), so that profiles tell the passes apart.  active
+// (optional): per-tile flag; a tile whose flag is 0 does nothing.
 template <class T, int CH, int PASS>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_word, const T* __restrict__ val,
        const unsigned short* __restrict__ idx, const T* __restrict__ gvec, int ng, int W,
-       double* __restrict__ part) {
+       double* __restrict__ part, const unsigned char* __restrict__ active) {
+    if (active && !active[blockIdx.x]) return;  // every LP of the tile has finished (batched engine)
     extern __shared__ __align__(16) unsigned char tile_lds[];
     T* lds = reinterpret_cast<T*>(tile_lds);
     const int tid = threadIdx.x;

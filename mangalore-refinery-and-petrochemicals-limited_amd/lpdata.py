@@ -259,6 +259,34 @@ def batch_sizes(count=512, lo=200, hi=20000, seed=42):
     return rng.integers(lo, hi + 1, size=count)
 
 
+def random_lp_sampled(m, n, k=5, seed=42):
+    """The random_lp recipe (k distinct uniform rows per column, N(0,1) values,
+    x0 ~ U[0,10], u = 20, b = A x0 + U[0,1], c ~ N(0,1)) with the rows drawn
+    by sampling and redrawing collisions at every size -- linear in n*k, for
+    generating many LPs quickly (a different random stream from random_lp)."""
+    rng = np.random.default_rng(seed)
+    k = min(k, m)
+    rows = rng.integers(0, m, size=(n, k))
+    for _ in range(256):
+        srt = np.sort(rows, axis=1)
+        dup = (np.diff(srt, axis=1) == 0).any(axis=1)
+        if not dup.any():
+            break
+        rows[dup] = rng.integers(0, m, size=(int(dup.sum()), k))
+    cols = np.repeat(np.arange(n), k)
+    vals = rng.standard_normal(n * k)
+    row_ptr, col_idx, val = _csr_from_cols(m, n, cols, rows.reshape(-1), vals)
+    x0 = rng.uniform(0.0, 10.0, n)
+    return _finish(m, n, row_ptr, col_idx, val, rng, x0, meta=dict(kind="random_sampled", seed=seed, k=k))
+
+
+def batch_lps(count=512, lo=200, hi=20000, seed=42, k=5):
+    """The batched workload: count LPs, LP i of size m_i ~ U{lo..hi} rows,
+    n_i = 3 m_i columns, k nonzeros per column, seed + 1 + i."""
+    sizes = batch_sizes(count, lo, hi, seed)
+    return [random_lp_sampled(int(mi), 3 * int(mi), k, seed + 1 + i) for i, mi in enumerate(sizes)]
+
+
 def benchmark10_model():
     """Deterministic 100 x 50 model of the reference's Benchmark 5-10
     (benchmarks/benchmark_main.cpp:139-151): maximise sum ((j % 7) + 1) x_j,

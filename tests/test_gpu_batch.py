@@ -1,0 +1,118 @@
+"""GPU parity of the batched PDLP engine (bo_pdlp_batch_*): many LPs of uneven
+size laid out block-diagonally and solved together.
+
+Every LP of a batch is judged on its own, by the oracle, as in
+tests/test_gpu_pdlp.py: FP64 rel-KKT of the returned point <= 1e-6 on the
+unscaled problem, objective within 1e-6 of the exact optima recorded in
+tests/golden/lp_optima.json (reference simplex chain and HiGHS), and the
+engine's own FP64 report agreeing with the oracle's.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+OPT = json.load(open(os.path.join(HERE, "golden", "lp_optima.json")))
+sys.path.insert(0, HERE)
+TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def instances():
+    from golden import make_golden
+    return make_golden.pdlp_instances()
+
+
+def check_lp(po, lp, x, y, s, f_star=None, name=""):
+    k = po.kkt(lp, x, y)
+    assert s["status"] == 0, (name, s)
+    assert k["rel_kkt"] <= TOL, (name, k)
+    assert k["max_bound_viol"] == 0.0
+    assert abs(k["rel_kkt"] - s["rel_kkt"]) <= 1e-9 + 1e-3 * k["rel_kkt"], (name, k["rel_kkt"], s["rel_kkt"])
+    if f_star is not None:
+        assert abs(k["primal_obj"] - f_star) / (1 + abs(f_star)) <= TOL, (name, k["primal_obj"], f_star)
+    return k
+
+
+@pytest.mark.parametrize("fp64", [0, 1])
+def test_batch_of_golden_instances(pkg, po, instances, fp64):
+    names = sorted(OPT)
+    lps = [instances[nm] for nm in names]
+    print("case: batch of", len(lps), "golden LPs, fp64", fp64, flush=True)
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=TOL, max_iterations=200000, fp64_iterates=fp64)
+    per, tot = b.solve()
+    print("batch", tot, flush=True)
+    assert tot["count"] == len(lps) and tot["solved"] == len(lps)
+    for k, nm in enumerate(names):
+        x, y = b.solution(k)
+        check_lp(po, lps[k], x, y, per[k], OPT[nm]["highs"], nm)
+        if "reference_simplex" in OPT[nm]:
+            fr = OPT[nm]["reference_simplex"]
+            assert abs(po.kkt(lps[k], x, y)["primal_obj"] - fr) / (1 + abs(fr)) <= TOL
+
+
+def test_batch_ragged_random(pkg, po, lpdata):
+    # uneven sizes (a single-row LP among them).  Every LP the batch reports
+    # solved must pass the oracle, and agree with the single-LP engine where
+    # both solve it.  Some of these small random LPs converge slowly under
+    # FP32 iterates in either engine (rounding order decides which), so the
+    # test asks for most, not all, within the 50,000-iteration cap.
+    rng = np.random.default_rng(3)
+    sizes = [1, 7] + [int(v) for v in rng.integers(20, 900, size=22)]
+    lps = [lpdata.random_lp_sampled(mi, 3 * mi, min(5, mi), 100 + i) for i, mi in enumerate(sizes)]
+    print("case: ragged batch", sizes, flush=True)
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=TOL, max_iterations=50000)
+    per, tot = b.solve()
+    print("batch", tot, flush=True)
+    assert tot["count"] == len(lps) and tot["solved"] >= 0.75 * len(lps)
+    for k, lp in enumerate(lps):
+        x, y = b.solution(k)
+        if per[k]["status"] != 0:
+            assert per[k]["status"] == 1 and per[k]["iterations"] == 50000
+            continue
+        kb = check_lp(po, lp, x, y, per[k], None, "lp%d" % k)
+        h = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=50000)
+        s = h.solve()
+        if s["status"] == 0:
+            xs, ys = h.solution()
+            ks = po.kkt(lp, xs, ys)
+            assert ks["rel_kkt"] <= TOL
+            assert abs(kb["primal_obj"] - ks["primal_obj"]) / (1 + abs(ks["primal_obj"])) <= 1e-5
+        h.close()
+
+
+def test_batch_deterministic_and_warm_start(pkg, po, lpdata):
+    lps = [lpdata.random_lp_sampled(mi, 3 * mi, 5, 7 + mi) for mi in (150, 400, 90, 260)]
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=TOL)
+    per1, tot1 = b.solve()
+    sol1 = [b.solution(k) for k in range(len(lps))]
+    per2, tot2 = b.solve()
+    sol2 = [b.solution(k) for k in range(len(lps))]
+    assert tot1["solved"] == len(lps)
+    for k in range(len(lps)):
+        assert per1[k]["iterations"] == per2[k]["iterations"]
+        assert np.array_equal(sol1[k][0], sol2[k][0]) and np.array_equal(sol1[k][1], sol2[k][1])
+    # warm start at the optimum: every LP stops at its first check
+    b.set_warm_start([s[0] for s in sol1], [s[1] for s in sol1])
+    per3, tot3 = b.solve()
+    assert tot3["solved"] == len(lps) and tot3["max_iterations"] == 0, tot3
+    for k in range(len(lps)):
+        x, y = b.solution(k)
+        assert po.kkt(lps[k], x, y)["rel_kkt"] <= TOL
+    b.set_warm_start(None, None)
+    per4, _ = b.solve()
+    assert [p["iterations"] for p in per4] == [p["iterations"] for p in per1]
+
+
+def test_batch_rejects_bad_lp(pkg, lpdata):
+    good = lpdata.random_lp(20, 50, 3, 1)
+    bad = lpdata.StdLP(good.m, good.n, good.row_ptr, good.col_idx.copy(), good.val, good.b, good.c, good.lb,
+                       good.ub, good.is_eq)
+    bad.col_idx[2] = good.n + 1
+    with pytest.raises(pkg.BoError) as e:
+        pkg.PdlpBatch([good, bad])
+    assert e.value.status == 4
