@@ -86,6 +86,22 @@ struct EpiStoreB {
     __device__ void operator()(int seg, const double* acc, double*) const { out[seg] = static_cast<TO>(acc[0]); }
 };
 
+// o0 = A v0, o1 = A v1 (the FP64 check products)
+struct EpiStore2 {
+    using TXv = double;
+    static constexpr int NV = 2;
+    static constexpr int Q = 0;
+    const double* v0;
+    const double* v1;
+    double* o0;
+    double* o1;
+    __device__ const double* vec(int v) const { return v ? v1 : v0; }
+    __device__ void operator()(int seg, const double* acc, double*) const {
+        o0[seg] = acc[0];
+        o1[seg] = acc[1];
+    }
+};
+
 // ---------------------------------------------------------------- elementwise
 __global__ void kb_fill_f64(double* p, double v, size_t n) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -225,29 +241,22 @@ kb_check_x(const Chunk* chunks, const PdlpState* st, const double* xk, const T* 
     }
 }
 
-// Row side, one thread per row (unscaled FP64 matrix, unscaled x).
-// partials per chunk: [0,1] ||viol||^2  [2,3] scaled  [4,5] b'y   (cur, cand)
+// Row side, from the products a0 = A xo_cur, a1 = A xo_cand (unscaled FP64
+// matrix).  partials per chunk: [0,1] ||viol||^2  [2,3] scaled  [4,5] b'y.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-kb_check_rows(const Chunk* chunks, const PdlpState* st, const int* rp, const int* ci, const double* rv,
-              const double* xo_cur, const double* xo_cand, const double* b, const unsigned char* eq, const double* rs,
-              const double* yk, const T* yp, double* yo_cur, double* yo_cand, double* ax_cand, double* part) {
+kb_check_rows(const Chunk* chunks, const PdlpState* st, const double* ar0, const double* ar1, const double* b,
+              const unsigned char* eq, const double* rs, const double* yk, const T* yp, double* yo_cur,
+              double* yo_cand, double* ax_cand, double* part) {
     __shared__ double sh[kBlock / kWave];
     const Chunk ch = chunks[blockIdx.x];
     const bool live = !st[ch.inst].done;
     const int i = ch.begin + static_cast<int>(threadIdx.x);
     double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (live && i < ch.end) {
-        double a0 = 0.0, a1 = 0.0;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const double v = rv[k];
-            const int j = ci[k];
-            a0 += v * xo_cur[j];
-            a1 += v * xo_cand[j];
-        }
         const double bi = b[i], ri = rs[i];
         const bool e = eq[i] != 0;
-        const double acc[2] = {a0, a1};
+        const double acc[2] = {ar0[i], ar1[i]};
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             const double r = acc[p] - bi;
@@ -255,7 +264,7 @@ kb_check_rows(const Chunk* chunks, const PdlpState* st, const int* rp, const int
             q[p] = v * v;
             q[2 + p] = (ri * v) * (ri * v);
         }
-        ax_cand[i] = ri * a1;
+        ax_cand[i] = ri * acc[1];
         double yc = ri * yk[i];
         if (!e && yc < 0.0) yc = 0.0;
         const double ya = ri * static_cast<double>(yp[i]);
@@ -271,28 +280,20 @@ kb_check_rows(const Chunk* chunks, const PdlpState* st, const int* rp, const int
     }
 }
 
-// Column side, one thread per column.
+// Column side, from the products a0 = A'yo_cur, a1 = A'yo_cand.
 // partials per chunk: [0,1] ||dual res||^2  [2,3] scaled  [4,5] sum l plam+ + u plam-
 __global__ void __launch_bounds__(kBlock)
-kb_check_cols(const Chunk* chunks, const PdlpState* st, const int* cp, const int* ri, const double* cv,
-              const double* yo_cur, const double* yo_cand, const double* c, const double* lb, const double* ub,
-              const double* cs, double* part) {
+kb_check_cols(const Chunk* chunks, const PdlpState* st, const double* ac0, const double* ac1, const double* c,
+              const double* lb, const double* ub, const double* cs, double* part) {
     __shared__ double sh[kBlock / kWave];
     const Chunk ch = chunks[blockIdx.x];
     const bool live = !st[ch.inst].done;
     const int j = ch.begin + static_cast<int>(threadIdx.x);
     double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (live && j < ch.end) {
-        double a0 = 0.0, a1 = 0.0;
-        for (int k = cp[j]; k < cp[j + 1]; ++k) {
-            const double v = cv[k];
-            const int i = ri[k];
-            a0 += v * yo_cur[i];
-            a1 += v * yo_cand[i];
-        }
         const double l = lb[j], u = ub[j], sj = cs[j];
         const bool lf = isfinite(l), uf = isfinite(u);
-        const double acc[2] = {a0, a1};
+        const double acc[2] = {ac0[j], ac1[j]};
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             const double lam = c[j] + acc[p];
@@ -491,6 +492,7 @@ struct Batch final : BatchBase {
     T *xp = nullptr, *yp = nullptr, *yg = nullptr, *pw_v = nullptr, *pw_u = nullptr;
     double *xo_cur = nullptr, *xo_cand = nullptr, *yo_cur = nullptr, *yo_cand = nullptr, *ax_cand = nullptr;
     double *wx = nullptr, *wy = nullptr;
+    double *ar0 = nullptr, *ar1 = nullptr, *ac0 = nullptr, *ac1 = nullptr;  // check products
     int has_warm = 0;
     // chunks
     Chunk *rch = nullptr, *cch = nullptr;
@@ -509,6 +511,7 @@ struct Batch final : BatchBase {
         bool built = false;
         int B = 0, W = 0, ntiles = 0, nslices = 0;
         long long pairs = 0;
+        int max_pairs = 0;  // most partials a segment adds
         size_t padded = 0;
         TileDesc* td = nullptr;
         unsigned* words = nullptr;
@@ -559,6 +562,7 @@ struct Batch final : BatchBase {
         t.nslices = static_cast<int>(h.slice_word.size());
         t.padded = h.padded;
         t.pairs = h.pairs;
+        t.max_pairs = h.max_seg_pairs;
         if ((e = mem.alloc(&t.td, h.tiles.size())) != hipSuccess) return e;
         if ((e = mem.alloc(&t.words, h.slice_word.size())) != hipSuccess) return e;
         if ((e = mem.alloc(&t.idx, h.padded)) != hipSuccess) return e;
@@ -664,7 +668,7 @@ struct Batch final : BatchBase {
         BO_TRY(mem.alloc(&rs, m));
         BO_TRY(mem.alloc(&rnorm, m));
         BO_TRY(mem.alloc(&b_hat, m));
-        for (double** v : {&yk, &y0, &ax, &ax0, &ax_cand, &yo_cur, &yo_cand, &wy}) BO_TRY(mem.alloc(v, m));
+        for (double** v : {&yk, &y0, &ax, &ax0, &ax_cand, &yo_cur, &yo_cand, &wy, &ar0, &ar1}) BO_TRY(mem.alloc(v, m));
         for (T** v : {&yp, &yg, &pw_u}) BO_TRY(mem.alloc(v, m));
         BO_TRY(mem.alloc(&c64, n));
         BO_TRY(mem.alloc(&lb64, n));
@@ -673,7 +677,7 @@ struct Batch final : BatchBase {
         BO_TRY(mem.alloc(&cs, n));
         BO_TRY(mem.alloc(&cnorm, n));
         for (T** v : {&c_hat, &l_hat, &u_hat, &xp, &pw_v}) BO_TRY(mem.alloc(v, n));
-        for (double** v : {&xk, &x0, &xo_cur, &xo_cand, &wx}) BO_TRY(mem.alloc(v, n));
+        for (double** v : {&xk, &x0, &xo_cur, &xo_cand, &wx, &ac0, &ac1}) BO_TRY(mem.alloc(v, n));
         BO_TRY(mem.alloc(&st, std::max(K, 1)));
         BO_TRY(mem.alloc(&running, 1));
         BO_TRY(up(rp, h_rp.data(), sizeof(int) * (m + 1)));
@@ -824,26 +828,20 @@ struct Batch final : BatchBase {
     }
     template <class Epi>
     void launch_reduce(const TilePass& tl, int nseg, const Epi& e) {
-        const int nslot = static_cast<int>(tl.pairs), npart = tl.nslices * kWave;
-        if (tl.B <= 4)
-            hipLaunchKernelGGL((k_tile_reduce<4, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
-                               tl.seg_ptr, tl.slot, nslot, tl.part, npart, e, nullptr);
-        else if (tl.B <= 8)
-            hipLaunchKernelGGL((k_tile_reduce<8, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
-                               tl.seg_ptr, tl.slot, nslot, tl.part, npart, e, nullptr);
-        else
-            hipLaunchKernelGGL((k_tile_reduce<16, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
-                               tl.seg_ptr, tl.slot, nslot, tl.part, npart, e, nullptr);
+        launch_tile_reduce(stream, tl.max_pairs, nseg, tl.seg_ptr, tl.slot, static_cast<int>(tl.pairs), tl.part,
+                           tl.nslices * kWave, e, nullptr);
     }
 
     void launch_check() {
         const unsigned gw = grid_for(static_cast<size_t>(std::max(K, 1)) * kWave);
         hipLaunchKernelGGL(kb_check_x<T>, dim3(ncch), dim3(kBlock), 0, stream, cch, st, xk, xp, cs, c64, lb64, ub64,
                            xo_cur, xo_cand, part_x);
-        hipLaunchKernelGGL(kb_check_rows<T>, dim3(nrch), dim3(kBlock), 0, stream, rch, st, rp, ci, rv64, xo_cur,
-                           xo_cand, b64, eq, rs, yk, yp, yo_cur, yo_cand, ax_cand, part_r);
-        hipLaunchKernelGGL(kb_check_cols, dim3(ncch), dim3(kBlock), 0, stream, cch, st, cp, ri, cv64, yo_cur, yo_cand,
-                           c64, lb64, ub64, cs, part_c);
+        rows<kSumProd>(rv64, EpiStore2{xo_cur, xo_cand, ar0, ar1});
+        hipLaunchKernelGGL(kb_check_rows<T>, dim3(nrch), dim3(kBlock), 0, stream, rch, st, ar0, ar1, b64, eq, rs, yk,
+                           yp, yo_cur, yo_cand, ax_cand, part_r);
+        cols<kSumProd>(cv64, EpiStore2{yo_cur, yo_cand, ac0, ac1});
+        hipLaunchKernelGGL(kb_check_cols, dim3(ncch), dim3(kBlock), 0, stream, cch, st, ac0, ac1, c64, lb64, ub64, cs,
+                           part_c);
         hipLaunchKernelGGL(kb_decide, dim3(gw), dim3(kBlock), 0, stream, K, st, cch_ptr, part_x, part_c, rch_ptr,
                            part_r);
         hipLaunchKernelGGL(kb_restart_x<T>, dim3(ncch), dim3(kBlock), 0, stream, cch, st, xk, x0, xp, part_dx);

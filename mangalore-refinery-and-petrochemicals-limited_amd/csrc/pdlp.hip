@@ -540,6 +540,7 @@ struct Engine final : EngineBase {
         int B = 0, W = 0, ntiles = 0, nslices = 0;
         unsigned lds_bytes = 0;  // dynamic LDS of the gather kernel
         long long pairs = 0;
+        int max_pairs = 0;  // most partials a segment adds
         size_t padded = 0;
         TileDesc* td = nullptr;
         unsigned* words = nullptr;  // slice words
@@ -773,6 +774,7 @@ struct Engine final : EngineBase {
         t.nslices = static_cast<int>(h.slice_word.size());
         t.padded = h.padded;
         t.pairs = h.pairs;
+        t.max_pairs = h.max_seg_pairs;
         if ((e = mem.alloc(&t.td, h.tiles.size())) != hipSuccess) return e;
         if ((e = mem.alloc(&t.words, h.slice_word.size())) != hipSuccess) return e;
         if ((e = mem.alloc(&t.idx, h.padded)) != hipSuccess) return e;
@@ -869,16 +871,8 @@ struct Engine final : EngineBase {
             hipLaunchKernelGGL((k_tile<T, kTileRows, PASS>), dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream,
                                t.td, t.words, t.val, t.idx, gvec, ng, t.W, t.part, nullptr);
         if (mid) hipEventRecord(mid, stream);
-        const int nslot = static_cast<int>(t.pairs), npart = t.nslices * kWave;
-        if (t.B <= 4)
-            hipLaunchKernelGGL((k_tile_reduce<4, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.seg_ptr,
-                               t.slot, nslot, t.part, npart, e, done);
-        else if (t.B <= 8)
-            hipLaunchKernelGGL((k_tile_reduce<8, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg, t.seg_ptr,
-                               t.slot, nslot, t.part, npart, e, done);
-        else
-            hipLaunchKernelGGL((k_tile_reduce<16, Epi>), dim3(grid_for(nseg)), dim3(kBlock), 0, stream, nseg,
-                               t.seg_ptr, t.slot, nslot, t.part, npart, e, done);
+        launch_tile_reduce(stream, t.max_pairs, nseg, t.seg_ptr, t.slot, static_cast<int>(t.pairs), t.part,
+                           t.nslices * kWave, e, done);
     }
 
     void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {

@@ -50,6 +50,7 @@ struct EpiPrimal {
     int t;             // step index within the interval
     const int* inst;   // nullptr: one LP
     __device__ const T* vec(int) const { return yg; }
+    __device__ bool skip(int j) const { return inst && st[inst[j]].done; }
     __device__ void operator()(int j, const double* acc, double*) const {
         const PdlpState* s = inst ? st + inst[j] : st;
         if (inst && s->done) return;
@@ -80,6 +81,7 @@ struct EpiDual {
     int t;
     const int* inst;   // nullptr: one LP
     __device__ const T* vec(int) const { return xp; }
+    __device__ bool skip(int i) const { return inst && st[inst[i]].done; }
     __device__ void operator()(int i, const double* acc, double*) const {
         const PdlpState* s = inst ? st + inst[i] : st;
         if (inst && s->done) return;

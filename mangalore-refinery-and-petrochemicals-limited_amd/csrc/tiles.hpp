@@ -67,6 +67,7 @@ struct HostTiles {
     size_t padded = 0;
     long long pairs = 0;
     int max_wave_slices = 0;
+    int max_seg_pairs = 0;              // most pairs any segment has (partials to add)
 };
 
 // Builds the tiled layout of a segment matrix (ptr over nseg segments, gidx
@@ -255,6 +256,7 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
     });
     h.slot.resize(pidx.size());
     for (size_t i = 0; i < pidx.size(); ++i) h.slot[i] = pair_slot[pidx[i]];
+    for (int s = 0; s < nseg; ++s) h.max_seg_pairs = std::max(h.max_seg_pairs, h.seg_ptr[s + 1] - h.seg_ptr[s]);
     return true;
 }
 
@@ -420,13 +422,14 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
 // Segment s: its partials in block order, then the pass epilogue.  The slot
 // and partial loads of up to U pairs are issued at once: raw buffer loads at
 // an out-of-range offset read 0, so no load sits under a per-lane branch.
+// An epilogue may skip a segment before anything is loaded (Epi::skip).
 template <int U, class Epi>
 __global__ void __launch_bounds__(kBlock)
 k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restrict__ slot, int nslot,
               const double* __restrict__ part, int npart, Epi epi, const int* __restrict__ done) {
     if (done && *done) return;
     const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= nseg) return;
+    if (s >= nseg || epi.skip(s)) return;
     constexpr unsigned kOut = 0x7FFFFFF0u;  // beyond every range below (ranges are < 2^31 bytes)
     const __amdgpu_buffer_rsrc_t rs = slice_rsrc(slot, nslot * 4);
     const __amdgpu_buffer_rsrc_t rp = slice_rsrc(part, npart * 8);
@@ -447,6 +450,28 @@ k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restr
     }
     double r[1] = {acc};
     epi(s, r, nullptr);
+}
+
+// Launch of the reduction with U covering the most pairs a segment has.
+template <class Epi>
+inline void launch_tile_reduce(hipStream_t stream, int max_pairs, int nseg, const int* seg_ptr, const unsigned* slot,
+                               int nslot, const double* part, int npart, const Epi& e, const int* done) {
+    const dim3 grid(grid_for(static_cast<size_t>(nseg))), block(kBlock);
+    if (max_pairs <= 1)
+        hipLaunchKernelGGL((k_tile_reduce<1, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
+                           e, done);
+    else if (max_pairs <= 2)
+        hipLaunchKernelGGL((k_tile_reduce<2, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
+                           e, done);
+    else if (max_pairs <= 4)
+        hipLaunchKernelGGL((k_tile_reduce<4, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
+                           e, done);
+    else if (max_pairs <= 8)
+        hipLaunchKernelGGL((k_tile_reduce<8, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
+                           e, done);
+    else
+        hipLaunchKernelGGL((k_tile_reduce<16, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
+                           e, done);
 }
 
 }  // namespace bo
