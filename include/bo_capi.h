@@ -242,6 +242,33 @@ bo_status bo_pdlp_batch_solve(struct bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo
 bo_status bo_pdlp_batch_get_solution(struct bo_pdlp_batch* h, int k, double* x, double* y);
 void bo_pdlp_batch_destroy(struct bo_pdlp_batch* h);
 
+/* ---- warm-start network (BF16 MLP on the matrix cores) -------------------- */
+/* No counterpart in the reference, which has no learned component.  A
+ * multilayer perceptron n_in -> width -> ... -> n_out (`layers` products,
+ * ReLU between them) with BF16 weights and activations, FP32 accumulation,
+ * FP32 master weights and Adam; it maps LP parameter changes to primal/dual
+ * warm starts for bo_pdlp_set_warm_start / bo_pdlp_batch_set_warm_start.
+ * All arrays are host memory, row-major, `batch` rows. */
+struct bo_mlp;
+bo_status bo_mlp_create(int n_in, int width, int n_out, int layers, int max_batch, unsigned long long seed,
+                        struct bo_mlp** out);
+/* Number of parameters: for each layer l, W_l (d_in x d_out, row-major) then b_l (d_out). */
+long long bo_mlp_num_params(const struct bo_mlp* h);
+bo_status bo_mlp_get_params(struct bo_mlp* h, float* params);
+bo_status bo_mlp_set_params(struct bo_mlp* h, const float* params); /* also resets Adam */
+bo_status bo_mlp_forward(struct bo_mlp* h, const float* x, int batch, float* y);
+/* Mean squared error of forward(x) against target, and its gradient (layout of the parameters). */
+bo_status bo_mlp_gradients(struct bo_mlp* h, const float* x, const float* target, int batch, double* loss,
+                           float* grads);
+/* One Adam step (beta 0.9 / 0.999) at learning rate lr; *loss before the step. */
+bo_status bo_mlp_train_step(struct bo_mlp* h, const float* x, const float* target, int batch, float lr,
+                            double* loss);
+void bo_mlp_destroy(struct bo_mlp* h);
+/* The product kernel alone: C[M x N] = op(A) op(B), BF16 operands given as raw
+ * 16-bit words, A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], FP32 C. */
+bo_status bo_gemm_bf16(int M, int N, int K, const unsigned short* A, long long sam, long long sak,
+                       const unsigned short* B, long long sbk, long long sbn, float* C);
+
 /* ---- full drop-in solve on a flattened LPModel ----------------------------- */
 /* LPModel (include/bharatopt/lp_model.hpp:60-127) as arrays: row-wise terms in
  * insertion order. con_sense: 0 LE, 1 EQ, 2 GE, 3 RANGED; var_type: 0 continuous,

@@ -130,6 +130,8 @@ EXPORTS = [
     "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
     "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels", "bo_pdlp_batch_create", "bo_pdlp_batch_set_warm_start",
     "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution", "bo_pdlp_batch_destroy",
+    "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
+    "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_destroy", "bo_gemm_bf16",
 ]
 
 
@@ -169,7 +171,7 @@ def lib():
         L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
         for name in EXPORTS:
             if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params",
-                            "bo_pdlp_batch_destroy"):
+                            "bo_pdlp_batch_destroy", "bo_mlp_destroy", "bo_mlp_num_params"):
                 getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -389,6 +391,104 @@ class PdlpBatch:
     def close(self):
         if self.handle:
             lib().bo_pdlp_batch_destroy(self.handle)
+            self.handle = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def f32_to_bf16_bits(a):
+    """float32 -> bfloat16 bit patterns (round to nearest even), as the kernels do."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    nan = np.isnan(np.asarray(a, np.float32))
+    if nan.any():
+        r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return r
+
+
+def bf16_bits_to_f32(b):
+    return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def gemm_bf16(a_bits, b_bits, trans_a=False, trans_b=False):
+    """C = op(A) op(B) on the matrix cores; A, B are 2-D uint16 BF16 bit arrays."""
+    a_bits = np.ascontiguousarray(a_bits, np.uint16)
+    b_bits = np.ascontiguousarray(b_bits, np.uint16)
+    if trans_a:
+        K, M = a_bits.shape
+        sam, sak = 1, M
+    else:
+        M, K = a_bits.shape
+        sam, sak = K, 1
+    if trans_b:
+        N, K2 = b_bits.shape
+        sbk, sbn = 1, K2
+    else:
+        K2, N = b_bits.shape
+        sbk, sbn = N, 1
+    assert K2 == K
+    c = np.zeros((M, N), np.float32)
+    ll = C.c_longlong
+    _check(lib().bo_gemm_bf16(M, N, K, a_bits.ctypes.data_as(P), ll(sam), ll(sak), b_bits.ctypes.data_as(P),
+                              ll(sbk), ll(sbn), _p(c)), "bo_gemm_bf16")
+    return c
+
+
+class Mlp:
+    """The warm-start network (bo_mlp_*)."""
+
+    def __init__(self, n_in, width, n_out, layers=4, max_batch=64, seed=1):
+        self.n_in, self.width, self.n_out, self.layers, self.max_batch = n_in, width, n_out, layers, max_batch
+        self.handle = P()
+        _check(lib().bo_mlp_create(n_in, width, n_out, layers, max_batch, C.c_ulonglong(seed),
+                                   C.byref(self.handle)), "bo_mlp_create")
+        lib().bo_mlp_num_params.restype = C.c_longlong
+        self.nparam = int(lib().bo_mlp_num_params(self.handle))
+
+    def shapes(self):
+        dims = [self.n_in] + [self.width] * (self.layers - 1) + [self.n_out]
+        return [(dims[l], dims[l + 1]) for l in range(self.layers)]
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        _check(lib().bo_mlp_get_params(self.handle, _p(p)), "bo_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparam
+        _check(lib().bo_mlp_set_params(self.handle, _p(p)), "bo_mlp_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        _check(lib().bo_mlp_forward(self.handle, _p(x), x.shape[0], _p(y)), "bo_mlp_forward")
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_double()
+        _check(lib().bo_mlp_gradients(self.handle, _p(x), _p(t), x.shape[0], C.byref(loss), _p(g)),
+               "bo_mlp_gradients")
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        _check(lib().bo_mlp_train_step(self.handle, _p(x), _p(t), x.shape[0], C.c_float(lr), C.byref(loss)),
+               "bo_mlp_train_step")
+        return loss.value
+
+    def close(self):
+        if self.handle:
+            lib().bo_mlp_destroy(self.handle)
             self.handle = P()
 
     def __del__(self):

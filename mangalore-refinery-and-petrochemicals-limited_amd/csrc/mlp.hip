@@ -1,0 +1,435 @@
+// mlp.hip -- the warm-start network: a BF16 multilayer perceptron (FP32
+// accumulation, FP32 master weights, Adam) trained and run by hand-written
+// kernels on the matrix cores.
+//
+// Layers: n_in -> width -> ... -> width -> n_out, ReLU between layers, no
+// activation on the output; loss = mean squared error.  Activations and
+// weights are BF16 in the products; every product accumulates in FP32.
+//
+// The product kernel (k_gemm) computes C[M x N] = op(A)[M x K] op(B)[K x N]
+// with v_mfma_f32_32x32x8_bf16: a 256-thread workgroup owns a 64 x 64 tile of
+// C, each wavefront a 32 x 32 quarter; K advances 32 at a time through LDS
+// (A and B tiles of 64 x 32 / 32 x 64 BF16).  Operand layout of the
+// instruction (lane l, 0..63): A row l % 32, k = 4 (l / 32) + 0..3; B column
+// l % 32, same k; result register r of lane l is C[8 (r / 4) + 4 (l / 32) +
+// r % 4][l % 32].  op(A) / op(B) read the stored matrix directly or
+// transposed (strides), so forward (X W), input gradients (dZ W') and weight
+// gradients (H' dZ) use the one kernel.  The epilogue adds a bias, applies
+// ReLU and writes FP32 and / or BF16.
+//
+// No counterpart exists in the reference (which has no learned component).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+
+namespace bo {
+
+bo_status ensure_device_public();
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bfloat16, round to nearest even (NaN kept quiet)
+__host__ __device__ inline unsigned short f2bf(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return static_cast<unsigned short>((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(unsigned short h) {
+    const unsigned u = static_cast<unsigned>(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+constexpr int kTm = 64, kTn = 64, kTk = 32;
+
+enum : int { kEpiNone = 0, kEpiBias = 1, kEpiRelu = 2 };
+
+// C = op(A) op(B) (+ bias, ReLU).  A(m,k) = A[m*sam + k*sak], B(k,n) =
+// B[k*sbk + n*sbn].  Out of range elements of the tiles are 0.
+template <int EPI>
+__global__ void __launch_bounds__(256)
+k_gemm(int M, int N, int K, const unsigned short* __restrict__ A, long long sam, long long sak,
+       const unsigned short* __restrict__ B, long long sbk, long long sbn, const float* __restrict__ bias,
+       float* __restrict__ C, unsigned short* __restrict__ Cb, long long ldc) {
+    __shared__ unsigned short As[kTm][kTk + 4];
+    __shared__ unsigned short Bs[kTk][kTn + 4];
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave, wave = tid / kWave;
+    const int bm = blockIdx.y * kTm, bn = blockIdx.x * kTn;
+    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += kTk) {
+        // 2048 elements of each tile, 8 per thread
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = tid + u * 256;
+            const int r = e / kTk, c = e % kTk;  // A tile: row r (m), column c (k)
+            const int m = bm + r, k = k0 + c;
+            As[r][c] = (m < M && k < K) ? A[m * sam + k * sak] : static_cast<unsigned short>(0);
+            const int rb = e / kTn, cb = e % kTn;  // B tile: row rb (k), column cb (n)
+            const int kb = k0 + rb, n = bn + cb;
+            Bs[rb][cb] = (kb < K && n < N) ? B[kb * sbk + n * sbn] : static_cast<unsigned short>(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kTk; kk += 8) {
+            bf16x4 a, b;
+            const int kq = kk + 4 * (lane / 32);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[j] = static_cast<short>(As[wm + lane % 32][kq + j]);
+                b[j] = static_cast<short>(Bs[kq + j][wn + lane % 32]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = bm + wm + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        const int col = bn + wn + lane % 32;
+        if (row < M && col < N) {
+            float v = acc[r];
+            if (EPI & kEpiBias) v += bias[col];
+            if (EPI & kEpiRelu) v = v > 0.0f ? v : 0.0f;
+            if (C) C[row * ldc + col] = v;
+            if (Cb) Cb[row * ldc + col] = f2bf(v);
+        }
+    }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, unsigned short* __restrict__ y, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// dY = 2 (Y - T) / count (FP32 and BF16); loss partial sums per block
+__global__ void __launch_bounds__(kBlock)
+k_mse(const float* __restrict__ y, const float* __restrict__ t, size_t n, float scale, unsigned short* __restrict__ dyb,
+      double* __restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (i < n) {
+        const float d = y[i] - t[i];
+        dyb[i] = f2bf(2.0f * d * scale);
+        l = static_cast<double>(d) * d;
+    }
+    const double s = block_sum(l, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_sum(const double* __restrict__ part, int n, double scale, double* __restrict__ out) {
+    __shared__ double sh[kBlock / kWave];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
+    const double s = block_sum(a, sh);
+    if (threadIdx.x == 0) *out = s * scale;
+}
+
+// dZ = dH * (H > 0), as BF16 for the next products
+__global__ void k_relu_bwd(const float* __restrict__ dh, const unsigned short* __restrict__ h,
+                           unsigned short* __restrict__ dz, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dz[i] = bf2f(h[i]) > 0.0f ? f2bf(dh[i]) : static_cast<unsigned short>(0);
+}
+
+// db[n] = sum over the batch of dZ[b][n] (BF16 values, FP32 sum in batch order)
+__global__ void k_colsum(const unsigned short* __restrict__ dz, int B, int N, float* __restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int b = 0; b < B; ++b) s += bf2f(dz[static_cast<size_t>(b) * N + n]);
+    db[n] = s;
+}
+
+// Adam (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 master weights, BF16 copy out
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, unsigned short* __restrict__ wb, size_t n, float lr, float bc1,
+                       float bc2) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+struct DevMemM {
+    std::vector<void*> ptrs;
+    ~DevMemM() {
+        for (void* p : ptrs) hipFree(p);
+    }
+    template <class U>
+    hipError_t alloc(U** p, size_t count) {
+        *p = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(U) * (count ? count : 1));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+}  // namespace
+}  // namespace bo
+
+using namespace bo;
+
+struct bo_mlp {
+    int layers = 0, n_in = 0, width = 0, n_out = 0, max_batch = 0;
+    std::vector<int> din, dout;  // per layer
+    std::vector<size_t> woff, boff;  // offsets of W_l (din x dout) and b_l in the flat parameter array
+    size_t nparam = 0;
+    long long steps = 0;
+    hipStream_t stream = nullptr;
+    DevMemM mem;
+    float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 master, Adam moments, gradients
+    unsigned short* wb = nullptr;                                    // BF16 copy of w (weights only used)
+    // activations (max_batch rows): x, hidden BF16 per layer, output FP32
+    unsigned short* xb = nullptr;
+    std::vector<unsigned short*> hb;  // layers - 1 hidden activations (BF16, post-ReLU)
+    float *y = nullptr, *t = nullptr, *xf = nullptr;
+    float* dh = nullptr;                // FP32 input gradient of the current layer (max_batch x max dim)
+    unsigned short *dzb = nullptr, *dzb2 = nullptr;  // BF16 gradients (ping-pong)
+    double *part = nullptr, *loss = nullptr;
+
+    ~bo_mlp() {
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    template <int EPI>
+    void gemm(int M, int N, int K, const unsigned short* A, long long sam, long long sak, const unsigned short* B,
+              long long sbk, long long sbn, const float* bias, float* C, unsigned short* Cb, long long ldc) {
+        dim3 grid((N + kTn - 1) / kTn, (M + kTm - 1) / kTm);
+        hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, stream, M, N, K, A, sam, sak, B, sbk, sbn, bias, C, Cb,
+                           ldc);
+    }
+
+    bo_status create(int nin, int wdt, int nout, int nl, int maxb, unsigned long long seed) {
+        layers = nl;
+        n_in = nin;
+        width = wdt;
+        n_out = nout;
+        max_batch = maxb;
+        for (int l = 0; l < layers; ++l) {
+            din.push_back(l == 0 ? n_in : width);
+            dout.push_back(l == layers - 1 ? n_out : width);
+        }
+        for (int l = 0; l < layers; ++l) {
+            woff.push_back(nparam);
+            nparam += static_cast<size_t>(din[l]) * dout[l];
+            boff.push_back(nparam);
+            nparam += dout[l];
+        }
+        BO_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        BO_TRY(mem.alloc(&w, nparam));
+        BO_TRY(mem.alloc(&m, nparam));
+        BO_TRY(mem.alloc(&v, nparam));
+        BO_TRY(mem.alloc(&g, nparam));
+        BO_TRY(mem.alloc(&wb, nparam));
+        const size_t maxd = static_cast<size_t>(std::max(std::max(n_in, n_out), width));
+        BO_TRY(mem.alloc(&xf, static_cast<size_t>(maxb) * n_in));
+        BO_TRY(mem.alloc(&xb, static_cast<size_t>(maxb) * n_in));
+        hb.assign(layers - 1, nullptr);
+        for (int l = 0; l + 1 < layers; ++l) BO_TRY(mem.alloc(&hb[l], static_cast<size_t>(maxb) * width));
+        BO_TRY(mem.alloc(&y, static_cast<size_t>(maxb) * n_out));
+        BO_TRY(mem.alloc(&t, static_cast<size_t>(maxb) * n_out));
+        BO_TRY(mem.alloc(&dh, static_cast<size_t>(maxb) * maxd));
+        BO_TRY(mem.alloc(&dzb, static_cast<size_t>(maxb) * maxd));
+        BO_TRY(mem.alloc(&dzb2, static_cast<size_t>(maxb) * maxd));
+        BO_TRY(mem.alloc(&part, grid_for(static_cast<size_t>(maxb) * n_out)));
+        BO_TRY(mem.alloc(&loss, 1));
+        // He-uniform initialisation of weights, zero biases (host, seeded)
+        std::vector<float> hw(nparam, 0.0f);
+        std::mt19937_64 rng(seed);
+        for (int l = 0; l < layers; ++l) {
+            const float a = std::sqrt(6.0f / static_cast<float>(din[l]));
+            std::uniform_real_distribution<float> U(-a, a);
+            for (size_t i = 0; i < static_cast<size_t>(din[l]) * dout[l]; ++i) hw[woff[l] + i] = U(rng);
+        }
+        return set_params(hw.data());
+    }
+
+    bo_status set_params(const float* hw) {
+        BO_TRY(hipMemcpyAsync(w, hw, sizeof(float) * nparam, hipMemcpyHostToDevice, stream));
+        BO_TRY(hipMemsetAsync(m, 0, sizeof(float) * nparam, stream));
+        BO_TRY(hipMemsetAsync(v, 0, sizeof(float) * nparam, stream));
+        hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(nparam)), dim3(kBlock), 0, stream, w, wb, nparam);
+        BO_TRY(hipStreamSynchronize(stream));
+        steps = 0;
+        return BO_SUCCESS;
+    }
+
+    void forward(int B) {
+        const unsigned short* in = xb;
+        for (int l = 0; l < layers; ++l) {
+            const int K = din[l], N = dout[l];
+            const unsigned short* W = wb + woff[l];
+            const float* bias = w + boff[l];  // FP32 master bias
+            if (l + 1 < layers)
+                gemm<kEpiBias | kEpiRelu>(B, N, K, in, K, 1, W, N, 1, bias, nullptr, hb[l], N);
+            else
+                gemm<kEpiBias>(B, N, K, in, K, 1, W, N, 1, bias, y, nullptr, N);
+            if (l + 1 < layers) in = hb[l];
+        }
+    }
+
+    // gradients of the mean squared error into g; loss into *loss
+    void backward(int B) {
+        const size_t ny = static_cast<size_t>(B) * n_out;
+        const unsigned nb = grid_for(ny);
+        hipLaunchKernelGGL(k_mse, dim3(nb), dim3(kBlock), 0, stream, y, t, ny, 1.0f / static_cast<float>(ny), dzb,
+                           part);
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(kBlock), 0, stream, part, static_cast<int>(nb),
+                           1.0 / static_cast<double>(ny), loss);
+        unsigned short* dz = dzb;       // gradient w.r.t. the pre-activation of layer l (BF16, B x dout)
+        unsigned short* other = dzb2;
+        for (int l = layers - 1; l >= 0; --l) {
+            const int K = din[l], N = dout[l];
+            const unsigned short* Hin = l == 0 ? xb : hb[l - 1];
+            // dW = Hin' dZ   (K x N), db = colsum dZ
+            gemm<kEpiNone>(K, N, B, Hin, 1, K, dz, N, 1, nullptr, g + woff[l], nullptr, N);
+            hipLaunchKernelGGL(k_colsum, dim3(grid_for(N)), dim3(kBlock), 0, stream, dz, B, N, g + boff[l]);
+            if (l == 0) break;
+            // dH = dZ W'   (B x K), then through the ReLU of layer l-1
+            gemm<kEpiNone>(B, K, N, dz, N, 1, wb + woff[l], 1, N, nullptr, dh, nullptr, K);
+            hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for(static_cast<size_t>(B) * K)), dim3(kBlock), 0, stream, dh,
+                               hb[l - 1], other, static_cast<size_t>(B) * K);
+            std::swap(dz, other);
+        }
+    }
+
+    void adam(float lr) {
+        ++steps;
+        const float bc1 = 1.0f - std::pow(0.9f, static_cast<float>(steps));
+        const float bc2 = 1.0f - std::pow(0.999f, static_cast<float>(steps));
+        hipLaunchKernelGGL(k_adam, dim3(grid_for(nparam)), dim3(kBlock), 0, stream, w, m, v, g, wb, nparam, lr, bc1,
+                           bc2);
+    }
+
+    bo_status upload_x(const float* x, int B) {
+        BO_TRY(hipMemcpyAsync(xf, x, sizeof(float) * B * n_in, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(static_cast<size_t>(B) * n_in)), dim3(kBlock), 0, stream, xf, xb,
+                           static_cast<size_t>(B) * n_in);
+        return BO_SUCCESS;
+    }
+};
+
+extern "C" {
+
+bo_status bo_mlp_create(int n_in, int width, int n_out, int layers, int max_batch, unsigned long long seed,
+                        bo_mlp** out) {
+    if (!out) return BO_INVALID_ARGUMENT;
+    *out = nullptr;
+    bo_status s = ensure_device_public();
+    if (s != BO_SUCCESS) return s;
+    if (n_in <= 0 || width <= 0 || n_out <= 0 || layers < 2 || max_batch <= 0) return BO_INVALID_ARGUMENT;
+    if (static_cast<long long>(max_batch) * std::max(std::max(n_in, n_out), width) >= (1LL << 31))
+        return BO_INVALID_ARGUMENT;
+    auto h = std::make_unique<bo_mlp>();
+    s = h->create(n_in, width, n_out, layers, max_batch, seed);
+    if (s != BO_SUCCESS) return s;
+    *out = h.release();
+    return BO_SUCCESS;
+}
+
+long long bo_mlp_num_params(const bo_mlp* h) { return h ? static_cast<long long>(h->nparam) : 0; }
+
+bo_status bo_mlp_get_params(bo_mlp* h, float* params) {
+    if (!h || !params) return BO_INVALID_ARGUMENT;
+    BO_TRY(hipMemcpyAsync(params, h->w, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
+    BO_TRY(hipStreamSynchronize(h->stream));
+    return BO_SUCCESS;
+}
+
+bo_status bo_mlp_set_params(bo_mlp* h, const float* params) {
+    if (!h || !params) return BO_INVALID_ARGUMENT;
+    return h->set_params(params);
+}
+
+bo_status bo_mlp_forward(bo_mlp* h, const float* x, int batch, float* y) {
+    if (!h || !x || !y || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
+    bo_status s = h->upload_x(x, batch);
+    if (s != BO_SUCCESS) return s;
+    h->forward(batch);
+    BO_TRY(hipMemcpyAsync(y, h->y, sizeof(float) * batch * h->n_out, hipMemcpyDeviceToHost, h->stream));
+    BO_TRY(hipStreamSynchronize(h->stream));
+    BO_TRY(hipGetLastError());
+    return BO_SUCCESS;
+}
+
+bo_status bo_mlp_gradients(bo_mlp* h, const float* x, const float* target, int batch, double* loss, float* grads) {
+    if (!h || !x || !target || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
+    bo_status s = h->upload_x(x, batch);
+    if (s != BO_SUCCESS) return s;
+    BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
+    h->forward(batch);
+    h->backward(batch);
+    if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (grads) BO_TRY(hipMemcpyAsync(grads, h->g, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
+    BO_TRY(hipStreamSynchronize(h->stream));
+    BO_TRY(hipGetLastError());
+    return BO_SUCCESS;
+}
+
+bo_status bo_mlp_train_step(bo_mlp* h, const float* x, const float* target, int batch, float lr, double* loss) {
+    if (!h || !x || !target || batch <= 0 || batch > h->max_batch || !(lr > 0.0f)) return BO_INVALID_ARGUMENT;
+    bo_status s = h->upload_x(x, batch);
+    if (s != BO_SUCCESS) return s;
+    BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
+    h->forward(batch);
+    h->backward(batch);
+    h->adam(lr);
+    if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    BO_TRY(hipStreamSynchronize(h->stream));
+    BO_TRY(hipGetLastError());
+    return BO_SUCCESS;
+}
+
+bo_status bo_gemm_bf16(int M, int N, int K, const unsigned short* A, long long sam, long long sak,
+                       const unsigned short* B, long long sbk, long long sbn, float* C) {
+    // host arrays; test and timing entry point of the product kernel
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return BO_INVALID_ARGUMENT;
+    bo_status s = ensure_device_public();
+    if (s != BO_SUCCESS) return s;
+    const long long na = (M - 1) * sam + (K - 1) * sak + 1, nb = (K - 1) * sbk + (N - 1) * sbn + 1;
+    unsigned short *dA = nullptr, *dB = nullptr;
+    float* dC = nullptr;
+    BO_TRY(hipMalloc(reinterpret_cast<void**>(&dA), sizeof(unsigned short) * na));
+    BO_TRY(hipMalloc(reinterpret_cast<void**>(&dB), sizeof(unsigned short) * nb));
+    BO_TRY(hipMalloc(reinterpret_cast<void**>(&dC), sizeof(float) * M * N));
+    BO_TRY(hipMemcpy(dA, A, sizeof(unsigned short) * na, hipMemcpyHostToDevice));
+    BO_TRY(hipMemcpy(dB, B, sizeof(unsigned short) * nb, hipMemcpyHostToDevice));
+    dim3 grid((N + kTn - 1) / kTn, (M + kTm - 1) / kTm);
+    hipLaunchKernelGGL(k_gemm<kEpiNone>, grid, dim3(256), 0, 0, M, N, K, dA, sam, sak, dB, sbk, sbn, nullptr, dC,
+                       nullptr, static_cast<long long>(N));
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(C, dC, sizeof(float) * M * N, hipMemcpyDeviceToHost);
+    hipFree(dA);
+    hipFree(dB);
+    hipFree(dC);
+    BO_TRY(e);
+    return BO_SUCCESS;
+}
+
+void bo_mlp_destroy(bo_mlp* h) { delete h; }
+
+}  // extern "C"
