@@ -10,7 +10,7 @@ with FP32 iterates until the FP64 rel-KKT of the unscaled problem is <= 1e-6,
 and the FP64 solution ready on the device (the host copy follows, untimed).
 An LP counts only if its FP64 rel-KKT <= 1e-6.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2|c3|c4]
                        [--no-cpu-baseline]
 Prints one JSON line.
 """
@@ -30,6 +30,7 @@ import __graft_entry__  # noqa: E402
 
 METRIC = "LPs solved/sec to rel-KKT 1e-6, warm vs cold PDHG, 1 GPU; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak
 
 WORKLOADS = {
     # name: (description, builder)
@@ -195,12 +196,165 @@ def bench_batch(pkg, args):
     print(json.dumps(out), flush=True)
 
 
+C4_FAMILY = dict(periods=12, blk_rows=2000, blk_cols=5000, inv_per_link=200, seed=42)
+
+
+def mlp_forward_roofline(net, batch, reps=20):
+    """Time of the network's forward pass (HIP events) and its roofline: FLOPs
+    2 B sum(d_in d_out); bytes = BF16 weights + FP32 biases + BF16
+    activations in and out of every layer.  At B = 64 the intensity is far
+    below the matrix cores' ridge (peak FLOP/s / peak B/s), so HBM bounds it."""
+    shapes = net.shapes()
+    flops = 2.0 * batch * sum(a * b for a, b in shapes)
+    nbytes = sum(2.0 * a * b + 4.0 * b + 2.0 * batch * (a + b) for a, b in shapes)
+    ms = net.time(0, batch, reps)
+    ridge = BF16_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    bound = "hbm" if flops / nbytes < ridge else "mfma"
+    if bound == "hbm":
+        ach, peak, unit = nbytes / (ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        ach, peak, unit = flops / (ms * 1e-3) / 1e12, BF16_PEAK_TFLOPS, "TFLOP/s"
+    return {"bound": bound, "kernel": "k_gemm + k_splitk_sum (network forward, batch %d)" % batch,
+            "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
+            "launch_ms": ms, "flops": flops, "algorithmic_bytes": nbytes,
+            "tflops": flops / (ms * 1e-3) / 1e12, "intensity_flop_per_byte": flops / nbytes}
+
+
+def bench_c4(pkg, args):
+    """Learned warm start over the parametric staircase family: training
+    data from cold batched solves, the network trained on the GPU, then the
+    held-out draws solved cold, from the network's warm start (inference time
+    included, cold fallback for any draw it leaves unsolved) and from the
+    training mean (the ablation without the network), in batches."""
+    ws = pkg.warmstart
+    t0 = time.perf_counter()
+    fam = pkg.lpdata.StaircaseFamily(**C4_FAMILY)
+    desc = ("staircase family periods=12 block=2000x5000 inv=200 seed=42; draw k: c = c0(1+0.2N(0,1)), "
+            "b = b0(1+U[-0.1,0.1]) from default_rng(k)")
+    log(f"workload c4: {desc}; m={fam.m} n={fam.n} nnz={fam.base.nnz}; built in {time.perf_counter() - t0:.1f}s")
+    params = dict(eps_rel_kkt=args.eps, max_iterations=args.max_iterations, use_graph=0 if args.no_graph else 1)
+    fs = ws.FamilySolver(pkg, fam, batch=args.c4_batch, **params)
+    train_ids = list(range(1, 1 + args.train_draws))
+    eval_ids = list(range(20001, 20001 + args.eval_draws))
+    for w in range(args.warmup):  # builds the batch layout, warms the kernels
+        r = fs.solve(eval_ids[:args.c4_batch], want_solutions=False)
+        log(f"warmup {w}: {r['solve_s']:.2f}s, solved {(r['status'] == 0).sum()}")
+
+    t0 = time.perf_counter()
+    tr = fs.solve(train_ids)
+    gen_s = time.perf_counter() - t0
+    ok = [i for i, st in enumerate(tr["status"]) if st == 0]
+    log(f"training data: {len(ok)}/{len(train_ids)} draws solved, {gen_s:.1f}s")
+    F = np.array([fam.features(train_ids[i]) for i in ok])
+    net = ws.WarmStartNet(pkg, fam.n, fam.m, width=1024, layers=4, max_batch=args.c4_batch, seed=1)
+    hist = net.fit(F, np.array([tr["x"][i] for i in ok]), np.array([tr["y"][i] for i in ok]),
+                   epochs=args.epochs, lr=args.lr, log=log)
+    del F, tr
+    log(f"training: val loss {hist['val'][0]:.4f} -> {hist['best_val']:.4f} (epoch {hist['best_epoch']}), "
+        f"{hist['seconds']:.1f}s")
+
+    cold = fs.solve(eval_ids)
+    log(f"cold: {cold['solve_s']:.2f}s solved {(cold['status'] == 0).sum()}")
+    Fe = np.array([fam.features(k) for k in eval_ids])
+    t0 = time.perf_counter()
+    Xn, Yn = net.predict(Fe)
+    infer_s = time.perf_counter() - t0
+    warm = fs.solve_warm_with_fallback(eval_ids, (Xn, Yn))
+    warm_s = warm["solve_s"] + infer_s
+    log(f"network warm: {warm_s:.2f}s (inference {infer_s:.2f}s) solved {(warm['status'] == 0).sum()} "
+        f"fallbacks {len(warm['fallback'])}")
+    Xm, Ym = net.mean_start(len(eval_ids))
+    mean = fs.solve_warm_with_fallback(eval_ids, (Xm, Ym))
+    log(f"mean warm: {mean['solve_s']:.2f}s solved {(mean['status'] == 0).sum()}")
+    fs.close()
+
+    # plain cold single-LP engine on a sample of the held-out draws
+    single_s, single_ok = 0.0, 0
+    for k in eval_ids[:args.single_sample]:
+        h = pkg.Pdlp(fam.draw(k), **params)
+        t0 = time.perf_counter()
+        s = h.solve()
+        single_s += time.perf_counter() - t0
+        single_ok += int(s["status"] == 0)
+        h.close()
+
+    ok = [i for i in range(len(eval_ids)) if cold["status"][i] == 0]  # distances to verified optima
+    X, Y = [cold["x"][i] for i in ok], [cold["y"][i] for i in ok]
+    zero = [np.clip(np.zeros(fam.n), fam.base.lb, fam.base.ub)] * len(ok)
+    dist = {"cold": ws.relative_distance(zero, [np.zeros(fam.m)] * len(ok), X, Y),
+            "mean": ws.relative_distance(Xm[ok], Ym[ok], X, Y), "network": ws.relative_distance(Xn[ok], Yn[ok], X, Y)}
+
+    def pct(v):
+        return [float(np.percentile(v, 50)), float(np.percentile(v, 90)), int(np.max(v))]
+
+    solved = {k: int((r["status"] == 0).sum()) for k, r in (("cold", cold), ("network", warm), ("mean", mean))}
+    rate = {"cold_batched": solved["cold"] / cold["solve_s"], "network_warm_batched": solved["network"] / warm_s,
+            "mean_warm_batched": solved["mean"] / mean["solve_s"],
+            "cold_single": single_ok / single_s if single_s > 0 else None}
+    mlp_roof = mlp_forward_roofline(net.net, args.c4_batch)
+    net.close()
+    out = {
+        "metric": METRIC,
+        "value": rate["network_warm_batched"],
+        "unit": "LPs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": warm_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": desc, "m": fam.m, "n": fam.n, "nnz": fam.base.nnz, "eps_rel_kkt": args.eps,
+                   "batch": args.c4_batch, "train_draws": len(train_ids), "eval_draws": len(eval_ids),
+                   "network": "BF16 MLP %d-1024-1024-1024-%d, FP32 accumulate, Adam" % (fam.n + fam.m, fam.n + fam.m),
+                   "iterates": "fp32", "checks": "fp64", "start": "learned warm start, cold fallback",
+                   "parallelism": "single-gpu"},
+        "step": "one pass over the held-out draws in batches (network inference + warm-started solves + fallbacks)",
+        "lps_per_s": rate,
+        "speedup": {"network_warm_batched_over_cold_batched": rate["network_warm_batched"] / rate["cold_batched"],
+                    "network_warm_batched_over_mean_warm_batched":
+                        rate["network_warm_batched"] / rate["mean_warm_batched"],
+                    "network_warm_batched_over_cold_single":
+                        rate["network_warm_batched"] / rate["cold_single"] if rate["cold_single"] else None},
+        "solved": solved,
+        "fallbacks": {"network": len(warm["fallback"]), "mean": len(mean["fallback"])},
+        "iterations_p50_p90_max": {"cold": pct(cold["iterations"]), "network": pct(warm["iterations"]),
+                                   "mean": pct(mean["iterations"])},
+        "start_distance_median_rel": dist,
+        "single_sample": args.single_sample,
+        "training": {"draws_solved": len(ok), "generation_s": gen_s, "epochs": args.epochs, "lr": args.lr,
+                     "val_loss_untrained": hist["val"][0], "val_loss_best": hist["best_val"],
+                     "best_epoch": hist["best_epoch"], "train_s": hist["seconds"]},
+        "inference_s": infer_s,
+        "roofline": mlp_roof,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (reference CPU PDHG, bounded sample) ...")
+        out["cpu_baseline"] = cpu_baseline(fam.draw(eval_ids[0]), int(np.median(cold["iterations"])))
+        from oracle import pyoracle as po
+        worst = 0.0
+        for i, k in enumerate(eval_ids):
+            if warm["status"][i] == 0:
+                worst = max(worst, po.kkt(fam.draw(k), warm["x"][i], warm["y"][i])["rel_kkt"])
+        out["checker_worst_rel_kkt_network_warm"] = worst
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS))
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS) + ["c4"])
+    ap.add_argument("--train-draws", type=int, default=2048, help="c4: training draws (seeds 1..)")
+    ap.add_argument("--eval-draws", type=int, default=1000, help="c4: held-out draws (seeds 20001..)")
+    ap.add_argument("--epochs", type=int, default=20, help="c4: training epochs")
+    ap.add_argument("--lr", type=float, default=1e-4, help="c4: Adam learning rate")
+    ap.add_argument("--c4-batch", type=int, default=64, help="c4: LPs per batched solve")
+    ap.add_argument("--single-sample", type=int, default=16,
+                    help="c4: held-out draws also solved one at a time by the single-LP engine")
     ap.add_argument("--max-iterations", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eps", type=float, default=1e-6)
@@ -227,6 +381,8 @@ def main():
         raise SystemExit("no GPU visible: " + info.status_message.decode())
     if args.workload in BATCH_WORKLOADS:
         return bench_batch(pkg, args)
+    if args.workload == "c4":
+        return bench_c4(pkg, args)
     desc, build = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     lp = build(pkg.lpdata)

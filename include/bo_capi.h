@@ -150,10 +150,16 @@ typedef struct bo_pdlp_params {
     double time_limit_s;       /* 0 = none */
     int verbose;
     int hot_columns;           /* entries of x+ kept in LDS by the CSR pass: -1 tune, 0 off */
+    long long stall_iterations; /* batched engine, FP32 iterates: an LP whose FP64 rel-KKT has not
+                                   halved within this many iterations stops as STALLED (4);
+                                   0 = never (default 4096) */
+    int fp64_polish;           /* batched engine: finish every STALLED LP with FP64 iterates from
+                                  its FP32 point, within its remaining iteration budget (default 1) */
 } bo_pdlp_params;
 
 typedef struct bo_pdlp_stats {
-    int status; /* 0 OPTIMAL (rel-KKT met), 1 MAX_ITERATIONS, 2 NUMERICAL_FAILURE, 3 TIME_LIMIT */
+    int status; /* 0 OPTIMAL (rel-KKT met), 1 MAX_ITERATIONS, 2 NUMERICAL_FAILURE, 3 TIME_LIMIT,
+                   4 STALLED (FP32 iterates stopped improving; see fp64_polish) */
     long long iterations;
     int restarts;
     int used_average;
@@ -172,6 +178,7 @@ typedef struct bo_pdlp_stats {
     int lanes_rows;        /* schedule used by the CSR pass (encoding of bo_pdlp_params) */
     int lanes_cols;        /* ... and by the CSC pass */
     int hot_columns;       /* LDS-cached entries of x+ in the CSR pass (0 = none) */
+    long long fp64_polish_iterations; /* of `iterations`, those run by the FP64 polish */
 } bo_pdlp_stats;
 
 /* Default parameters. */
@@ -235,6 +242,15 @@ bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params
 /* Warm start for every LP: x and y concatenated in batch order (sum n, sum m
  * entries, original space); NULL, NULL clears it. */
 bo_status bo_pdlp_batch_set_warm_start(struct bo_pdlp_batch* h, const double* x, const double* y);
+/* Starting primal weight of every LP (count entries, > 0; 0 keeps the default
+ * ||c_hat|| / ||b_hat|| of the scaled LP); NULL clears them.  Part of a warm
+ * start: the weight a solve of a similar LP ended with (bo_pdlp_stats.primal_weight). */
+bo_status bo_pdlp_batch_set_primal_weights(struct bo_pdlp_batch* h, const double* w);
+/* New costs c (sum n entries) and / or right-hand sides b (sum m entries) for
+ * every LP, batch order, original space; NULL keeps that part.  The matrices,
+ * bounds, row senses and the device layout stay, so a parametric family of
+ * LPs over one matrix is re-solved without building the batch again. */
+bo_status bo_pdlp_batch_set_objective_rhs(struct bo_pdlp_batch* h, const double* c, const double* b);
 /* per_lp (count entries, may be NULL): the stats of every LP, as a single solve reports
  * them; their solve_ms is the batch's wall time. */
 bo_status bo_pdlp_batch_solve(struct bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total);
@@ -263,6 +279,10 @@ bo_status bo_mlp_gradients(struct bo_mlp* h, const float* x, const float* target
 /* One Adam step (beta 0.9 / 0.999) at learning rate lr; *loss before the step. */
 bo_status bo_mlp_train_step(struct bo_mlp* h, const float* x, const float* target, int batch, float lr,
                             double* loss);
+/* Mean time (ms, HIP events on the network's stream) of one forward pass
+ * (what 0) or forward + backward pass (what 1) over `batch` rows of the
+ * device-resident input (the last one uploaded, zeros at first). */
+bo_status bo_mlp_time(struct bo_mlp* h, int what, int batch, int reps, double* ms);
 void bo_mlp_destroy(struct bo_mlp* h);
 /* The product kernel alone: C[M x N] = op(A) op(B), BF16 operands given as raw
  * 16-bit words, A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], FP32 C. */

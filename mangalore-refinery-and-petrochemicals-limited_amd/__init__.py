@@ -18,6 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import lpdata  # noqa: F401  (re-exported for callers)
+from . import warmstart  # noqa: F401
 from .lpdata import FlatModel, StdLP
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -75,7 +76,7 @@ class PdlpParams(C.Structure):
                 ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("check_every", C.c_int),
                 ("lanes_rows", C.c_int), ("lanes_cols", C.c_int), ("fp64_iterates", C.c_int),
                 ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int),
-                ("hot_columns", C.c_int)]
+                ("hot_columns", C.c_int), ("stall_iterations", C.c_longlong), ("fp64_polish", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -84,7 +85,7 @@ class PdlpStats(C.Structure):
                     "rel_kkt", "rel_primal", "rel_dual", "rel_gap", "primal_obj", "dual_obj",
                     "primal_weight", "step_size", "setup_ms", "solve_ms", "spmv_csr_bytes",
                     "spmv_csc_bytes")] + [("lanes_rows", C.c_int), ("lanes_cols", C.c_int),
-                                          ("hot_columns", C.c_int)]
+                                          ("hot_columns", C.c_int), ("fp64_polish_iterations", C.c_longlong)]
 
 
 class PdlpPassInfo(C.Structure):
@@ -129,9 +130,10 @@ EXPORTS = [
     "bo_pdlp_set_warm_start", "bo_pdlp_solve", "bo_pdlp_get_solution", "bo_pdlp_destroy",
     "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
     "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels", "bo_pdlp_batch_create", "bo_pdlp_batch_set_warm_start",
-    "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution", "bo_pdlp_batch_destroy",
+    "bo_pdlp_batch_set_primal_weights",
+    "bo_pdlp_batch_set_objective_rhs", "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution", "bo_pdlp_batch_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
-    "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_destroy", "bo_gemm_bf16",
+    "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_time", "bo_mlp_destroy", "bo_gemm_bf16",
 ]
 
 
@@ -374,6 +376,23 @@ class PdlpBatch:
         y = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in ys]))
         _check(lib().bo_pdlp_batch_set_warm_start(self.handle, _p(x), _p(y)), "bo_pdlp_batch_set_warm_start")
 
+    def set_objective_rhs(self, cs=None, bs=None):
+        """New c and / or b for every LP (lists in batch order; None keeps them)."""
+        c = None if cs is None else np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in cs]))
+        b = None if bs is None else np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in bs]))
+        assert c is None or c.size == sum(lp.n for lp in self.lps)
+        assert b is None or b.size == sum(lp.m for lp in self.lps)
+        _check(lib().bo_pdlp_batch_set_objective_rhs(self.handle, _p(c), _p(b)), "bo_pdlp_batch_set_objective_rhs")
+
+    def set_primal_weights(self, w=None):
+        """Starting primal weight per LP (0 = default); None clears them."""
+        if w is None:
+            _check(lib().bo_pdlp_batch_set_primal_weights(self.handle, None), "bo_pdlp_batch_set_primal_weights")
+            return
+        w = np.ascontiguousarray(w, np.float64)
+        assert w.size == len(self.lps)
+        _check(lib().bo_pdlp_batch_set_primal_weights(self.handle, _p(w)), "bo_pdlp_batch_set_primal_weights")
+
     def solve(self):
         per = (PdlpStats * len(self.lps))()
         tot = PdlpBatchStats()
@@ -478,6 +497,15 @@ class Mlp:
                "bo_mlp_gradients")
         return loss.value, g
 
+    def loss(self, x, t):
+        """Mean squared error of forward(x) against t (no gradient copy)."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        _check(lib().bo_mlp_gradients(self.handle, _p(x), _p(t), x.shape[0], C.byref(loss), None),
+               "bo_mlp_gradients")
+        return loss.value
+
     def train_step(self, x, t, lr):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
@@ -485,6 +513,12 @@ class Mlp:
         _check(lib().bo_mlp_train_step(self.handle, _p(x), _p(t), x.shape[0], C.c_float(lr), C.byref(loss)),
                "bo_mlp_train_step")
         return loss.value
+
+    def time(self, what, batch, reps=10):
+        """ms per forward (what 0) or forward + backward (what 1) pass on the device."""
+        ms = C.c_double()
+        _check(lib().bo_mlp_time(self.handle, int(what), int(batch), int(reps), C.byref(ms)), "bo_mlp_time")
+        return ms.value
 
     def close(self):
         if self.handle:

@@ -467,6 +467,9 @@ std::vector<Chunk> make_chunks(const std::vector<int>& off, std::vector<int>& pt
 struct BatchBase {
     virtual ~BatchBase() = default;
     virtual bo_status set_warm_start(const double* x, const double* y) = 0;
+    virtual bo_status set_primal_weights(const double* w) = 0;
+    virtual bo_status set_objective_rhs(const double* c, const double* b) = 0;
+    virtual void set_iteration_limits(const long long* lim) = 0;
     virtual bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) = 0;
     virtual bo_status get_solution(int k, double* x, double* y) = 0;
 };
@@ -652,6 +655,7 @@ struct Batch final : BatchBase {
             s.c_norm = cn[k];
             s.eps = prm.eps_rel_kkt;
             s.max_iter = prm.max_iterations;
+            s.stall_window = sizeof(T) == sizeof(float) ? prm.stall_iterations : 0;
         }
         // device memory
         BO_TRY(mem.alloc(&rp, m + 1));
@@ -878,6 +882,48 @@ struct Batch final : BatchBase {
         return BO_SUCCESS;
     }
 
+    bo_status set_primal_weights(const double* w) override {
+        if (w)
+            for (int k = 0; k < K; ++k)
+                if (!(w[k] >= 0.0) || !std::isfinite(w[k])) return BO_INVALID_ARGUMENT;
+        BO_TRY(hipStreamSynchronize(stream));  // st_init may still feed an upload
+        for (int k = 0; k < K; ++k) st_init[k].omega_init = w ? w[k] : 0.0;
+        return BO_SUCCESS;
+    }
+
+    // Iteration budget of every LP (at most max_iterations of the parameters).
+    void set_iteration_limits(const long long* lim) override {
+        for (int k = 0; k < K; ++k) st_init[k].max_iter = std::min(lim[k], prm.max_iterations);
+    }
+
+    // New costs and / or right-hand sides for every LP (original space, batch
+    // order); the matrices, bounds, row senses and the layout stay.
+    bo_status set_objective_rhs(const double* c, const double* b) override {
+        if (c)
+            for (int j = 0; j < n; ++j)
+                if (!std::isfinite(c[j])) return BO_INVALID_ARGUMENT;
+        if (b)
+            for (int i = 0; i < m; ++i)
+                if (!std::isfinite(b[i])) return BO_INVALID_ARGUMENT;
+        BO_TRY(hipStreamSynchronize(stream));
+        if (c) BO_TRY(up(c64, c, sizeof(double) * n));
+        if (b) BO_TRY(up(b64, b, sizeof(double) * m));
+        for (int k = 0; k < K; ++k) {
+            if (c) {
+                double c2 = 0.0;
+                for (int j = coff[k]; j < coff[k + 1]; ++j) c2 += c[j] * c[j];
+                st_init[k].c_norm = std::sqrt(c2);
+            }
+            if (b) {
+                double b2 = 0.0;
+                for (int i = roff[k]; i < roff[k + 1]; ++i) b2 += b[i] * b[i];
+                st_init[k].b_norm = std::sqrt(b2);
+            }
+        }
+        BO_TRY(hipStreamSynchronize(stream));
+        return BO_SUCCESS;
+    }
+
     bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) override {
         auto t0 = std::chrono::high_resolution_clock::now();
         bo_status s0 = prepare();
@@ -887,7 +933,6 @@ struct Batch final : BatchBase {
         BO_TRY(hipStreamSynchronize(stream));
         const int Kc = prm.check_every > 0 ? prm.check_every : 64;
         long long iters = 0;
-        long long intervals = 0;
         while (*running_host > 0) {
             const long long left = prm.max_iterations - iters;
             const int k = static_cast<int>(std::min<long long>(Kc, left));
@@ -910,7 +955,6 @@ struct Batch final : BatchBase {
             }
             BO_TRY(hipGetLastError());
             iters += k;
-            ++intervals;
             BO_TRY(hipMemcpyAsync(running_host, running, sizeof(int), hipMemcpyDeviceToHost, stream));
             BO_TRY(hipStreamSynchronize(stream));
             if (prm.time_limit_s > 0.0 &&
@@ -994,8 +1038,47 @@ bool valid_lp(const bo_lp& lp) {
 }  // namespace
 }  // namespace bo
 
+namespace bo {
+namespace {
+
+// Owned copy of one LP, kept for the FP64 polish of LPs the FP32 engine
+// stops as STALLED (the caller's arrays need not outlive create).
+struct HostLp {
+    int m = 0, n = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> val, b, c, lb, ub;
+    std::vector<unsigned char> eq;
+    explicit HostLp(const bo_lp& lp)
+        : m(lp.m), n(lp.n), rp(lp.row_ptr, lp.row_ptr + (lp.m > 0 ? lp.m + 1 : 0)),
+          ci(lp.col_idx, lp.col_idx + lp.nnz), val(lp.val, lp.val + lp.nnz), b(lp.b, lp.b + lp.m),
+          c(lp.c, lp.c + lp.n), lb(lp.lb, lp.lb + lp.n), ub(lp.ub, lp.ub + lp.n), eq(lp.is_eq, lp.is_eq + lp.m) {}
+    bo_lp view() const {
+        bo_lp v{};
+        v.m = m;
+        v.n = n;
+        v.nnz = static_cast<long long>(val.size());
+        v.row_ptr = rp.data();
+        v.col_idx = ci.data();
+        v.val = val.data();
+        v.b = b.data();
+        v.c = c.data();
+        v.lb = lb.data();
+        v.ub = ub.data();
+        v.is_eq = eq.data();
+        return v;
+    }
+};
+
+}  // namespace
+}  // namespace bo
+
 struct bo_pdlp_batch {
     std::unique_ptr<bo::BatchBase> eng;
+    bo_pdlp_params prm{};
+    std::vector<bo::HostLp> lps;           // kept when the FP64 polish may run
+    std::unique_ptr<bo::BatchBase> pol;    // FP64 engine of the last solve's polished LPs
+    std::vector<int> pol_of;               // LP -> its index in pol, or -1
+    bool polishing() const { return !prm.fp64_iterates && prm.fp64_polish && prm.stall_iterations > 0; }
 };
 
 using namespace bo;
@@ -1031,6 +1114,12 @@ bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params
         h->eng = std::move(e);
     }
     if (st != BO_SUCCESS) return st;
+    h->prm = p;
+    h->pol_of.assign(count, -1);
+    if (h->polishing()) {
+        h->lps.reserve(count);
+        for (int k = 0; k < count; ++k) h->lps.emplace_back(lps[k]);
+    }
     *out = h.release();
     return BO_SUCCESS;
 }
@@ -1040,13 +1129,103 @@ bo_status bo_pdlp_batch_set_warm_start(bo_pdlp_batch* h, const double* x, const 
     return h->eng->set_warm_start(x, y);
 }
 
+bo_status bo_pdlp_batch_set_primal_weights(bo_pdlp_batch* h, const double* w) {
+    if (!h) return BO_INVALID_ARGUMENT;
+    return h->eng->set_primal_weights(w);
+}
+
+bo_status bo_pdlp_batch_set_objective_rhs(bo_pdlp_batch* h, const double* c, const double* b) {
+    if (!h) return BO_INVALID_ARGUMENT;
+    bo_status s = h->eng->set_objective_rhs(c, b);
+    if (s != BO_SUCCESS) return s;
+    size_t oc = 0, ob = 0;
+    for (HostLp& lp : h->lps) {
+        if (c) std::copy(c + oc, c + oc + lp.n, lp.c.begin());
+        if (b) std::copy(b + ob, b + ob + lp.m, lp.b.begin());
+        oc += lp.n;
+        ob += lp.m;
+    }
+    return BO_SUCCESS;
+}
+
+// The FP32 batch, then (fp64_polish) one FP64 batch over the LPs it stopped
+// as STALLED, each warm-started from its FP32 point with its last primal
+// weight and the iterations it has left.
 bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
     if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->solve(per_lp, total);
+    auto t0 = std::chrono::high_resolution_clock::now();
+    const int K = static_cast<int>(h->pol_of.size());
+    std::vector<bo_pdlp_stats> per(K);
+    bo_pdlp_batch_stats tot{};
+    h->pol.reset();
+    std::fill(h->pol_of.begin(), h->pol_of.end(), -1);
+    bo_status s = h->eng->solve(per.data(), &tot);
+    if (s != BO_SUCCESS) return s;
+    std::vector<int> stalled;
+    if (h->polishing())
+        for (int k = 0; k < K; ++k)
+            if (per[k].status == 4) stalled.push_back(k);
+    if (!stalled.empty()) {
+        const int P = static_cast<int>(stalled.size());
+        std::vector<bo_lp> views(P);
+        std::vector<long long> lim(P);
+        std::vector<double> w(P), wx, wy;
+        long long most = 0;
+        for (int j = 0; j < P; ++j) {
+            const int k = stalled[j];
+            views[j] = h->lps[k].view();
+            lim[j] = h->prm.max_iterations - per[k].iterations;
+            most = std::max(most, lim[j]);
+            w[j] = per[k].primal_weight;
+            std::vector<double> x(h->lps[k].n), y(h->lps[k].m);
+            s = h->eng->get_solution(k, x.data(), y.data());
+            if (s != BO_SUCCESS) return s;
+            wx.insert(wx.end(), x.begin(), x.end());
+            wy.insert(wy.end(), y.begin(), y.end());
+        }
+        if (most > 0) {
+            bo_pdlp_params p = h->prm;
+            p.fp64_iterates = 1;
+            p.max_iterations = most;
+            auto e = std::make_unique<Batch<double>>();
+            s = e->create(views.data(), P, &p);
+            if (s != BO_SUCCESS) return s;
+            e->set_iteration_limits(lim.data());
+            if ((s = e->set_primal_weights(w.data())) != BO_SUCCESS) return s;
+            if ((s = e->set_warm_start(wx.data(), wy.data())) != BO_SUCCESS) return s;
+            std::vector<bo_pdlp_stats> per2(P);
+            bo_pdlp_batch_stats tot2{};
+            if ((s = e->solve(per2.data(), &tot2)) != BO_SUCCESS) return s;
+            for (int j = 0; j < P; ++j) {
+                const int k = stalled[j];
+                const long long fp32_its = per[k].iterations;
+                per[k] = per2[j];
+                per[k].iterations += fp32_its;
+                per[k].fp64_polish_iterations = per2[j].iterations;
+                h->pol_of[k] = j;
+            }
+            h->pol = std::move(e);
+        }
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+    tot.solved = 0;
+    tot.max_iterations = 0;
+    tot.sum_iterations = 0;
+    for (int k = 0; k < K; ++k) {
+        per[k].solve_ms = ms;
+        tot.solved += per[k].status == 0;
+        tot.max_iterations = std::max(tot.max_iterations, per[k].iterations);
+        tot.sum_iterations += per[k].iterations;
+    }
+    tot.solve_ms = ms;
+    if (per_lp) std::copy(per.begin(), per.end(), per_lp);
+    if (total) *total = tot;
+    return BO_SUCCESS;
 }
 
 bo_status bo_pdlp_batch_get_solution(bo_pdlp_batch* h, int k, double* x, double* y) {
-    if (!h) return BO_INVALID_ARGUMENT;
+    if (!h || k < 0 || k >= static_cast<int>(h->pol_of.size())) return BO_INVALID_ARGUMENT;
+    if (h->pol && h->pol_of[k] >= 0) return h->pol->get_solution(h->pol_of[k], x, y);
     return h->eng->get_solution(k, x, y);
 }
 

@@ -20,7 +20,9 @@
 // No counterpart exists in the reference (which has no learned component).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -54,63 +56,174 @@ __host__ __device__ inline float bf2f(unsigned short h) {
 }
 
 constexpr int kTm = 64, kTn = 64, kTk = 32;
+constexpr int kLd = kTk + 8;      // LDS row pitch in BF16 elements (80 B: 16-byte aligned rows)
+constexpr int kMaxSplit = 64;     // K splits of one product at most
+constexpr int kFillTarget = 1024; // workgroups a split product aims for (4 per CU)
 
 enum : int { kEpiNone = 0, kEpiBias = 1, kEpiRelu = 2 };
 
-// C = op(A) op(B) (+ bias, ReLU).  A(m,k) = A[m*sam + k*sak], B(k,n) =
-// B[k*sbk + n*sbn].  Out of range elements of the tiles are 0.
-template <int EPI>
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+// Operand layouts: the stored matrix is contiguous along k (0) or along its
+// other dimension (1).  One thread stages 8 consecutive elements of that
+// contiguous direction: a 16-byte load when all 8 are in range and `vec`
+// (unit stride, 8-element aligned rows and base), else 8 guarded loads.
+//
+// LDS tiles hold both operands as [row of C or column of C][k], k
+// contiguous, so each lane reads its 4 k-values with one 8-byte LDS load.
+template <int L>
+__device__ inline void stage(unsigned short (*T)[kLd], const unsigned short* __restrict__ G, long long s_r, long long s_k,
+                             int r0, int R, int k0, int kend, bool vec) {
+    const int t = threadIdx.x;
+    if (L == 0) {  // k contiguous: 64 rows x 32 k, thread -> row t/4, k 8*(t%4)..+7
+        const int r = t >> 2, c = (t & 3) * 8;
+        const int gr = r0 + r, gk = k0 + c;
+        us8 v;
+        if (vec && gr < R && gk + 7 < kend) {
+            v = *reinterpret_cast<const us8*>(G + gr * s_r + gk);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[e] = (gr < R && gk + e < kend) ? G[gr * s_r + static_cast<long long>(gk + e) * s_k]
+                                                 : static_cast<unsigned short>(0);
+        }
+        *reinterpret_cast<us8*>(&T[r][c]) = v;
+    } else {  // row-direction contiguous: thread -> k t/8, rows 8*(t%8)..+7
+        const int kk = t >> 3, r = (t & 7) * 8;
+        const int gk = k0 + kk, gr = r0 + r;
+        us8 v;
+        if (vec && gk < kend && gr + 7 < R) {
+            v = *reinterpret_cast<const us8*>(G + static_cast<long long>(gk) * s_k + gr);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[e] = (gk < kend && gr + e < R) ? G[static_cast<long long>(gr + e) * s_r + gk * s_k]
+                                                 : static_cast<unsigned short>(0);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) T[r + e][kk] = v[e];
+    }
+}
+
+// C = op(A) op(B) (+ bias, ReLU): A(m,k) = A[m*sam + k*sak], B(k,n) =
+// B[k*sbk + n*sbn]; a 256-thread workgroup owns a 64 x 64 tile of C, each
+// wavefront a 32 x 32 quarter.  Split K (gridDim.z > 1): workgroup z sums k
+// in [z*kc, (z+1)*kc) and stores its raw FP32 tile to ws[z][M][N]; the
+// epilogue then runs in k_splitk_sum.
+template <int EPI, int LA, int LB>
 __global__ void __launch_bounds__(256)
-k_gemm(int M, int N, int K, const unsigned short* __restrict__ A, long long sam, long long sak,
-       const unsigned short* __restrict__ B, long long sbk, long long sbn, const float* __restrict__ bias,
-       float* __restrict__ C, unsigned short* __restrict__ Cb, long long ldc) {
-    __shared__ unsigned short As[kTm][kTk + 4];
-    __shared__ unsigned short Bs[kTk][kTn + 4];
-    const int tid = threadIdx.x;
-    const int lane = tid % kWave, wave = tid / kWave;
+k_gemm(int M, int N, int K, int kc, const unsigned short* __restrict__ A, long long sam, long long sak, bool va,
+       const unsigned short* __restrict__ B, long long sbk, long long sbn, bool vb, const float* __restrict__ bias,
+       float* __restrict__ C, unsigned short* __restrict__ Cb, long long ldc, float* __restrict__ ws) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[kTm][kLd];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[kTn][kLd];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int bm = blockIdx.y * kTm, bn = blockIdx.x * kTn;
     const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    const int kb = blockIdx.z * kc;
+    const int ke = min(K, kb + kc);
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += kTk) {
-        // 2048 elements of each tile, 8 per thread
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = tid + u * 256;
-            const int r = e / kTk, c = e % kTk;  // A tile: row r (m), column c (k)
-            const int m = bm + r, k = k0 + c;
-            As[r][c] = (m < M && k < K) ? A[m * sam + k * sak] : static_cast<unsigned short>(0);
-            const int rb = e / kTn, cb = e % kTn;  // B tile: row rb (k), column cb (n)
-            const int kb = k0 + rb, n = bn + cb;
-            Bs[rb][cb] = (kb < K && n < N) ? B[kb * sbk + n * sbn] : static_cast<unsigned short>(0);
-        }
+    for (int k0 = kb; k0 < ke; k0 += kTk) {
+        // A tile rows are m (stride sam), B tile rows are n (stride sbn)
+        stage<LA>(As, A, sam, sak, bm, M, k0, ke, va);
+        stage<LB>(Bs, B, sbn, sbk, bn, N, k0, ke, vb);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kTk; kk += 8) {
-            bf16x4 a, b;
             const int kq = kk + 4 * (lane / 32);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                a[j] = static_cast<short>(As[wm + lane % 32][kq + j]);
-                b[j] = static_cast<short>(Bs[kq + j][wn + lane % 32]);
-            }
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + lane % 32][kq]);
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + lane % 32][kq]);
             acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
     }
+    const bool split = gridDim.z > 1;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = bm + wm + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
         const int col = bn + wn + lane % 32;
         if (row < M && col < N) {
             float v = acc[r];
+            if (split) {
+                ws[(static_cast<size_t>(blockIdx.z) * M + row) * N + col] = v;
+                continue;
+            }
             if (EPI & kEpiBias) v += bias[col];
             if (EPI & kEpiRelu) v = v > 0.0f ? v : 0.0f;
             if (C) C[row * ldc + col] = v;
             if (Cb) Cb[row * ldc + col] = f2bf(v);
         }
     }
+}
+
+// Sum of the S split partials in order (deterministic), then the epilogue.
+template <int EPI>
+__global__ void k_splitk_sum(int M, int N, int S, const float* __restrict__ ws, const float* __restrict__ bias,
+                             float* __restrict__ C, unsigned short* __restrict__ Cb, long long ldc) {
+    const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t MN = static_cast<size_t>(M) * N;
+    if (e >= MN) return;
+    const int row = static_cast<int>(e / N), col = static_cast<int>(e % N);
+    float v = 0.0f;
+    for (int z = 0; z < S; ++z) v += ws[z * MN + e];
+    if (EPI & kEpiBias) v += bias[col];
+    if (EPI & kEpiRelu) v = v > 0.0f ? v : 0.0f;
+    if (C) C[row * ldc + col] = v;
+    if (Cb) Cb[row * ldc + col] = f2bf(v);
+}
+
+// K split of an M x N x K product: enough workgroups to fill the chip, at
+// least 256 k per split, kc a multiple of the 32-wide k step.
+struct SplitK {
+    int S, kc;
+};
+inline SplitK split_for(int M, int N, int K) {
+    const long long tiles = static_cast<long long>((M + kTm - 1) / kTm) * ((N + kTn - 1) / kTn);
+    int S = 1;
+    if (tiles < kFillTarget / 2 && K >= 512) {
+        S = static_cast<int>(std::min<long long>((kFillTarget + tiles - 1) / tiles, K / 256));
+        S = std::max(1, std::min(S, kMaxSplit));
+    }
+    int kc = (K + S - 1) / S;
+    kc = (kc + kTk - 1) / kTk * kTk;
+    S = (K + kc - 1) / kc;
+    return SplitK{S, kc};
+}
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Launch of one product on `stream`; ws holds split partials (ws_floats
+// floats at least S * M * N, checked).
+template <int EPI>
+hipError_t launch_gemm(hipStream_t stream, int M, int N, int K, const unsigned short* A, long long sam, long long sak,
+                       const unsigned short* B, long long sbk, long long sbn, const float* bias, float* C,
+                       unsigned short* Cb, long long ldc, float* ws, size_t ws_floats) {
+    const SplitK sk = split_for(M, N, K);
+    if (sk.S > 1 && static_cast<size_t>(sk.S) * M * N > ws_floats) return hipErrorInvalidValue;
+    dim3 grid((N + kTn - 1) / kTn, (M + kTm - 1) / kTm, sk.S);
+    const int la = sak == 1 ? 0 : 1, lb = sbk == 1 ? 0 : 1;
+    const bool va = aligned16(A) && (la == 0 ? (sak == 1 && sam % 8 == 0) : (sam == 1 && sak % 8 == 0));
+    const bool vb = aligned16(B) && (lb == 0 ? (sbk == 1 && sbn % 8 == 0) : (sbn == 1 && sbk % 8 == 0));
+#define BO_GEMM_LAUNCH(LA_, LB_)                                                                                  \
+    hipLaunchKernelGGL((k_gemm<EPI, LA_, LB_>), grid, dim3(256), 0, stream, M, N, K, sk.kc, A, sam, sak, va, B, sbk, \
+                       sbn, vb, bias, C, Cb, ldc, ws)
+    if (la == 0 && lb == 0)
+        BO_GEMM_LAUNCH(0, 0);
+    else if (la == 0)
+        BO_GEMM_LAUNCH(0, 1);
+    else if (lb == 0)
+        BO_GEMM_LAUNCH(1, 0);
+    else
+        BO_GEMM_LAUNCH(1, 1);
+#undef BO_GEMM_LAUNCH
+    if (sk.S > 1) {
+        const size_t MN = static_cast<size_t>(M) * N;
+        hipLaunchKernelGGL(k_splitk_sum<EPI>, dim3(grid_for(MN)), dim3(kBlock), 0, stream, M, N, sk.S, ws, bias, C,
+                           Cb, ldc);
+    }
+    return hipGetLastError();
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, unsigned short* __restrict__ y, size_t n) {
@@ -210,6 +323,9 @@ struct bo_mlp {
     float* dh = nullptr;                // FP32 input gradient of the current layer (max_batch x max dim)
     unsigned short *dzb = nullptr, *dzb2 = nullptr;  // BF16 gradients (ping-pong)
     double *part = nullptr, *loss = nullptr;
+    float* ws = nullptr;  // split-K partials
+    size_t ws_floats = 0;
+    hipError_t err = hipSuccess;  // first launch error of the current call
 
     ~bo_mlp() {
         if (stream) hipStreamDestroy(stream);
@@ -218,9 +334,24 @@ struct bo_mlp {
     template <int EPI>
     void gemm(int M, int N, int K, const unsigned short* A, long long sam, long long sak, const unsigned short* B,
               long long sbk, long long sbn, const float* bias, float* C, unsigned short* Cb, long long ldc) {
-        dim3 grid((N + kTn - 1) / kTn, (M + kTm - 1) / kTm);
-        hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, stream, M, N, K, A, sam, sak, B, sbk, sbn, bias, C, Cb,
-                           ldc);
+        const hipError_t e = launch_gemm<EPI>(stream, M, N, K, A, sam, sak, B, sbk, sbn, bias, C, Cb, ldc, ws, ws_floats);
+        if (err == hipSuccess) err = e;
+    }
+
+    // split-K workspace for every product of a batch of up to max_batch rows
+    size_t ws_needed() const {
+        size_t need = 0;
+        auto take = [&](int M, int N, int K) {
+            const SplitK sk = split_for(M, N, K);
+            if (sk.S > 1) need = std::max(need, static_cast<size_t>(sk.S) * M * N);
+        };
+        for (int B = 1; B <= max_batch; ++B)
+            for (int l = 0; l < layers; ++l) {
+                take(B, dout[l], din[l]);   // forward
+                take(din[l], dout[l], B);   // weight gradient
+                take(B, din[l], dout[l]);   // input gradient
+            }
+        return need;
     }
 
     bo_status create(int nin, int wdt, int nout, int nl, int maxb, unsigned long long seed) {
@@ -257,6 +388,9 @@ struct bo_mlp {
         BO_TRY(mem.alloc(&dzb2, static_cast<size_t>(maxb) * maxd));
         BO_TRY(mem.alloc(&part, grid_for(static_cast<size_t>(maxb) * n_out)));
         BO_TRY(mem.alloc(&loss, 1));
+        ws_floats = ws_needed();
+        if (ws_floats) BO_TRY(mem.alloc(&ws, ws_floats));
+        BO_TRY(hipMemsetAsync(xb, 0, sizeof(unsigned short) * maxb * n_in, stream));
         // He-uniform initialisation of weights, zero biases (host, seeded)
         std::vector<float> hw(nparam, 0.0f);
         std::mt19937_64 rng(seed);
@@ -317,6 +451,12 @@ struct bo_mlp {
         }
     }
 
+    hipError_t take_error() {
+        const hipError_t e = err != hipSuccess ? err : hipGetLastError();
+        err = hipSuccess;
+        return e;
+    }
+
     void adam(float lr) {
         ++steps;
         const float bc1 = 1.0f - std::pow(0.9f, static_cast<float>(steps));
@@ -369,7 +509,9 @@ bo_status bo_mlp_forward(bo_mlp* h, const float* x, int batch, float* y) {
     if (!h || !x || !y || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
     bo_status s = h->upload_x(x, batch);
     if (s != BO_SUCCESS) return s;
+    h->err = hipSuccess;
     h->forward(batch);
+    BO_TRY(h->take_error());
     BO_TRY(hipMemcpyAsync(y, h->y, sizeof(float) * batch * h->n_out, hipMemcpyDeviceToHost, h->stream));
     BO_TRY(hipStreamSynchronize(h->stream));
     BO_TRY(hipGetLastError());
@@ -381,8 +523,10 @@ bo_status bo_mlp_gradients(bo_mlp* h, const float* x, const float* target, int b
     bo_status s = h->upload_x(x, batch);
     if (s != BO_SUCCESS) return s;
     BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
+    h->err = hipSuccess;
     h->forward(batch);
     h->backward(batch);
+    BO_TRY(h->take_error());
     if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (grads) BO_TRY(hipMemcpyAsync(grads, h->g, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
     BO_TRY(hipStreamSynchronize(h->stream));
@@ -395,8 +539,10 @@ bo_status bo_mlp_train_step(bo_mlp* h, const float* x, const float* target, int 
     bo_status s = h->upload_x(x, batch);
     if (s != BO_SUCCESS) return s;
     BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
+    h->err = hipSuccess;
     h->forward(batch);
     h->backward(batch);
+    BO_TRY(h->take_error());
     h->adam(lr);
     if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     BO_TRY(hipStreamSynchronize(h->stream));
@@ -404,29 +550,54 @@ bo_status bo_mlp_train_step(bo_mlp* h, const float* x, const float* target, int 
     return BO_SUCCESS;
 }
 
+bo_status bo_mlp_time(bo_mlp* h, int what, int batch, int reps, double* ms) {
+    if (!h || !ms || (what != 0 && what != 1) || batch <= 0 || batch > h->max_batch || reps <= 0)
+        return BO_INVALID_ARGUMENT;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BO_TRY(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        h->err = hipSuccess;
+        for (int r = 0; r <= reps; ++r) {  // one untimed pass first
+            if (r == 1) e = hipEventRecord(e0, h->stream);
+            h->forward(batch);
+            if (what == 1) h->backward(batch);
+        }
+        if (e == hipSuccess) e = h->take_error();
+        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        *ms = static_cast<double>(t) / reps;
+    }
+    hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    BO_TRY(e);
+    return BO_SUCCESS;
+}
+
 bo_status bo_gemm_bf16(int M, int N, int K, const unsigned short* A, long long sam, long long sak,
                        const unsigned short* B, long long sbk, long long sbn, float* C) {
     // host arrays; test and timing entry point of the product kernel
-    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return BO_INVALID_ARGUMENT;
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || sam < 0 || sak < 0 || sbk < 0 || sbn < 0)
+        return BO_INVALID_ARGUMENT;
     bo_status s = ensure_device_public();
     if (s != BO_SUCCESS) return s;
     const long long na = (M - 1) * sam + (K - 1) * sak + 1, nb = (K - 1) * sbk + (N - 1) * sbn + 1;
+    const SplitK sk = split_for(M, N, K);
+    const size_t nws = sk.S > 1 ? static_cast<size_t>(sk.S) * M * N : 0;
+    DevMemM mem;
     unsigned short *dA = nullptr, *dB = nullptr;
-    float* dC = nullptr;
-    BO_TRY(hipMalloc(reinterpret_cast<void**>(&dA), sizeof(unsigned short) * na));
-    BO_TRY(hipMalloc(reinterpret_cast<void**>(&dB), sizeof(unsigned short) * nb));
-    BO_TRY(hipMalloc(reinterpret_cast<void**>(&dC), sizeof(float) * M * N));
+    float *dC = nullptr, *dW = nullptr;
+    BO_TRY(mem.alloc(&dA, na));
+    BO_TRY(mem.alloc(&dB, nb));
+    BO_TRY(mem.alloc(&dC, static_cast<size_t>(M) * N));
+    if (nws) BO_TRY(mem.alloc(&dW, nws));
     BO_TRY(hipMemcpy(dA, A, sizeof(unsigned short) * na, hipMemcpyHostToDevice));
     BO_TRY(hipMemcpy(dB, B, sizeof(unsigned short) * nb, hipMemcpyHostToDevice));
-    dim3 grid((N + kTn - 1) / kTn, (M + kTm - 1) / kTm);
-    hipLaunchKernelGGL(k_gemm<kEpiNone>, grid, dim3(256), 0, 0, M, N, K, dA, sam, sak, dB, sbk, sbn, nullptr, dC,
-                       nullptr, static_cast<long long>(N));
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(C, dC, sizeof(float) * M * N, hipMemcpyDeviceToHost);
-    hipFree(dA);
-    hipFree(dB);
-    hipFree(dC);
-    BO_TRY(e);
+    BO_TRY((launch_gemm<kEpiNone>(nullptr, M, N, K, dA, sam, sak, dB, sbk, sbn, nullptr, dC, nullptr,
+                                  static_cast<long long>(N), dW, nws)));
+    BO_TRY(hipMemcpy(C, dC, sizeof(float) * M * N, hipMemcpyDeviceToHost));
     return BO_SUCCESS;
 }
 

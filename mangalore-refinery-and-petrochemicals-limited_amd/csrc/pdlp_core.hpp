@@ -16,6 +16,10 @@ struct PdlpState {
     // [0] current iterate z, [1] candidate T(z)
     double rel_kkt[2], rel_primal[2], rel_dual[2], rel_gap[2], pobj[2], dobj[2], kkt_w[2];
     double sigma_max;
+    double omega_init;  // > 0: starting primal weight (a warm start's), else ||c_hat|| / ||b_hat||
+    double best_kkt;    // lowest rel-KKT seen, updated when it halves
+    long long best_kkt_iter;  // iteration of that update
+    long long stall_window;   // > 0: stop as STALLED after this many iterations without a halving
     long long iter, iter_last_restart, max_iter;
     int done, status, restart, restart_to_cand, best, restarts, checks, pad;
 };
@@ -105,7 +109,7 @@ __device__ inline void init_state_core(PdlpState* st, double pw_u2, double c2, d
     st->sigma_max = smax;
     st->eta = smax > 0.0 ? 0.95 / smax : 1.0;
     const double cn = sqrt(c2), bn = sqrt(b2);
-    st->omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    st->omega = st->omega_init > 0.0 ? st->omega_init : ((cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0);
     st->tau = st->eta / st->omega;
     st->sigma = st->eta * st->omega;
     st->iter = 0;
@@ -119,14 +123,17 @@ __device__ inline void init_state_core(PdlpState* st, double pw_u2, double c2, d
     st->checks = 0;
     st->kkt_last_restart = 0.0;
     st->kkt_prev_candidate = INFINITY;
+    st->best_kkt = INFINITY;
+    st->best_kkt_iter = 0;
 }
 
 // The check's sums q (FP64, unscaled problem):
 //   [0,1] c'x  [2,3] ||primal violation||^2  [4,5] the same, scaled  [6,7] b'y
 //   [8,9] ||dual residual||^2  [10,11] the same, scaled  [12,13] sum l plam+ + u plam-
 // for the current iterate [even] and the candidate T(z) [odd]; then the
-// stopping test, the KKT restart rule (beta 0.2 / 0.8 / 0.36) and the
-// bookkeeping.
+// stopping test, the stall test (no halving of the best rel-KKT within
+// stall_window iterations), the KKT restart rule (beta 0.2 / 0.8 / 0.36) and
+// the bookkeeping.
 __device__ inline void decide_core(PdlpState* st, const double* q) {
     const double w = st->omega;
     bool bad = false;
@@ -163,6 +170,14 @@ __device__ inline void decide_core(PdlpState* st, const double* q) {
     if (st->iter >= st->max_iter) {
         st->done = 1;
         st->status = 1;
+        return;
+    }
+    if (st->rel_kkt[best] < 0.5 * st->best_kkt) {
+        st->best_kkt = st->rel_kkt[best];
+        st->best_kkt_iter = st->iter;
+    } else if (st->stall_window > 0 && st->iter - st->best_kkt_iter >= st->stall_window) {
+        st->done = 1;
+        st->status = 4;
         return;
     }
     if (st->iter == 0) {  // starting point

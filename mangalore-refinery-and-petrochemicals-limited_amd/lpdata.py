@@ -287,6 +287,42 @@ def batch_lps(count=512, lo=200, hi=20000, seed=42, k=5):
     return [random_lp_sampled(int(mi), 3 * int(mi), k, seed + 1 + i) for i, mi in enumerate(sizes)]
 
 
+class StaircaseFamily:
+    """Parametric family over one fixed staircase matrix (the learned
+    warm-start workload): draw k has
+
+        c = c0 * (1 + 0.2 N(0,1)),   b = b0 * (1 + U[-0.1, 0.1])
+
+    elementwise, from ``default_rng(k)``, with A, bounds and row senses of
+    ``staircase_lp(periods, blk_rows, blk_cols, inv_per_link, seed)``.  The
+    features of a draw are its normalised perturbations
+    (c / c0 - 1) / 0.2 ~ N(0,1) and (b / b0 - 1) / 0.1 ~ U[-1,1] (0 where b0 is 0),
+    n + m values."""
+
+    def __init__(self, periods=12, blk_rows=2000, blk_cols=5000, inv_per_link=200, seed=42):
+        self.base = staircase_lp(periods, blk_rows, blk_cols, inv_per_link, seed)
+        self.m, self.n = self.base.m, self.base.n
+        self.meta = dict(kind="staircase_family", periods=periods, blk_rows=blk_rows, blk_cols=blk_cols,
+                         inv_per_link=inv_per_link, seed=seed, nnz=self.base.nnz)
+
+    def perturbation(self, k):
+        rng = np.random.default_rng(k)
+        ec = rng.standard_normal(self.n)
+        eb = rng.uniform(-1.0, 1.0, self.m)
+        return ec, eb
+
+    def features(self, k):
+        ec, eb = self.perturbation(k)
+        eb = np.where(self.base.b != 0.0, eb, 0.0)
+        return np.concatenate([ec, eb]).astype(np.float32)
+
+    def draw(self, k):
+        ec, eb = self.perturbation(k)
+        b0 = self.base
+        return StdLP(self.m, self.n, b0.row_ptr, b0.col_idx, b0.val, b0.b * (1.0 + 0.1 * eb),
+                     b0.c * (1.0 + 0.2 * ec), b0.lb, b0.ub, b0.is_eq, meta=dict(self.meta, draw=int(k)))
+
+
 def benchmark10_model():
     """Deterministic 100 x 50 model of the reference's Benchmark 5-10
     (benchmarks/benchmark_main.cpp:139-151): maximise sum ((j % 7) + 1) x_j,

@@ -22,7 +22,8 @@ def _bf(pkg, a):
 
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
-@pytest.mark.parametrize("M,N,K", [(70, 33, 45), (64, 64, 32), (5, 130, 300), (129, 7, 1)])
+@pytest.mark.parametrize("M,N,K", [(70, 33, 45), (64, 64, 32), (5, 130, 300), (129, 7, 1),
+                                   (64, 100, 5000), (33, 70, 4001)])  # the last two split K
 def test_gemm_matches_exact_product(pkg, ta, tb, M, N, K):
     rng = np.random.default_rng(M * 7 + N * 3 + K)
     a = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
@@ -40,9 +41,10 @@ def test_gemm_matches_exact_product(pkg, ta, tb, M, N, K):
     assert np.all(np.abs(c - ref) <= 2e-6 * scale + 1e-30)
 
 
-def test_network_gradients_match_oracle(pkg):
+@pytest.mark.parametrize("n_in,width,n_out", [(37, 64, 29), (700, 96, 600)])  # the second splits K
+def test_network_gradients_match_oracle(pkg, n_in, width, n_out):
     rng = np.random.default_rng(5)
-    n_in, width, n_out, B = 37, 64, 29, 8
+    B = 8
     net = pkg.Mlp(n_in, width, n_out, layers=4, max_batch=16, seed=3)
     shapes = net.shapes()
     p = rng.standard_normal(net.nparam).astype(np.float32) * 0.3
