@@ -220,6 +220,18 @@ def mlp_forward_roofline(net, batch, reps=20):
             "tflops": flops / (ms * 1e-3) / 1e12, "intensity_flop_per_byte": flops / nbytes}
 
 
+def family_pass_bytes(m, n, nnz, lanes=64, tsize=4):
+    """Algorithmic bytes of one family-engine product pass over `lanes` LPs:
+    the matrix once (index + value) and its pointers, the gathered vector
+    once (all lanes), and the epilogue's reads and writes per (segment, LP):
+    primal (CSC, x update): x_k r/w, x_0, c_hat, x+ -> 32 B (FP32 x+), plus
+    the shared bounds; dual (CSR, y update): A x r/w, A x_0, y_k r/w, y_0,
+    b_hat, y+, y gathered copy -> 60 B, plus the row sense."""
+    primal = (4 + tsize) * nnz + 4 * (n + 1) + tsize * lanes * m + lanes * (24 + 2 * tsize) * n + 2 * tsize * n
+    dual = (4 + tsize) * nnz + 4 * (m + 1) + tsize * lanes * n + lanes * (48 + 3 * tsize) * m + m
+    return primal, dual
+
+
 def bench_c4(pkg, args):
     """Learned warm start over the parametric staircase family: training
     data from cold batched solves, the network trained on the GPU, then the
@@ -229,21 +241,27 @@ def bench_c4(pkg, args):
     ws = pkg.warmstart
     t0 = time.perf_counter()
     fam = pkg.lpdata.StaircaseFamily(**C4_FAMILY)
+    ok_train = 0
     desc = ("staircase family periods=12 block=2000x5000 inv=200 seed=42; draw k: c = c0(1+0.2N(0,1)), "
             "b = b0(1+U[-0.1,0.1]) from default_rng(k)")
     log(f"workload c4: {desc}; m={fam.m} n={fam.n} nnz={fam.base.nnz}; built in {time.perf_counter() - t0:.1f}s")
     params = dict(eps_rel_kkt=args.eps, max_iterations=args.max_iterations, use_graph=0 if args.no_graph else 1)
-    fs = ws.FamilySolver(pkg, fam, batch=args.c4_batch, **params)
+    fs = ws.FamilySolver(pkg, fam, batch=args.c4_batch, engine="family", **params)
     train_ids = list(range(1, 1 + args.train_draws))
     eval_ids = list(range(20001, 20001 + args.eval_draws))
-    for w in range(args.warmup):  # builds the batch layout, warms the kernels
+    fam_ms = None
+    for w in range(max(1, args.warmup)):  # builds the engine, warms the kernels
         r = fs.solve(eval_ids[:args.c4_batch], want_solutions=False)
         log(f"warmup {w}: {r['solve_s']:.2f}s, solved {(r['status'] == 0).sum()}")
+        if len(r["status"]) == args.c4_batch:  # pass times with every lane at work
+            fam_ms = fs._engine(args.c4_batch).time_passes(20)
+            log(f"family passes (ms): primal {fam_ms[0]:.4f} dual {fam_ms[1]:.4f}")
 
     t0 = time.perf_counter()
     tr = fs.solve(train_ids)
     gen_s = time.perf_counter() - t0
     ok = [i for i, st in enumerate(tr["status"]) if st == 0]
+    ok_train = len(ok)
     log(f"training data: {len(ok)}/{len(train_ids)} draws solved, {gen_s:.1f}s")
     F = np.array([fam.features(train_ids[i]) for i in ok])
     net = ws.WarmStartNet(pkg, fam.n, fam.m, width=1024, layers=4, max_batch=args.c4_batch, seed=1)
@@ -254,7 +272,11 @@ def bench_c4(pkg, args):
         f"{hist['seconds']:.1f}s")
 
     cold = fs.solve(eval_ids)
-    log(f"cold: {cold['solve_s']:.2f}s solved {(cold['status'] == 0).sum()}")
+    log(f"cold (family engine): {cold['solve_s']:.2f}s solved {(cold['status'] == 0).sum()}")
+    fb = ws.FamilySolver(pkg, fam, batch=args.c4_batch, engine="batch", **params)
+    cold_bd = fb.solve(eval_ids[:args.block_sample], want_solutions=False)
+    fb.close()
+    log(f"cold (block-diagonal batched engine, {args.block_sample} draws): {cold_bd['solve_s']:.2f}s")
     Fe = np.array([fam.features(k) for k in eval_ids])
     t0 = time.perf_counter()
     Xn, Yn = net.predict(Fe)
@@ -288,14 +310,25 @@ def bench_c4(pkg, args):
         return [float(np.percentile(v, 50)), float(np.percentile(v, 90)), int(np.max(v))]
 
     solved = {k: int((r["status"] == 0).sum()) for k, r in (("cold", cold), ("network", warm), ("mean", mean))}
-    rate = {"cold_batched": solved["cold"] / cold["solve_s"], "network_warm_batched": solved["network"] / warm_s,
-            "mean_warm_batched": solved["mean"] / mean["solve_s"],
+    rate = {"cold_family": solved["cold"] / cold["solve_s"], "network_warm_family": solved["network"] / warm_s,
+            "mean_warm_family": solved["mean"] / mean["solve_s"],
+            "cold_block_diagonal_batched": int((cold_bd["status"] == 0).sum()) / cold_bd["solve_s"],
             "cold_single": single_ok / single_s if single_s > 0 else None}
     mlp_roof = mlp_forward_roofline(net.net, args.c4_batch)
     net.close()
+    roof = None
+    if fam_ms is not None:
+        pb, db = family_pass_bytes(fam.m, fam.n, fam.base.nnz, args.c4_batch)
+        kname, ms, nb = ("family_primal_pass", fam_ms[0], pb) if fam_ms[0] >= fam_ms[1] else \
+            ("family_dual_pass", fam_ms[1], db)
+        gbps = nb / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": kname, "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbps / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": nb, "launch_ms": ms,
+                "passes_ms": {"primal": fam_ms[0], "dual": fam_ms[1]},
+                "passes_gbps": {"primal": pb / (fam_ms[0] * 1e-3) / 1e9, "dual": db / (fam_ms[1] * 1e-3) / 1e9}}
     out = {
         "metric": METRIC,
-        "value": rate["network_warm_batched"],
+        "value": rate["network_warm_family"],
         "unit": "LPs/s",
         "n_gpus": 1,
         "steps": 1,
@@ -313,22 +346,25 @@ def bench_c4(pkg, args):
                    "parallelism": "single-gpu"},
         "step": "one pass over the held-out draws in batches (network inference + warm-started solves + fallbacks)",
         "lps_per_s": rate,
-        "speedup": {"network_warm_batched_over_cold_batched": rate["network_warm_batched"] / rate["cold_batched"],
-                    "network_warm_batched_over_mean_warm_batched":
-                        rate["network_warm_batched"] / rate["mean_warm_batched"],
-                    "network_warm_batched_over_cold_single":
-                        rate["network_warm_batched"] / rate["cold_single"] if rate["cold_single"] else None},
+        "speedup": {"network_warm_family_over_cold_family": rate["network_warm_family"] / rate["cold_family"],
+                    "network_warm_family_over_mean_warm_family":
+                        rate["network_warm_family"] / rate["mean_warm_family"],
+                    "network_warm_family_over_cold_single":
+                        rate["network_warm_family"] / rate["cold_single"] if rate["cold_single"] else None,
+                    "cold_family_over_cold_block_diagonal":
+                        rate["cold_family"] / rate["cold_block_diagonal_batched"]},
         "solved": solved,
         "fallbacks": {"network": len(warm["fallback"]), "mean": len(mean["fallback"])},
         "iterations_p50_p90_max": {"cold": pct(cold["iterations"]), "network": pct(warm["iterations"]),
                                    "mean": pct(mean["iterations"])},
         "start_distance_median_rel": dist,
         "single_sample": args.single_sample,
-        "training": {"draws_solved": len(ok), "generation_s": gen_s, "epochs": args.epochs, "lr": args.lr,
+        "training": {"draws_solved": ok_train, "generation_s": gen_s, "epochs": args.epochs, "lr": args.lr,
                      "val_loss_untrained": hist["val"][0], "val_loss_best": hist["best_val"],
                      "best_epoch": hist["best_epoch"], "train_s": hist["seconds"]},
         "inference_s": infer_s,
-        "roofline": mlp_roof,
+        "roofline": roof,
+        "network_roofline": mlp_roof,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (reference CPU PDHG, bounded sample) ...")
@@ -353,6 +389,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=20, help="c4: training epochs")
     ap.add_argument("--lr", type=float, default=1e-4, help="c4: Adam learning rate")
     ap.add_argument("--c4-batch", type=int, default=64, help="c4: LPs per batched solve")
+    ap.add_argument("--block-sample", type=int, default=128,
+                    help="c4: held-out draws also solved cold by the block-diagonal batched engine")
     ap.add_argument("--single-sample", type=int, default=16,
                     help="c4: held-out draws also solved one at a time by the single-LP engine")
     ap.add_argument("--max-iterations", type=int, default=200000)

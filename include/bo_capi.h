@@ -258,6 +258,34 @@ bo_status bo_pdlp_batch_solve(struct bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo
 bo_status bo_pdlp_batch_get_solution(struct bo_pdlp_batch* h, int k, double* x, double* y);
 void bo_pdlp_batch_destroy(struct bo_pdlp_batch* h);
 
+/* ---- parametric family: one matrix, up to 64 LPs ------------------------- */
+/* No counterpart in the reference, which solves one LP per call
+ * (src/first_order_solver.cpp:15-291).  LPs that share A, bounds and row
+ * senses and differ in c and b are solved together: vectors are stored
+ * LP-minor, one wavefront per row / column with one LP per lane, so each
+ * matrix entry is read once per pass for all of them.  The scaling and the
+ * step-size estimate depend on A alone and are computed once, at create.
+ * Arrays are host memory; c and x are count x n, b and y count x m, row-major
+ * (one row per LP).  Statistics and the FP64 polish are those of the batched
+ * engine. */
+struct bo_pdlp_family;
+/* The family of `lp`'s matrix; room for `capacity` (1..64) LPs, all starting as lp. */
+bo_status bo_pdlp_family_create(const bo_lp* lp, int capacity, const bo_pdlp_params* params,
+                                struct bo_pdlp_family** out);
+/* The next solves' LPs: count (1..capacity) cost and right-hand-side rows;
+ * both NULL keeps the current ones (count unchanged).  Clears the warm start. */
+bo_status bo_pdlp_family_set_lps(struct bo_pdlp_family* h, int count, const double* c, const double* b);
+/* Starting points (count x n, count x m, original space); NULL, NULL clears them. */
+bo_status bo_pdlp_family_set_warm_start(struct bo_pdlp_family* h, const double* x, const double* y);
+bo_status bo_pdlp_family_solve(struct bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total);
+/* Solutions of the last solve, original space (count x n, count x m; either may be NULL). */
+bo_status bo_pdlp_family_get_solutions(struct bo_pdlp_family* h, double* x, double* y);
+/* Mean time (ms, HIP events on the engine's stream) of one primal product
+ * pass (CSC, x update) and one dual pass (CSR, y update) over all `count`
+ * LPs, run on the state the last solve left; ms2[0] primal, ms2[1] dual. */
+bo_status bo_pdlp_family_time_passes(struct bo_pdlp_family* h, int reps, double* ms2);
+void bo_pdlp_family_destroy(struct bo_pdlp_family* h);
+
 /* ---- warm-start network (BF16 MLP on the matrix cores) -------------------- */
 /* No counterpart in the reference, which has no learned component.  A
  * multilayer perceptron n_in -> width -> ... -> n_out (`layers` products,

@@ -132,6 +132,8 @@ EXPORTS = [
     "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels", "bo_pdlp_batch_create", "bo_pdlp_batch_set_warm_start",
     "bo_pdlp_batch_set_primal_weights",
     "bo_pdlp_batch_set_objective_rhs", "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution", "bo_pdlp_batch_destroy",
+    "bo_pdlp_family_create", "bo_pdlp_family_set_lps", "bo_pdlp_family_set_warm_start", "bo_pdlp_family_solve",
+    "bo_pdlp_family_get_solutions", "bo_pdlp_family_time_passes", "bo_pdlp_family_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
     "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_time", "bo_mlp_destroy", "bo_gemm_bf16",
 ]
@@ -173,7 +175,8 @@ def lib():
         L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
         for name in EXPORTS:
             if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params",
-                            "bo_pdlp_batch_destroy", "bo_mlp_destroy", "bo_mlp_num_params"):
+                            "bo_pdlp_batch_destroy", "bo_pdlp_family_destroy", "bo_mlp_destroy",
+                            "bo_mlp_num_params"):
                 getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -410,6 +413,65 @@ class PdlpBatch:
     def close(self):
         if self.handle:
             lib().bo_pdlp_batch_destroy(self.handle)
+            self.handle = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PdlpFamily:
+    """Up to 64 LPs over one matrix (bo_pdlp_family_*): `base` gives A, bounds
+    and row senses; set_lps swaps in the LPs' costs and right-hand sides."""
+
+    def __init__(self, base: StdLP, capacity=64, **params):
+        self.base = base
+        self._h = _LpHolder(base)
+        self.params = default_pdlp_params(**params)
+        self.handle = P()
+        _check(lib().bo_pdlp_family_create(C.byref(self._h.s), int(capacity), C.byref(self.params),
+                                           C.byref(self.handle)), "bo_pdlp_family_create")
+        self.count = capacity
+
+    def set_lps(self, cs, bs):
+        """Costs and right-hand sides of the next solves (sequences of rows)."""
+        c = np.ascontiguousarray(np.asarray(cs, np.float64).reshape(len(cs), self.base.n))
+        b = np.ascontiguousarray(np.asarray(bs, np.float64).reshape(len(bs), self.base.m))
+        assert c.shape[0] == b.shape[0]
+        _check(lib().bo_pdlp_family_set_lps(self.handle, c.shape[0], _p(c), _p(b)), "bo_pdlp_family_set_lps")
+        self.count = c.shape[0]
+
+    def set_warm_start(self, xs=None, ys=None):
+        if xs is None and ys is None:
+            _check(lib().bo_pdlp_family_set_warm_start(self.handle, None, None), "bo_pdlp_family_set_warm_start")
+            return
+        x = np.ascontiguousarray(np.asarray(xs, np.float64).reshape(self.count, self.base.n))
+        y = np.ascontiguousarray(np.asarray(ys, np.float64).reshape(self.count, self.base.m))
+        _check(lib().bo_pdlp_family_set_warm_start(self.handle, _p(x), _p(y)), "bo_pdlp_family_set_warm_start")
+
+    def solve(self):
+        per = (PdlpStats * self.count)()
+        tot = PdlpBatchStats()
+        _check(lib().bo_pdlp_family_solve(self.handle, per, C.byref(tot)), "bo_pdlp_family_solve")
+        return ([{k: getattr(p, k) for k, _ in PdlpStats._fields_} for p in per],
+                {k: getattr(tot, k) for k, _ in PdlpBatchStats._fields_})
+
+    def solutions(self):
+        x = np.zeros((self.count, self.base.n))
+        y = np.zeros((self.count, self.base.m))
+        _check(lib().bo_pdlp_family_get_solutions(self.handle, _p(x), _p(y)), "bo_pdlp_family_get_solutions")
+        return x, y
+
+    def time_passes(self, reps=20):
+        ms = np.zeros(2)
+        _check(lib().bo_pdlp_family_time_passes(self.handle, int(reps), _p(ms)), "bo_pdlp_family_time_passes")
+        return float(ms[0]), float(ms[1])
+
+    def close(self):
+        if self.handle:
+            lib().bo_pdlp_family_destroy(self.handle)
             self.handle = P()
 
     def __del__(self):

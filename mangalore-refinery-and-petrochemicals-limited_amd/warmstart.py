@@ -8,12 +8,13 @@ warm start is a dual-simplex basis for branch and bound
 Pipeline (all heavy work on the GPU, through libbo_hip.so):
 
 * ``FamilySolver`` solves draws of a family (``lpdata.StaircaseFamily``) in
-  batches with the batched PDLP engine: one ``PdlpBatch`` per batch size,
-  built once over copies of the family's matrix; every batch swaps in its
-  draws' costs and right-hand sides (``bo_pdlp_batch_set_objective_rhs``),
-  optionally a warm start, and solves.  A draw the warm-started solve leaves
-  unsolved is solved again from the cold start (the fallback); the engine's
-  FP64 rel-KKT of the unscaled LP decides every result.
+  batches of up to 64: by default with the family engine (``PdlpFamily``:
+  one matrix, one LP per lane), or with the block-diagonal batched engine
+  (``PdlpBatch``, one per batch size, built once over copies of the
+  family's matrix).  Every batch swaps in its draws' costs and right-hand
+  sides, optionally a warm start, and solves.  A draw the warm-started solve
+  leaves unsolved is solved again from the cold start (the fallback); the
+  engine's FP64 rel-KKT of the unscaled LP decides every result.
 * ``WarmStartNet`` is the BF16 multilayer perceptron of bo_mlp_* (forward,
   backward and Adam kernels on the matrix cores).  It maps the normalised
   perturbations of a draw (n + m features) to its primal / dual solution,
@@ -31,14 +32,45 @@ import numpy as np
 class FamilySolver:
     """Batched solves of family draws, reusing one device layout per batch size."""
 
-    def __init__(self, pkg, family, batch=64, **params):
-        self.pkg, self.family, self.batch, self.params = pkg, family, batch, params
+    def __init__(self, pkg, family, batch=64, engine="family", **params):
+        assert engine in ("family", "batch") and (engine == "batch" or batch <= 64)
+        self.pkg, self.family, self.batch, self.engine, self.params = pkg, family, batch, engine, params
         self._engines = {}
 
     def _engine(self, k):
-        if k not in self._engines:
-            self._engines[k] = self.pkg.PdlpBatch([self.family.base] * k, **self.params)
-        return self._engines[k]
+        key = "family" if self.engine == "family" else k
+        if key not in self._engines:
+            if self.engine == "family":
+                self._engines[key] = self.pkg.PdlpFamily(self.family.base, capacity=self.batch, **self.params)
+            else:
+                self._engines[key] = self.pkg.PdlpBatch([self.family.base] * k, **self.params)
+        return self._engines[key]
+
+    def _run_chunk(self, chunk, warm_rows, want_solutions):
+        eng = self._engine(len(chunk))
+        draws = [self.family.draw(k) for k in chunk]
+        if self.engine == "family":
+            eng.set_lps([d.c for d in draws], [d.b for d in draws])
+        else:
+            eng.set_objective_rhs([d.c for d in draws], [d.b for d in draws])
+        if warm_rows is None:
+            eng.set_warm_start(None, None)
+        else:
+            eng.set_warm_start(*warm_rows)
+        t0 = time.perf_counter()
+        p, _ = eng.solve()
+        dt = time.perf_counter() - t0
+        xs, ys = [], []
+        if want_solutions:
+            if self.engine == "family":
+                X, Y = eng.solutions()
+                xs, ys = list(X), list(Y)
+            else:
+                for i in range(len(chunk)):
+                    x, y = eng.solution(i)
+                    xs.append(x)
+                    ys.append(y)
+        return p, dt, xs, ys
 
     def solve(self, ids, warm=None, want_solutions=True):
         """Solve draws `ids` (cold, or from warm = (X, Y) rows aligned with ids).
@@ -51,23 +83,13 @@ class FamilySolver:
         solve_s = 0.0
         for s in range(0, len(ids), self.batch):
             chunk = ids[s:s + self.batch]
-            eng = self._engine(len(chunk))
-            draws = [self.family.draw(k) for k in chunk]
-            eng.set_objective_rhs([d.c for d in draws], [d.b for d in draws])
-            if warm is None:
-                eng.set_warm_start(None, None)
-            else:
-                eng.set_warm_start([warm[0][s + i] for i in range(len(chunk))],
-                                   [warm[1][s + i] for i in range(len(chunk))])
-            t0 = time.perf_counter()
-            p, _ = eng.solve()
-            solve_s += time.perf_counter() - t0
+            rows = None if warm is None else ([warm[0][s + i] for i in range(len(chunk))],
+                                              [warm[1][s + i] for i in range(len(chunk))])
+            p, dt, x, y = self._run_chunk(chunk, rows, want_solutions)
+            solve_s += dt
             per.extend(p)
-            if want_solutions:
-                for i in range(len(chunk)):
-                    x, y = eng.solution(i)
-                    xs.append(x)
-                    ys.append(y)
+            xs.extend(x)
+            ys.extend(y)
         return dict(per=per, x=xs, y=ys, solve_s=solve_s,
                     iterations=np.array([q["iterations"] for q in per]),
                     status=np.array([q["status"] for q in per]))

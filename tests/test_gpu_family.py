@@ -1,0 +1,114 @@
+"""GPU parity of the parametric-family engine (bo_pdlp_family_*): up to 64
+LPs over one matrix, vectors LP-minor, one LP per lane.
+
+Every LP is judged by the oracle's FP64 KKT evaluator (rel-KKT <= 1e-6 on the
+unscaled LP, bounds exact), its objective must match the block-diagonal
+batched engine's on the same draw, and the engine's own FP64 report must
+agree with the oracle's.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def family(pkg):
+    return pkg.lpdata.StaircaseFamily(periods=12, blk_rows=100, blk_cols=250, inv_per_link=10, seed=7)
+
+
+def _check(po, lp, x, y, s, name):
+    k = po.kkt(lp, x, y)
+    assert s["status"] == 0, (name, s)
+    assert k["rel_kkt"] <= TOL, (name, k)
+    assert k["max_bound_viol"] == 0.0
+    assert abs(k["rel_kkt"] - s["rel_kkt"]) <= 1e-9 + 1e-3 * k["rel_kkt"], (name, k["rel_kkt"], s["rel_kkt"])
+    return k
+
+
+@pytest.mark.parametrize("fp64", [0, 1])
+def test_family_matches_batch_engine(pkg, po, family, fp64):
+    ids = list(range(1, 41))  # 40 LPs: lanes 40..63 idle
+    draws = [family.draw(k) for k in ids]
+    print("case: family of", len(ids), "draws, fp64", fp64, flush=True)
+    f = pkg.PdlpFamily(family.base, capacity=64, eps_rel_kkt=TOL, max_iterations=60000, fp64_iterates=fp64)
+    f.set_lps([d.c for d in draws], [d.b for d in draws])
+    per, tot = f.solve()
+    print("family", tot, flush=True)
+    X, Y = f.solutions()
+    b = pkg.PdlpBatch(draws, eps_rel_kkt=TOL, max_iterations=60000, fp64_iterates=fp64)
+    perb, totb = b.solve()
+    print("batch", totb, flush=True)
+    assert tot["solved"] == len(ids)
+    for i, d in enumerate(draws):
+        k = _check(po, d, X[i], Y[i], per[i], "draw%d" % ids[i])
+        if perb[i]["status"] == 0:
+            xb, yb = b.solution(i)
+            kb = po.kkt(d, xb, yb)
+            assert abs(k["primal_obj"] - kb["primal_obj"]) <= 1e-5 * (1 + abs(kb["primal_obj"]))
+    its_f = np.array([p["iterations"] for p in per])
+    its_b = np.array([p["iterations"] for p in perb])
+    print("iterations family median %d batch median %d" % (np.median(its_f), np.median(its_b)), flush=True)
+    b.close()
+    f.close()
+
+
+def test_family_reuse_warm_start_and_count(pkg, po, family):
+    f = pkg.PdlpFamily(family.base, capacity=8, eps_rel_kkt=TOL, max_iterations=60000)
+    d1 = [family.draw(k) for k in (101, 102, 103)]
+    f.set_lps([d.c for d in d1], [d.b for d in d1])
+    per1, tot1 = f.solve()
+    X1, Y1 = f.solutions()
+    per1b, _ = f.solve()  # same LPs again: same run
+    X1b, Y1b = f.solutions()
+    assert [p["iterations"] for p in per1] == [p["iterations"] for p in per1b]
+    assert np.array_equal(X1, X1b) and np.array_equal(Y1, Y1b)
+    for i, d in enumerate(d1):
+        _check(po, d, X1[i], Y1[i], per1[i], "reuse%d" % i)
+    # warm start at the optimum: every LP stops at its first check
+    f.set_warm_start(X1, Y1)
+    per2, tot2 = f.solve()
+    assert tot2["solved"] == 3 and tot2["max_iterations"] == 0, tot2
+    # a different count of different LPs in the same handle
+    d2 = [family.draw(k) for k in range(201, 209)]
+    f.set_lps([d.c for d in d2], [d.b for d in d2])
+    per3, tot3 = f.solve()
+    X3, Y3 = f.solutions()
+    assert len(per3) == 8
+    for i, d in enumerate(d2):
+        if per3[i]["status"] == 0:
+            _check(po, d, X3[i], Y3[i], per3[i], "count8_%d" % i)
+    assert tot3["solved"] >= 7
+    f.close()
+
+
+def test_family_fp64_polish(pkg, po, family):
+    # draws whose FP32 iterates stall short of 1e-6 (see the batched engine's
+    # polish test): the family engine hands them to its FP64 engine
+    ids = [31, 143, 185, 1, 2]
+    draws = [family.draw(k) for k in ids]
+    f = pkg.PdlpFamily(family.base, capacity=len(ids), eps_rel_kkt=TOL, max_iterations=40000)
+    f.set_lps([d.c for d in draws], [d.b for d in draws])
+    per, tot = f.solve()
+    print("polish", [(p["status"], p["iterations"], p["fp64_polish_iterations"]) for p in per], flush=True)
+    X, Y = f.solutions()
+    assert tot["solved"] == len(ids)
+    for i, d in enumerate(draws):
+        assert per[i]["iterations"] <= 40000
+        _check(po, d, X[i], Y[i], per[i], "polish%d" % ids[i])
+    f.close()
+
+
+def test_family_rejects_bad_input(pkg, family):
+    with pytest.raises(pkg.BoError):
+        pkg.PdlpFamily(family.base, capacity=65)
+    f = pkg.PdlpFamily(family.base, capacity=4)
+    d = family.draw(5)
+    with pytest.raises(pkg.BoError):
+        f.set_lps([d.c] * 5, [d.b] * 5)  # above capacity
+    c = d.c.copy()
+    c[3] = np.nan
+    with pytest.raises(pkg.BoError):
+        f.set_lps([c], [d.b])
+    f.close()

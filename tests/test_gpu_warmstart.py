@@ -40,7 +40,7 @@ def _verify(po, family, ids, r, name, min_solved=1.0):
 
 def test_objective_rhs_swap_matches_fresh_batch(pkg, po, family):
     ws = pkg.warmstart
-    fs = ws.FamilySolver(pkg, family, batch=8, eps_rel_kkt=TOL, max_iterations=100000)
+    fs = ws.FamilySolver(pkg, family, batch=8, engine="batch", eps_rel_kkt=TOL, max_iterations=100000)
     ids = list(range(1, 17))
     print("case: swap c/b into a built batch, draws", ids, flush=True)
     r = fs.solve(ids)
