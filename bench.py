@@ -222,13 +222,13 @@ def mlp_forward_roofline(net, batch, reps=20):
 
 def family_pass_bytes(m, n, nnz, lanes=64, tsize=4):
     """Algorithmic bytes of one family-engine product pass over `lanes` LPs:
-    the matrix once (index + value) and its pointers, the gathered vector
+    the matrix once (int32 index + FP64 scaled value) and its pointers, the gathered vector
     once (all lanes), and the epilogue's reads and writes per (segment, LP):
     primal (CSC, x update): x_k r/w, x_0, c_hat, x+ -> 32 B (FP32 x+), plus
     the shared bounds; dual (CSR, y update): A x r/w, A x_0, y_k r/w, y_0,
     b_hat, y+, y gathered copy -> 60 B, plus the row sense."""
-    primal = (4 + tsize) * nnz + 4 * (n + 1) + tsize * lanes * m + lanes * (24 + 2 * tsize) * n + 2 * tsize * n
-    dual = (4 + tsize) * nnz + 4 * (m + 1) + tsize * lanes * n + lanes * (48 + 3 * tsize) * m + m
+    primal = 12 * nnz + 4 * (n + 1) + tsize * lanes * m + lanes * (24 + 2 * tsize) * n + 2 * tsize * n
+    dual = 12 * nnz + 4 * (m + 1) + tsize * lanes * n + lanes * (48 + 3 * tsize) * m + m
     return primal, dual
 
 

@@ -150,10 +150,10 @@ typedef struct bo_pdlp_params {
     double time_limit_s;       /* 0 = none */
     int verbose;
     int hot_columns;           /* entries of x+ kept in LDS by the CSR pass: -1 tune, 0 off */
-    long long stall_iterations; /* batched engine, FP32 iterates: an LP whose FP64 rel-KKT has not
-                                   halved within this many iterations stops as STALLED (4);
-                                   0 = never (default 4096) */
-    int fp64_polish;           /* batched engine: finish every STALLED LP with FP64 iterates from
+    long long stall_iterations; /* batched and family engines, FP32 iterates: an LP whose FP64 rel-KKT
+                                   has not halved within max(this, iterations / 2) iterations stops
+                                   as STALLED (4); 0 = never (default 4096) */
+    int fp64_polish;           /* batched and family engines: finish every STALLED LP with FP64 iterates from
                                   its FP32 point, within its remaining iteration budget (default 1) */
 } bo_pdlp_params;
 

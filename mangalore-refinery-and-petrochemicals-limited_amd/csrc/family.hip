@@ -15,6 +15,9 @@
 // The scaling (Ruiz, then Pock-Chambolle) and the power-iteration estimate of
 // ||A_hat||_2 depend on the matrix alone: they are computed once, at create,
 // on the host.  Each LP's primal weight starts from its own ||c_hat|| / ||b_hat||.
+// The scaled matrix is kept in FP64: shared by all LPs, its 12 bytes per
+// entry are a few percent of a pass, and its products carry no rounding of
+// A (the iterates and gathered vectors are FP32).
 //
 // No counterpart exists in the reference, which solves one LP per call
 // (src/first_order_solver.cpp:15-291).
@@ -555,7 +558,7 @@ void host_scaling(int m, int n, const std::vector<int>& rp, const std::vector<in
 }
 
 // ||A_hat v||^2 after kPowerIters rounds of v <- A'A v / ||.|| from a hashed
-// positive start, on the working-precision scaled values.
+// positive start.
 double host_power(int m, int n, const std::vector<int>& rp, const std::vector<int>& ci, const std::vector<double>& a) {
     std::vector<double> v(n), u(m);
     for (int j = 0; j < n; ++j) {
@@ -617,7 +620,7 @@ struct Family final : FamilyBase {
     hipStream_t stream = nullptr;
     DevMemF mem;
     int *rp = nullptr, *ci = nullptr, *cp = nullptr, *ri = nullptr;
-    T *rv = nullptr, *cv = nullptr;
+    double *rv = nullptr, *cv = nullptr;  // scaled values, FP64 (the matrix is shared: 4 more bytes per entry)
     double *rv64 = nullptr, *cv64 = nullptr;
     double *rs = nullptr, *cs = nullptr, *lb64 = nullptr, *ub64 = nullptr;
     T *l_hat = nullptr, *u_hat = nullptr;
@@ -697,16 +700,12 @@ struct Family final : FamilyBase {
         }
         std::vector<double> hrs, hcs;
         host_scaling(m, n, h_rp, h_ci, h_rv, prm.ruiz_iterations, prm.pock_chambolle != 0, hrs, hcs);
-        std::vector<T> h_rs32(nnz), h_cs32(nnz);
-        std::vector<double> h_aw(nnz);  // working-precision scaled values (power iteration)
+        std::vector<double> h_rsc(nnz), h_csc(nnz);  // scaled values
         for (int i = 0; i < m; ++i)
-            for (int q = h_rp[i]; q < h_rp[i + 1]; ++q) {
-                h_rs32[q] = static_cast<T>(hrs[i] * h_rv[q] * hcs[h_ci[q]]);
-                h_aw[q] = static_cast<double>(h_rs32[q]);
-            }
+            for (int q = h_rp[i]; q < h_rp[i + 1]; ++q) h_rsc[q] = hrs[i] * h_rv[q] * hcs[h_ci[q]];
         for (int j = 0; j < n; ++j)
-            for (int q = h_cp[j]; q < h_cp[j + 1]; ++q) h_cs32[q] = static_cast<T>(hcs[j] * h_cv[q] * hrs[h_ri[q]]);
-        pw_u2 = host_power(m, n, h_rp, h_ci, h_aw);
+            for (int q = h_cp[j]; q < h_cp[j + 1]; ++q) h_csc[q] = hcs[j] * h_cv[q] * hrs[h_ri[q]];
+        pw_u2 = host_power(m, n, h_rp, h_ci, h_rsc);
         std::vector<T> hl(n), hu(n);
         for (int j = 0; j < n; ++j) {
             hl[j] = static_cast<T>(lp.lb[j] / hcs[j]);
@@ -746,11 +745,11 @@ struct Family final : FamilyBase {
         BO_TRY(mem.alloc(&running, 1));
         BO_TRY(up(rp, h_rp.data(), sizeof(int) * (m + 1)));
         BO_TRY(up(ci, h_ci.data(), sizeof(int) * nnz));
-        BO_TRY(up(rv, h_rs32.data(), sizeof(T) * nnz));
+        BO_TRY(up(rv, h_rsc.data(), sizeof(double) * nnz));
         BO_TRY(up(rv64, h_rv.data(), sizeof(double) * nnz));
         BO_TRY(up(cp, h_cp.data(), sizeof(int) * (n + 1)));
         BO_TRY(up(ri, h_ri.data(), sizeof(int) * nnz));
-        BO_TRY(up(cv, h_cs32.data(), sizeof(T) * nnz));
+        BO_TRY(up(cv, h_csc.data(), sizeof(double) * nnz));
         BO_TRY(up(cv64, h_cv.data(), sizeof(double) * nnz));
         BO_TRY(up(rs, hrs.data(), sizeof(double) * m));
         BO_TRY(up(cs, hcs.data(), sizeof(double) * n));
@@ -838,11 +837,11 @@ struct Family final : FamilyBase {
 
     void launch_primal(int t) {
         EpiFPrimal<T> e{yg, cf, c_hat, l_hat, u_hat, xk, x0, xp, t};
-        hipLaunchKernelGGL((kf_spmm<T, EpiFPrimal<T>>), dim3(spmm_grid(n)), dim3(256), 0, stream, n, cp, ri, cv, e);
+        hipLaunchKernelGGL((kf_spmm<double, EpiFPrimal<T>>), dim3(spmm_grid(n)), dim3(256), 0, stream, n, cp, ri, cv, e);
     }
     void launch_dual(int t) {
         EpiFDual<T> e{xp, cf, b_hat, eq, ax, ax0, yk, y0, yp, yg, t};
-        hipLaunchKernelGGL((kf_spmm<T, EpiFDual<T>>), dim3(spmm_grid(m)), dim3(256), 0, stream, m, rp, ci, rv, e);
+        hipLaunchKernelGGL((kf_spmm<double, EpiFDual<T>>), dim3(spmm_grid(m)), dim3(256), 0, stream, m, rp, ci, rv, e);
     }
 
     void launch_check() {
@@ -887,7 +886,7 @@ struct Family final : FamilyBase {
                            static_cast<int>(gn), part_b, static_cast<int>(gm));
         hipLaunchKernelGGL(kf_coef, dim3(1), dim3(kL), 0, stream, st, cf, running);
         // A x0 (the starting point is also the first anchor)
-        hipLaunchKernelGGL((kf_spmm<T, EpiFStore<T>>), dim3(spmm_grid(m)), dim3(256), 0, stream, m, rp, ci, rv,
+        hipLaunchKernelGGL((kf_spmm<double, EpiFStore<T>>), dim3(spmm_grid(m)), dim3(256), 0, stream, m, rp, ci, rv,
                            EpiFStore<T>{xp, ax, ax0});
         BO_TRY(hipGetLastError());
         return BO_SUCCESS;

@@ -19,7 +19,7 @@ struct PdlpState {
     double omega_init;  // > 0: starting primal weight (a warm start's), else ||c_hat|| / ||b_hat||
     double best_kkt;    // lowest rel-KKT seen, updated when it halves
     long long best_kkt_iter;  // iteration of that update
-    long long stall_window;   // > 0: stop as STALLED after this many iterations without a halving
+    long long stall_window;   // > 0: stop as STALLED after max(this, iter / 2) iterations without a halving
     long long iter, iter_last_restart, max_iter;
     int done, status, restart, restart_to_cand, best, restarts, checks, pad;
 };
@@ -132,8 +132,9 @@ __device__ inline void init_state_core(PdlpState* st, double pw_u2, double c2, d
 //   [8,9] ||dual residual||^2  [10,11] the same, scaled  [12,13] sum l plam+ + u plam-
 // for the current iterate [even] and the candidate T(z) [odd]; then the
 // stopping test, the stall test (no halving of the best rel-KKT within
-// stall_window iterations), the KKT restart rule (beta 0.2 / 0.8 / 0.36) and
-// the bookkeeping.
+// max(stall_window, iter / 2) iterations: restart intervals grow with the
+// iteration count, and so do plateaus of a converging run), the KKT restart
+// rule (beta 0.2 / 0.8 / 0.36) and the bookkeeping.
 __device__ inline void decide_core(PdlpState* st, const double* q) {
     const double w = st->omega;
     bool bad = false;
@@ -175,7 +176,8 @@ __device__ inline void decide_core(PdlpState* st, const double* q) {
     if (st->rel_kkt[best] < 0.5 * st->best_kkt) {
         st->best_kkt = st->rel_kkt[best];
         st->best_kkt_iter = st->iter;
-    } else if (st->stall_window > 0 && st->iter - st->best_kkt_iter >= st->stall_window) {
+    } else if (st->stall_window > 0 &&
+               st->iter - st->best_kkt_iter >= (st->stall_window > st->iter / 2 ? st->stall_window : st->iter / 2)) {
         st->done = 1;
         st->status = 4;
         return;

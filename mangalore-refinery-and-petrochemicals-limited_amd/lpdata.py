@@ -297,16 +297,27 @@ class StaircaseFamily:
     ``staircase_lp(periods, blk_rows, blk_cols, inv_per_link, seed)``.  The
     features of a draw are its normalised perturbations
     (c / c0 - 1) / 0.2 ~ N(0,1) and (b / b0 - 1) / 0.1 ~ U[-1,1] (0 where b0 is 0),
-    n + m values."""
+    n + m values.  With rank > 0 the perturbations of draw k are instead
+    fixed mixtures of `rank` latent N(0,1) factors z from default_rng(k)
+    (c: U_c z, each entry ~N(0,1); b: tanh(U_b z); U from default_rng(seed + 1)):
+    a family whose solutions move along a few directions only."""
 
-    def __init__(self, periods=12, blk_rows=2000, blk_cols=5000, inv_per_link=200, seed=42):
+    def __init__(self, periods=12, blk_rows=2000, blk_cols=5000, inv_per_link=200, seed=42, rank=0):
         self.base = staircase_lp(periods, blk_rows, blk_cols, inv_per_link, seed)
         self.m, self.n = self.base.m, self.base.n
+        self.rank = rank
         self.meta = dict(kind="staircase_family", periods=periods, blk_rows=blk_rows, blk_cols=blk_cols,
-                         inv_per_link=inv_per_link, seed=seed, nnz=self.base.nnz)
+                         inv_per_link=inv_per_link, seed=seed, nnz=self.base.nnz, rank=rank)
+        if rank:
+            g = np.random.default_rng(seed + 1)
+            self._uc = g.standard_normal((self.n, rank)) / np.sqrt(rank)
+            self._ub = g.standard_normal((self.m, rank)) / np.sqrt(rank)
 
     def perturbation(self, k):
         rng = np.random.default_rng(k)
+        if self.rank:
+            z = rng.standard_normal(self.rank)
+            return self._uc @ z, np.tanh(self._ub @ z)
         ec = rng.standard_normal(self.n)
         eb = rng.uniform(-1.0, 1.0, self.m)
         return ec, eb

@@ -8,9 +8,14 @@
   on cold solutions of training draws, against the cold start and against
   the training mean as warm starts, on held-out draws.  Every solve of every
   variant is judged by the oracle's FP64 KKT evaluator (rel-KKT <= 1e-6) and
-  its objective must match the cold solve's.  The test asserts that the
-  network's starting points are closer to the optimum than the cold start
-  and than the mean, and prints the iteration counts of each variant.
+  its objective must match the cold solve's.  On the low-rank variant of the
+  family (perturbations mixing 4 latent factors: solutions the network can
+  learn from 1,000 draws) the test asserts that the network's starting
+  points are much closer to the optimum than the training mean and that
+  they save iterations over both the mean and the cold start.  On the
+  full-rank family (i.i.d. perturbation of every cost and right-hand side)
+  the network cannot generalise from this little data; there the test only
+  asserts that training never makes it worse than the mean it starts from.
 * The cold fallback: a warm start given too few iterations falls back to a
   cold solve and still returns verified optima.
 """
@@ -24,6 +29,11 @@ TOL = 1e-6
 @pytest.fixture(scope="module")
 def family(pkg):
     return pkg.lpdata.StaircaseFamily(periods=12, blk_rows=100, blk_cols=250, inv_per_link=10, seed=7)
+
+
+@pytest.fixture(scope="module")
+def lowrank(pkg):
+    return pkg.lpdata.StaircaseFamily(periods=12, blk_rows=100, blk_cols=250, inv_per_link=10, seed=7, rank=4)
 
 
 def _verify(po, family, ids, r, name, min_solved=1.0):
@@ -56,23 +66,24 @@ def test_objective_rhs_swap_matches_fresh_batch(pkg, po, family):
     fs.close()
 
 
-def test_learned_warm_start_ablation(pkg, po, family):
+@pytest.mark.parametrize("which", ["lowrank", "fullrank"])
+def test_learned_warm_start_ablation(pkg, po, family, lowrank, which):
     ws = pkg.warmstart
-    # A few draws of this family stall short of 1e-6 with FP32 iterates in any
-    # start; they end at the iteration cap (status 1) and are left out of
-    # training and of the comparisons.
+    family = lowrank if which == "lowrank" else family
+    # Draws the engine cannot finish within the cap (status 1) are left out
+    # of training and of the comparisons.
     fs = ws.FamilySolver(pkg, family, batch=64, eps_rel_kkt=TOL, max_iterations=40000)
-    train_ids = list(range(1, 513))
+    train_ids = list(range(1, 1025 if which == "lowrank" else 513))
     test_ids = list(range(100001, 100129))
-    print("case: ablation, family m=%d n=%d nnz=%d, %d training draws, %d held-out"
-          % (family.m, family.n, family.base.nnz, len(train_ids), len(test_ids)), flush=True)
+    print("case: ablation (%s), family m=%d n=%d nnz=%d, %d training draws, %d held-out"
+          % (which, family.m, family.n, family.base.nnz, len(train_ids), len(test_ids)), flush=True)
     tr = fs.solve(train_ids)
     ok = [i for i in range(len(train_ids)) if tr["status"][i] == 0]
     print("training draws solved: %d / %d" % (len(ok), len(train_ids)), flush=True)
     assert len(ok) >= 0.95 * len(train_ids)
     F = np.array([family.features(train_ids[i]) for i in ok])
     net = ws.WarmStartNet(pkg, family.n, family.m, width=1024, layers=4, max_batch=64, seed=3)
-    hist = net.fit(F, np.array([tr["x"][i] for i in ok]), np.array([tr["y"][i] for i in ok]), epochs=30,
+    hist = net.fit(F, np.array([tr["x"][i] for i in ok]), np.array([tr["y"][i] for i in ok]), epochs=40,
                    lr=1e-4, log=print)
     print("training: val loss %.4f -> best %.4f at epoch %d, %.1fs"
           % (hist["val"][0], hist["best_val"], hist["best_epoch"], hist["seconds"]), flush=True)
@@ -108,9 +119,14 @@ def test_learned_warm_start_ablation(pkg, po, family):
     print("iteration ratio cold/network %.3f, mean/network %.3f"
           % (its["cold"].sum() / its["network"].sum(), its["mean"].sum() / its["network"].sum()), flush=True)
     assert d_net < d_cold
-    # the network is never further from the optimum than the training mean
-    # (its untrained state, which training keeps unless validation improves)
-    assert d_net <= d_mean * (1 + 1e-9)
+    if which == "lowrank":
+        assert hist["best_val"] < 0.6 * hist["val"][0]
+        assert d_net < 0.6 * d_mean
+        assert its["network"].sum() < its["mean"].sum() and its["network"].sum() < its["cold"].sum()
+    else:
+        # never further from the optimum than the training mean (its untrained
+        # state, which training keeps unless validation improves)
+        assert d_net <= d_mean * (1 + 1e-9)
     net.close()
     fs.close()
 
