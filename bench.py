@@ -164,6 +164,16 @@ def bench_batch(pkg, args):
     total = sum(times)
     solved = sum(t["solved"] for t in tots)
     its = np.array([p["iterations"] for p in per])
+    roof, table = None, {}
+    if tots[-1]["tiled"]:
+        ms4, nb4 = b.time_kernels(20)
+        for k, role in enumerate(KERNEL_ROLES):
+            table[role] = {"ms": float(ms4[k]), "bytes": float(nb4[k]), "gbps": nb4[k] / (ms4[k] * 1e-3) / 1e9}
+        kname = max(table, key=lambda k: table[k]["ms"])
+        dom = table[kname]
+        roof = {"bound": "hbm", "kernel": kname, "achieved": dom["gbps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dom["gbps"] / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom["bytes"],
+                "launch_ms": dom["ms"], "note": "every LP running (the first steps of a solve)"}
     out = {
         "metric": METRIC,
         "value": solved / total,
@@ -182,7 +192,10 @@ def bench_batch(pkg, args):
                    "start": "cold", "parallelism": "single-gpu"},
         "solved_per_step": tots[-1]["solved"],
         "iterations_p50_p90_max": [float(np.percentile(its, 50)), float(np.percentile(its, 90)), int(its.max())],
-        "roofline": None,
+        "main_loop_steps": tots[-1]["loop_steps"],
+        "tail": {"lps": tots[-1]["tail_lps"], "fp64_polished": tots[-1]["polished"], "ms": tots[-1]["tail_ms"]},
+        "kernels": table,
+        "roofline": roof,
         "setup_s": setup_s,
     }
     if not args.no_cpu_baseline:

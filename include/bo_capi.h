@@ -144,7 +144,9 @@ typedef struct bo_pdlp_params {
                                   slices) + reduce kernels, cut for k tiles per CU,
                                   -c stream kernel with an LDS budget of c (1024/2048/4096)
                                   nonzeros per workgroup, 0 = autotune */
-    int lanes_cols;            /* CSC pass schedule, same encoding */
+    int lanes_cols;            /* CSC pass schedule, same encoding; the batched engine uses the tiled
+                                  layout for a pass whose code is <= 0 and the segment kernel
+                                  otherwise */
     int fp64_iterates;         /* 1: run the iteration in FP64 (error comparison run) */
     int use_graph;             /* 1: replay each check interval as a hipGraph */
     double time_limit_s;       /* 0 = none */
@@ -155,6 +157,9 @@ typedef struct bo_pdlp_params {
                                    as STALLED (4); 0 = never (default 4096) */
     int fp64_polish;           /* batched and family engines: finish every STALLED LP with FP64 iterates from
                                   its FP32 point, within its remaining iteration budget (default 1) */
+    int compact_tail;          /* batched engine: once at most 1/8 of a batch's LPs still run, continue
+                                  them in a compact batch (segment kernels), from their points, primal
+                                  weights and remaining budgets, repeatedly (default 1) */
 } bo_pdlp_params;
 
 typedef struct bo_pdlp_stats {
@@ -235,6 +240,9 @@ typedef struct bo_pdlp_batch_stats {
     long long loop_steps;      /* steps the batch loop ran */
     double solve_ms;           /* host wall time of bo_pdlp_batch_solve */
     int tiled;                 /* 1: both passes on the tiled schedule */
+    int polished;              /* LPs finished by the FP64 polish */
+    int tail_lps;              /* LPs handed to the tail rounds (compact batches, FP64 polish) */
+    double tail_ms;            /* host wall time of the tail rounds (set-up included), part of solve_ms */
 } bo_pdlp_batch_stats;
 struct bo_pdlp_batch;
 bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params* params,
@@ -256,6 +264,10 @@ bo_status bo_pdlp_batch_set_objective_rhs(struct bo_pdlp_batch* h, const double*
 bo_status bo_pdlp_batch_solve(struct bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total);
 /* Solution of LP k in its original space (x[n_k], y[m_k]). */
 bo_status bo_pdlp_batch_get_solution(struct bo_pdlp_batch* h, int k, double* x, double* y);
+/* Mean time (ms, HIP events on the batch's stream) of each iteration kernel with every LP running,
+ * on the state the last solve left: ms4 = [CSC gather, CSC reduce + primal step, CSR gather,
+ * CSR reduce + dual step]; bytes4 = their algorithmic bytes.  Needs the tiled layout of both passes. */
+bo_status bo_pdlp_batch_time_kernels(struct bo_pdlp_batch* h, int reps, double* ms4, double* bytes4);
 void bo_pdlp_batch_destroy(struct bo_pdlp_batch* h);
 
 /* ---- parametric family: one matrix, up to 64 LPs ------------------------- */

@@ -76,7 +76,8 @@ class PdlpParams(C.Structure):
                 ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("check_every", C.c_int),
                 ("lanes_rows", C.c_int), ("lanes_cols", C.c_int), ("fp64_iterates", C.c_int),
                 ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int),
-                ("hot_columns", C.c_int), ("stall_iterations", C.c_longlong), ("fp64_polish", C.c_int)]
+                ("hot_columns", C.c_int), ("stall_iterations", C.c_longlong), ("fp64_polish", C.c_int),
+                ("compact_tail", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -97,7 +98,7 @@ class PdlpPassInfo(C.Structure):
 class PdlpBatchStats(C.Structure):
     _fields_ = [("count", C.c_int), ("solved", C.c_int), ("max_iterations", C.c_longlong),
                 ("sum_iterations", C.c_longlong), ("loop_steps", C.c_longlong), ("solve_ms", C.c_double),
-                ("tiled", C.c_int)]
+                ("tiled", C.c_int), ("polished", C.c_int), ("tail_lps", C.c_int), ("tail_ms", C.c_double)]
 
 
 class BoModel(C.Structure):
@@ -131,7 +132,8 @@ EXPORTS = [
     "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
     "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels", "bo_pdlp_batch_create", "bo_pdlp_batch_set_warm_start",
     "bo_pdlp_batch_set_primal_weights",
-    "bo_pdlp_batch_set_objective_rhs", "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution", "bo_pdlp_batch_destroy",
+    "bo_pdlp_batch_set_objective_rhs", "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution",
+    "bo_pdlp_batch_time_kernels", "bo_pdlp_batch_destroy",
     "bo_pdlp_family_create", "bo_pdlp_family_set_lps", "bo_pdlp_family_set_warm_start", "bo_pdlp_family_solve",
     "bo_pdlp_family_get_solutions", "bo_pdlp_family_time_passes", "bo_pdlp_family_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
@@ -378,6 +380,12 @@ class PdlpBatch:
         x = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in xs]))
         y = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64) for v in ys]))
         _check(lib().bo_pdlp_batch_set_warm_start(self.handle, _p(x), _p(y)), "bo_pdlp_batch_set_warm_start")
+
+    def time_kernels(self, reps=20):
+        """ms and algorithmic bytes of [CSC gather, CSC reduce, CSR gather, CSR reduce], all LPs running."""
+        ms, nb = np.zeros(4), np.zeros(4)
+        _check(lib().bo_pdlp_batch_time_kernels(self.handle, int(reps), _p(ms), _p(nb)), "bo_pdlp_batch_time_kernels")
+        return ms, nb
 
     def set_objective_rhs(self, cs=None, bs=None):
         """New c and / or b for every LP (lists in batch order; None keeps them)."""

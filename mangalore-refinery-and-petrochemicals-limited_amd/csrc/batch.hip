@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -95,7 +96,9 @@ struct EpiStore2 {
     const double* v1;
     double* o0;
     double* o1;
+    const unsigned char* wg_live;  // 0 for a workgroup whose LPs all stopped
     __device__ const double* vec(int v) const { return v ? v1 : v0; }
+    __device__ bool wg_dead(int b) const { return !wg_live[b]; }
     __device__ void operator()(int seg, const double* acc, double*) const {
         o0[seg] = acc[0];
         o1[seg] = acc[1];
@@ -414,6 +417,20 @@ __global__ void kb_tile_active(int ntiles, const TileDesc* tiles, const int* ins
     active[t] = a;
 }
 
+// Workgroup flags: workgroup b of a launch that gives `per` consecutive
+// segments to each workgroup is live while any LP among its segments runs
+// (segments are in LP order, so its LPs are inst[first] .. inst[last]).
+__global__ void kb_wg_live(int nwg, int per, int nseg, const int* inst, const PdlpState* st, unsigned char* flags) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nwg) return;
+    const int s0 = b * per;
+    const int s1 = min(nseg, s0 + per) - 1;
+    unsigned char a = 0;
+    if (s0 <= s1)
+        for (int k = inst[s0]; k <= inst[s1] && !a; ++k) a = st[k].done ? 0 : 1;
+    flags[b] = a;
+}
+
 __global__ void kb_count_running(int K, const PdlpState* st, int* running) {
     __shared__ double sh[kBlock / kWave];
     double c = 0.0;
@@ -470,6 +487,8 @@ struct BatchBase {
     virtual bo_status set_primal_weights(const double* w) = 0;
     virtual bo_status set_objective_rhs(const double* c, const double* b) = 0;
     virtual void set_iteration_limits(const long long* lim) = 0;
+    virtual void set_stop_running(int r) = 0;
+    virtual bo_status time_kernels(int reps, double* ms4, double* bytes4) = 0;
     virtual bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) = 0;
     virtual bo_status get_solution(int k, double* x, double* y) = 0;
 };
@@ -509,6 +528,9 @@ struct Batch final : BatchBase {
     int* running = nullptr;
     int* running_host = nullptr;
     BinTable bt_r{}, bt_c{};
+    // live flags of the workgroups of the reduce (256 segments each) and
+    // check-product (bt_r / bt_c) launches
+    unsigned char *live_red_r = nullptr, *live_red_c = nullptr, *live_chk_r = nullptr, *live_chk_c = nullptr;
     // tiled passes
     struct TilePass {
         bool built = false;
@@ -530,6 +552,7 @@ struct Batch final : BatchBase {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_k = 0;
+    int stop_running = 0;
 
     ~Batch() override {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
@@ -722,12 +745,17 @@ struct Batch final : BatchBase {
         BO_TRY(hipStreamSynchronize(stream));
         bt_r = one_bin(m, 4);
         bt_c = one_bin(n, 4);
+        BO_TRY(mem.alloc(&live_red_r, grid_for(m)));
+        BO_TRY(mem.alloc(&live_red_c, grid_for(n)));
+        BO_TRY(mem.alloc(&live_chk_r, bt_r.total_blocks));
+        BO_TRY(mem.alloc(&live_chk_c, bt_c.total_blocks));
         int dev = 0;
         BO_TRY(hipGetDevice(&dev));
         BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        BO_TRY(build_tiles_pass(0, h_rp, h_ci));
-        BO_TRY(build_tiles_pass(1, h_cp, h_ri));
+        // a positive schedule code keeps that pass on the segment kernel (no tile layout to build)
+        if (prm.lanes_rows <= 0) BO_TRY(build_tiles_pass(0, h_rp, h_ci));
+        if (prm.lanes_cols <= 0) BO_TRY(build_tiles_pass(1, h_cp, h_ri));
         return BO_SUCCESS;
     }
 
@@ -802,13 +830,14 @@ struct Batch final : BatchBase {
             if (tp[pass].built)
                 hipLaunchKernelGGL(kb_tile_active, dim3(grid_for(tp[pass].ntiles)), dim3(kBlock), 0, stream,
                                    tp[pass].ntiles, tp[pass].td, pass == 0 ? inst_r : inst_c, st, tp[pass].active);
+        launch_live_flags();
         BO_TRY(hipGetLastError());
         return BO_SUCCESS;
     }
 
     void launch_primal(int t) {
-        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, inst_c};
         const TilePass& tl = tp[1];
+        EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, inst_c, tl.built ? live_red_c : live_chk_c};
         if (tl.built) {
             hipLaunchKernelGGL((k_tile<T, kTileRows, 3>), dim3(tl.ntiles), dim3(kTileThreads),
                                static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, yg, m,
@@ -819,8 +848,8 @@ struct Batch final : BatchBase {
         }
     }
     void launch_dual(int t) {
-        EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, inst_r};
         const TilePass& tl = tp[0];
+        EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, inst_r, tl.built ? live_red_r : live_chk_r};
         if (tl.built) {
             hipLaunchKernelGGL((k_tile<T, kTileRows, 2>), dim3(tl.ntiles), dim3(kTileThreads),
                                static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, xp, n,
@@ -830,6 +859,18 @@ struct Batch final : BatchBase {
             rows<kSumProd>(rv, e);
         }
     }
+    void launch_live_flags() {
+        const int pr = kBlock, pc = kBlock, qr = kBlock / 4, qc = kBlock / 4;  // one_bin(., 4): 64 segments
+        hipLaunchKernelGGL(kb_wg_live, dim3(grid_for(grid_for(m))), dim3(kBlock), 0, stream,
+                           static_cast<int>(grid_for(m)), pr, m, inst_r, st, live_red_r);
+        hipLaunchKernelGGL(kb_wg_live, dim3(grid_for(grid_for(n))), dim3(kBlock), 0, stream,
+                           static_cast<int>(grid_for(n)), pc, n, inst_c, st, live_red_c);
+        hipLaunchKernelGGL(kb_wg_live, dim3(grid_for(bt_r.total_blocks)), dim3(kBlock), 0, stream,
+                           bt_r.total_blocks, qr, m, inst_r, st, live_chk_r);
+        hipLaunchKernelGGL(kb_wg_live, dim3(grid_for(bt_c.total_blocks)), dim3(kBlock), 0, stream,
+                           bt_c.total_blocks, qc, n, inst_c, st, live_chk_c);
+    }
+
     template <class Epi>
     void launch_reduce(const TilePass& tl, int nseg, const Epi& e) {
         launch_tile_reduce(stream, tl.max_pairs, nseg, tl.seg_ptr, tl.slot, static_cast<int>(tl.pairs), tl.part,
@@ -840,10 +881,10 @@ struct Batch final : BatchBase {
         const unsigned gw = grid_for(static_cast<size_t>(std::max(K, 1)) * kWave);
         hipLaunchKernelGGL(kb_check_x<T>, dim3(ncch), dim3(kBlock), 0, stream, cch, st, xk, xp, cs, c64, lb64, ub64,
                            xo_cur, xo_cand, part_x);
-        rows<kSumProd>(rv64, EpiStore2{xo_cur, xo_cand, ar0, ar1});
+        rows<kSumProd>(rv64, EpiStore2{xo_cur, xo_cand, ar0, ar1, live_chk_r});
         hipLaunchKernelGGL(kb_check_rows<T>, dim3(nrch), dim3(kBlock), 0, stream, rch, st, ar0, ar1, b64, eq, rs, yk,
                            yp, yo_cur, yo_cand, ax_cand, part_r);
-        cols<kSumProd>(cv64, EpiStore2{yo_cur, yo_cand, ac0, ac1});
+        cols<kSumProd>(cv64, EpiStore2{yo_cur, yo_cand, ac0, ac1, live_chk_c});
         hipLaunchKernelGGL(kb_check_cols, dim3(ncch), dim3(kBlock), 0, stream, cch, st, ac0, ac1, c64, lb64, ub64, cs,
                            part_c);
         hipLaunchKernelGGL(kb_decide, dim3(gw), dim3(kBlock), 0, stream, K, st, cch_ptr, part_x, part_c, rch_ptr,
@@ -857,6 +898,7 @@ struct Batch final : BatchBase {
             if (tp[pass].built)
                 hipLaunchKernelGGL(kb_tile_active, dim3(grid_for(tp[pass].ntiles)), dim3(kBlock), 0, stream,
                                    tp[pass].ntiles, tp[pass].td, pass == 0 ? inst_r : inst_c, st, tp[pass].active);
+        launch_live_flags();
         hipLaunchKernelGGL(kb_count_running, dim3(1), dim3(kBlock), 0, stream, K, st, running);
     }
 
@@ -890,6 +932,77 @@ struct Batch final : BatchBase {
         for (int k = 0; k < K; ++k) st_init[k].omega_init = w ? w[k] : 0.0;
         return BO_SUCCESS;
     }
+
+    // Mean time of each iteration kernel with every LP running, on the state
+    // the last solve left (restored afterwards): [CSC gather, CSC reduce +
+    // primal epilogue, CSR gather, CSR reduce + dual epilogue], HIP events on
+    // the engine's stream; bytes4: their algorithmic bytes (matrix entries in
+    // the tiled format, the gathered vector once, slice words, tile table,
+    // partials; the reduce's pointers, slots, partials, row / column -> LP
+    // map and epilogue reads and writes).
+    bo_status time_kernels(int reps, double* ms4, double* bytes4) override {
+        if (!tp[0].built || !tp[1].built || reps <= 0) return BO_INVALID_ARGUMENT;
+        std::vector<PdlpState> keep(K), live;
+        BO_TRY(hipMemcpyAsync(keep.data(), st, sizeof(PdlpState) * K, hipMemcpyDeviceToHost, stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        live = keep;
+        for (PdlpState& q : live) q.done = 0;
+        auto flags = [&]() {
+            for (int pass = 0; pass < 2; ++pass)
+                hipLaunchKernelGGL(kb_tile_active, dim3(grid_for(tp[pass].ntiles)), dim3(kBlock), 0, stream,
+                                   tp[pass].ntiles, tp[pass].td, pass == 0 ? inst_r : inst_c, st, tp[pass].active);
+            launch_live_flags();
+        };
+        BO_TRY(hipMemcpyAsync(st, live.data(), sizeof(PdlpState) * K, hipMemcpyHostToDevice, stream));
+        flags();
+        hipEvent_t ev[5];
+        for (hipEvent_t& e : ev) BO_TRY(hipEventCreate(&e));
+        double acc[4] = {0, 0, 0, 0};
+        for (int r = 0; r < reps; ++r) {
+            const TilePass& tc = tp[1];
+            const TilePass& tr = tp[0];
+            BO_TRY(hipEventRecord(ev[0], stream));
+            hipLaunchKernelGGL((k_tile<T, kTileRows, 3>), dim3(tc.ntiles), dim3(kTileThreads),
+                               static_cast<unsigned>(tc.W * sizeof(T)), stream, tc.td, tc.words, tc.val, tc.idx, yg, m,
+                               tc.W, tc.part, tc.active);
+            BO_TRY(hipEventRecord(ev[1], stream));
+            launch_reduce(tc, n, EpiPrimal<T>{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, 0, inst_c, live_red_c});
+            BO_TRY(hipEventRecord(ev[2], stream));
+            hipLaunchKernelGGL((k_tile<T, kTileRows, 2>), dim3(tr.ntiles), dim3(kTileThreads),
+                               static_cast<unsigned>(tr.W * sizeof(T)), stream, tr.td, tr.words, tr.val, tr.idx, xp, n,
+                               tr.W, tr.part, tr.active);
+            BO_TRY(hipEventRecord(ev[3], stream));
+            launch_reduce(tr, m, EpiDual<T>{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, 0, inst_r, live_red_r});
+            BO_TRY(hipEventRecord(ev[4], stream));
+            BO_TRY(hipEventSynchronize(ev[4]));
+            for (int q = 0; q < 4; ++q) {
+                float t = 0.0f;
+                BO_TRY(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+                acc[q] += t;
+            }
+        }
+        for (hipEvent_t& e : ev) hipEventDestroy(e);
+        BO_TRY(hipMemcpyAsync(st, keep.data(), sizeof(PdlpState) * K, hipMemcpyHostToDevice, stream));
+        flags();
+        BO_TRY(hipStreamSynchronize(stream));
+        const double tb = sizeof(T);
+        for (int q = 0; q < 4; ++q) ms4[q] = acc[q] / reps;
+        for (int pass = 0; pass < 2; ++pass) {
+            const TilePass& t = tp[pass];
+            const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
+            const double epi = pass == 0 ? 49.0 + 3.0 * tb : 24.0 + 4.0 * tb;
+            const double g = (tb + 2.0) * static_cast<double>(nnz) + tb * ng + 4.0 * t.nslices +
+                             static_cast<double>(sizeof(TileDesc)) * t.ntiles + 8.0 * static_cast<double>(t.pairs);
+            const double rd = 4.0 * (nseg + 1) + 12.0 * static_cast<double>(t.pairs) + (epi + 4.0) * nseg;
+            bytes4[pass == 1 ? 0 : 2] = g;
+            bytes4[pass == 1 ? 1 : 3] = rd;
+        }
+        return BO_SUCCESS;
+    }
+
+    // The solve loop returns once at most this many LPs still run (they keep
+    // status 1 with fewer iterations than their budget).
+    void set_stop_running(int r) override { stop_running = r; }
 
     // Iteration budget of every LP (at most max_iterations of the parameters).
     void set_iteration_limits(const long long* lim) override {
@@ -933,7 +1046,7 @@ struct Batch final : BatchBase {
         BO_TRY(hipStreamSynchronize(stream));
         const int Kc = prm.check_every > 0 ? prm.check_every : 64;
         long long iters = 0;
-        while (*running_host > 0) {
+        while (*running_host > stop_running) {
             const long long left = prm.max_iterations - iters;
             const int k = static_cast<int>(std::min<long long>(Kc, left));
             if (k <= 0) break;
@@ -957,6 +1070,10 @@ struct Batch final : BatchBase {
             iters += k;
             BO_TRY(hipMemcpyAsync(running_host, running, sizeof(int), hipMemcpyDeviceToHost, stream));
             BO_TRY(hipStreamSynchronize(stream));
+            if (prm.verbose >= 2)
+                std::fprintf(stderr, "batch: step %lld running %d at %.3f ms\n", iters, *running_host,
+                             std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0)
+                                 .count());
             if (prm.time_limit_s > 0.0 &&
                 std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count() >
                     prm.time_limit_s)
@@ -1075,10 +1192,13 @@ struct HostLp {
 struct bo_pdlp_batch {
     std::unique_ptr<bo::BatchBase> eng;
     bo_pdlp_params prm{};
-    std::vector<bo::HostLp> lps;           // kept when the FP64 polish may run
-    std::unique_ptr<bo::BatchBase> pol;    // FP64 engine of the last solve's polished LPs
-    std::vector<int> pol_of;               // LP -> its index in pol, or -1
+    int count = 0;
+    std::vector<bo::HostLp> lps;             // kept for the tail rounds
+    std::vector<std::vector<double>> fx, fy;  // solutions of LPs finished in a tail round
+    std::vector<char> in_tail;               // 1: the LP's last solution is in fx / fy
     bool polishing() const { return !prm.fp64_iterates && prm.fp64_polish && prm.stall_iterations > 0; }
+    bool compacting() const { return prm.compact_tail != 0; }
+    bool keep_lps() const { return polishing() || compacting(); }
 };
 
 using namespace bo;
@@ -1115,8 +1235,11 @@ bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params
     }
     if (st != BO_SUCCESS) return st;
     h->prm = p;
-    h->pol_of.assign(count, -1);
-    if (h->polishing()) {
+    h->count = count;
+    h->in_tail.assign(count, 0);
+    h->fx.resize(count);
+    h->fy.resize(count);
+    if (h->keep_lps()) {
         h->lps.reserve(count);
         for (int k = 0; k < count; ++k) h->lps.emplace_back(lps[k]);
     }
@@ -1148,74 +1271,147 @@ bo_status bo_pdlp_batch_set_objective_rhs(bo_pdlp_batch* h, const double* c, con
     return BO_SUCCESS;
 }
 
-// The FP32 batch, then (fp64_polish) one FP64 batch over the LPs it stopped
-// as STALLED, each warm-started from its FP32 point with its last primal
-// weight and the iterations it has left.
+}  // extern "C"
+
+namespace {
+
+// An LP handed from one round to the next: its starting point, primal
+// weight and the iterations it has used.
+struct TailLp {
+    int k;
+    long long used, fp64_used;
+    double w;
+    std::vector<double> x, y;
+};
+
+bool tail_from_round(const bo_pdlp_stats& s, long long lim) { return s.status == 1 && s.iterations < lim; }
+
+// One compact round over `in` (segment kernels, no tile layout): LPs that
+// finish are final (stats and solution kept on the handle), LPs cut by the
+// round's stop rule go to `again`, LPs that stall (FP32 rounds) go to `to64`.
+template <class T>
+bo_status tail_round(bo_pdlp_batch* h, std::vector<TailLp>& in, std::vector<TailLp>& again, std::vector<TailLp>& to64,
+                     std::vector<bo_pdlp_stats>& per) {
+    const bool is64 = sizeof(T) == sizeof(double);
+    const int P = static_cast<int>(in.size());
+    auto rc0 = std::chrono::high_resolution_clock::now();
+    std::vector<bo_lp> views(P);
+    std::vector<long long> lim(P);
+    std::vector<double> w(P), wx, wy;
+    long long most = 0;
+    for (int j = 0; j < P; ++j) {
+        views[j] = h->lps[in[j].k].view();
+        lim[j] = h->prm.max_iterations - in[j].used;
+        most = std::max(most, lim[j]);
+        w[j] = in[j].w;
+        wx.insert(wx.end(), in[j].x.begin(), in[j].x.end());
+        wy.insert(wy.end(), in[j].y.begin(), in[j].y.end());
+    }
+    bo_pdlp_params p = h->prm;
+    p.fp64_iterates = is64 ? 1 : 0;
+    p.max_iterations = std::max(1LL, most);
+    p.lanes_rows = p.lanes_cols = 4;  // segment kernels: no host tile layout for a few LPs
+    Batch<T> e;
+    bo_status s = e.create(views.data(), P, &p);
+    if (s != BO_SUCCESS) return s;
+    e.set_iteration_limits(lim.data());
+    e.set_stop_running(h->compacting() && P >= 16 ? P / 8 : 0);
+    if ((s = e.set_primal_weights(w.data())) != BO_SUCCESS) return s;
+    if ((s = e.set_warm_start(wx.data(), wy.data())) != BO_SUCCESS) return s;
+    std::vector<bo_pdlp_stats> per2(P);
+    bo_pdlp_batch_stats tot2{};
+    auto r0 = std::chrono::high_resolution_clock::now();
+    if ((s = e.solve(per2.data(), &tot2)) != BO_SUCCESS) return s;
+    if (h->prm.verbose >= 2)
+        std::fprintf(stderr, "batch tail round: %s, %d LPs, %lld steps, solved %d, %.1f ms (set-up %.1f ms)\n",
+                     is64 ? "fp64" : "fp32", P, tot2.loop_steps, tot2.solved,
+                     std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - r0).count(),
+                     std::chrono::duration<double, std::milli>(r0 - rc0).count());
+    for (int j = 0; j < P; ++j) {
+        const int k = in[j].k;
+        std::vector<double> x(h->lps[k].n), y(h->lps[k].m);
+        if ((s = e.get_solution(j, x.data(), y.data())) != BO_SUCCESS) return s;
+        TailLp t{k, in[j].used + per2[j].iterations, in[j].fp64_used + (is64 ? per2[j].iterations : 0),
+                 per2[j].primal_weight, std::move(x), std::move(y)};
+        if (tail_from_round(per2[j], lim[j])) {
+            again.push_back(std::move(t));
+        } else if (per2[j].status == 4 && !is64 && h->polishing()) {
+            to64.push_back(std::move(t));
+        } else {
+            per[k] = per2[j];
+            per[k].iterations = t.used;
+            per[k].fp64_polish_iterations = t.fp64_used;
+            h->fx[k] = std::move(t.x);
+            h->fy[k] = std::move(t.y);
+            h->in_tail[k] = 1;
+        }
+    }
+    return BO_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+// The main batch, then the tail: with compact_tail the main batch returns
+// once at most 1/8 of its LPs still run, and those continue in compact
+// batches (segment kernels, no tile layout) that again hand over their last
+// eighth, until every LP has stopped; with fp64_polish an LP whose FP32
+// iterates stall continues in an FP64 batch.  Every LP carries over its
+// point, its primal weight and the iterations it has used.
 bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
     if (!h) return BO_INVALID_ARGUMENT;
     auto t0 = std::chrono::high_resolution_clock::now();
-    const int K = static_cast<int>(h->pol_of.size());
+    const int K = h->count;
     std::vector<bo_pdlp_stats> per(K);
     bo_pdlp_batch_stats tot{};
-    h->pol.reset();
-    std::fill(h->pol_of.begin(), h->pol_of.end(), -1);
+    std::fill(h->in_tail.begin(), h->in_tail.end(), 0);
+    h->eng->set_stop_running(h->compacting() && K >= 16 ? K / 8 : 0);
     bo_status s = h->eng->solve(per.data(), &tot);
     if (s != BO_SUCCESS) return s;
-    std::vector<int> stalled;
-    if (h->polishing())
-        for (int k = 0; k < K; ++k)
-            if (per[k].status == 4) stalled.push_back(k);
-    if (!stalled.empty()) {
-        const int P = static_cast<int>(stalled.size());
-        std::vector<bo_lp> views(P);
-        std::vector<long long> lim(P);
-        std::vector<double> w(P), wx, wy;
-        long long most = 0;
-        for (int j = 0; j < P; ++j) {
-            const int k = stalled[j];
-            views[j] = h->lps[k].view();
-            lim[j] = h->prm.max_iterations - per[k].iterations;
-            most = std::max(most, lim[j]);
-            w[j] = per[k].primal_weight;
-            std::vector<double> x(h->lps[k].n), y(h->lps[k].m);
-            s = h->eng->get_solution(k, x.data(), y.data());
-            if (s != BO_SUCCESS) return s;
-            wx.insert(wx.end(), x.begin(), x.end());
-            wy.insert(wy.end(), y.begin(), y.end());
-        }
-        if (most > 0) {
-            bo_pdlp_params p = h->prm;
-            p.fp64_iterates = 1;
-            p.max_iterations = most;
-            auto e = std::make_unique<Batch<double>>();
-            s = e->create(views.data(), P, &p);
-            if (s != BO_SUCCESS) return s;
-            e->set_iteration_limits(lim.data());
-            if ((s = e->set_primal_weights(w.data())) != BO_SUCCESS) return s;
-            if ((s = e->set_warm_start(wx.data(), wy.data())) != BO_SUCCESS) return s;
-            std::vector<bo_pdlp_stats> per2(P);
-            bo_pdlp_batch_stats tot2{};
-            if ((s = e->solve(per2.data(), &tot2)) != BO_SUCCESS) return s;
-            for (int j = 0; j < P; ++j) {
-                const int k = stalled[j];
-                const long long fp32_its = per[k].iterations;
-                per[k] = per2[j];
-                per[k].iterations += fp32_its;
-                per[k].fp64_polish_iterations = per2[j].iterations;
-                h->pol_of[k] = j;
+    std::vector<TailLp> same, p64;
+    const bool main64 = h->prm.fp64_iterates != 0;
+    for (int k = 0; k < K; ++k) {
+        const bool cut = h->compacting() && tail_from_round(per[k], h->prm.max_iterations);
+        const bool stalled = per[k].status == 4 && h->polishing();
+        if (!cut && !stalled) continue;
+        TailLp t{k, per[k].iterations, main64 ? per[k].iterations : 0, per[k].primal_weight, {}, {}};
+        t.x.resize(h->lps[k].n);
+        t.y.resize(h->lps[k].m);
+        if ((s = h->eng->get_solution(k, t.x.data(), t.y.data())) != BO_SUCCESS) return s;
+        (stalled ? p64 : same).push_back(std::move(t));
+    }
+    if (!same.empty() || !p64.empty()) {
+        auto tp0 = std::chrono::high_resolution_clock::now();
+        int tail_lps = static_cast<int>(same.size() + p64.size());
+        std::vector<TailLp> unused;
+        while (!same.empty() || !p64.empty()) {
+            if (!same.empty()) {
+                std::vector<TailLp> in;
+                in.swap(same);
+                s = main64 ? tail_round<double>(h, in, same, unused, per) : tail_round<float>(h, in, same, p64, per);
+                if (s != BO_SUCCESS) return s;
             }
-            h->pol = std::move(e);
+            if (!p64.empty()) {
+                std::vector<TailLp> in;
+                in.swap(p64);
+                if ((s = tail_round<double>(h, in, p64, unused, per)) != BO_SUCCESS) return s;
+            }
         }
+        tot.tail_lps = tail_lps;
+        tot.tail_ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tp0).count();
     }
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
     tot.solved = 0;
     tot.max_iterations = 0;
     tot.sum_iterations = 0;
+    tot.polished = 0;
     for (int k = 0; k < K; ++k) {
         per[k].solve_ms = ms;
         tot.solved += per[k].status == 0;
         tot.max_iterations = std::max(tot.max_iterations, per[k].iterations);
         tot.sum_iterations += per[k].iterations;
+        tot.polished += (!main64 && per[k].fp64_polish_iterations > 0) ? 1 : 0;
     }
     tot.solve_ms = ms;
     if (per_lp) std::copy(per.begin(), per.end(), per_lp);
@@ -1223,9 +1419,18 @@ bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_b
     return BO_SUCCESS;
 }
 
+bo_status bo_pdlp_batch_time_kernels(bo_pdlp_batch* h, int reps, double* ms4, double* bytes4) {
+    if (!h || !ms4 || !bytes4) return BO_INVALID_ARGUMENT;
+    return h->eng->time_kernels(reps, ms4, bytes4);
+}
+
 bo_status bo_pdlp_batch_get_solution(bo_pdlp_batch* h, int k, double* x, double* y) {
-    if (!h || k < 0 || k >= static_cast<int>(h->pol_of.size())) return BO_INVALID_ARGUMENT;
-    if (h->pol && h->pol_of[k] >= 0) return h->pol->get_solution(h->pol_of[k], x, y);
+    if (!h || k < 0 || k >= h->count) return BO_INVALID_ARGUMENT;
+    if (h->in_tail[k]) {
+        if (x) std::copy(h->fx[k].begin(), h->fx[k].end(), x);
+        if (y) std::copy(h->fy[k].begin(), h->fy[k].end(), y);
+        return BO_SUCCESS;
+    }
     return h->eng->get_solution(k, x, y);
 }
 

@@ -44,6 +44,17 @@ inline unsigned grid_for(size_t n, int block = kBlock) {
 }
 
 // ---- wave / block reductions (fixed tree order => deterministic) ----------
+// Whole-workgroup skip: an epilogue with a member wg_dead(block) can tell a
+// kernel that every segment of its workgroup belongs to finished work.
+template <class E>
+__device__ inline auto wg_dead(const E& e, int block, int) -> decltype(e.wg_dead(block)) {
+    return e.wg_dead(block);
+}
+template <class E>
+__device__ inline bool wg_dead(const E&, int, long) {
+    return false;
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);

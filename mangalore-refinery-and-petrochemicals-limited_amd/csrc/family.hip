@@ -59,7 +59,8 @@ __device__ inline double halpern_at(const LaneCoef* cf, int lane, int t) {
 }
 
 // ----------------------------------------------------------------- products
-constexpr int kXcds = 8;  // L2 domains of the chip; workgroup b runs on XCD b % 8
+constexpr int kXcds = 8;    // L2 domains of the chip; workgroup b runs on XCD b % 8
+constexpr int kUnroll = 8;  // matrix entries in flight per wavefront
 
 // Launch grid of a product pass over nseg segments: a multiple of kXcds.
 inline unsigned spmm_grid(int nseg) {
@@ -86,35 +87,37 @@ kf_spmm(int nseg, const int* __restrict__ ptr, const int* __restrict__ idx, cons
     using TV = typename Epi::TV;
     const TV* v0 = epi.vec(0) + lane;
     const TV* v1 = epi.vec(Epi::NV - 1) + lane;
+    // the epilogue's own operands are fetched first, so their latency hides
+    // behind the gathers
+    const typename Epi::Pre pre = epi.load(s, lane);
     double a0 = 0.0, a1 = 0.0;
-    int q = b;
-    for (; q + 4 <= e; q += 4) {
-        const int j0 = idx[q], j1 = idx[q + 1], j2 = idx[q + 2], j3 = idx[q + 3];
-        const double w0 = static_cast<double>(val[q]), w1 = static_cast<double>(val[q + 1]);
-        const double w2 = static_cast<double>(val[q + 2]), w3 = static_cast<double>(val[q + 3]);
-        const size_t o0 = static_cast<size_t>(j0) * kL, o1 = static_cast<size_t>(j1) * kL;
-        const size_t o2 = static_cast<size_t>(j2) * kL, o3 = static_cast<size_t>(j3) * kL;
-        const double x0 = static_cast<double>(v0[o0]), x1 = static_cast<double>(v0[o1]);
-        const double x2 = static_cast<double>(v0[o2]), x3 = static_cast<double>(v0[o3]);
-        a0 += w0 * x0;
-        a0 += w1 * x1;
-        a0 += w2 * x2;
-        a0 += w3 * x3;
-        if (Epi::NV == 2) {
-            a1 += w0 * static_cast<double>(v1[o0]);
-            a1 += w1 * static_cast<double>(v1[o1]);
-            a1 += w2 * static_cast<double>(v1[o2]);
-            a1 += w3 * static_cast<double>(v1[o3]);
+    // kUnroll entries per round trip; entries past the segment (the arrays
+    // carry kUnroll entries of padding) get weight 0.  The sum runs in
+    // entry order.
+    for (int q = b; q < e; q += kUnroll) {
+        int j[kUnroll];
+        double w[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            j[u] = idx[q + u];
+            const double a = static_cast<double>(val[q + u]);
+            w[u] = q + u < e ? a : 0.0;
         }
-    }
-    for (; q < e; ++q) {
-        const size_t o = static_cast<size_t>(idx[q]) * kL;
-        const double w = static_cast<double>(val[q]);
-        a0 += w * static_cast<double>(v0[o]);
-        if (Epi::NV == 2) a1 += w * static_cast<double>(v1[o]);
+        double x[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) x[u] = static_cast<double>(v0[static_cast<size_t>(j[u]) * kL]);
+        if (Epi::NV == 2) {
+            double z[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) z[u] = static_cast<double>(v1[static_cast<size_t>(j[u]) * kL]);
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) a1 += w[u] * z[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) a0 += w[u] * x[u];
     }
     const double acc[2] = {a0, a1};
-    epi(s, lane, acc);
+    epi(s, lane, acc, pre);
 }
 
 // Column j of LP `lane`: primal step from A'y, Halpern update of x.
@@ -131,17 +134,22 @@ struct EpiFPrimal {
     const double* x0;
     T* xp;
     int t;
+    struct Pre {
+        double xo, x0, c;
+        T lo, hi;
+    };
     __device__ const T* vec(int) const { return yg; }
     __device__ bool live(int lane) const { return cf->live[lane] != 0; }
-    __device__ void operator()(int j, int lane, const double* acc) const {
-        if (!cf->live[lane]) return;
+    __device__ Pre load(int j, int lane) const {
         const size_t e = static_cast<size_t>(j) * kL + lane;
-        const double xo = xk[e];
-        const T xt = clampT(static_cast<T>(xo - cf->tau[lane] * (static_cast<double>(c_hat[e]) + acc[0])), l_hat[j],
-                            u_hat[j]);
+        return Pre{xk[e], x0[e], static_cast<double>(c_hat[e]), l_hat[j], u_hat[j]};
+    }
+    __device__ void operator()(int j, int lane, const double* acc, const Pre& p) const {
+        const size_t e = static_cast<size_t>(j) * kL + lane;
+        const T xt = clampT(static_cast<T>(p.xo - cf->tau[lane] * (p.c + acc[0])), p.lo, p.hi);
         xp[e] = xt;
         const double w = halpern_at(cf, lane, t);
-        xk[e] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[e];
+        xk[e] = w * (2.0 * static_cast<double>(xt) - p.xo) + (1.0 - w) * p.x0;
     }
 };
 
@@ -161,21 +169,27 @@ struct EpiFDual {
     T* yp;
     T* yg;
     int t;
+    struct Pre {
+        double axo, yo, b, y0, ax0;
+        bool eq;
+    };
     __device__ const T* vec(int) const { return xp; }
     __device__ bool live(int lane) const { return cf->live[lane] != 0; }
-    __device__ void operator()(int i, int lane, const double* acc) const {
-        if (!cf->live[lane]) return;
+    __device__ Pre load(int i, int lane) const {
         const size_t e = static_cast<size_t>(i) * kL + lane;
-        const double axo = ax[e], yo = yk[e];
-        double yn = yo + cf->sigma[lane] * (2.0 * acc[0] - axo - static_cast<double>(b_hat[e]));
-        if (!eq[i] && yn < 0.0) yn = 0.0;
+        return Pre{ax[e], yk[e], static_cast<double>(b_hat[e]), y0[e], ax0[e], eq[i] != 0};
+    }
+    __device__ void operator()(int i, int lane, const double* acc, const Pre& p) const {
+        const size_t e = static_cast<size_t>(i) * kL + lane;
+        double yn = p.yo + cf->sigma[lane] * (2.0 * acc[0] - p.axo - p.b);
+        if (!p.eq && yn < 0.0) yn = 0.0;
         const T yt = static_cast<T>(yn);
         yp[e] = yt;
         const double w = halpern_at(cf, lane, t);
-        const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[e];
+        const double ynew = w * (2.0 * static_cast<double>(yt) - p.yo) + (1.0 - w) * p.y0;
         yk[e] = ynew;
         yg[e] = static_cast<T>(ynew);
-        ax[e] = w * (2.0 * acc[0] - axo) + (1.0 - w) * ax0[e];
+        ax[e] = w * (2.0 * acc[0] - p.axo) + (1.0 - w) * p.ax0;
     }
 };
 
@@ -187,9 +201,11 @@ struct EpiFStore {
     const TV* v;
     double* out0;
     double* out1;  // may be null
+    struct Pre {};
     __device__ const TV* vec(int) const { return v; }
     __device__ bool live(int) const { return true; }
-    __device__ void operator()(int s, int lane, const double* acc) const {
+    __device__ Pre load(int, int) const { return Pre{}; }
+    __device__ void operator()(int s, int lane, const double* acc, const Pre&) const {
         const size_t e = static_cast<size_t>(s) * kL + lane;
         out0[e] = acc[0];
         if (out1) out1[e] = acc[0];
@@ -205,10 +221,11 @@ struct EpiFCheck2 {
     const LaneCoef* cf;
     double* out0;
     double* out1;
+    struct Pre {};
     __device__ const double* vec(int k) const { return k == 0 ? v0 : v1; }
     __device__ bool live(int lane) const { return cf->live[lane] != 0; }
-    __device__ void operator()(int s, int lane, const double* acc) const {
-        if (!cf->live[lane]) return;
+    __device__ Pre load(int, int) const { return Pre{}; }
+    __device__ void operator()(int s, int lane, const double* acc, const Pre&) const {
         const size_t e = static_cast<size_t>(s) * kL + lane;
         out0[e] = acc[0];
         out1[e] = acc[1];
@@ -714,13 +731,20 @@ struct Family final : FamilyBase {
         // device
         const size_t NL = static_cast<size_t>(n) * kL, ML = static_cast<size_t>(m) * kL;
         BO_TRY(mem.alloc(&rp, m + 1));
-        BO_TRY(mem.alloc(&ci, nnz));
-        BO_TRY(mem.alloc(&rv, nnz));
-        BO_TRY(mem.alloc(&rv64, nnz));
+        BO_TRY(mem.alloc(&ci, nnz + kUnroll));
+        BO_TRY(mem.alloc(&rv, nnz + kUnroll));
+        BO_TRY(mem.alloc(&rv64, nnz + kUnroll));
         BO_TRY(mem.alloc(&cp, n + 1));
-        BO_TRY(mem.alloc(&ri, nnz));
-        BO_TRY(mem.alloc(&cv, nnz));
-        BO_TRY(mem.alloc(&cv64, nnz));
+        BO_TRY(mem.alloc(&ri, nnz + kUnroll));
+        BO_TRY(mem.alloc(&cv, nnz + kUnroll));
+        BO_TRY(mem.alloc(&cv64, nnz + kUnroll));
+        // the padding: index 0, value 0 (read by the last round trip of the last segment)
+        BO_TRY(hipMemsetAsync(ci + nnz, 0, sizeof(int) * kUnroll, stream));
+        BO_TRY(hipMemsetAsync(ri + nnz, 0, sizeof(int) * kUnroll, stream));
+        BO_TRY(hipMemsetAsync(rv + nnz, 0, sizeof(double) * kUnroll, stream));
+        BO_TRY(hipMemsetAsync(cv + nnz, 0, sizeof(double) * kUnroll, stream));
+        BO_TRY(hipMemsetAsync(rv64 + nnz, 0, sizeof(double) * kUnroll, stream));
+        BO_TRY(hipMemsetAsync(cv64 + nnz, 0, sizeof(double) * kUnroll, stream));
         BO_TRY(mem.alloc(&rs, m));
         BO_TRY(mem.alloc(&cs, n));
         BO_TRY(mem.alloc(&lb64, n));
@@ -1156,6 +1180,7 @@ bo_status bo_pdlp_family_solve(bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp
         for (int k = 0; k < K; ++k)
             if (per[k].status == 4 && per[k].iterations < h->prm.max_iterations) stalled.push_back(k);
     if (!stalled.empty()) {
+        auto tp0 = std::chrono::high_resolution_clock::now();
         const int P = static_cast<int>(stalled.size());
         if (!h->pol) {
             bo_pdlp_params p = h->prm;
@@ -1196,6 +1221,10 @@ bo_status bo_pdlp_family_solve(bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp
             per[k].fp64_polish_iterations = per2[j].iterations;
             h->pol_of[k] = j;
         }
+        tot.polished = P;
+        tot.tail_lps = P;
+        tot.tail_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tp0).count();
     }
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
     tot.solved = 0;

@@ -1270,6 +1270,7 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
     p->hot_columns = -1;
     p->stall_iterations = 4096;
     p->fp64_polish = 1;
+    p->compact_tail = 1;
 }
 
 bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp** out) {

@@ -53,8 +53,10 @@ struct EpiPrimal {
     T* xp;             // x+ (gathered by the CSR pass)
     int t;             // step index within the interval
     const int* inst;   // nullptr: one LP
+    const unsigned char* wg_live = nullptr;  // batched: 0 for a reduce workgroup whose LPs all stopped
     __device__ const T* vec(int) const { return yg; }
     __device__ bool skip(int j) const { return inst && st[inst[j]].done; }
+    __device__ bool wg_dead(int b) const { return wg_live && !wg_live[b]; }
     __device__ void operator()(int j, const double* acc, double*) const {
         const PdlpState* s = inst ? st + inst[j] : st;
         if (inst && s->done) return;
@@ -84,8 +86,10 @@ struct EpiDual {
     T* yg;             // yk rounded to T (gathered by the next CSC pass)
     int t;
     const int* inst;   // nullptr: one LP
+    const unsigned char* wg_live = nullptr;  // batched: 0 for a reduce workgroup whose LPs all stopped
     __device__ const T* vec(int) const { return xp; }
     __device__ bool skip(int i) const { return inst && st[inst[i]].done; }
+    __device__ bool wg_dead(int b) const { return wg_live && !wg_live[b]; }
     __device__ void operator()(int i, const double* acc, double*) const {
         const PdlpState* s = inst ? st + inst[i] : st;
         if (inst && s->done) return;

@@ -177,6 +177,7 @@ k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx
            const TV* __restrict__ val, Epi epi, double* __restrict__ partials,
            const int* __restrict__ done) {
     if (done && *done) return;
+    if (wg_dead(epi, static_cast<int>(blockIdx.x), 0)) return;
     constexpr int Q = Epi::Q;
     const int vblock = static_cast<int>(blockIdx.x);
     int bin = 0;

@@ -428,6 +428,7 @@ __global__ void __launch_bounds__(kBlock)
 k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restrict__ slot, int nslot,
               const double* __restrict__ part, int npart, Epi epi, const int* __restrict__ done) {
     if (done && *done) return;
+    if (wg_dead(epi, static_cast<int>(blockIdx.x), 0)) return;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= nseg || epi.skip(s)) return;
     constexpr unsigned kOut = 0x7FFFFFF0u;  // beyond every range below (ranges are < 2^31 bytes)
