@@ -263,6 +263,19 @@ __device__ inline void sum_lane_parts(const double* part, int nblocks, int lane,
         for (int k = 0; k < NQ; ++k) out[k] += part[(static_cast<size_t>(bk) * NQ + k) * kL + lane];
 }
 
+// Quantity q of this lane summed over the workgroups' partials, with 8
+// interleaved accumulators combined in a fixed order (one wavefront per
+// quantity keeps the loads of a check's sums in flight together).
+__device__ inline double sum_lane_q(const double* part, int nblocks, int nq, int q, int lane) {
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int b = 0;
+    for (; b + 8 <= nblocks; b += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += part[(static_cast<size_t>(b + u) * nq + q) * kL + lane];
+    for (; b < nblocks; ++b) a[0] += part[(static_cast<size_t>(b) * nq + q) * kL + lane];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 __device__ inline int first_seg() { return static_cast<int>(blockIdx.x) * kSegWaves + static_cast<int>(threadIdx.x) / kWave; }
 __device__ inline int seg_stride() { return static_cast<int>(gridDim.x) * kSegWaves; }
 
@@ -432,23 +445,21 @@ kf_check_cols(int n, const LaneCoef* cf, const double* ac0, const double* ac1, c
     store_lane_sums<6>(q, part);
 }
 
-// One thread per lane: the check sums, then the stopping test, stall test and
+// 14 wavefronts (1024 threads): wavefront q adds check quantity q for all
+// lanes; then one thread per lane runs the stopping test, stall test and
 // restart rule of the single-LP engine.
-__global__ void kf_decide(PdlpState* st, const double* px, const double* pr, const double* pc, int ncol_blocks,
-                          int nrow_blocks) {
-    const int lane = static_cast<int>(threadIdx.x);
-    if (lane >= kL || st[lane].done) return;
-    double x[2], r[6], c[6], a[14];
-    sum_lane_parts<2>(px, ncol_blocks, lane, x);
-    sum_lane_parts<6>(pr, nrow_blocks, lane, r);
-    sum_lane_parts<6>(pc, ncol_blocks, lane, c);
-    a[0] = x[0];
-    a[1] = x[1];
+__global__ void __launch_bounds__(1024)
+kf_decide(PdlpState* st, const double* px, const double* pr, const double* pc, int ncol_blocks, int nrow_blocks) {
+    __shared__ double sh[14][kL];
+    const int lane = static_cast<int>(threadIdx.x) % kWave, w = static_cast<int>(threadIdx.x) / kWave;
+    if (w < 2) sh[w][lane] = sum_lane_q(px, ncol_blocks, 2, w, lane);
+    else if (w < 8) sh[w][lane] = sum_lane_q(pr, nrow_blocks, 6, w - 2, lane);
+    else if (w < 14) sh[w][lane] = sum_lane_q(pc, ncol_blocks, 6, w - 8, lane);
+    __syncthreads();
+    if (w != 0 || st[lane].done) return;
+    double a[14];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        a[2 + k] = r[k];
-        a[8 + k] = c[k];
-    }
+    for (int q = 0; q < 14; ++q) a[q] = sh[q][lane];
     decide_core(st + lane, a);
 }
 
@@ -497,14 +508,15 @@ kf_restart_y(int m, const PdlpState* st, double* yk, double* y0, const T* yp, T*
     store_lane_sums<1>(q, part);
 }
 
+// Two wavefronts: ||dx||^2 and ||dy||^2 of every lane, then the primal-weight update.
 __global__ void kf_restart_finish(PdlpState* st, const double* pdx, int ncol_blocks, const double* pdy,
                                   int nrow_blocks) {
-    const int lane = static_cast<int>(threadIdx.x);
-    if (lane >= kL || st[lane].done || !st[lane].restart) return;
-    double dx2, dy2;
-    sum_lane_parts<1>(pdx, ncol_blocks, lane, &dx2);
-    sum_lane_parts<1>(pdy, nrow_blocks, lane, &dy2);
-    restart_finish_core(st + lane, dx2, dy2);
+    __shared__ double sh[2][kL];
+    const int lane = static_cast<int>(threadIdx.x) % kWave, w = static_cast<int>(threadIdx.x) / kWave;
+    sh[w][lane] = w == 0 ? sum_lane_q(pdx, ncol_blocks, 1, 0, lane) : sum_lane_q(pdy, nrow_blocks, 1, 0, lane);
+    __syncthreads();
+    if (w != 0 || st[lane].done || !st[lane].restart) return;
+    restart_finish_core(st + lane, sh[0][lane], sh[1][lane]);
 }
 
 // ------------------------------------------------------------- data movement
@@ -880,13 +892,13 @@ struct Family final : FamilyBase {
                            EpiFCheck2{yo_cur, yo_cand, cf, ac0, ac1});
         hipLaunchKernelGGL(kf_check_cols, dim3(gn), dim3(256), 0, stream, n, cf, ac0, ac1, c64, lb64, ub64, cs,
                            part_c);
-        hipLaunchKernelGGL(kf_decide, dim3(1), dim3(kL), 0, stream, st, part_a, part_b, part_c, static_cast<int>(gn),
+        hipLaunchKernelGGL(kf_decide, dim3(1), dim3(1024), 0, stream, st, part_a, part_b, part_c, static_cast<int>(gn),
                            static_cast<int>(gm));
         hipLaunchKernelGGL(kf_restart_x<T>, dim3(gn), dim3(256), 0, stream, n, st, xk, x0, xp, part_a);
         hipLaunchKernelGGL(kf_restart_y<T>, dim3(gm), dim3(256), 0, stream, m, st, yk, y0, yp, yg, ax, ax0, ax_cand,
                            part_b);
-        hipLaunchKernelGGL(kf_restart_finish, dim3(1), dim3(kL), 0, stream, st, part_a, static_cast<int>(gn), part_b,
-                           static_cast<int>(gm));
+        hipLaunchKernelGGL(kf_restart_finish, dim3(1), dim3(2 * kL), 0, stream, st, part_a, static_cast<int>(gn),
+                           part_b, static_cast<int>(gm));
         hipLaunchKernelGGL(kf_coef, dim3(1), dim3(kL), 0, stream, st, cf, running);
     }
 
