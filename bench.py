@@ -171,9 +171,12 @@ def bench_batch(pkg, args):
             table[role] = {"ms": float(ms4[k]), "bytes": float(nb4[k]), "gbps": nb4[k] / (ms4[k] * 1e-3) / 1e9}
         kname = max(table, key=lambda k: table[k]["ms"])
         dom = table[kname]
+        traffic = _traffic_from_profiles(args.workload)
         roof = {"bound": "hbm", "kernel": kname, "achieved": dom["gbps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": dom["gbps"] / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom["bytes"],
-                "launch_ms": dom["ms"], "note": "every LP running (the first steps of a solve)"}
+                "frac": dom["gbps"] / HBM_PEAK_GBS,
+                "traffic": None if traffic is None or kname not in traffic else traffic[kname]["bytes"],
+                "algorithmic_bytes": dom["bytes"], "launch_ms": dom["ms"],
+                "note": "every LP running (the first steps of a solve)"}
     out = {
         "metric": METRIC,
         "value": solved / total,
@@ -335,8 +338,11 @@ def bench_c4(pkg, args):
         kname, ms, nb = ("family_primal_pass", fam_ms[0], pb) if fam_ms[0] >= fam_ms[1] else \
             ("family_dual_pass", fam_ms[1], db)
         gbps = nb / (ms * 1e-3) / 1e9
+        traffic = _traffic_from_profiles("c4")
         roof = {"bound": "hbm", "kernel": kname, "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": gbps / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": nb, "launch_ms": ms,
+                "frac": gbps / HBM_PEAK_GBS,
+                "traffic": None if traffic is None or kname not in traffic else traffic[kname]["bytes"],
+                "algorithmic_bytes": nb, "launch_ms": ms,
                 "passes_ms": {"primal": fam_ms[0], "dual": fam_ms[1]},
                 "passes_gbps": {"primal": pb / (fam_ms[0] * 1e-3) / 1e9, "dual": db / (fam_ms[1] * 1e-3) / 1e9}}
     out = {
