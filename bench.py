@@ -501,6 +501,7 @@ def main():
             "workload": desc, "m": lp.m, "n": lp.n, "nnz": lp.nnz, "eps_rel_kkt": args.eps,
             "iterates": "fp32", "checks": "fp64", "start": "cold",
             "lanes_rows": int(last["lanes_rows"]), "lanes_cols": int(last["lanes_cols"]),
+            "check_every": int(last["check_every"]),
             "parallelism": "single-gpu",
         },
         "solved": solved,

@@ -86,7 +86,8 @@ class PdlpStats(C.Structure):
                     "rel_kkt", "rel_primal", "rel_dual", "rel_gap", "primal_obj", "dual_obj",
                     "primal_weight", "step_size", "setup_ms", "solve_ms", "spmv_csr_bytes",
                     "spmv_csc_bytes")] + [("lanes_rows", C.c_int), ("lanes_cols", C.c_int),
-                                          ("hot_columns", C.c_int), ("fp64_polish_iterations", C.c_longlong)]
+                                          ("hot_columns", C.c_int), ("fp64_polish_iterations", C.c_longlong),
+                                          ("check_every", C.c_int)]
 
 
 class PdlpPassInfo(C.Structure):

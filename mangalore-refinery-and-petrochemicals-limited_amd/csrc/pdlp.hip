@@ -723,7 +723,7 @@ struct Engine final : EngineBase {
         // launch plans: lanes > 0 fixes the segment kernel's nonzeros-per-lane
         // target, lanes < 0 selects the stream kernel with LDS budget -lanes,
         // 0 lets the tuner choose
-        const bool tune = prm.lanes_rows == 0 || prm.lanes_cols == 0 || prm.hot_columns < 0;
+        const bool tune = prm.lanes_rows == 0 || prm.lanes_cols == 0 || prm.hot_columns < 0 || prm.check_every <= 0;
         plan_rows = fixed_plan(0, prm.lanes_rows);
         plan_cols = fixed_plan(1, prm.lanes_cols);
         if (prm.hot_columns > 0 && plan_rows.cap == 0 && plan_rows.tiled == 0) {
@@ -942,7 +942,7 @@ struct Engine final : EngineBase {
     // ms4 (optional): average of [CSC gather, CSC reduce, CSR gather, CSR
     // reduce]; a single-kernel pass reports its time as gather, 0 as reduce.
     bo_status time_pair(const PassPlan& pr, const PassPlan& pc, int reps, double* csr_ms, double* csc_ms,
-                        double* ms4 = nullptr) {
+                        double* ms4 = nullptr, double* step_ms = nullptr) {
         std::vector<hipEvent_t> ev(4 * static_cast<size_t>(reps) + 1);
         for (auto& e : ev) BO_TRY(hipEventCreate(&e));
         launch_primal(pc, nullptr, 0);
@@ -968,6 +968,7 @@ struct Engine final : EngineBase {
         if (csr_ms) *csr_ms = (acc[2] + acc[3]) / reps;
         if (ms4)
             for (int q = 0; q < 4; ++q) ms4[q] = acc[q] / reps;
+        if (step_ms) *step_ms = (acc[0] + acc[1] + acc[2] + acc[3]) / reps;
         return BO_SUCCESS;
     }
 
@@ -1025,6 +1026,27 @@ struct Engine final : EngineBase {
         }
         plan_rows = pr;
         plan_cols = pc;
+        if (prm.check_every <= 0) {
+            // check cadence: 128 steps when one FP64 check costs at least
+            // four iteration steps (it then stays under 3% of a solve), else 64
+            double step_ms = 0.0;
+            bo_status s = time_pair(plan_rows, plan_cols, 4, nullptr, nullptr, nullptr, &step_ms);
+            if (s != BO_SUCCESS) return s;
+            hipEvent_t e0, e1;
+            BO_TRY(hipEventCreate(&e0));
+            BO_TRY(hipEventCreate(&e1));
+            launch_check();
+            BO_TRY(hipEventRecord(e0, stream));
+            for (int r = 0; r < 3; ++r) launch_check();
+            BO_TRY(hipEventRecord(e1, stream));
+            BO_TRY(hipEventSynchronize(e1));
+            float t = 0.0f;
+            BO_TRY(hipEventElapsedTime(&t, e0, e1));
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            const double check_ms = t / 3.0;
+            prm.check_every = check_ms >= 4.0 * step_ms ? 128 : 64;
+        }
         return BO_SUCCESS;
     }
 
@@ -1183,6 +1205,7 @@ struct Engine final : EngineBase {
             out->lanes_rows = prm.lanes_rows;
             out->lanes_cols = prm.lanes_cols;
             out->hot_columns = plan_rows.hot;
+            out->check_every = prm.check_every;
         }
         return BO_SUCCESS;
     }
@@ -1260,7 +1283,7 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
     p->eps_rel_kkt = 1e-6;
     p->ruiz_iterations = 10;
     p->pock_chambolle = 1;
-    p->check_every = 64;
+    p->check_every = 0;
     p->lanes_rows = 0;
     p->lanes_cols = 0;
     p->fp64_iterates = 0;
