@@ -240,11 +240,12 @@ def family_pass_bytes(m, n, nnz, lanes=64, tsize=4):
     """Algorithmic bytes of one family-engine product pass over `lanes` LPs:
     the matrix once (int32 index + FP64 scaled value) and its pointers, the gathered vector
     once (all lanes), and the epilogue's reads and writes per (segment, LP):
-    primal (CSC, x update): x_k r/w, x_0, c_hat, x+ -> 32 B (FP32 x+), plus
-    the shared bounds; dual (CSR, y update): A x r/w, A x_0, y_k r/w, y_0,
-    b_hat, y+, y gathered copy -> 60 B, plus the row sense."""
-    primal = 12 * nnz + 4 * (n + 1) + tsize * lanes * m + lanes * (24 + 2 * tsize) * n + 2 * tsize * n
-    dual = 12 * nnz + 4 * (m + 1) + tsize * lanes * n + lanes * (48 + 3 * tsize) * m + m
+    primal (CSC, x update): x_k r/w (FP64), anchor x_0, c_hat, x+ (FP32)
+    -> 28 B, plus the shared bounds; dual (CSR, y update): A x r/w and y_k
+    r/w (FP64), anchors A x_0 and y_0, b_hat, y+, y gathered copy (FP32)
+    -> 52 B, plus the row sense."""
+    primal = 12 * nnz + 4 * (n + 1) + tsize * lanes * m + lanes * (16 + 3 * tsize) * n + 2 * tsize * n
+    dual = 12 * nnz + 4 * (m + 1) + tsize * lanes * n + lanes * (32 + 5 * tsize) * m + m
     return primal, dual
 
 
@@ -256,10 +257,14 @@ def bench_c4(pkg, args):
     training mean (the ablation without the network), in batches."""
     ws = pkg.warmstart
     t0 = time.perf_counter()
-    fam = pkg.lpdata.StaircaseFamily(**C4_FAMILY)
+    fam = pkg.lpdata.StaircaseFamily(**C4_FAMILY, rank=args.c4_rank)
     ok_train = 0
     desc = ("staircase family periods=12 block=2000x5000 inv=200 seed=42; draw k: c = c0(1+0.2N(0,1)), "
             "b = b0(1+U[-0.1,0.1]) from default_rng(k)")
+    if args.c4_rank:
+        desc = (f"staircase family periods=12 block=2000x5000 inv=200 seed=42, low-rank variant: draw k mixes "
+                f"{args.c4_rank} latent N(0,1) factors from default_rng(k) (c = c0(1+0.2 U_c z), "
+                f"b = b0(1+0.1 tanh(U_b z)))")
     log(f"workload c4: {desc}; m={fam.m} n={fam.n} nnz={fam.base.nnz}; built in {time.perf_counter() - t0:.1f}s")
     params = dict(eps_rel_kkt=args.eps, max_iterations=args.max_iterations, use_graph=0 if args.no_graph else 1)
     fs = ws.FamilySolver(pkg, fam, batch=args.c4_batch, engine="family", **params)
@@ -408,6 +413,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=20, help="c4: training epochs")
     ap.add_argument("--lr", type=float, default=1e-4, help="c4: Adam learning rate")
     ap.add_argument("--c4-batch", type=int, default=64, help="c4: LPs per batched solve")
+    ap.add_argument("--c4-rank", type=int, default=0,
+                    help="c4: 0 = the family as specified; r > 0 = its low-rank variant (r latent factors)")
     ap.add_argument("--block-sample", type=int, default=128,
                     help="c4: held-out draws also solved cold by the block-diagonal batched engine")
     ap.add_argument("--single-sample", type=int, default=16,

@@ -131,7 +131,7 @@ struct EpiFPrimal {
     const T* l_hat;
     const T* u_hat;
     double* xk;
-    const double* x0;
+    const T* x0;       // Halpern anchor (working precision: it enters with weight 1 / (k + 2))
     T* xp;
     int t;
     struct Pre {
@@ -142,7 +142,7 @@ struct EpiFPrimal {
     __device__ bool live(int lane) const { return cf->live[lane] != 0; }
     __device__ Pre load(int j, int lane) const {
         const size_t e = static_cast<size_t>(j) * kL + lane;
-        return Pre{xk[e], x0[e], static_cast<double>(c_hat[e]), l_hat[j], u_hat[j]};
+        return Pre{xk[e], static_cast<double>(x0[e]), static_cast<double>(c_hat[e]), l_hat[j], u_hat[j]};
     }
     __device__ void operator()(int j, int lane, const double* acc, const Pre& p) const {
         const size_t e = static_cast<size_t>(j) * kL + lane;
@@ -163,9 +163,9 @@ struct EpiFDual {
     const T* b_hat;
     const unsigned char* eq;
     double* ax;
-    const double* ax0;
+    const T* ax0;      // anchors in working precision
     double* yk;
-    const double* y0;
+    const T* y0;
     T* yp;
     T* yg;
     int t;
@@ -177,7 +177,8 @@ struct EpiFDual {
     __device__ bool live(int lane) const { return cf->live[lane] != 0; }
     __device__ Pre load(int i, int lane) const {
         const size_t e = static_cast<size_t>(i) * kL + lane;
-        return Pre{ax[e], yk[e], static_cast<double>(b_hat[e]), y0[e], ax0[e], eq[i] != 0};
+        return Pre{ax[e], yk[e], static_cast<double>(b_hat[e]), static_cast<double>(y0[e]), static_cast<double>(ax0[e]),
+                   eq[i] != 0};
     }
     __device__ void operator()(int i, int lane, const double* acc, const Pre& p) const {
         const size_t e = static_cast<size_t>(i) * kL + lane;
@@ -200,7 +201,7 @@ struct EpiFStore {
     static constexpr int NV = 1;
     const TV* v;
     double* out0;
-    double* out1;  // may be null
+    TV* out1;  // may be null
     struct Pre {};
     __device__ const TV* vec(int) const { return v; }
     __device__ bool live(int) const { return true; }
@@ -208,7 +209,7 @@ struct EpiFStore {
     __device__ void operator()(int s, int lane, const double* acc, const Pre&) const {
         const size_t e = static_cast<size_t>(s) * kL + lane;
         out0[e] = acc[0];
-        if (out1) out1[e] = acc[0];
+        if (out1) out1[e] = static_cast<TV>(acc[0]);
     }
 };
 
@@ -285,7 +286,7 @@ __device__ inline int seg_stride() { return static_cast<int>(gridDim.x) * kSegWa
 template <class T>
 __global__ void __launch_bounds__(256)
 kf_scale_cols(int n, const double* cs, const double* c, const T* l_hat, const T* u_hat, const double* wx,
-              int has_warm, T* c_hat, double* xk, double* x0, T* xp, double* part) {
+              int has_warm, T* c_hat, double* xk, T* x0, T* xp, double* part) {
     const int lane = static_cast<int>(threadIdx.x) % kWave;
     double q[1] = {0.0};
     for (int j = first_seg(); j < n; j += seg_stride()) {
@@ -296,7 +297,7 @@ kf_scale_cols(int n, const double* cs, const double* c, const T* l_hat, const T*
         q[0] += static_cast<double>(ch) * static_cast<double>(ch);
         const double xs = clampT(has_warm ? wx[e] / sj : 0.0, static_cast<double>(l_hat[j]), static_cast<double>(u_hat[j]));
         xk[e] = xs;
-        x0[e] = xs;
+        x0[e] = static_cast<T>(xs);
         xp[e] = static_cast<T>(xs);
     }
     store_lane_sums<1>(q, part);
@@ -305,7 +306,7 @@ kf_scale_cols(int n, const double* cs, const double* c, const T* l_hat, const T*
 template <class T>
 __global__ void __launch_bounds__(256)
 kf_scale_rows(int m, const double* rs, const double* b, const unsigned char* eq, const double* wy, int has_warm,
-              T* b_hat, double* yk, double* y0, T* yg, T* yp, double* part) {
+              T* b_hat, double* yk, T* y0, T* yg, T* yp, double* part) {
     const int lane = static_cast<int>(threadIdx.x) % kWave;
     double q[1] = {0.0};
     for (int i = first_seg(); i < m; i += seg_stride()) {
@@ -317,7 +318,7 @@ kf_scale_rows(int m, const double* rs, const double* b, const unsigned char* eq,
         double ys = has_warm ? wy[e] / ri : 0.0;
         if (!eq[i] && ys < 0.0) ys = 0.0;
         yk[e] = ys;
-        y0[e] = ys;
+        y0[e] = static_cast<T>(ys);
         yg[e] = static_cast<T>(ys);
         yp[e] = static_cast<T>(ys);
     }
@@ -467,7 +468,7 @@ kf_decide(PdlpState* st, const double* px, const double* pr, const double* pc, i
 // anchor; lane sums of ||dx||^2 and ||dy||^2 of the anchor move.
 template <class T>
 __global__ void __launch_bounds__(256)
-kf_restart_x(int n, const PdlpState* st, double* xk, double* x0, const T* xp, double* part) {
+kf_restart_x(int n, const PdlpState* st, double* xk, T* x0, const T* xp, double* part) {
     const int lane = static_cast<int>(threadIdx.x) % kWave;
     const PdlpState& s = st[lane];
     double q[1] = {0.0};
@@ -475,17 +476,17 @@ kf_restart_x(int n, const PdlpState* st, double* xk, double* x0, const T* xp, do
         for (int j = first_seg(); j < n; j += seg_stride()) {
             const size_t e = static_cast<size_t>(j) * kL + lane;
             const double xn = s.restart_to_cand ? static_cast<double>(xp[e]) : xk[e];
-            const double d = xn - x0[e];
+            const double d = xn - static_cast<double>(x0[e]);
             q[0] += d * d;
             xk[e] = xn;
-            x0[e] = xn;
+            x0[e] = static_cast<T>(xn);
         }
     store_lane_sums<1>(q, part);
 }
 
 template <class T>
 __global__ void __launch_bounds__(256)
-kf_restart_y(int m, const PdlpState* st, double* yk, double* y0, const T* yp, T* yg, double* ax, double* ax0,
+kf_restart_y(int m, const PdlpState* st, double* yk, T* y0, const T* yp, T* yg, double* ax, T* ax0,
              const double* ax_cand, double* part) {
     const int lane = static_cast<int>(threadIdx.x) % kWave;
     const PdlpState& s = st[lane];
@@ -498,12 +499,12 @@ kf_restart_y(int m, const PdlpState* st, double* yk, double* y0, const T* yp, T*
                 yn = static_cast<double>(yp[e]);
                 ax[e] = ax_cand[e];
             }
-            const double d = yn - y0[e];
+            const double d = yn - static_cast<double>(y0[e]);
             q[0] += d * d;
             yk[e] = yn;
-            y0[e] = yn;
+            y0[e] = static_cast<T>(yn);
             yg[e] = static_cast<T>(yn);
-            ax0[e] = ax[e];
+            ax0[e] = static_cast<T>(ax[e]);
         }
     store_lane_sums<1>(q, part);
 }
@@ -658,7 +659,8 @@ struct Family final : FamilyBase {
     // LP-minor per-LP data
     double *c64 = nullptr, *b64 = nullptr;
     T *c_hat = nullptr, *b_hat = nullptr;
-    double *xk = nullptr, *x0 = nullptr, *yk = nullptr, *y0 = nullptr, *ax = nullptr, *ax0 = nullptr;
+    double *xk = nullptr, *yk = nullptr, *ax = nullptr;
+    T *x0 = nullptr, *y0 = nullptr, *ax0 = nullptr;  // Halpern anchors
     T *xp = nullptr, *yp = nullptr, *yg = nullptr;
     double *xo_cur = nullptr, *xo_cand = nullptr, *yo_cur = nullptr, *yo_cand = nullptr;
     double *ar0 = nullptr, *ar1 = nullptr, *ac0 = nullptr, *ac1 = nullptr, *ax_cand = nullptr;
@@ -764,9 +766,11 @@ struct Family final : FamilyBase {
         BO_TRY(mem.alloc(&l_hat, n));
         BO_TRY(mem.alloc(&u_hat, n));
         BO_TRY(mem.alloc(&eq, m));
-        for (double** v : {&c64, &xk, &x0, &xo_cur, &xo_cand, &ac0, &ac1, &wx}) BO_TRY(mem.alloc(v, NL));
-        for (double** v : {&b64, &yk, &y0, &ax, &ax0, &yo_cur, &yo_cand, &ar0, &ar1, &ax_cand, &wy})
-            BO_TRY(mem.alloc(v, ML));
+        for (double** v : {&c64, &xk, &xo_cur, &xo_cand, &ac0, &ac1, &wx}) BO_TRY(mem.alloc(v, NL));
+        for (double** v : {&b64, &yk, &ax, &yo_cur, &yo_cand, &ar0, &ar1, &ax_cand, &wy}) BO_TRY(mem.alloc(v, ML));
+        BO_TRY(mem.alloc(&x0, NL));
+        BO_TRY(mem.alloc(&y0, ML));
+        BO_TRY(mem.alloc(&ax0, ML));
         BO_TRY(mem.alloc(&c_hat, NL));
         BO_TRY(mem.alloc(&xp, NL));
         BO_TRY(mem.alloc(&b_hat, ML));
