@@ -55,8 +55,8 @@ __host__ __device__ inline float bf2f(unsigned short h) {
     return f;
 }
 
-constexpr int kTm = 64, kTn = 64, kTk = 32;
-constexpr int kLd = kTk + 8;      // LDS row pitch in BF16 elements (80 B: 16-byte aligned rows)
+constexpr int kTm = 64, kTn = 64, kTk = 64;
+constexpr int kLd = kTk + 8;      // LDS row pitch in BF16 elements (144 B: 16-byte aligned rows)
 constexpr int kMaxSplit = 64;     // K splits of one product at most
 constexpr int kFillTarget = 1024; // workgroups a split product aims for (4 per CU)
 
@@ -65,51 +65,65 @@ enum : int { kEpiNone = 0, kEpiBias = 1, kEpiRelu = 2 };
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 // Operand layouts: the stored matrix is contiguous along k (0) or along its
-// other dimension (1).  One thread stages 8 consecutive elements of that
-// contiguous direction: a 16-byte load when all 8 are in range and `vec`
-// (unit stride, 8-element aligned rows and base), else 8 guarded loads.
+// other dimension (1).  A 64 x 64 tile is 512 runs of 8 consecutive
+// elements along the contiguous direction; thread t stages runs t and
+// t + 256: a 16-byte load when all 8 are in range and `vec` (unit stride,
+// 8-element aligned rows and base), else 8 guarded loads.  Loading (into
+// registers) and storing (into LDS) are separate so the next tile's loads are
+// in flight while the current tile is multiplied.
 //
 // LDS tiles hold both operands as [row of C or column of C][k], k
 // contiguous, so each lane reads its 4 k-values with one 8-byte LDS load.
 template <int L>
-__device__ inline void stage(unsigned short (*T)[kLd], const unsigned short* __restrict__ G, long long s_r, long long s_k,
-                             int r0, int R, int k0, int kend, bool vec) {
-    const int t = threadIdx.x;
-    if (L == 0) {  // k contiguous: 64 rows x 32 k, thread -> row t/4, k 8*(t%4)..+7
-        const int r = t >> 2, c = (t & 3) * 8;
-        const int gr = r0 + r, gk = k0 + c;
-        us8 v;
-        if (vec && gr < R && gk + 7 < kend) {
-            v = *reinterpret_cast<const us8*>(G + gr * s_r + gk);
-        } else {
+__device__ inline void tile_load(us8* v, const unsigned short* __restrict__ G, long long s_r, long long s_k, int r0,
+                                 int R, int k0, int kend, bool vec) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (gr < R && gk + e < kend) ? G[gr * s_r + static_cast<long long>(gk + e) * s_k]
-                                                 : static_cast<unsigned short>(0);
+    for (int u = 0; u < 2; ++u) {
+        const int c = static_cast<int>(threadIdx.x) + 256 * u;
+        if (L == 0) {  // k contiguous: run c -> row c / 8, k 8 (c % 8) .. + 7
+            const int gr = r0 + (c >> 3), gk = k0 + (c & 7) * 8;
+            if (vec && gr < R && gk + 7 < kend) {
+                v[u] = *reinterpret_cast<const us8*>(G + gr * s_r + gk);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    v[u][e] = (gr < R && gk + e < kend) ? G[gr * s_r + static_cast<long long>(gk + e) * s_k]
+                                                        : static_cast<unsigned short>(0);
+            }
+        } else {  // row-direction contiguous: run c -> k c / 8, rows 8 (c % 8) .. + 7
+            const int gk = k0 + (c >> 3), gr = r0 + (c & 7) * 8;
+            if (vec && gk < kend && gr + 7 < R) {
+                v[u] = *reinterpret_cast<const us8*>(G + static_cast<long long>(gk) * s_k + gr);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    v[u][e] = (gk < kend && gr + e < R) ? G[static_cast<long long>(gr + e) * s_r + gk * s_k]
+                                                        : static_cast<unsigned short>(0);
+            }
         }
-        *reinterpret_cast<us8*>(&T[r][c]) = v;
-    } else {  // row-direction contiguous: thread -> k t/8, rows 8*(t%8)..+7
-        const int kk = t >> 3, r = (t & 7) * 8;
-        const int gk = k0 + kk, gr = r0 + r;
-        us8 v;
-        if (vec && gk < kend && gr + 7 < R) {
-            v = *reinterpret_cast<const us8*>(G + static_cast<long long>(gk) * s_k + gr);
+    }
+}
+
+template <int L>
+__device__ inline void tile_store(unsigned short (*T)[kLd], const us8* v) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int c = static_cast<int>(threadIdx.x) + 256 * u;
+        if (L == 0) {
+            *reinterpret_cast<us8*>(&T[c >> 3][(c & 7) * 8]) = v[u];
         } else {
+            const int kk = c >> 3, r = (c & 7) * 8;
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (gk < kend && gr + e < R) ? G[static_cast<long long>(gr + e) * s_r + gk * s_k]
-                                                 : static_cast<unsigned short>(0);
+            for (int e = 0; e < 8; ++e) T[r + e][kk] = v[u][e];
         }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) T[r + e][kk] = v[e];
     }
 }
 
 // C = op(A) op(B) (+ bias, ReLU): A(m,k) = A[m*sam + k*sak], B(k,n) =
 // B[k*sbk + n*sbn]; a 256-thread workgroup owns a 64 x 64 tile of C, each
-// wavefront a 32 x 32 quarter.  Split K (gridDim.z > 1): workgroup z sums k
-// in [z*kc, (z+1)*kc) and stores its raw FP32 tile to ws[z][M][N]; the
-// epilogue then runs in k_splitk_sum.
+// wavefront a 32 x 32 quarter, k in steps of 64.  Split K (gridDim.z > 1):
+// workgroup z sums k in [z*kc, (z+1)*kc) and stores its raw FP32 tile to
+// ws[z][M][N]; the epilogue then runs in k_splitk_sum.
 template <int EPI, int LA, int LB>
 __global__ void __launch_bounds__(256)
 k_gemm(int M, int N, int K, int kc, const unsigned short* __restrict__ A, long long sam, long long sak, bool va,
@@ -125,11 +139,19 @@ k_gemm(int M, int N, int K, int kc, const unsigned short* __restrict__ A, long l
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    us8 ra[2], rb[2];
+    if (kb < ke) {
+        tile_load<LA>(ra, A, sam, sak, bm, M, kb, ke, va);  // A tile rows are m (stride sam)
+        tile_load<LB>(rb, B, sbn, sbk, bn, N, kb, ke, vb);  // B tile rows are n (stride sbn)
+    }
     for (int k0 = kb; k0 < ke; k0 += kTk) {
-        // A tile rows are m (stride sam), B tile rows are n (stride sbn)
-        stage<LA>(As, A, sam, sak, bm, M, k0, ke, va);
-        stage<LB>(Bs, B, sbn, sbk, bn, N, k0, ke, vb);
+        tile_store<LA>(As, ra);
+        tile_store<LB>(Bs, rb);
         __syncthreads();
+        if (k0 + kTk < ke) {  // the next tile's loads overlap this tile's products
+            tile_load<LA>(ra, A, sam, sak, bm, M, k0 + kTk, ke, va);
+            tile_load<LB>(rb, B, sbn, sbk, bn, N, k0 + kTk, ke, vb);
+        }
 #pragma unroll
         for (int kk = 0; kk < kTk; kk += 8) {
             const int kq = kk + 4 * (lane / 32);
