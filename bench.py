@@ -74,7 +74,7 @@ def kernel_table(h, reps):
 
 
 def spmv_bytes(r, c, z):
-    """Survey formula for one SpMV: values+indices, row pointers, x, y."""
+    """Algorithmic bytes of one SpMV: values + indices, row pointers, x, y (each once)."""
     return 8.0 * z + 4.0 * (r + 1) + 4.0 * c + 4.0 * r
 
 
@@ -435,7 +435,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(independent LPs shard across GPUs without communication; see DESIGN.md)",
+              "(independent LPs shard across GPUs without communication)",
               file=sys.stderr)
         sys.exit(2)
 

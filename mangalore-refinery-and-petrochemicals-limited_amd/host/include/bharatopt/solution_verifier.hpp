@@ -1,4 +1,4 @@
-// solution_verifier.hpp -- FP64 judge of a candidate solution against the
+// solution_verifier.hpp -- FP64 verifier of a candidate solution against the
 // original model, interface of the reference's include/bharatopt/solution_verifier.hpp:13-65.
 #ifndef BHARATOPT_SOLUTION_VERIFIER_HPP
 #define BHARATOPT_SOLUTION_VERIFIER_HPP

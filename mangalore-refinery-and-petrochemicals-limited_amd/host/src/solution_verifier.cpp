@@ -1,5 +1,5 @@
 // solution_verifier.cpp -- bounds, integrality, row activities and objective of
-// a candidate, all in FP64 on the original model; the verdict rules are the
+// a candidate, all in FP64 on the original model; the acceptance rules are the
 // reference's (src/solution_verifier.cpp:42-215).
 #include <bharatopt/solution_verifier.hpp>
 

@@ -75,7 +75,7 @@ int orc_pdhg_ref(int m, int n, const int* rp, const int* ci, const double* v,
                  const unsigned char* is_eq, const orc_pdhg_opts* o,
                  double* x_out, double* y_out, orc_pdhg_out* out);
 
-/* ---- FP64 judge ---------------------------------------------------------- */
+/* ---- FP64 verifier ------------------------------------------------------- */
 typedef struct {
     double r_primal;        /* || [A_E x - b_E ; (A_I x - b_I)^+] ||_2              */
     double r_dual;          /* || lambda - proj_Lambda(lambda) ||_2 (+ ||min(y_I,0)||) */

@@ -194,7 +194,7 @@ int ref_simplex_solve(const ref_model* fm, double* x_out, double* obj_out, int* 
     }
 }
 
-// SolutionVerifier::verify (FP64 judge): returns verified flag and max violations.
+// SolutionVerifier::verify (FP64 verifier): returns verified flag and max violations.
 int ref_verify(const ref_model* fm, const double* x, double tol, double* max_cons_viol,
                double* max_lb_viol, double* max_ub_viol, double* obj_recomputed) {
     try {

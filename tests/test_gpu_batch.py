@@ -1,7 +1,7 @@
 """GPU parity of the batched PDLP engine (bo_pdlp_batch_*): many LPs of uneven
 size laid out block-diagonally and solved together.
 
-Every LP of a batch is judged on its own, by the oracle, as in
+Every LP of a batch is checked on its own, by the oracle, as in
 tests/test_gpu_pdlp.py: FP64 rel-KKT of the returned point <= 1e-6 on the
 unscaled problem, objective within 1e-6 of the exact optima recorded in
 tests/golden/lp_optima.json (reference simplex chain and HiGHS), and the

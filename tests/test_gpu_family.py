@@ -1,7 +1,7 @@
 """GPU parity of the parametric-family engine (bo_pdlp_family_*): up to 64
 LPs over one matrix, vectors LP-minor, one LP per lane.
 
-Every LP is judged by the oracle's FP64 KKT evaluator (rel-KKT <= 1e-6 on the
+Every LP is checked by the oracle's FP64 KKT evaluator (rel-KKT <= 1e-6 on the
 unscaled LP, bounds exact), its objective must match the block-diagonal
 batched engine's on the same draw, and the engine's own FP64 report must
 agree with the oracle's.

@@ -1,6 +1,6 @@
 """GPU parity of the PDLP engine (FP32 iterates, FP64 checks) through the C ABI.
 
-The engine's answer is judged by the oracle, never by itself:
+The engine's answer is checked by the oracle, never by itself:
 * the oracle's FP64 KKT evaluator (tests/test_oracle.py pins it) recomputes
   rel-KKT of the returned (x, y) on the unscaled problem: must be <= 1e-6,
   the tolerance of the benchmark metric;

@@ -7,7 +7,7 @@
 * The ablation: a BF16 network trained on the GPU by the library's kernels
   on cold solutions of training draws, against the cold start and against
   the training mean as warm starts, on held-out draws.  Every solve of every
-  variant is judged by the oracle's FP64 KKT evaluator (rel-KKT <= 1e-6) and
+  variant is checked by the oracle's FP64 KKT evaluator (rel-KKT <= 1e-6) and
   its objective must match the cold solve's.  On the low-rank variant of the
   family (perturbations mixing 4 latent factors: solutions the network can
   learn from 1,000 draws) the test asserts that the network's starting

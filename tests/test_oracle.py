@@ -6,7 +6,7 @@
   the FP64 result vectors, tests/golden/ref_pdhg_kat.json, and live against
   oracle/_ref when present);
 * the sparse KATs of tests/test_sparse_matrix.cpp / test_gpu_foundation.cpp;
-* the FP64 KKT judge: closed forms, HiGHS optimal primal-dual pairs, weak
+* the FP64 KKT evaluator: closed forms, HiGHS optimal primal-dual pairs, weak
   duality, and planted mistakes that it must flag.
 """
 import hashlib
@@ -151,7 +151,7 @@ def test_hand_lp_optimum(po, lpdata, models):
     assert abs(-(lp.c @ r.x) - 64 / 3) < 1e-2
 
 
-# ---------------------------------------------------------------- KKT judge
+# ---------------------------------------------------------------- KKT evaluator
 def _hand_lp(lpdata):
     mb = lpdata.ModelBuilder(sense=0)  # min -3x - 5y, 2x + y <= 8, x + 2y <= 8
     x = mb.add_variable(0.0, np.inf, -3.0)
