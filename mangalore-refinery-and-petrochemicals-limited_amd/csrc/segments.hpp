@@ -236,7 +236,7 @@ k_segments_hot(BinTable bt, const int* __restrict__ ptr, const int* __restrict__
             case 64: seg_process<MODE, 64, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
             default:
                 seg_process<MODE, 256, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot);
-                __syncthreads();  // the one-segment path reuses its LDS reduction slots next round
+                __syncthreads();  // the one-segment path reuses its LDS reduction slots on its next trip
                 break;
         }
     }
