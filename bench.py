@@ -533,12 +533,29 @@ def main():
         },
         "setup_s": setup_s,
     }
+    if args.workload == "c0":
+        # the reference's own algorithm (diagonal steps, ergodic average, its
+        # stop rule) in FP64 on the GPU, bit-equal to the reference loop
+        t0 = time.perf_counter()
+        r = pkg.pdhg_ref_solve(lp, max_iterations=50000)
+        out["reference_pdhg_on_gpu"] = {"status": int(r.status), "iterations": int(r.iterations),
+                                        "wall_ms": (time.perf_counter() - t0) * 1e3,
+                                        "kernel_ms": float(r.kernel_time_ms),
+                                        "constraint_violation": float(r.constraint_violation)}
     if not args.no_cpu_baseline:
         log("cpu baseline (reference CPU PDHG, bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline(lp, int(last["iterations"]))
         from oracle import pyoracle as po
         k = po.kkt(lp, x, y)  # the checker: FP64 rel-KKT of the returned solution
         out["checker_rel_kkt"] = k["rel_kkt"]
+        if args.workload == "c0":
+            # the reference's CPU loop to its own stop rule on the same LP
+            t0 = time.perf_counter()
+            if po.ref_lib() is not None:
+                rr = po.ref_fo_solve(lp.to_flat(), max_iterations=50000)
+            else:
+                rr = po.pdhg_ref(lp, max_iterations=50000)
+            out["reference_pdhg_on_cpu_ms"] = (time.perf_counter() - t0) * 1e3
     print(json.dumps(out), flush=True)
 
 

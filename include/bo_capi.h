@@ -162,6 +162,9 @@ typedef struct bo_pdlp_params {
     int compact_tail;          /* batched engine: once at most 1/8 of a batch's LPs still run, continue
                                   them in a compact batch (segment kernels), from their points, primal
                                   weights and remaining budgets, repeatedly (default 1) */
+    int single_workgroup;      /* single-LP engine: 1 run the whole solve in one workgroup (no kernel
+                                  launch per pass), 0 never, -1 (default) the tuner times both on LPs
+                                  of up to 2^18 nonzeros without a time limit */
 } bo_pdlp_params;
 
 typedef struct bo_pdlp_stats {
@@ -187,6 +190,7 @@ typedef struct bo_pdlp_stats {
     int hot_columns;       /* LDS-cached entries of x+ in the CSR pass (0 = none) */
     long long fp64_polish_iterations; /* of `iterations`, those run by the FP64 polish */
     int check_every;       /* check cadence used (single-LP engine) */
+    int single_workgroup;  /* 1: the solve ran in one workgroup (single-LP engine) */
 } bo_pdlp_stats;
 
 /* Default parameters. */

@@ -77,7 +77,7 @@ class PdlpParams(C.Structure):
                 ("lanes_rows", C.c_int), ("lanes_cols", C.c_int), ("fp64_iterates", C.c_int),
                 ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int),
                 ("hot_columns", C.c_int), ("stall_iterations", C.c_longlong), ("fp64_polish", C.c_int),
-                ("compact_tail", C.c_int)]
+                ("compact_tail", C.c_int), ("single_workgroup", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -87,7 +87,7 @@ class PdlpStats(C.Structure):
                     "primal_weight", "step_size", "setup_ms", "solve_ms", "spmv_csr_bytes",
                     "spmv_csc_bytes")] + [("lanes_rows", C.c_int), ("lanes_cols", C.c_int),
                                           ("hot_columns", C.c_int), ("fp64_polish_iterations", C.c_longlong),
-                                          ("check_every", C.c_int)]
+                                          ("check_every", C.c_int), ("single_workgroup", C.c_int)]
 
 
 class PdlpPassInfo(C.Structure):

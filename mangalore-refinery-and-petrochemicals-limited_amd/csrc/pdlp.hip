@@ -388,6 +388,162 @@ __global__ void k_restart_finish(PdlpState* st, const double* pdx, int nbx, cons
     restart_finish_core(st, dx2, dy2);
 }
 
+
+// ------------------------------------------------------- one-workgroup solve
+constexpr long long kSmallTuneNnz = 1LL << 18;  // LPs up to this size try the one-workgroup solve
+constexpr int kSmallMaxLds = 96 * 1024;         // LDS for the gathered vectors of a one-workgroup solve
+
+// A small LP's whole solve in one workgroup of 1024 threads: the step-0
+// check, then intervals of K steps, each followed by the FP64 check, the
+// decision and the restart, exactly as the multi-kernel loop runs them, with
+// workgroup barriers in place of kernel launches.  The multi-kernel loop pays
+// a few microseconds of launch latency per pass, which dominates LPs of a
+// few thousand nonzeros.  Threads stride over rows / columns and add each
+// segment in order in FP64, four entries in flight.
+constexpr int kSmallThreads = 1024;
+
+template <class T>
+struct SmallArgs {
+    int m, n, K;
+    const int *rp, *ci, *cp, *ri;
+    const T *rv, *cv;
+    const double *rv64, *cv64;
+    EpiPrimal<T> ep;
+    EpiDual<T> ed;
+    EpiCheckRows<T> er;
+    EpiCheckCols ec;
+    const double *cs, *c64, *lb64, *ub64;
+    double *xo_cur, *xo_cand;
+    double *x0, *y0, *ax0;  // anchors (written at restarts)
+    PdlpState* st;
+    int lds;  // 1: copies of x+ and y (the gathered vectors) live in LDS, n + m entries of T
+};
+
+template <class TM, class TV>
+__device__ inline double seg_dot(int b, int e, const int* __restrict__ idx, const TM* __restrict__ val,
+                                 const TV* __restrict__ v) {
+    double a = 0.0;
+    int q = b;
+    for (; q + 4 <= e; q += 4) {
+        const double w0 = static_cast<double>(val[q]), w1 = static_cast<double>(val[q + 1]);
+        const double w2 = static_cast<double>(val[q + 2]), w3 = static_cast<double>(val[q + 3]);
+        const double x0 = static_cast<double>(v[idx[q]]), x1 = static_cast<double>(v[idx[q + 1]]);
+        const double x2 = static_cast<double>(v[idx[q + 2]]), x3 = static_cast<double>(v[idx[q + 3]]);
+        a += w0 * x0;
+        a += w1 * x1;
+        a += w2 * x2;
+        a += w3 * x3;
+    }
+    for (; q < e; ++q) a += static_cast<double>(val[q]) * static_cast<double>(v[idx[q]]);
+    return a;
+}
+
+template <class T>
+__device__ inline void small_check(const SmallArgs<T>& a, double* sh, T* ly) {
+    const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+    PdlpState* st = a.st;
+    double p[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) p[k] = 0.0;
+    for (int j = tid; j < a.n; j += nt) {  // unscaled x: current point, candidate
+        const double sj = a.cs[j], l = a.lb64[j], u = a.ub64[j];
+        const double xc = clampT(sj * a.ep.xk[j], l, u);
+        const double xa = clampT(sj * static_cast<double>(a.ep.xp[j]), l, u);
+        a.xo_cur[j] = xc;
+        a.xo_cand[j] = xa;
+        p[0] += a.c64[j] * xc;
+        p[1] += a.c64[j] * xa;
+    }
+    __syncthreads();
+    for (int i = tid; i < a.m; i += nt) {  // rows: A xo (FP64), violations, duals
+        const int b = a.rp[i], e = a.rp[i + 1];
+        const double acc[2] = {seg_dot(b, e, a.ci, a.rv64, a.xo_cur), seg_dot(b, e, a.ci, a.rv64, a.xo_cand)};
+        a.er(i, acc, p + 2);
+    }
+    __syncthreads();
+    for (int j = tid; j < a.n; j += nt) {  // columns: A'yo (FP64), dual residuals
+        const int b = a.cp[j], e = a.cp[j + 1];
+        const double acc[2] = {seg_dot(b, e, a.ri, a.cv64, a.er.yo_cur), seg_dot(b, e, a.ri, a.cv64, a.er.yo_cand)};
+        a.ec(j, acc, p + 8);
+    }
+    double q[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) q[k] = block_sum(p[k], sh);
+    if (tid == 0 && !st->done) decide_core(st, q);
+    __syncthreads();
+    if (st->done || !st->restart) return;
+    const bool cand = st->restart_to_cand != 0;
+    double dx = 0.0, dy = 0.0;
+    for (int j = tid; j < a.n; j += nt) {
+        const double xn = cand ? static_cast<double>(a.ep.xp[j]) : a.ep.xk[j];
+        const double d = xn - a.x0[j];
+        dx += d * d;
+        a.ep.xk[j] = xn;
+        a.x0[j] = xn;
+    }
+    for (int i = tid; i < a.m; i += nt) {
+        double yn = a.ed.yk[i];
+        if (cand) {
+            yn = static_cast<double>(a.ed.yp[i]);
+            a.ed.ax[i] = a.er.ax_cand[i];
+        }
+        const double d = yn - a.y0[i];
+        dy += d * d;
+        a.ed.yk[i] = yn;
+        a.y0[i] = yn;
+        a.ed.yg[i] = static_cast<T>(yn);
+        if (ly) ly[i] = static_cast<T>(yn);
+        a.ax0[i] = a.ed.ax[i];
+    }
+    const double dx2 = block_sum(dx, sh);
+    const double dy2 = block_sum(dy, sh);
+    if (tid == 0) restart_finish_core(st, dx2, dy2);
+    __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(kSmallThreads) k_small(SmallArgs<T> a) {
+    __shared__ double sh[kSmallThreads / kWave];
+    extern __shared__ __attribute__((aligned(16))) unsigned char small_lds[];
+    const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+    PdlpState* st = a.st;
+    T* lx = a.lds ? reinterpret_cast<T*>(small_lds) : nullptr;  // x+ copy
+    T* ly = a.lds ? lx + a.n : nullptr;                          // y copy
+    if (a.lds) {
+        for (int j = tid; j < a.n; j += nt) lx[j] = a.ep.xp[j];
+        for (int i = tid; i < a.m; i += nt) ly[i] = a.ed.yg[i];
+    }
+    small_check(a, sh, ly);  // step 0 (its barriers order the copies above)
+    const T* gy = a.lds ? ly : a.ep.yg;
+    const T* gx = a.lds ? lx : a.ed.xp;
+    while (!st->done) {
+        const long long left = st->max_iter - st->iter;
+        const int k = static_cast<int>(left < a.K ? left : a.K);
+        if (k <= 0) break;
+        for (int t = 0; t < k; ++t) {
+            EpiPrimal<T> ep = a.ep;
+            ep.t = t;
+            for (int j = tid; j < a.n; j += nt) {
+                const double acc = seg_dot(a.cp[j], a.cp[j + 1], a.ri, a.cv, gy);
+                ep(j, &acc, nullptr);
+                if (lx) lx[j] = ep.xp[j];
+            }
+            __syncthreads();
+            EpiDual<T> ed = a.ed;
+            ed.t = t;
+            for (int i = tid; i < a.m; i += nt) {
+                const double acc = seg_dot(a.rp[i], a.rp[i + 1], a.ci, a.rv, gx);
+                ed(i, &acc, nullptr);
+                if (ly) ly[i] = ed.yg[i];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) st->iter += k;
+        __syncthreads();
+        small_check(a, sh, ly);
+    }
+}
+
 // ---------------------------------------------------------------- host side
 struct DevMem {
     std::vector<void*> ptrs;
@@ -532,6 +688,8 @@ struct Engine final : EngineBase {
     PdlpState* st_host = nullptr;  // pinned, read back after each interval
     PdlpState* st_init = nullptr;  // pinned, staged for the upload in prepare()
     PassPlan plan_rows{}, plan_cols{};  // CSR (dual) pass and CSC (primal) pass
+    bool one_wg = false;                // the whole solve in one workgroup (k_small)
+    bool small_lds_set = false;         // k_small's dynamic LDS limit raised
     int* sblk[2][3] = {};               // stream blocks per pass (rows, cols) and LDS budget
     int nsblk[2][3] = {};
     // tiled layout per pass (0: CSR pass gathering x+, 1: CSC pass gathering y)
@@ -746,6 +904,12 @@ struct Engine final : EngineBase {
         prm.lanes_rows = plan_code(plan_rows);
         prm.lanes_cols = plan_code(plan_cols);
         prm.hot_columns = plan_rows.hot;
+        if (prm.single_workgroup > 0) {
+            one_wg = true;
+        } else if (prm.single_workgroup < 0 && nnz <= kSmallTuneNnz && prm.time_limit_s <= 0.0) {
+            bo_status s3 = tune_one_wg();
+            if (s3 != BO_SUCCESS) return s3;
+        }
         return BO_SUCCESS;
     }
 
@@ -1050,6 +1214,98 @@ struct Engine final : EngineBase {
         return BO_SUCCESS;
     }
 
+    SmallArgs<T> small_args() {
+        SmallArgs<T> a{};
+        a.m = m;
+        a.n = n;
+        a.K = prm.check_every > 0 ? prm.check_every : 64;
+        a.rp = rp;
+        a.ci = ci;
+        a.cp = cp;
+        a.ri = ri;
+        a.rv = rv;
+        a.cv = cv;
+        a.rv64 = rv64;
+        a.cv64 = cv64;
+        a.ep = EpiPrimal<T>{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, 0, nullptr};
+        a.ed = EpiDual<T>{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, 0, nullptr};
+        a.er = EpiCheckRows<T>{xo_cur, xo_cand, b64, eq, rs, yk, yp, yo_cur, yo_cand, ax_cand};
+        a.ec = EpiCheckCols{yo_cur, yo_cand, c64, lb64, ub64, cs};
+        a.cs = cs;
+        a.c64 = c64;
+        a.lb64 = lb64;
+        a.ub64 = ub64;
+        a.xo_cur = xo_cur;
+        a.xo_cand = xo_cand;
+        a.x0 = x0;
+        a.y0 = y0;
+        a.ax0 = ax0;
+        a.st = st;
+        return a;
+    }
+
+    // LDS for the gathered vectors when they fit beside the kernel's own
+    // (the attribute is raised once, at the first use)
+    unsigned small_lds_bytes() const {
+        const size_t b = (static_cast<size_t>(n) + m) * sizeof(T);
+        return b <= static_cast<size_t>(kSmallMaxLds) ? static_cast<unsigned>(b) : 0u;
+    }
+
+    void launch_one_wg() {
+        SmallArgs<T> a = small_args();
+        const unsigned lds = small_lds_bytes();
+        a.lds = lds > 0 ? 1 : 0;
+        if (lds > 0 && !small_lds_set) {
+            if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kSmallMaxLds)) ==
+                hipSuccess)
+                small_lds_set = true;
+        }
+        if (!small_lds_set) {
+            a.lds = 0;
+            hipLaunchKernelGGL(k_small<T>, dim3(1), dim3(kSmallThreads), 0, stream, a);
+        } else {
+            hipLaunchKernelGGL(k_small<T>, dim3(1), dim3(kSmallThreads), a.lds ? lds : 0u, stream, a);
+        }
+    }
+
+    // One workgroup or the multi-kernel loop: both run 4 check intervals from
+    // the cold start; the one-workgroup solve is kept when it takes less than
+    // 0.8 of the other (the loop's hipGraph replay saves some launch time).
+    bo_status tune_one_wg() {
+        const long long keep = prm.max_iterations;
+        const int K = prm.check_every > 0 ? prm.check_every : 64;
+        prm.max_iterations = 4LL * K;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float t_multi = 0.0f, t_one = 0.0f;
+        bo_status s = BO_SUCCESS;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess && (s = prepare()) == BO_SUCCESS) {
+            e = hipEventRecord(e0, stream);
+            launch_check();
+            for (int r = 0; r < 4; ++r) launch_interval(K);
+            if (e == hipSuccess) e = hipEventRecord(e1, stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t_multi, e0, e1);
+        }
+        if (e == hipSuccess && s == BO_SUCCESS && (s = prepare()) == BO_SUCCESS) {
+            e = hipEventRecord(e0, stream);
+            launch_one_wg();
+            if (e == hipSuccess) e = hipEventRecord(e1, stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t_one, e0, e1);
+            if (e == hipSuccess) e = hipGetLastError();
+        }
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        prm.max_iterations = keep;
+        if (s != BO_SUCCESS) return s;
+        BO_TRY(e);
+        one_wg = t_one < 0.8f * t_multi;
+        return BO_SUCCESS;
+    }
+
     bo_status set_warm_start(const double* wx_h, const double* wy_h) override {
         if (!wx_h && !wy_h) {
             has_warm = 0;
@@ -1144,13 +1400,19 @@ struct Engine final : EngineBase {
         auto t0 = std::chrono::high_resolution_clock::now();
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
-        launch_check();  // step 0: KKT of the starting point
+        if (!one_wg) launch_check();  // step 0: KKT of the starting point (k_small runs its own)
         BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
         BO_TRY(hipStreamSynchronize(stream));
         auto t1 = std::chrono::high_resolution_clock::now();
         const int K = prm.check_every > 0 ? prm.check_every : 64;
         long long iters = 0;
-        while (!st_host->done) {
+        if (one_wg) {
+            launch_one_wg();
+            BO_TRY(hipGetLastError());
+            BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
+            BO_TRY(hipStreamSynchronize(stream));
+        }
+        while (!one_wg && !st_host->done) {
             const long long left = prm.max_iterations - iters;
             const int k = static_cast<int>(std::min<long long>(K, left));
             if (k <= 0) break;
@@ -1206,6 +1468,7 @@ struct Engine final : EngineBase {
             out->lanes_cols = prm.lanes_cols;
             out->hot_columns = plan_rows.hot;
             out->check_every = prm.check_every;
+            out->single_workgroup = one_wg ? 1 : 0;
         }
         return BO_SUCCESS;
     }
@@ -1294,6 +1557,7 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
     p->stall_iterations = 4096;
     p->fp64_polish = 1;
     p->compact_tail = 1;
+    p->single_workgroup = -1;
 }
 
 bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp** out) {

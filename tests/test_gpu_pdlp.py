@@ -248,6 +248,48 @@ def test_dropin_pdlp_engine(pkg, po, lpdata):
     np.testing.assert_allclose(r["y"], [1 / 3, 7 / 3], atol=1e-5)
 
 
+@pytest.mark.parametrize("fp64", [0, 1])
+@pytest.mark.parametrize("name", sorted(OPT))
+def test_pdlp_single_workgroup(pkg, po, instances, name, fp64):
+    # the whole solve in one workgroup (k_small): same stopping rule, same
+    # optimum as the multi-kernel loop
+    lp = instances[name]
+    print("case: one workgroup", name, "fp64", fp64, flush=True)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=200000, single_workgroup=1, fp64_iterates=fp64)
+    s = h.solve()
+    assert s["single_workgroup"] == 1
+    x, y = h.solution()
+    k = check_solution(po, lp, x, y, s, name)
+    h.close()
+    g = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=200000, single_workgroup=0, fp64_iterates=fp64)
+    sg = g.solve()
+    assert sg["single_workgroup"] == 0 and sg["status"] == 0
+    xg, yg = g.solution()
+    kg = po.kkt(lp, xg, yg)
+    assert abs(k["primal_obj"] - kg["primal_obj"]) <= 2e-6 * (1 + abs(kg["primal_obj"]))
+    g.close()
+
+
+def test_pdlp_single_workgroup_limits_and_repeat(pkg, po, instances):
+    name = sorted(OPT)[0]
+    lp = instances[name]
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=200000, single_workgroup=1)
+    s1 = h.solve()
+    x1, y1 = h.solution()
+    s2 = h.solve()
+    x2, y2 = h.solution()
+    assert s1["iterations"] == s2["iterations"]
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+    h.set_warm_start(x1, y1)  # at the optimum: stops at the first check
+    s3 = h.solve()
+    assert s3["status"] == 0 and s3["iterations"] == 0
+    h.close()
+    c = pkg.Pdlp(lp, eps_rel_kkt=1e-12, max_iterations=100, single_workgroup=1)
+    s4 = c.solve()
+    assert s4["status"] == 1 and s4["iterations"] == 100
+    c.close()
+
+
 def test_pdlp_rejects_bad_csr(pkg, lpdata):
     lp = lpdata.random_lp(20, 50, 3, 1)
     bad = lpdata.StdLP(lp.m, lp.n, lp.row_ptr, lp.col_idx.copy(), lp.val, lp.b, lp.c, lp.lb, lp.ub, lp.is_eq)
