@@ -147,6 +147,128 @@ __global__ void k_ref_viol(int m, const int* __restrict__ rp, const int* __restr
     reduce_max_to(amax, v2);
 }
 
+
+// ---------------------------------------------------- one-workgroup loop
+// The same loop for small LPs in one workgroup of 1024 threads, with
+// workgroup barriers in place of kernel launches and host round trips:
+// identical per-row / per-column arithmetic (so the same bits), maxima
+// reduced exactly (order-independent), the stop test on the device.  y and
+// x_hat, the vectors the passes gather, are mirrored in LDS.
+constexpr int kRefSmallThreads = 1024;
+constexpr int kRefSmallMaxEntries = 7680;  // n + m of LPs that run this way (their gathered vectors: 60 KB)
+constexpr long long kRefSmallMaxNnz = 1LL << 16;
+
+struct RefSmallArgs {
+    int m, n;
+    const int *rp, *ci, *cp, *ri;
+    const double *rv, *cv, *b, *c, *lb, *ub, *tau, *sigma;
+    const unsigned char* eq;
+    double *x, *y, *xsum, *xhat, *xavg;
+    long long max_iter, check_freq;
+    double feas_tol;
+    double* out;  // [0] iterations [1] loop iterations [2] status [3] constraint violation [4] bound violation [5] avg count
+};
+
+__device__ inline double block_max_ref(double v, double* sh) {
+    v = wave_max_ref(v);
+    const int lane = static_cast<int>(threadIdx.x) % kWave, wid = static_cast<int>(threadIdx.x) / kWave;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) t = ref_max(t, sh[w]);
+    return t;
+}
+
+__global__ void __launch_bounds__(kRefSmallThreads) k_ref_small(RefSmallArgs a) {
+    __shared__ double sh[kRefSmallThreads / kWave];
+    __shared__ double ly[kRefSmallMaxEntries];  // y copy [0, m), x_hat copy [m, m + n)
+    double* lxh = ly + a.m;
+    const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+    for (int i = tid; i < a.m; i += nt) ly[i] = a.y[i];
+    __syncthreads();
+    long long iter = 0, avg_count = 0;
+    int status = 1;
+    double best = 0.0, bviol = 0.0;
+    while (iter < a.max_iter) {
+        iter++;
+        const bool track = (iter % a.check_freq == 0 || iter == a.max_iter);
+        double dp = 0.0, dd = 0.0;
+        for (int j = tid; j < a.n; j += nt) {  // A'y + primal step + ergodic sum + extrapolation
+            double aty = 0.0;
+            for (int k = a.cp[j]; k < a.cp[j + 1]; ++k) aty = __dadd_rn(aty, __dmul_rn(a.cv[k], ly[a.ri[k]]));
+            const double xo = a.x[j];
+            const double grad = __dadd_rn(a.c[j], aty);
+            const double next_x = __dsub_rn(xo, __dmul_rn(a.tau[j], grad));
+            const double new_x = dclamp(next_x, a.lb[j], a.ub[j]);
+            dp = ref_max(dp, ref_max(0.0, fabs(__dsub_rn(new_x, xo))));
+            a.x[j] = new_x;
+            a.xsum[j] = __dadd_rn(a.xsum[j], new_x);
+            const double xh = __dsub_rn(__dmul_rn(2.0, new_x), xo);
+            a.xhat[j] = xh;
+            lxh[j] = xh;
+        }
+        avg_count++;
+        __syncthreads();
+        for (int i = tid; i < a.m; i += nt) {  // A x_hat + dual step
+            double ax = 0.0;
+            for (int k = a.rp[i]; k < a.rp[i + 1]; ++k) ax = __dadd_rn(ax, __dmul_rn(a.rv[k], lxh[a.ci[k]]));
+            const double yo = ly[i];
+            const double next_y = __dadd_rn(yo, __dmul_rn(a.sigma[i], __dsub_rn(ax, a.b[i])));
+            const double new_y = (!a.eq[i]) ? ref_max(0.0, next_y) : next_y;
+            dd = ref_max(dd, ref_max(0.0, fabs(__dsub_rn(new_y, yo))));
+            a.y[i] = new_y;
+            ly[i] = new_y;
+        }
+        __syncthreads();
+        if (!track) continue;
+        const double pmax = block_max_ref(dp, sh);
+        const double dmax = block_max_ref(dd, sh);
+        double bv = 0.0;
+        for (int j = tid; j < a.n; j += nt) {  // x_avg and the bound violation of x
+            const double xj = a.x[j];
+            if (xj < a.lb[j] - 1e-12) bv = ref_max(bv, a.lb[j] - xj);
+            if (xj > a.ub[j] + 1e-12) bv = ref_max(bv, xj - a.ub[j]);
+            a.xavg[j] = dclamp(a.xsum[j] / static_cast<double>(avg_count), a.lb[j], a.ub[j]);
+        }
+        __syncthreads();
+        double v1 = 0.0, v2 = 0.0;
+        for (int i = tid; i < a.m; i += nt) {  // row violations of x and x_avg
+            double s1 = 0.0, s2 = 0.0;
+            for (int k = a.rp[i]; k < a.rp[i + 1]; ++k) {
+                const double w = a.rv[k];
+                const int j = a.ci[k];
+                s1 = __dadd_rn(s1, __dmul_rn(w, a.x[j]));
+                s2 = __dadd_rn(s2, __dmul_rn(w, a.xavg[j]));
+            }
+            v1 = ref_max(v1, row_viol(s1, a.b[i], a.eq[i]));
+            v2 = ref_max(v2, row_viol(s2, a.b[i], a.eq[i]));
+        }
+        bviol = block_max_ref(bv, sh);
+        const double max_viol = block_max_ref(v1, sh);
+        const double avg_viol = block_max_ref(v2, sh);
+        best = (avg_viol < max_viol) ? avg_viol : max_viol;  // std::min
+        if (isnan(best) || best > 1e15) {
+            status = 2;
+            break;
+        }
+        const bool feasible = best <= a.feas_tol && bviol <= a.feas_tol;
+        const bool stationary = pmax <= 1e-7 && dmax <= 1e-7;
+        if (feasible && stationary && iter >= 100) {
+            status = 0;
+            break;
+        }
+    }
+    if (tid == 0) {
+        a.out[0] = static_cast<double>(iter);
+        a.out[1] = static_cast<double>(iter);
+        a.out[2] = static_cast<double>(status);
+        a.out[3] = best;
+        a.out[4] = bviol;
+        a.out[5] = static_cast<double>(avg_count);
+    }
+}
+
 struct DevBufs {
     std::vector<void*> ptrs;
     ~DevBufs() {
@@ -277,7 +399,26 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
     bool converged = false, failed = false;
     unsigned long long red[8];
 
-    while (iter < o->max_iterations) {
+    const bool one_wg = static_cast<long long>(n) + m <= kRefSmallMaxEntries && nnz <= kRefSmallMaxNnz;
+    if (one_wg && o->max_iterations > 0) {
+        double* d_out = nullptr;
+        BO_TRY(mem.alloc(&d_out, 8));
+        RefSmallArgs a{m, n, d_rp, d_ci, d_cp, d_ri, d_rv, d_cv, d_b, d_c, d_lb, d_ub, d_tau, d_sigma, d_eq,
+                       d_x, d_y, d_xsum, d_xhat, d_xavg, o->max_iterations, check_freq, o->feasibility_tolerance,
+                       d_out};
+        hipLaunchKernelGGL(k_ref_small, dim3(1), dim3(kRefSmallThreads), 0, 0, a);
+        BO_TRY(hipGetLastError());
+        double out[8];
+        BO_TRY(hipMemcpy(out, d_out, sizeof(out), hipMemcpyDeviceToHost));
+        iter = static_cast<long long>(out[0]);
+        avg_count = static_cast<long long>(out[5]);
+        const int status = static_cast<int>(out[2]);
+        converged = status == 0;
+        failed = status == 2;
+        res->constraint_violation = out[3];
+        res->bound_violation = out[4];
+    }
+    while (!one_wg && iter < o->max_iterations) {
         iter++;
         const int track = (iter % check_freq == 0 || iter == o->max_iterations) ? 1 : 0;
         if (track) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d_red, 8);

@@ -113,6 +113,10 @@ def test_benchmark10_on_gpu(pkg, po, lpdata):
     dict(m=60, n=200, k=5, seed=7, it=50000),
     dict(m=1, n=4, k=1, seed=3, it=300),
     dict(m=500, n=120, k=7, seed=8, it=700),
+    # beyond the one-workgroup limits (n + m <= 7680, nnz <= 65536): the
+    # kernel-per-pass loop
+    dict(m=3000, n=6000, k=3, seed=11, it=400),
+    dict(m=400, n=1200, k=60, seed=13, it=300),
 ])
 def test_pdhg_ref_bitequal_random(pkg, po, lpdata, case):
     print("case:", case, flush=True)
