@@ -124,6 +124,8 @@ typedef struct bo_ref_result {
     int used_average;
     double kernel_time_ms;
     double transfer_time_ms;
+    int loop; /* how the loop ran: 0 kernel per pass, 1 one workgroup (matrix in memory),
+                 2 one workgroup with the matrix resident in LDS */
 } bo_ref_result;
 
 /* The PDHG loop of CPUFirstOrderSolver::solve (src/first_order_solver.cpp:103-279)

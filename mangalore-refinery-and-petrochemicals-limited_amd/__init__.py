@@ -68,7 +68,7 @@ class RefResult(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("loop_iterations", C.c_longlong),
                 ("constraint_violation", C.c_double), ("bound_violation", C.c_double),
                 ("used_average", C.c_int), ("kernel_time_ms", C.c_double),
-                ("transfer_time_ms", C.c_double)]
+                ("transfer_time_ms", C.c_double), ("loop", C.c_int)]
 
 
 class PdlpParams(C.Structure):
@@ -278,6 +278,7 @@ class RefSolve:
     kernel_time_ms: float
     x: np.ndarray
     y: np.ndarray
+    loop: int = 0  # 0 kernel per pass, 1 one workgroup, 2 one workgroup with the matrix in LDS
 
 
 def pdhg_ref_solve(lp: StdLP, max_iterations=50000, feasibility_tolerance=1e-7,
@@ -293,7 +294,7 @@ def pdhg_ref_solve(lp: StdLP, max_iterations=50000, feasibility_tolerance=1e-7,
     _check(lib().bo_pdhg_ref_solve(C.byref(h.s), C.byref(o), _p(x), _p(y), C.byref(r)),
            "bo_pdhg_ref_solve")
     return RefSolve(r.status, r.iterations, r.loop_iterations, r.constraint_violation,
-                    r.bound_violation, bool(r.used_average), r.kernel_time_ms, x[:lp.n], y[:lp.m])
+                    r.bound_violation, bool(r.used_average), r.kernel_time_ms, x[:lp.n], y[:lp.m], r.loop)
 
 
 def default_pdlp_params(**kw) -> PdlpParams:

@@ -159,7 +159,7 @@ constexpr int kRefSmallMaxEntries = 7680;  // n + m of LPs that run this way (th
 constexpr long long kRefSmallMaxNnz = 1LL << 16;
 
 struct RefSmallArgs {
-    int m, n;
+    int m, n, nnz;
     const int *rp, *ci, *cp, *ri;
     const double *rv, *cv, *b, *c, *lb, *ub, *tau, *sigma;
     const unsigned char* eq;
@@ -168,6 +168,34 @@ struct RefSmallArgs {
     double feas_tol;
     double* out;  // [0] iterations [1] loop iterations [2] status [3] constraint violation [4] bound violation [5] avg count
 };
+
+// sum_k vals[k] * vec[idx[k]] over [k0, k1) in ascending k with no fused
+// multiply-adds.  The loads and products of up to eight entries are issued
+// ahead; the additions stay one at a time in the reference's order, so the
+// result is the same bits as the plain loop.
+template <int W>
+__device__ inline void dot_chunk(double& s, int k, const double* vals, const int* idx, const double* vec) {
+    int i[W];
+    double p[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) i[u] = idx[k + u];
+#pragma unroll
+    for (int u = 0; u < W; ++u) p[u] = __dmul_rn(vals[k + u], vec[i[u]]);
+#pragma unroll
+    for (int u = 0; u < W; ++u) s = __dadd_rn(s, p[u]);
+}
+
+__device__ inline double ordered_dot(int k0, int k1, const double* vals, const int* idx, const double* vec) {
+    double s = 0.0;
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) dot_chunk<8>(s, k, vals, idx, vec);
+    if (k + 4 <= k1) {
+        dot_chunk<4>(s, k, vals, idx, vec);
+        k += 4;
+    }
+    for (; k < k1; ++k) s = __dadd_rn(s, __dmul_rn(vals[k], vec[idx[k]]));
+    return s;
+}
 
 __device__ inline double block_max_ref(double v, double* sh) {
     v = wave_max_ref(v);
@@ -190,13 +218,14 @@ __global__ void __launch_bounds__(kRefSmallThreads) k_ref_small(RefSmallArgs a) 
     long long iter = 0, avg_count = 0;
     int status = 1;
     double best = 0.0, bviol = 0.0;
+    long long to_check = a.check_freq;
     while (iter < a.max_iter) {
         iter++;
-        const bool track = (iter % a.check_freq == 0 || iter == a.max_iter);
+        const bool track = (--to_check == 0 || iter == a.max_iter);  // iter % check_freq == 0 || last
+        if (to_check == 0) to_check = a.check_freq;
         double dp = 0.0, dd = 0.0;
         for (int j = tid; j < a.n; j += nt) {  // A'y + primal step + ergodic sum + extrapolation
-            double aty = 0.0;
-            for (int k = a.cp[j]; k < a.cp[j + 1]; ++k) aty = __dadd_rn(aty, __dmul_rn(a.cv[k], ly[a.ri[k]]));
+            const double aty = ordered_dot(a.cp[j], a.cp[j + 1], a.cv, a.ri, ly);
             const double xo = a.x[j];
             const double grad = __dadd_rn(a.c[j], aty);
             const double next_x = __dsub_rn(xo, __dmul_rn(a.tau[j], grad));
@@ -211,8 +240,7 @@ __global__ void __launch_bounds__(kRefSmallThreads) k_ref_small(RefSmallArgs a) 
         avg_count++;
         __syncthreads();
         for (int i = tid; i < a.m; i += nt) {  // A x_hat + dual step
-            double ax = 0.0;
-            for (int k = a.rp[i]; k < a.rp[i + 1]; ++k) ax = __dadd_rn(ax, __dmul_rn(a.rv[k], lxh[a.ci[k]]));
+            const double ax = ordered_dot(a.rp[i], a.rp[i + 1], a.rv, a.ci, lxh);
             const double yo = ly[i];
             const double next_y = __dadd_rn(yo, __dmul_rn(a.sigma[i], __dsub_rn(ax, a.b[i])));
             const double new_y = (!a.eq[i]) ? ref_max(0.0, next_y) : next_y;
@@ -258,6 +286,166 @@ __global__ void __launch_bounds__(kRefSmallThreads) k_ref_small(RefSmallArgs a) 
             status = 0;
             break;
         }
+    }
+    if (tid == 0) {
+        a.out[0] = static_cast<double>(iter);
+        a.out[1] = static_cast<double>(iter);
+        a.out[2] = static_cast<double>(status);
+        a.out[3] = best;
+        a.out[4] = bviol;
+        a.out[5] = static_cast<double>(avg_count);
+    }
+}
+
+// The one-workgroup loop with the whole LP in LDS: both orientations of the
+// matrix and the gathered vectors (y, x_hat; x at checks) live in LDS, and
+// each thread keeps the data of the R columns and R rows it owns (j = tid +
+// r * 1024) in registers, so x, x_sum and y reach memory only at checks and
+// at the end.  The sparse dots load four entries ahead and then add them in
+// the reference's order, one at a time: the same additions, the same bits.
+constexpr int kRefLdsMaxPerThread = 2;  // 4 would spill registers
+
+inline size_t ref_lds_bytes(long long m, long long n, long long nnz) {
+    return sizeof(double) * static_cast<size_t>(m + n + 2 * nnz) + sizeof(int) * static_cast<size_t>(m + n + 2 + 2 * nnz);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kRefSmallThreads) k_ref_lds(RefSmallArgs a) {
+    extern __shared__ double dyn[];
+    __shared__ double sh[kRefSmallThreads / kWave];
+    const int m = a.m, n = a.n, nnz = a.nnz;
+    double* ly = dyn;                                // y [m]
+    double* lxh = ly + m;                            // x_hat [n]; x during a check
+    double* lrv = lxh + n;                           // CSR values [nnz]
+    double* lcv = lrv + nnz;                         // CSC values [nnz]
+    int* lrp = reinterpret_cast<int*>(lcv + nnz);    // [m + 1]
+    int* lcp = lrp + m + 1;                          // [n + 1]
+    int* lci = lcp + n + 1;                          // [nnz]
+    int* lri = lci + nnz;                            // [nnz]
+    const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+    for (int k = tid; k < nnz; k += nt) {
+        lrv[k] = a.rv[k];
+        lcv[k] = a.cv[k];
+        lci[k] = a.ci[k];
+        lri[k] = a.ri[k];
+    }
+    for (int i = tid; i <= m; i += nt) lrp[i] = a.rp[i];
+    for (int j = tid; j <= n; j += nt) lcp[j] = a.cp[j];
+    for (int i = tid; i < m; i += nt) ly[i] = a.y[i];
+    double x[R], xs[R], cc[R], tau[R], lo[R], hi[R];
+    double y[R], bb[R], sg[R];
+    bool eq[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = tid + r * nt, i = tid + r * nt;
+        const bool cj = j < n, ri = i < m;
+        x[r] = cj ? a.x[j] : 0.0;
+        xs[r] = cj ? a.xsum[j] : 0.0;
+        cc[r] = cj ? a.c[j] : 0.0;
+        tau[r] = cj ? a.tau[j] : 0.0;
+        lo[r] = cj ? a.lb[j] : 0.0;
+        hi[r] = cj ? a.ub[j] : 0.0;
+        y[r] = ri ? a.y[i] : 0.0;
+        bb[r] = ri ? a.b[i] : 0.0;
+        sg[r] = ri ? a.sigma[i] : 0.0;
+        eq[r] = ri ? (a.eq[i] != 0) : false;
+    }
+    __syncthreads();
+    long long iter = 0, avg_count = 0;
+    int status = 1;
+    double best = 0.0, bviol = 0.0;
+    long long to_check = a.check_freq;
+    while (iter < a.max_iter) {
+        iter++;
+        const bool track = (--to_check == 0 || iter == a.max_iter);  // iter % check_freq == 0 || last
+        if (to_check == 0) to_check = a.check_freq;
+        double dp = 0.0, dd = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // A'y + primal step + ergodic sum + extrapolation
+            const int j = tid + r * nt;
+            if (j < n) {
+                const double aty = ordered_dot(lcp[j], lcp[j + 1], lcv, lri, ly);
+                const double xo = x[r];
+                const double grad = __dadd_rn(cc[r], aty);
+                const double next_x = __dsub_rn(xo, __dmul_rn(tau[r], grad));
+                const double new_x = dclamp(next_x, lo[r], hi[r]);
+                dp = ref_max(dp, ref_max(0.0, fabs(__dsub_rn(new_x, xo))));
+                x[r] = new_x;
+                xs[r] = __dadd_rn(xs[r], new_x);
+                lxh[j] = __dsub_rn(__dmul_rn(2.0, new_x), xo);
+            }
+        }
+        avg_count++;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // A x_hat + dual step
+            const int i = tid + r * nt;
+            if (i < m) {
+                const double ax = ordered_dot(lrp[i], lrp[i + 1], lrv, lci, lxh);
+                const double yo = y[r];
+                const double next_y = __dadd_rn(yo, __dmul_rn(sg[r], __dsub_rn(ax, bb[r])));
+                const double new_y = (!eq[r]) ? ref_max(0.0, next_y) : next_y;
+                dd = ref_max(dd, ref_max(0.0, fabs(__dsub_rn(new_y, yo))));
+                y[r] = new_y;
+                ly[i] = new_y;
+            }
+        }
+        __syncthreads();
+        if (!track) continue;
+        const double pmax = block_max_ref(dp, sh);
+        const double dmax = block_max_ref(dd, sh);
+        double bv = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // x (into LDS), x_avg and the bound violation of x
+            const int j = tid + r * nt;
+            if (j < n) {
+                const double xj = x[r];
+                if (xj < lo[r] - 1e-12) bv = ref_max(bv, lo[r] - xj);
+                if (xj > hi[r] + 1e-12) bv = ref_max(bv, xj - hi[r]);
+                lxh[j] = xj;
+                a.xavg[j] = dclamp(xs[r] / static_cast<double>(avg_count), lo[r], hi[r]);
+            }
+        }
+        __syncthreads();
+        double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // row violations of x and x_avg
+            const int i = tid + r * nt;
+            if (i < m) {
+                double s1 = 0.0, s2 = 0.0;
+                for (int k = lrp[i]; k < lrp[i + 1]; ++k) {
+                    const double w = lrv[k];
+                    const int j = lci[k];
+                    s1 = __dadd_rn(s1, __dmul_rn(w, lxh[j]));
+                    s2 = __dadd_rn(s2, __dmul_rn(w, a.xavg[j]));
+                }
+                v1 = ref_max(v1, row_viol(s1, bb[r], eq[r] ? 1 : 0));
+                v2 = ref_max(v2, row_viol(s2, bb[r], eq[r] ? 1 : 0));
+            }
+        }
+        bviol = block_max_ref(bv, sh);
+        const double max_viol = block_max_ref(v1, sh);
+        const double avg_viol = block_max_ref(v2, sh);
+        best = (avg_viol < max_viol) ? avg_viol : max_viol;  // std::min
+        if (isnan(best) || best > 1e15) {
+            status = 2;
+            break;
+        }
+        const bool feasible = best <= a.feas_tol && bviol <= a.feas_tol;
+        const bool stationary = pmax <= 1e-7 && dmax <= 1e-7;
+        if (feasible && stationary && iter >= 100) {
+            status = 0;
+            break;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = tid + r * nt, i = tid + r * nt;
+        if (j < n) {
+            a.x[j] = x[r];
+            a.xsum[j] = xs[r];
+        }
+        if (i < m) a.y[i] = y[r];
     }
     if (tid == 0) {
         a.out[0] = static_cast<double>(iter);
@@ -399,14 +587,34 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
     bool converged = false, failed = false;
     unsigned long long red[8];
 
-    const bool one_wg = static_cast<long long>(n) + m <= kRefSmallMaxEntries && nnz <= kRefSmallMaxNnz;
+    // Which loop: the whole LP in one workgroup's LDS, one workgroup over the
+    // matrix in memory, or a kernel per pass with the checks on the host.
+    int dev = 0, lds_max = 0;
+    BO_TRY(hipGetDevice(&dev));
+    BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t lds_bytes = ref_lds_bytes(m, n, nnz);
+    const int per_thread = static_cast<int>(((m > n ? m : n) + kRefSmallThreads - 1) / kRefSmallThreads);
+    const bool lds_fit = per_thread <= kRefLdsMaxPerThread && nnz < (1LL << 30) &&
+                         lds_bytes + 1024 <= static_cast<size_t>(lds_max);
+    const bool one_wg = lds_fit || (static_cast<long long>(n) + m <= kRefSmallMaxEntries && nnz <= kRefSmallMaxNnz);
+    res->loop = lds_fit ? 2 : (one_wg ? 1 : 0);
     if (one_wg && o->max_iterations > 0) {
         double* d_out = nullptr;
         BO_TRY(mem.alloc(&d_out, 8));
-        RefSmallArgs a{m, n, d_rp, d_ci, d_cp, d_ri, d_rv, d_cv, d_b, d_c, d_lb, d_ub, d_tau, d_sigma, d_eq,
-                       d_x, d_y, d_xsum, d_xhat, d_xavg, o->max_iterations, check_freq, o->feasibility_tolerance,
-                       d_out};
-        hipLaunchKernelGGL(k_ref_small, dim3(1), dim3(kRefSmallThreads), 0, 0, a);
+        RefSmallArgs a{m, n, static_cast<int>(nnz), d_rp, d_ci, d_cp, d_ri, d_rv, d_cv, d_b, d_c, d_lb, d_ub,
+                       d_tau, d_sigma, d_eq, d_x, d_y, d_xsum, d_xhat, d_xavg, o->max_iterations, check_freq,
+                       o->feasibility_tolerance, d_out};
+        if (lds_fit) {
+            const void* fn = per_thread <= 1 ? reinterpret_cast<const void*>(&k_ref_lds<1>)
+                                             : reinterpret_cast<const void*>(&k_ref_lds<2>);
+            BO_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes)));
+            if (per_thread <= 1)
+                hipLaunchKernelGGL(k_ref_lds<1>, dim3(1), dim3(kRefSmallThreads), lds_bytes, 0, a);
+            else
+                hipLaunchKernelGGL(k_ref_lds<2>, dim3(1), dim3(kRefSmallThreads), lds_bytes, 0, a);
+        } else {
+            hipLaunchKernelGGL(k_ref_small, dim3(1), dim3(kRefSmallThreads), 0, 0, a);
+        }
         BO_TRY(hipGetLastError());
         double out[8];
         BO_TRY(hipMemcpy(out, d_out, sizeof(out), hipMemcpyDeviceToHost));

@@ -109,19 +109,24 @@ def test_benchmark10_on_gpu(pkg, po, lpdata):
 
 
 @pytest.mark.parametrize("case", [
-    dict(m=300, n=1000, k=5, seed=42, it=2000),
-    dict(m=60, n=200, k=5, seed=7, it=50000),
-    dict(m=1, n=4, k=1, seed=3, it=300),
-    dict(m=500, n=120, k=7, seed=8, it=700),
-    # beyond the one-workgroup limits (n + m <= 7680, nnz <= 65536): the
-    # kernel-per-pass loop
-    dict(m=3000, n=6000, k=3, seed=11, it=400),
-    dict(m=400, n=1200, k=60, seed=13, it=300),
+    # loop 2: the whole LP resident in one workgroup's LDS (one or two
+    # columns / rows per thread); loop 1: one workgroup over the matrix in
+    # memory (n + m <= 7680, nnz <= 65536); loop 0: a kernel per pass
+    dict(m=300, n=1000, k=5, seed=42, it=2000, loop=2),
+    dict(m=60, n=200, k=5, seed=7, it=50000, loop=2),
+    dict(m=1, n=4, k=1, seed=3, it=300, loop=2),
+    dict(m=500, n=120, k=7, seed=8, it=700, loop=2),
+    dict(m=200, n=1800, k=2, seed=19, it=600, loop=2),
+    dict(m=2000, n=5000, k=3, seed=17, it=500, loop=1),
+    dict(m=3000, n=6000, k=3, seed=11, it=400, loop=0),
+    dict(m=400, n=1200, k=60, seed=13, it=300, loop=0),
 ])
 def test_pdhg_ref_bitequal_random(pkg, po, lpdata, case):
     print("case:", case, flush=True)
     lp = lpdata.random_lp(case["m"], case["n"], case["k"], case["seed"])
-    _same(pkg.pdhg_ref_solve(lp, max_iterations=case["it"]), po.pdhg_ref(lp, max_iterations=case["it"]))
+    g = pkg.pdhg_ref_solve(lp, max_iterations=case["it"])
+    assert g.loop == case["loop"]
+    _same(g, po.pdhg_ref(lp, max_iterations=case["it"]))
 
 
 def test_pdhg_ref_bitequal_staircase(pkg, po, lpdata):
@@ -149,6 +154,23 @@ def test_pdhg_ref_edge_columns(pkg, po):
         np.array([1.0, -1.0, 0.5, 2.0, -0.3]), np.array([0.0, -np.inf, 0.0, -np.inf, 1.0]),
         np.array([10.0, np.inf, 3.0, np.inf, 1.0]), np.array([0, 1, 0, 0], np.uint8))
     _same(pkg.pdhg_ref_solve(lp, max_iterations=2500), po.pdhg_ref(lp, max_iterations=2500))
+
+
+@pytest.mark.parametrize("step", [1.0, 10.0])
+def test_pdhg_ref_numerical_failure(pkg, po, step):
+    # fixed steps too long for the free columns: the iterates blow up and the
+    # first check reports NUMERICAL_FAILURE (iterations 0, loop stopped at 100)
+    lp = pkg.lpdata.StdLP(
+        4, 5, np.array([0, 2, 4, 4, 6], np.int32), np.array([0, 1, 1, 3, 0, 3], np.int32),
+        np.array([1.0, 2.0, -1.0, 1.0, 3.0, -2.0]), np.array([4.0, 1.0, 0.0, 2.0]),
+        np.array([1.0, -1.0, 0.5, 2.0, -0.3]), np.array([0.0, -np.inf, 0.0, -np.inf, 1.0]),
+        np.array([10.0, np.inf, 3.0, np.inf, 1.0]), np.array([0, 1, 0, 0], np.uint8))
+    g = pkg.pdhg_ref_solve(lp, max_iterations=3000, adaptive_step_size=False, primal_step_size=step,
+                           dual_step_size=step)
+    o = po.pdhg_ref(lp, max_iterations=3000, adaptive=False, primal_step=step, dual_step=step)
+    assert g.loop == 2
+    assert (g.status, g.iterations, g.loop_iterations) == (o.status, o.iterations, o.loop_iterations) == (2, 0, 100)
+    assert g.constraint_violation == o.constraint_violation
 
 
 # ---------------------------------------------------------------- whole drop-in
