@@ -17,6 +17,9 @@
  *                          PDLP (Applegate et al. 2021), pinned by closed forms and by
  *                          HiGHS optimal primal/dual pairs in tests/
  *   orc_max_violation     src/first_order_solver.cpp:202-212 (max row / bound violation)
+ *   orc_pdlp_omp          the restarted Halpern PDHG of the GPU engine in FP64 on
+ *                          all host cores (no reference counterpart; the multi-core
+ *                          CPU baseline of the new algorithm; libpdlp_omp.so)
  *
  * Parity pins (tests/test_oracle_*.py): the reference's own KATs
  * (tests/test_first_order_solver.cpp, tests/test_sparse_matrix.cpp,
@@ -94,6 +97,35 @@ void orc_kkt(int m, int n, const int* rp, const int* ci, const double* v,
              const double* b, const double* c, const double* lb, const double* ub,
              const unsigned char* is_eq, const double* x, const double* y,
              orc_kkt_out* out);
+
+/* ---- the new algorithm on all host cores (pdlp_omp.c, libpdlp_omp.so) ---- */
+typedef struct {
+    long long max_iterations;
+    double eps_rel_kkt;     /* stop when the FP64 rel-KKT (orc_kkt's definition) <= this */
+    int check_every;        /* steps between checks (64) */
+    int ruiz_iterations;    /* 10 */
+    int pock_chambolle;     /* 1 */
+    int threads;            /* OpenMP threads; <= 0: all */
+} orc_pdlp_opts;
+
+typedef struct {
+    int status;             /* 0 rel-KKT reached, 1 iteration limit, 2 numerical failure */
+    long long iterations;
+    int restarts;
+    double rel_kkt, rel_primal, rel_dual, rel_gap, primal_obj, dual_obj;
+    int threads;
+    double step_size;       /* eta = 0.95 / ||A_hat||_2 */
+    double primal_weight;   /* omega at the end */
+    double prep_s;          /* CSC copy of the matrix (a one-time layout cost) */
+    double setup_s;         /* prep + scaling + power iteration */
+    double solve_s;         /* the whole call, setup included */
+} orc_pdlp_out;
+
+/* Restarted, reflected Halpern PDHG in FP64 with OpenMP; x_out[n], y_out[m]
+ * (unscaled, the better of z and T(z) at the last check).  Returns status. */
+int orc_pdlp_omp(int m, int n, const int* rp, const int* ci, const double* v, const double* b,
+                 const double* c, const double* lb, const double* ub, const unsigned char* is_eq,
+                 const orc_pdlp_opts* o, double* x_out, double* y_out, orc_pdlp_out* out);
 
 /* Reference-style violation measure (src/first_order_solver.cpp:202-212). */
 void orc_max_violation(int m, int n, const int* rp, const int* ci, const double* v,

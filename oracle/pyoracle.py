@@ -48,6 +48,52 @@ def oracle_lib():
     return _ORACLE
 
 
+class OrcPdlpOpts(C.Structure):
+    _fields_ = [("max_iterations", C.c_longlong), ("eps_rel_kkt", C.c_double), ("check_every", C.c_int),
+                ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("threads", C.c_int)]
+
+
+class OrcPdlpOut(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("restarts", C.c_int),
+                ("rel_kkt", C.c_double), ("rel_primal", C.c_double), ("rel_dual", C.c_double),
+                ("rel_gap", C.c_double), ("primal_obj", C.c_double), ("dual_obj", C.c_double),
+                ("threads", C.c_int), ("step_size", C.c_double), ("primal_weight", C.c_double),
+                ("prep_s", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double)]
+
+
+_PDLP_OMP = None
+
+
+def pdlp_omp_lib():
+    global _PDLP_OMP
+    if _PDLP_OMP is None:
+        path = os.path.join(HERE, "libpdlp_omp.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+        lib = C.CDLL(path)
+        lib.orc_pdlp_omp.restype = C.c_int
+        lib.orc_pdlp_omp.argtypes = [C.c_int, C.c_int] + [P] * 8 + [P, P, P, P]
+        _PDLP_OMP = lib
+    return _PDLP_OMP
+
+
+def pdlp_omp(lp, max_iterations=100000, eps_rel_kkt=1e-6, check_every=64, ruiz_iterations=10,
+             pock_chambolle=True, threads=0):
+    """The restarted Halpern PDHG in FP64 on `threads` host cores (0: all).
+    Returns (stats dict, x, y)."""
+    lib = pdlp_omp_lib()
+    o = OrcPdlpOpts(max_iterations, eps_rel_kkt, check_every, ruiz_iterations, 1 if pock_chambolle else 0,
+                    threads)
+    out = OrcPdlpOut()
+    x = np.zeros(max(lp.n, 1))
+    y = np.zeros(max(lp.m, 1))
+    eq = np.ascontiguousarray(lp.is_eq, dtype=np.uint8)
+    lib.orc_pdlp_omp(lp.m, lp.n, _p(_i32(lp.row_ptr)), _p(_i32(lp.col_idx)), _p(_f64(lp.val)),
+                     _p(_f64(lp.b)), _p(_f64(lp.c)), _p(_f64(lp.lb)), _p(_f64(lp.ub)), _p(eq),
+                     C.byref(o), _p(x), _p(y), C.byref(out))
+    return {k: getattr(out, k) for k, _ in OrcPdlpOut._fields_}, x[:lp.n], y[:lp.m]
+
+
 def ref_lib():
     """The reference build, or None when it is not present."""
     global _REF

@@ -68,6 +68,30 @@ def test_pdlp_fp64_iterates(pkg, po, instances):
     check_solution(po, lp, x, y, s, name)
 
 
+@pytest.mark.parametrize("name", ["random_300x1000_s42", "random_300x1000_s44", "staircase_3x200x500_s3",
+                                  "zipf_2000x5000_s9"])
+@pytest.mark.parametrize("one_wg", [0, 1])
+def test_pdlp_fp64_follows_the_cpu_restatement(pkg, po, instances, name, one_wg):
+    # The engine in FP64 against oracle/pdlp_omp.c, an independent FP64
+    # restatement of the same algorithm (scaling, power iteration, steps,
+    # checks, restart rule, primal weight): only summation orders differ, so
+    # the runs restart alike and stop within a few check intervals of each
+    # other, at the same optimum.
+    lp = instances[name]
+    print("case:", name, "one workgroup", one_wg, flush=True)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, fp64_iterates=1, check_every=64, single_workgroup=one_wg)
+    s = h.solve()
+    x, y = h.solution()
+    h.close()
+    c, xc, yc = po.pdlp_omp(lp, eps_rel_kkt=TOL, check_every=64, threads=2)
+    print("gpu", s["iterations"], s["restarts"], "cpu", c["iterations"], c["restarts"], flush=True)
+    assert s["status"] == 0 and c["status"] == 0
+    assert abs(s["iterations"] - c["iterations"]) <= max(256, 0.1 * c["iterations"])
+    assert abs(s["step_size"] - c["step_size"]) <= 1e-9 * c["step_size"]  # same power iteration
+    kg, kc = po.kkt(lp, x, y), po.kkt(lp, xc, yc)
+    assert abs(kg["primal_obj"] - kc["primal_obj"]) / (1 + abs(kc["primal_obj"])) <= 2e-6
+
+
 def test_pdlp_warm_start_at_optimum(pkg, po, instances):
     lp = instances["random_300x1000_s43"]
     h = pkg.Pdlp(lp, eps_rel_kkt=TOL)

@@ -7,7 +7,10 @@
   oracle/_ref when present);
 * the sparse KATs of tests/test_sparse_matrix.cpp / test_gpu_foundation.cpp;
 * the FP64 KKT evaluator: closed forms, HiGHS optimal primal-dual pairs, weak
-  duality, and planted mistakes that it must flag.
+  duality, and planted mistakes that it must flag;
+* the OpenMP restatement of the restarted Halpern PDHG (oracle/pdlp_omp.c):
+  its own stop test agrees with the evaluator, and its objectives match the
+  exact optima of tests/golden/lp_optima.json.
 """
 import hashlib
 import json
@@ -222,3 +225,41 @@ def test_kkt_equality_rows_and_free_columns(po):
     # free column with nonzero reduced cost -> dual residual
     k = po.kkt(L, [1.5, 0.5], [0.0])
     assert abs(k["r_dual"] - 1.0) < 1e-15
+
+
+# ------------------------------------------- the new algorithm on host cores
+OPT = json.load(open(os.path.join(HERE, "golden", "lp_optima.json")))
+
+
+@pytest.mark.parametrize("name", sorted(OPT))
+def test_pdlp_omp_reaches_tolerance_and_optimum(po, name):
+    """oracle/pdlp_omp.c (the multi-core CPU baseline of the restarted
+    Halpern PDHG): it must stop at rel-KKT <= 1e-6 by its own FP64 check, the
+    independent evaluator must agree on the returned point, and the objective
+    must match the exact optima (HiGHS, and the reference's simplex chain
+    where it ran) to 1e-6 relative."""
+    import sys
+    sys.path.insert(0, HERE)
+    from golden import make_golden
+    lp = make_golden.pdlp_instances()[name]
+    st, x, y = po.pdlp_omp(lp, threads=2)
+    assert st["status"] == 0 and st["rel_kkt"] <= 1e-6, st
+    k = po.kkt(lp, x, y)
+    assert abs(k["rel_kkt"] - st["rel_kkt"]) <= 1e-9 + 1e-3 * k["rel_kkt"]
+    assert k["max_bound_viol"] == 0.0
+    for key in ("highs", "reference_simplex"):
+        if key in OPT[name]:
+            f = OPT[name][key]
+            assert abs(k["primal_obj"] - f) / (1 + abs(f)) <= 1e-6, (key, k["primal_obj"], f)
+
+
+def test_pdlp_omp_iteration_limit_and_threads(po, lpdata):
+    lp = lpdata.random_lp(300, 1000, 5, 42)
+    st, _, _ = po.pdlp_omp(lp, max_iterations=100, threads=1)
+    assert st["status"] == 1 and st["iterations"] == 100  # 64 + 36: the last interval is cut
+    a, xa, _ = po.pdlp_omp(lp, threads=1)
+    b, xb, _ = po.pdlp_omp(lp, threads=4)
+    assert a["status"] == b["status"] == 0 and b["threads"] == 4
+    # reductions combine in another order with more threads; the path barely moves
+    assert abs(a["iterations"] - b["iterations"]) <= 256
+    assert np.max(np.abs(xa - xb)) <= 1e-3 * (1 + np.max(np.abs(xa)))
