@@ -192,7 +192,8 @@ typedef struct bo_pdlp_stats {
     int hot_columns;       /* LDS-cached entries of x+ in the CSR pass (0 = none) */
     long long fp64_polish_iterations; /* of `iterations`, those run by the FP64 polish */
     int check_every;       /* check cadence used (single-LP engine) */
-    int single_workgroup;  /* 1: the solve ran in one workgroup (single-LP engine) */
+    int single_workgroup;  /* the solve ran in one workgroup (single-LP engine): 1 over the matrix in
+                              memory, 2 with the scaled matrix resident in LDS */
 } bo_pdlp_stats;
 
 /* Default parameters. */

@@ -404,7 +404,7 @@ constexpr int kSmallThreads = 1024;
 
 template <class T>
 struct SmallArgs {
-    int m, n, K;
+    int m, n, K, nnz;
     const int *rp, *ci, *cp, *ri;
     const T *rv, *cv;
     const double *rv64, *cv64;
@@ -537,6 +537,127 @@ __global__ void __launch_bounds__(kSmallThreads) k_small(SmallArgs<T> a) {
                 if (ly) ly[i] = ed.yg[i];
             }
             __syncthreads();
+        }
+        if (tid == 0) st->iter += k;
+        __syncthreads();
+        small_check(a, sh, ly);
+    }
+}
+
+// The one-workgroup solve with the whole scaled matrix resident in LDS (both
+// orientations, T values and 32-bit indices) beside the gathered vectors,
+// and each thread's R columns and R rows (j = tid + r * 1024) of Halpern
+// state in registers.  The steps are those of EpiPrimal / EpiDual with the
+// same arithmetic; the state goes back to memory for each check
+// (small_check, unchanged) and is reloaded after it, since a restart moves it.
+constexpr int kSmallLdsMaxPerThread = 2;
+
+template <class T>
+size_t small_resident_bytes(long long m, long long n, long long nnz) {
+    return sizeof(T) * static_cast<size_t>(2 * nnz + n + m) + sizeof(int) * static_cast<size_t>(m + n + 2 + 2 * nnz);
+}
+
+template <class T, int R>
+__global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
+    __shared__ double sh[kSmallThreads / kWave];
+    extern __shared__ __attribute__((aligned(16))) unsigned char small_lds[];
+    const int m = a.m, n = a.n, nnz = a.nnz;
+    const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+    PdlpState* st = a.st;
+    T* lrv = reinterpret_cast<T*>(small_lds);       // CSR values [nnz]
+    T* lcv = lrv + nnz;                             // CSC values [nnz]
+    T* lx = lcv + nnz;                              // x+ [n]
+    T* ly = lx + n;                                 // y copy [m]
+    int* lrp = reinterpret_cast<int*>(ly + m);      // [m + 1]
+    int* lcp = lrp + m + 1;                         // [n + 1]
+    int* lci = lcp + n + 1;                         // [nnz]
+    int* lri = lci + nnz;                           // [nnz]
+    for (int k = tid; k < nnz; k += nt) {
+        lrv[k] = a.rv[k];
+        lcv[k] = a.cv[k];
+        lci[k] = a.ci[k];
+        lri[k] = a.ri[k];
+    }
+    for (int i = tid; i <= m; i += nt) lrp[i] = a.rp[i];
+    for (int j = tid; j <= n; j += nt) lcp[j] = a.cp[j];
+    for (int j = tid; j < n; j += nt) lx[j] = a.ep.xp[j];
+    for (int i = tid; i < m; i += nt) ly[i] = a.ed.yg[i];
+    small_check(a, sh, ly);  // step 0 (its barriers order the copies above)
+    while (!st->done) {
+        const long long left = st->max_iter - st->iter;
+        const int k = static_cast<int>(left < a.K ? left : a.K);
+        if (k <= 0) break;
+        // the data and state as the last check left them (nothing stays in
+        // registers across a check)
+        T cc[R], lo[R], hi[R], bb[R], xpr[R], ypr[R];
+        bool eq[R];
+        double xk[R], x0[R], yk[R], y0[R], ax[R], ax0[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + r * nt, i = tid + r * nt;
+            cc[r] = j < n ? a.ep.c[j] : T(0);
+            lo[r] = j < n ? a.ep.lb[j] : T(0);
+            hi[r] = j < n ? a.ep.ub[j] : T(0);
+            xpr[r] = j < n ? a.ep.xp[j] : T(0);
+            bb[r] = i < m ? a.ed.b[i] : T(0);
+            eq[r] = i < m ? a.ed.eq[i] != 0 : false;
+            ypr[r] = i < m ? a.ed.yp[i] : T(0);
+            xk[r] = j < n ? a.ep.xk[j] : 0.0;
+            x0[r] = j < n ? a.x0[j] : 0.0;
+            yk[r] = i < m ? a.ed.yk[i] : 0.0;
+            y0[r] = i < m ? a.y0[i] : 0.0;
+            ax[r] = i < m ? a.ed.ax[i] : 0.0;
+            ax0[r] = i < m ? a.ax0[i] : 0.0;
+        }
+        const double tau = st->tau, sigma = st->sigma;
+        const long long base = st->iter - st->iter_last_restart;
+        for (int t = 0; t < k; ++t) {
+            const double kk = static_cast<double>(base + t);
+            const double w = (kk + 1.0) / (kk + 2.0);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {  // column j: x+ = proj(x - tau (c + A'y)), Halpern update of x
+                const int j = tid + r * nt;
+                if (j < n) {
+                    const double acc = seg_dot(lcp[j], lcp[j + 1], lri, lcv, ly);
+                    const double xo = xk[r];
+                    const T xt = clampT(static_cast<T>(xo - tau * (static_cast<double>(cc[r]) + acc)), lo[r], hi[r]);
+                    xpr[r] = xt;
+                    lx[j] = xt;
+                    xk[r] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[r];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {  // row i: y+ = proj(y + sigma (A(2x+ - x) - b)), Halpern update of y, A x
+                const int i = tid + r * nt;
+                if (i < m) {
+                    const double acc = seg_dot(lrp[i], lrp[i + 1], lci, lrv, lx);
+                    const double axo = ax[r], yo = yk[r];
+                    double yn = yo + sigma * (2.0 * acc - axo - static_cast<double>(bb[r]));
+                    if (!eq[r] && yn < 0.0) yn = 0.0;
+                    const T yt = static_cast<T>(yn);
+                    ypr[r] = yt;
+                    const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[r];
+                    yk[r] = ynew;
+                    ly[i] = static_cast<T>(ynew);
+                    ax[r] = w * (2.0 * acc - axo) + (1.0 - w) * ax0[r];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // back to memory for the check
+            const int j = tid + r * nt, i = tid + r * nt;
+            if (j < n) {
+                a.ep.xk[j] = xk[r];
+                a.ep.xp[j] = xpr[r];
+            }
+            if (i < m) {
+                a.ed.yk[i] = yk[r];
+                a.ed.yp[i] = ypr[r];
+                a.ed.yg[i] = static_cast<T>(yk[r]);
+                a.ed.ax[i] = ax[r];
+            }
         }
         if (tid == 0) st->iter += k;
         __syncthreads();
@@ -1219,6 +1340,7 @@ struct Engine final : EngineBase {
         a.m = m;
         a.n = n;
         a.K = prm.check_every > 0 ? prm.check_every : 64;
+        a.nnz = static_cast<int>(nnz);
         a.rp = rp;
         a.ci = ci;
         a.cp = cp;
@@ -1251,7 +1373,30 @@ struct Engine final : EngineBase {
         return b <= static_cast<size_t>(kSmallMaxLds) ? static_cast<unsigned>(b) : 0u;
     }
 
+    // 0: the matrix does not fit the LDS (or too many columns / rows per
+    // thread); else the columns / rows per thread of k_small_lds
+    int small_resident_per_thread() const {
+        const long long per = ((m > n ? m : n) + kSmallThreads - 1) / kSmallThreads;
+        if (per > kSmallLdsMaxPerThread || nnz >= (1LL << 30)) return 0;
+        if (small_resident_bytes<T>(m, n, nnz) + 1024 > static_cast<size_t>(lds_max)) return 0;
+        return per < 1 ? 1 : static_cast<int>(per);
+    }
+
     void launch_one_wg() {
+        if (const int per = small_resident_per_thread()) {
+            SmallArgs<T> a = small_args();
+            const size_t bytes = small_resident_bytes<T>(m, n, nnz);
+            const void* fn = per == 1 ? reinterpret_cast<const void*>(&k_small_lds<T, 1>)
+                                      : reinterpret_cast<const void*>(&k_small_lds<T, 2>);
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) ==
+                hipSuccess) {
+                if (per == 1)
+                    hipLaunchKernelGGL((k_small_lds<T, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                else
+                    hipLaunchKernelGGL((k_small_lds<T, 2>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                return;
+            }
+        }
         SmallArgs<T> a = small_args();
         const unsigned lds = small_lds_bytes();
         a.lds = lds > 0 ? 1 : 0;
@@ -1468,7 +1613,7 @@ struct Engine final : EngineBase {
             out->lanes_cols = prm.lanes_cols;
             out->hot_columns = plan_rows.hot;
             out->check_every = prm.check_every;
-            out->single_workgroup = one_wg ? 1 : 0;
+            out->single_workgroup = one_wg ? (small_resident_per_thread() ? 2 : 1) : 0;
         }
         return BO_SUCCESS;
     }

@@ -87,7 +87,10 @@ def test_pdlp_fp64_follows_the_cpu_restatement(pkg, po, instances, name, one_wg)
     print("gpu", s["iterations"], s["restarts"], "cpu", c["iterations"], c["restarts"], flush=True)
     assert s["status"] == 0 and c["status"] == 0
     assert abs(s["iterations"] - c["iterations"]) <= max(256, 0.1 * c["iterations"])
-    assert abs(s["step_size"] - c["step_size"]) <= 1e-9 * c["step_size"]  # same power iteration
+    # the same 40 power-iteration rounds; the engine seeds them in its storage
+    # order (columns sorted by length), so the estimates of ||A_hat||_2 agree
+    # to the iteration's convergence (1e-4 on the Zipf LP), not to rounding
+    assert abs(s["step_size"] - c["step_size"]) <= 1e-3 * c["step_size"]
     kg, kc = po.kkt(lp, x, y), po.kkt(lp, xc, yc)
     assert abs(kg["primal_obj"] - kc["primal_obj"]) / (1 + abs(kc["primal_obj"])) <= 2e-6
 
@@ -281,7 +284,10 @@ def test_pdlp_single_workgroup(pkg, po, instances, name, fp64):
     print("case: one workgroup", name, "fp64", fp64, flush=True)
     h = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=200000, single_workgroup=1, fp64_iterates=fp64)
     s = h.solve()
-    assert s["single_workgroup"] == 1
+    # 2: the scaled matrix (both orientations) fits the LDS beside the vectors
+    resident = (lp.m <= 2048 and lp.n <= 2048 and
+                (8 if fp64 else 4) * (2 * lp.nnz + lp.m + lp.n) + 4 * (lp.m + lp.n + 2 + 2 * lp.nnz) + 1024 <= 160 * 1024)
+    assert s["single_workgroup"] == (2 if resident else 1)
     x, y = h.solution()
     k = check_solution(po, lp, x, y, s, name)
     h.close()
