@@ -583,6 +583,8 @@ def main():
         "rel_kkt": last["rel_kkt"],
         "primal_objective": last["primal_obj"],
         "ms_per_iteration": iter_ms,
+        "device_setup_ms": last["setup_ms"],  # scaling + power iteration + starting check, inside ms_per_step
+        "single_workgroup": int(last["single_workgroup"]),
         "kernels": table,
         "pass_layout": {"csr": info[0], "csc": info[1]},
         "spmv_gbps": {  # SpMV bytes of the reference formula over each whole pass
