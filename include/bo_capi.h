@@ -404,6 +404,34 @@ bo_status bo_presolve_stats(const bo_model* model, int* status, int* reduced_var
 bo_status bo_first_order_solve(const bo_model* model, const bo_fo_options* opts,
                                bo_fo_result* result, double* primal_out, double* dual_out);
 
+/* ---- MPS reader ------------------------------------------------------------ */
+/* MpsParser::parse_string / parse_file (include/bharatopt/mps_parser.hpp:40-49,
+ * src/mps_parser.cpp:44-529) into a model handle whose flattened LPModel
+ * (bo_model above) feeds bo_first_order_solve. */
+typedef struct bo_mps_info {
+    int status; /* MpsParseStatus ordinal: 0 SUCCESS, 1 FILE_NOT_FOUND, 2 MISSING_ENDATA,
+                   3 UNKNOWN_SECTION, 4 MALFORMED_ROW, 5 MALFORMED_COLUMN, 6 MALFORMED_BOUND,
+                   7 UNCLOSED_INTEGER_MARKER, 8 INVALID_NUMERIC_VALUE */
+    int num_vars;
+    int num_cons;
+    long long nonzeros; /* coefficients placed (MpsParseResult::nonzeros_parsed) */
+    int integer_vars;
+    int binary_vars;
+    double parse_time_ms;
+    char problem_name[128];
+    char message[256];
+} bo_mps_info;
+
+struct bo_mps;
+/* text != NULL: parse that text; else read the file at `path`.  Returns
+ * BO_SUCCESS whenever the reader ran: info->status tells whether the text was
+ * a model, and *model is set (non-NULL) only when it was. */
+bo_status bo_mps_read(const char* text, const char* path, bo_mps_info* info, struct bo_mps** model);
+/* The model as a bo_model whose arrays belong to the handle (valid until
+ * bo_mps_destroy). */
+bo_status bo_mps_model(const struct bo_mps* model, bo_model* view);
+void bo_mps_destroy(struct bo_mps* model);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,16 @@ class BoModel(C.Structure):
                 ("term_var", P), ("term_val", P)]
 
 
+class MpsInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("num_vars", C.c_int), ("num_cons", C.c_int), ("nonzeros", C.c_longlong),
+                ("integer_vars", C.c_int), ("binary_vars", C.c_int), ("parse_time_ms", C.c_double),
+                ("problem_name", C.c_char * 128), ("message", C.c_char * 256)]
+
+
+MPS_STATUS = ["SUCCESS", "FILE_NOT_FOUND", "MISSING_ENDATA", "UNKNOWN_SECTION", "MALFORMED_ROW",
+              "MALFORMED_COLUMN", "MALFORMED_BOUND", "UNCLOSED_INTEGER_MARKER", "INVALID_NUMERIC_VALUE"]
+
+
 class FoOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_longlong), ("feasibility_tolerance", C.c_double),
                 ("optimality_tolerance", C.c_double), ("primal_step_size", C.c_double),
@@ -139,6 +149,7 @@ EXPORTS = [
     "bo_pdlp_family_get_solutions", "bo_pdlp_family_time_passes", "bo_pdlp_family_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
     "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_time", "bo_mlp_destroy", "bo_gemm_bf16",
+    "bo_mps_read", "bo_mps_model", "bo_mps_destroy",
 ]
 
 
@@ -174,6 +185,10 @@ def lib():
         L.bo_pdlp_destroy.restype = None
         L.bo_pdlp_time_spmv.argtypes = [P, C.c_int, P, P]
         L.bo_first_order_solve.argtypes = [P, P, P, P, P]
+        L.bo_mps_read.argtypes = [C.c_char_p, C.c_char_p, P, C.POINTER(P)]
+        L.bo_mps_model.argtypes = [P, P]
+        L.bo_mps_destroy.argtypes = [P]
+        L.bo_mps_destroy.restype = None
         L.bo_presolve_stats.argtypes = [P, P, P, P, P, P]
         L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
         for name in EXPORTS:
@@ -624,6 +639,46 @@ def copy_kernel_ms(nbytes, reps=20):
     ms = C.c_double()
     _check(lib().bo_copy_kernel_calibrate(int(nbytes), int(reps), C.byref(ms)), "bo_copy_kernel_calibrate")
     return ms.value
+
+
+def read_mps(text=None, path=None):
+    """MpsParser::parse_string (text) or parse_file (path) through the C ABI.
+    Returns (info dict, FlatModel or None when the text is not a model)."""
+    L = lib()
+    info = MpsInfo()
+    h = P()
+    st = L.bo_mps_read(None if text is None else text.encode(), None if path is None else path.encode(),
+                       C.byref(info), C.byref(h))
+    out = {k: getattr(info, k) for k, _ in MpsInfo._fields_}
+    out["problem_name"] = info.problem_name.decode()
+    out["message"] = info.message.decode()
+    out["status_name"] = MPS_STATUS[info.status] if 0 <= info.status < len(MPS_STATUS) else str(info.status)
+    if st != 0:
+        raise BoError(st, "bo_mps_read: " + out["message"])
+    if not h:
+        return out, None
+    try:
+        v = BoModel()
+        _check(L.bo_mps_model(h, C.byref(v)), "bo_mps_model")
+        nv, nc = v.num_vars, v.num_cons
+
+        def arr(ptr, ctype, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).astype(dtype, copy=True)
+
+        tp = arr(v.term_ptr, C.c_longlong, nc + 1, np.int64)
+        nnz = int(tp[-1]) if nc > 0 else 0
+        fm = FlatModel(num_vars=nv, num_cons=nc, sense=v.sense, obj_offset=v.obj_offset,
+                       lb=arr(v.lb, C.c_double, nv, np.float64), ub=arr(v.ub, C.c_double, nv, np.float64),
+                       obj=arr(v.obj, C.c_double, nv, np.float64), var_type=arr(v.var_type, C.c_int, nv, np.int32),
+                       con_sense=arr(v.con_sense, C.c_int, nc, np.int32), rhs=arr(v.rhs, C.c_double, nc, np.float64),
+                       range_upper=arr(v.range_upper, C.c_double, nc, np.float64), term_ptr=tp,
+                       term_var=arr(v.term_var, C.c_int, nnz, np.int32),
+                       term_val=arr(v.term_val, C.c_double, nnz, np.float64))
+    finally:
+        L.bo_mps_destroy(h)
+    return out, fm
 
 
 def presolve_stats(fm: FlatModel):

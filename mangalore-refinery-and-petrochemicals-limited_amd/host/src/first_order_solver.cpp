@@ -218,16 +218,34 @@ FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
         converged = r.status == 0;
         result.stats.iterations = static_cast<size_t>(r.iterations);
     } else {
+        // Precision: LPs up to 2^18 nonzeros run with FP64 iterates (they run
+        // in one workgroup, where FP64 costs about the same, and badly scaled
+        // models such as Netlib's kb2 or share2b do not converge with FP32
+        // iterates); larger ones run FP32 first and, if that stalls, go on
+        // in FP64 from where it stopped, for the iterations left.
         bo_pdlp_params p;
         bo_pdlp_default_params(&p);
         p.max_iterations = static_cast<long long>(options_.max_iterations);
         p.eps_rel_kkt = options_.relative_kkt_tolerance;
+        p.fp64_iterates = lp.nnz <= (1LL << 18) ? 1 : 0;
         bo_pdlp* h = nullptr;
         bo_status s = bo_pdlp_create(&lp, &p, &h);
         bo_pdlp_stats st{};
         if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
         if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
         bo_pdlp_destroy(h);
+        if (s == BO_SUCCESS && st.status == 4 && st.iterations < p.max_iterations) {
+            const long long used = st.iterations;
+            p.fp64_iterates = 1;
+            p.max_iterations -= used;
+            h = nullptr;
+            s = bo_pdlp_create(&lp, &p, &h);
+            if (s == BO_SUCCESS) s = bo_pdlp_set_warm_start(h, x.data(), y.data());
+            if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
+            if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
+            bo_pdlp_destroy(h);
+            st.iterations += used;
+        }
         if (s != BO_SUCCESS) {
             result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
             result.message = device_error(s);

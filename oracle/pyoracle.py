@@ -94,6 +94,34 @@ def pdlp_omp(lp, max_iterations=100000, eps_rel_kkt=1e-6, check_every=64, ruiz_i
     return {k: getattr(out, k) for k, _ in OrcPdlpOut._fields_}, x[:lp.n], y[:lp.m]
 
 
+def ref_mps(text):
+    """The reference's MpsParser::parse_string on `text`, flattened like
+    bo_model: dict(status, sense, obj_offset, lb, ub, obj, var_type, con_sense,
+    rhs, range_upper, term_ptr, term_var, term_val); None without oracle/_ref."""
+    lib = ref_lib()
+    if lib is None:
+        return None
+    raw = text.encode()
+    dims = np.zeros(5, np.int64)
+    off = C.c_double()
+    if lib.ref_mps_parse(raw, _p(dims), C.byref(off), -1, *([None] * 10)) != 0:
+        raise RuntimeError("reference MpsParser raised")
+    nv, nc, nt = int(dims[1]), int(dims[2]), int(dims[3])
+    a = dict(lb=np.zeros(max(nv, 1)), ub=np.zeros(max(nv, 1)), obj=np.zeros(max(nv, 1)),
+             var_type=np.zeros(max(nv, 1), np.int32), con_sense=np.zeros(max(nc, 1), np.int32),
+             rhs=np.zeros(max(nc, 1)), range_upper=np.zeros(max(nc, 1)), term_ptr=np.zeros(nc + 1, np.int64),
+             term_var=np.zeros(max(nt, 1), np.int32), term_val=np.zeros(max(nt, 1)))
+    rc = lib.ref_mps_parse(raw, _p(dims), C.byref(off), nt, _p(a["lb"]), _p(a["ub"]), _p(a["obj"]),
+                           _p(a["var_type"]), _p(a["con_sense"]), _p(a["rhs"]), _p(a["range_upper"]),
+                           _p(a["term_ptr"]), _p(a["term_var"]), _p(a["term_val"]))
+    if rc != 0:
+        raise RuntimeError("reference MpsParser raised")
+    out = {k: v[:{"lb": nv, "ub": nv, "obj": nv, "var_type": nv, "con_sense": nc, "rhs": nc,
+                  "range_upper": nc, "term_ptr": nc + 1, "term_var": nt, "term_val": nt}[k]] for k, v in a.items()}
+    out.update(status=int(dims[0]), sense=int(dims[4]), obj_offset=off.value)
+    return out
+
+
 def ref_lib():
     """The reference build, or None when it is not present."""
     global _REF
@@ -112,6 +140,8 @@ def ref_lib():
         lib.ref_verify.argtypes = [P, P, C.c_double, P, P, P, P]
         lib.ref_csr_spmv.restype = C.c_longlong
         lib.ref_csr_spmv.argtypes = [C.c_int, C.c_int, C.c_longlong, P, P, P, P, P, P, P]
+        lib.ref_mps_parse.restype = C.c_int
+        lib.ref_mps_parse.argtypes = [C.c_char_p, P, P, C.c_longlong] + [P] * 10
         _REF = lib
     return _REF
 

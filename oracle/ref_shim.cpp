@@ -8,7 +8,9 @@
 //   DualRevisedSimplex / RevisedSimplex (oracle chain of src/conformance_runner.cpp:81-101)
 //   SolutionVerifier::verify          include/bharatopt/solution_verifier.hpp:46-65
 //   CSRMatrix::from_coo / multiply / multiply_transpose  src/sparse_matrix.cpp
+//   MpsParser::parse_string           include/bharatopt/mps_parser.hpp:40-49
 #include <bharatopt/dual_revised_simplex.hpp>
+#include <bharatopt/mps_parser.hpp>
 #include <bharatopt/first_order_solver.hpp>
 #include <bharatopt/presolve.hpp>
 #include <bharatopt/revised_simplex.hpp>
@@ -226,6 +228,56 @@ long long ref_csr_spmv(int rows, int cols, long long nnz, const int* ri, const i
         if (x && y) a.multiply(x, y);
         if (xt && yt) a.multiply_transpose(xt, yt);
         return static_cast<long long>(a.nnz());
+    } catch (...) {
+        return -1;
+    }
+}
+
+// MpsParser::parse_string, the model flattened like ref_model into caller
+// buffers.  dims[0..4] = status, vars, cons, terms (row-wise, insertion
+// order), sense.  Call with cap_terms < 0 for the sizes only.  Returns 0, or
+// -1 on an exception or buffers too small.
+int ref_mps_parse(const char* text, long long* dims, double* offset, long long cap_terms, double* lb, double* ub,
+                  double* obj, int* vtype, int* csense, double* rhs, double* rup, long long* tptr, int* tvar,
+                  double* tval) {
+    try {
+        MpsParseResult r = MpsParser().parse_string(text);
+        const LPModel& m = r.model;
+        long long terms = 0;
+        for (size_t i = 0; i < m.num_constraints(); ++i)
+            terms += static_cast<long long>(m.get_constraint(static_cast<index_t>(i)).terms.size());
+        dims[0] = static_cast<long long>(r.status);
+        dims[1] = static_cast<long long>(m.num_variables());
+        dims[2] = static_cast<long long>(m.num_constraints());
+        dims[3] = terms;
+        dims[4] = m.sense() == ObjectiveSense::MAXIMIZE ? 1 : 0;
+        *offset = m.obj_offset();
+        if (cap_terms < 0) return 0;
+        if (cap_terms < terms) return -1;
+        for (size_t j = 0; j < m.num_variables(); ++j) {
+            const Variable& v = m.get_variable(static_cast<index_t>(j));
+            lb[j] = v.lower_bound;
+            ub[j] = v.upper_bound;
+            obj[j] = v.obj_coeff;
+            vtype[j] = v.type == VariableType::BINARY ? 2 : (v.type == VariableType::INTEGER ? 1 : 0);
+        }
+        long long t = 0;
+        tptr[0] = 0;
+        for (size_t i = 0; i < m.num_constraints(); ++i) {
+            const Constraint& c = m.get_constraint(static_cast<index_t>(i));
+            csense[i] = c.sense == ConstraintSense::EQUAL ? 1
+                        : c.sense == ConstraintSense::GREATER_EQUAL ? 2
+                        : c.sense == ConstraintSense::RANGED ? 3 : 0;
+            rhs[i] = c.rhs;
+            rup[i] = c.range_upper;
+            for (const auto& term : c.terms) {
+                tvar[t] = static_cast<int>(term.first);
+                tval[t] = term.second;
+                ++t;
+            }
+            tptr[i + 1] = t;
+        }
+        return 0;
     } catch (...) {
         return -1;
     }

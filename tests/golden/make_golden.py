@@ -12,8 +12,15 @@ and scipy's HiGHS are available):
                       independent exact solvers: the reference's simplex
                       chain (dual -> primal Dantzig -> primal Bland) where it
                       is tractable, and HiGHS (scipy.optimize.linprog).
+* mps_golden.json     for every MPS file under mps/ (the reference's
+                      conformance set, example1 and the refinery demo, copied
+                      as data): the reference MpsParser's model (status,
+                      sizes, SHA-256 of the flattened arrays) and the LP
+                      optimum of that model by HiGHS (integrality relaxed,
+                      objective in the model's sense with its offset).
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py        (ref_pdhg_kat, lp_optima)
+        python tests/golden/make_golden.py mps    (mps_golden)
 """
 import hashlib
 import json
@@ -148,5 +155,71 @@ def main():
         json.dump(opt, f, indent=1, sort_keys=True)
 
 
+MPS_KEYS = ("lb", "ub", "obj", "var_type", "con_sense", "rhs", "range_upper", "term_ptr", "term_var", "term_val")
+
+
+def flat_sha(d):
+    """SHA-256 over the flattened model arrays in a fixed dtype per field."""
+    h = hashlib.sha256()
+    for k in MPS_KEYS:
+        dt = np.int64 if k in ("var_type", "con_sense", "term_ptr", "term_var") else np.float64
+        h.update(np.ascontiguousarray(np.asarray(d[k]), dt).tobytes())
+    h.update(np.array([d["sense"]], np.int64).tobytes())
+    h.update(np.array([d["obj_offset"]], np.float64).tobytes())
+    return h.hexdigest()
+
+
+def highs_flat(d):
+    """LP optimum of a flattened model by HiGHS: (status name, objective)."""
+    from scipy.optimize import linprog
+    from scipy.sparse import csr_matrix
+    nv, nc = len(d["lb"]), len(d["rhs"])
+    A = csr_matrix((d["term_val"], d["term_var"], d["term_ptr"]), shape=(nc, nv))
+    cs, rhs, ru = d["con_sense"], d["rhs"], d["range_upper"]
+    ub_rows, ub_rhs, eq_rows, eq_rhs = [], [], [], []
+    for i in range(nc):
+        if cs[i] == 0:
+            ub_rows.append(A[i]); ub_rhs.append(rhs[i])
+        elif cs[i] == 2:
+            ub_rows.append(-A[i]); ub_rhs.append(-rhs[i])
+        elif cs[i] == 1:
+            eq_rows.append(A[i]); eq_rhs.append(rhs[i])
+        else:  # ranged: rhs <= a'x <= range_upper
+            ub_rows.append(A[i]); ub_rhs.append(ru[i])
+            ub_rows.append(-A[i]); ub_rhs.append(-rhs[i])
+    from scipy.sparse import vstack
+    sign = -1.0 if d["sense"] == 1 else 1.0
+    res = linprog(sign * np.asarray(d["obj"]), A_ub=vstack(ub_rows) if ub_rows else None,
+                  b_ub=ub_rhs if ub_rows else None, A_eq=vstack(eq_rows) if eq_rows else None,
+                  b_eq=eq_rhs if eq_rows else None, bounds=list(zip(d["lb"], d["ub"])), method="highs",
+                  options=dict(primal_feasibility_tolerance=1e-10, dual_feasibility_tolerance=1e-10))
+    name = {0: "optimal", 2: "infeasible", 3: "unbounded"}.get(res.status, "other")
+    return name, (float(sign * res.fun + d["obj_offset"]) if res.status == 0 else None)
+
+
+def mps_main():
+    if po.ref_lib() is None:
+        raise SystemExit("oracle/_ref is not built; run make -C oracle first")
+    out = {}
+    folder = os.path.join(HERE, "mps")
+    for fname in sorted(os.listdir(folder)):
+        if not fname.endswith(".mps"):
+            continue
+        d = po.ref_mps(open(os.path.join(folder, fname)).read())
+        entry = dict(status=d["status"], num_vars=len(d["lb"]), num_cons=len(d["rhs"]),
+                     terms=int(d["term_ptr"][-1]) if len(d["term_ptr"]) else 0, sense=d["sense"],
+                     obj_offset=d["obj_offset"], model_sha=flat_sha(d),
+                     integer_vars=int(np.sum(np.asarray(d["var_type"]) > 0)))
+        if d["status"] == 0:
+            entry["highs_status"], entry["highs_objective"] = highs_flat(d)
+        out[fname] = entry
+        print(fname, entry, flush=True)
+    with open(os.path.join(HERE, "mps_golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "mps":
+        mps_main()
+    else:
+        main()
