@@ -601,6 +601,13 @@ def main():
         },
         "setup_s": setup_s,
     }
+    if int(last["single_workgroup"]) > 0:
+        out["roofline"]["note"] = (
+            "the solve ran in one workgroup (single_workgroup=%d: %s), which reads the matrix from HBM once "
+            "and is bound by one CU's LDS and barrier latency, not by HBM; the kernels above are the "
+            "multi-kernel loop's passes on the same LP" % (
+                int(last["single_workgroup"]),
+                "matrix resident in LDS" if int(last["single_workgroup"]) == 2 else "matrix in memory"))
     if args.workload == "c0":
         # the reference's own algorithm (diagonal steps, ergodic average, its
         # stop rule) in FP64 on the GPU, bit-equal to the reference loop

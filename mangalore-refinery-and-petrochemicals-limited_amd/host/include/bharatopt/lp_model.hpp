@@ -86,6 +86,8 @@ public:
     const std::vector<Constraint>& constraints() const { return cons_; }
 
 private:
+    friend class PresolveEngine;  // hands over its working copy when it removed nothing
+
     std::string name_;
     ObjectiveSense sense_{ObjectiveSense::MINIMIZE};
     real_t obj_offset_{0.0};

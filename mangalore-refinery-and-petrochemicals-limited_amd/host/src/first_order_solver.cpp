@@ -60,8 +60,41 @@ StandardForm build_standard_form(const LPModel& model) {
     }
     sf.b.assign(sf.m, 0.0);
     sf.is_eq.assign(sf.m, 0);
-    COOMatrix coo(static_cast<size_t>(sf.m), static_cast<size_t>(sf.n));
-    int next_extra = sf.m_model;
+    // CSR straight from the rows, with CSRMatrix::from_coo's rule per row:
+    // columns ascending (stable, so repeated columns add in insertion order),
+    // repeated columns summed, exact zeros dropped.  The ranged rows' extra
+    // "-a'x <= -lower" rows follow the model's rows in constraint order.
+    std::vector<index_t> ptr(static_cast<size_t>(sf.m) + 1, 0), idx;
+    std::vector<real_t> val;
+    size_t total = 0;
+    for (const auto& c : model.constraints())
+        total += c.terms.size() * (c.sense == ConstraintSense::RANGED ? 2 : 1);
+    idx.reserve(total);
+    val.reserve(total);
+    std::vector<std::pair<index_t, real_t>> row;
+    auto append_row = [&](const Constraint& con, real_t mult, int r) {
+        row.clear();
+        for (const auto& t : con.terms) {
+            if (t.first < 0 || t.first >= sf.n)
+                throw std::out_of_range("COO triplet index out of range during compression.");
+            row.emplace_back(t.first, t.second * mult);
+        }
+        auto by_col = [](const std::pair<index_t, real_t>& a, const std::pair<index_t, real_t>& b) {
+            return a.first < b.first;
+        };
+        if (!std::is_sorted(row.begin(), row.end(), by_col)) std::stable_sort(row.begin(), row.end(), by_col);
+        for (size_t k = 0; k < row.size();) {
+            real_t sum = row[k].second;
+            size_t q = k + 1;
+            for (; q < row.size() && row[q].first == row[k].first; ++q) sum += row[q].second;
+            if (sum != 0.0) {
+                idx.push_back(row[k].first);
+                val.push_back(sum);
+            }
+            k = q;
+        }
+        ptr[static_cast<size_t>(r) + 1] = static_cast<index_t>(idx.size());
+    };
     for (int i = 0; i < sf.m_model; ++i) {
         const Constraint& con = model.constraints()[static_cast<size_t>(i)];
         real_t mult = 1.0;
@@ -73,15 +106,16 @@ StandardForm build_standard_form(const LPModel& model) {
             sf.is_eq[i] = 1;
         } else if (con.sense == ConstraintSense::RANGED) {
             rhs = con.range_upper;  // a'x <= upper here, -a'x <= -lower in the extra row
-            const int e = next_extra++;
-            sf.b[e] = -con.rhs;
+            sf.b[sf.m_model + static_cast<int>(sf.lower_of.size())] = -con.rhs;
             sf.lower_of.push_back(i);
-            for (const auto& t : con.terms) coo.add_entry(e, t.first, -t.second);
         }
         sf.b[i] = rhs;
-        for (const auto& t : con.terms) coo.add_entry(i, t.first, t.second * mult);
+        append_row(con, mult, i);
     }
-    sf.A = CSRMatrix::from_coo(coo);
+    for (size_t e = 0; e < sf.lower_of.size(); ++e)
+        append_row(model.constraints()[static_cast<size_t>(sf.lower_of[e])], -1.0, sf.m_model + static_cast<int>(e));
+    sf.A = CSRMatrix(static_cast<size_t>(sf.m), static_cast<size_t>(sf.n), std::move(ptr), std::move(idx),
+                     std::move(val));
     return sf;
 }
 

@@ -84,6 +84,9 @@ ValidationResult ModelValidator::validate(const LPModel& model) const {
     }
 
     const auto& cons = model.constraints();
+    // repeated terms: the row that last used each variable (one array for all
+    // rows instead of a set per row)
+    std::vector<index_t> last_row(vars.size(), -1);
     std::unordered_set<std::string> cnames;
     cnames.reserve(cons.size());
     for (size_t i = 0; i < cons.size(); ++i) {
@@ -101,7 +104,6 @@ ValidationResult ModelValidator::validate(const LPModel& model) const {
             else if (!std::isnan(c.rhs) && c.rhs > c.range_upper)
                 r.add_error("CONSTRAINT", "Ranged constraint lower bound > upper range.", c.name, id);
         }
-        std::unordered_set<index_t> seen;
         for (const auto& t : c.terms) {
             if (t.first < 0 || static_cast<size_t>(t.first) >= vars.size()) {
                 r.add_error("COEFFICIENT",
@@ -114,7 +116,10 @@ ValidationResult ModelValidator::validate(const LPModel& model) const {
                 r.add_error("COEFFICIENT", "Constraint term coefficient is infinite.", c.name, id);
             else if (t.second == 0.0)
                 r.add_warning("COEFFICIENT", "Constraint contains explicit zero coefficient.", c.name, id);
-            if (!seen.insert(t.first).second)
+            index_t& last = last_row[static_cast<size_t>(t.first)];
+            const bool repeated = last == id;
+            last = id;
+            if (repeated)
                 r.add_warning("COEFFICIENT",
                               "Constraint contains duplicate terms for variable #" + std::to_string(t.first) + ".",
                               c.name, id);

@@ -134,11 +134,13 @@ PresolveResult PresolveEngine::presolve(const LPModel& orig, const PresolveOptio
     res.postsolve.initialize(orig);
     res.stats.original_vars = orig.num_variables();
     res.stats.original_cons = orig.num_constraints();
-    auto finish = [&](PresolveStatus s) {
+    // every return goes through here; res is not used afterwards, so it is
+    // moved out rather than copied (two models and the postsolve record)
+    auto finish = [&](PresolveStatus s) -> PresolveResult {
         res.status = s;
         res.stats.presolve_time_ms =
             std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
-        return res;
+        return std::move(res);
     };
 
     const size_t nv = orig.num_variables(), nc = orig.num_constraints();
@@ -339,14 +341,45 @@ PresolveResult PresolveEngine::presolve(const LPModel& orig, const PresolveOptio
     LPModel red("reduced_" + orig.name());
     red.set_sense(sense);
     red.set_obj_offset(w.offset);
-    std::vector<index_t> to_red(nv, -1);
-    for (size_t j = 0; j < nv; ++j) {
+    // Nothing removed (every variable and row active, every row non-empty,
+    // every term on a valid variable): the working copy is the reduced model,
+    // handed over without rebuilding it -- the same variables, rows and terms
+    // in the same order as the general path below, whose add_variable /
+    // add_constraint calls it replaces (names re-registered, so a repeated
+    // name still throws).
+    bool kept_all = std::all_of(w.var_on.begin(), w.var_on.end(), [](char on) { return on != 0; }) &&
+                    std::all_of(w.con_on.begin(), w.con_on.end(), [](char on) { return on != 0; });
+    for (size_t i = 0; kept_all && i < nc; ++i) {
+        kept_all = !w.cons[i].terms.empty();
+        for (const auto& t : w.cons[i].terms) kept_all = kept_all && t.first >= 0 && static_cast<size_t>(t.first) < nv;
+    }
+    if (kept_all) {
+        red.var_lookup_.reserve(nv);
+        for (size_t j = 0; j < nv; ++j) {
+            Variable& v = w.vars[j];
+            v.index = static_cast<index_t>(j);
+            if (!red.var_lookup_.emplace(v.name, v.index).second)
+                throw std::invalid_argument("Variable with name '" + v.name + "' already exists in LPModel.");
+            res.postsolve.record_var_map(v.index, v.index);
+        }
+        red.con_lookup_.reserve(nc);
+        for (size_t i = 0; i < nc; ++i) {
+            Constraint& c = w.cons[i];
+            c.index = static_cast<index_t>(i);
+            if (!red.con_lookup_.emplace(c.name, c.index).second)
+                throw std::invalid_argument("Constraint with name '" + c.name + "' already exists in LPModel.");
+        }
+        red.vars_ = std::move(w.vars);
+        red.cons_ = std::move(w.cons);
+    }
+    std::vector<index_t> to_red(kept_all ? 0 : nv, -1);
+    for (size_t j = 0; !kept_all && j < nv; ++j) {
         if (!w.var_on[j]) continue;
         const Variable& v = w.vars[j];
         to_red[j] = red.add_variable(v.name, v.lower_bound, v.upper_bound, v.obj_coeff, v.type);
         res.postsolve.record_var_map(static_cast<index_t>(j), to_red[j]);
     }
-    for (size_t i = 0; i < nc; ++i) {
+    for (size_t i = 0; !kept_all && i < nc; ++i) {
         if (!w.con_on[i]) continue;
         const Constraint& c = w.cons[i];
         std::vector<std::pair<index_t, real_t>> terms;
