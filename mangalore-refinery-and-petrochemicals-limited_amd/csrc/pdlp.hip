@@ -416,7 +416,9 @@ struct SmallArgs {
     double *xo_cur, *xo_cand;
     double *x0, *y0, *ax0;  // anchors (written at restarts)
     PdlpState* st;
-    int lds;  // 1: copies of x+ and y (the gathered vectors) live in LDS, n + m entries of T
+    int lds;    // 1: copies of x+ and y (the gathered vectors) live in LDS, n + m entries of T
+    int power;  // k_small_lds: 1 = run the power iteration, the state's start and A x0 in the kernel
+    const double* scal;  // [2] ||c_hat||^2, [3] ||b_hat||^2
 };
 
 template <class TM, class TV>
@@ -580,7 +582,58 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
     }
     for (int i = tid; i <= m; i += nt) lrp[i] = a.rp[i];
     for (int j = tid; j <= n; j += nt) lcp[j] = a.cp[j];
-    for (int j = tid; j < n; j += nt) lx[j] = a.ep.xp[j];
+    if (a.power) {
+        // ||A_hat||_2 by 40 rounds of power iteration on the LDS matrix (the
+        // launches of the multi-kernel set-up, same start vector and
+        // roundings: v, u held in T in the x+ / y slots), then the state's
+        // step sizes and A x0 with its anchor copy
+        T* v = lx;
+        T* u = ly;
+        __syncthreads();
+        double acc2 = 0.0;
+        for (int j = tid; j < n; j += nt) {
+            unsigned h = static_cast<unsigned>(j) * 2654435761u + 12345u;
+            h ^= h >> 13;
+            h *= 0x5bd1e995u;
+            h ^= h >> 15;
+            v[j] = static_cast<T>(0.5 + static_cast<double>(h & 0xffffu) / 65536.0);
+            acc2 += static_cast<double>(v[j]) * static_cast<double>(v[j]);
+        }
+        double nv = sqrt(block_sum(acc2, sh));
+        for (int j = tid; j < n; j += nt)
+            if (nv > 0.0) v[j] = static_cast<T>(static_cast<double>(v[j]) / nv);
+        double u2 = 0.0;
+        for (int it = 0; it < 40; ++it) {
+            __syncthreads();
+            double pu = 0.0;
+            for (int i = tid; i < m; i += nt) {
+                const double acc = seg_dot(lrp[i], lrp[i + 1], lci, lrv, v);
+                u[i] = static_cast<T>(acc);
+                pu += acc * acc;
+            }
+            u2 = block_sum(pu, sh);
+            double pv = 0.0;
+            for (int j = tid; j < n; j += nt) {
+                const double acc = seg_dot(lcp[j], lcp[j + 1], lri, lcv, u);
+                v[j] = static_cast<T>(acc);
+                pv += acc * acc;
+            }
+            nv = sqrt(block_sum(pv, sh));
+            for (int j = tid; j < n; j += nt)
+                if (nv > 0.0) v[j] = static_cast<T>(static_cast<double>(v[j]) / nv);
+        }
+        if (tid == 0) init_state_core(st, u2, a.scal[2], a.scal[3]);
+        __syncthreads();
+        for (int j = tid; j < n; j += nt) lx[j] = a.ep.xp[j];
+        __syncthreads();
+        for (int i = tid; i < m; i += nt) {
+            const double acc = seg_dot(lrp[i], lrp[i + 1], lci, lrv, lx);
+            a.ed.ax[i] = acc;
+            a.ax0[i] = acc;
+        }
+    } else {
+        for (int j = tid; j < n; j += nt) lx[j] = a.ep.xp[j];
+    }
     for (int i = tid; i < m; i += nt) ly[i] = a.ed.yg[i];
     small_check(a, sh, ly);  // step 0 (its barriers order the copies above)
     while (!st->done) {
@@ -811,6 +864,7 @@ struct Engine final : EngineBase {
     PassPlan plan_rows{}, plan_cols{};  // CSR (dual) pass and CSC (primal) pass
     bool one_wg = false;                // the whole solve in one workgroup (k_small)
     bool small_lds_set = false;         // k_small's dynamic LDS limit raised
+    bool power_pending = false;         // prepare() left the power iteration and the start to k_small_lds
     int* sblk[2][3] = {};               // stream blocks per pass (rows, cols) and LDS budget
     int nsblk[2][3] = {};
     // tiled layout per pass (0: CSR pass gathering x+, 1: CSC pass gathering y)
@@ -1385,6 +1439,8 @@ struct Engine final : EngineBase {
     void launch_one_wg() {
         if (const int per = small_resident_per_thread()) {
             SmallArgs<T> a = small_args();
+            a.power = power_pending ? 1 : 0;
+            a.scal = scal;
             const size_t bytes = small_resident_bytes<T>(m, n, nnz);
             const void* fn = per == 1 ? reinterpret_cast<const void*>(&k_small_lds<T, 1>)
                                       : reinterpret_cast<const void*>(&k_small_lds<T, 2>);
@@ -1394,8 +1450,15 @@ struct Engine final : EngineBase {
                     hipLaunchKernelGGL((k_small_lds<T, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
                 else
                     hipLaunchKernelGGL((k_small_lds<T, 2>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                power_pending = false;
                 return;
             }
+        }
+        if (power_pending) {  // the LDS-resident kernel could not be set up: the launches do it
+            const BinTable br = make_bins(row_len_sorted, 4);
+            launch_power(br);
+            launch_start(br);
+            power_pending = false;
         }
         SmallArgs<T> a = small_args();
         const unsigned lds = small_lds_bytes();
@@ -1434,7 +1497,7 @@ struct Engine final : EngineBase {
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             if (e == hipSuccess) e = hipEventElapsedTime(&t_multi, e0, e1);
         }
-        if (e == hipSuccess && s == BO_SUCCESS && (s = prepare()) == BO_SUCCESS) {
+        if (e == hipSuccess && s == BO_SUCCESS && (s = prepare(small_resident_per_thread() > 0)) == BO_SUCCESS) {
             e = hipEventRecord(e0, stream);
             launch_one_wg();
             if (e == hipSuccess) e = hipEventRecord(e1, stream);
@@ -1469,7 +1532,7 @@ struct Engine final : EngineBase {
     }
 
     // Scaling, step size, starting point, A x0: everything device-side.
-    bo_status prepare() {
+    bo_status prepare(bool kernel_power = false) {
         const unsigned gm = grid_for(m), gn = grid_for(n);
         hipLaunchKernelGGL(k_fill_f64, dim3(gm), dim3(kBlock), 0, stream, rs, 1.0, static_cast<size_t>(m));
         hipLaunchKernelGGL(k_fill_f64, dim3(gn), dim3(kBlock), 0, stream, cs, 1.0, static_cast<size_t>(n));
@@ -1504,22 +1567,9 @@ struct Engine final : EngineBase {
                            has_warm, c_hat, l_hat, u_hat, xk, x0, xp);
         hipLaunchKernelGGL(k_scale_rows<T>, dim3(gm), dim3(kBlock), 0, stream, m, rs, b64, eq, wy, has_warm,
                            b_hat, yk, y0, yg, yp);
-        // power iteration for ||A_hat||_2
-        hipLaunchKernelGGL(k_pw_init<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n);
-        hipLaunchKernelGGL(k_sq_partial<T>, dim3(nb_x), dim3(kBlock), 0, stream, pw_v, n, part_dx);
-        hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_dx, nb_x, 1, 0, scal + 1);
-        hipLaunchKernelGGL(k_pw_normalize<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n, scal + 1);
-        for (int it = 0; it < 40; ++it) {
-            EpiStore<T, T> eu{pw_v, pw_u};
-            launch_rows<kSumProd>(br, rv, eu, part_r, nullptr);
-            hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_r, br.total_blocks, 1, 0,
-                               scal + 0);
-            EpiStore<T, T> ew{pw_u, pw_v};
-            launch_cols<kSumProd>(bc, cv, ew, part_c, nullptr);
-            hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_c, bc.total_blocks, 1, 0,
-                               scal + 1);
-            hipLaunchKernelGGL(k_pw_normalize<T>, dim3(gn), dim3(kBlock), 0, stream, pw_v, n, scal + 1);
-        }
+        // power iteration for ||A_hat||_2 (the LDS-resident one-workgroup
+        // solve runs it itself: kernel_power)
+        if (!kernel_power) launch_power(br);
         hipLaunchKernelGGL(k_sq_partial<T>, dim3(nb_x), dim3(kBlock), 0, stream, c_hat, n, part_dx);
         hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_dx, nb_x, 1, 0, scal + 2);
         hipLaunchKernelGGL(k_sq_partial<T>, dim3(nb_y), dim3(kBlock), 0, stream, b_hat, m, part_dy);
@@ -1532,18 +1582,44 @@ struct Engine final : EngineBase {
         st_init->eps = prm.eps_rel_kkt;
         st_init->max_iter = prm.max_iterations;
         BO_TRY(hipMemcpyAsync(st, st_init, sizeof(PdlpState), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, stream, st, scal + 0, scal + 2, scal + 3);
-        // A x0 (the starting point is also the first anchor)
-        EpiStore<T, double> ea{xp, ax};
-        launch_rows<kSumProd>(br, rv, ea, part_r, nullptr);
-        hipLaunchKernelGGL(k_copy_f64, dim3(gm), dim3(kBlock), 0, stream, ax0, ax, static_cast<size_t>(m));
+        power_pending = kernel_power;
+        if (!kernel_power) launch_start(br);
         BO_TRY(hipGetLastError());
         return BO_SUCCESS;
     }
 
+    // ||A_hat||_2 by power iteration into scal[0]
+    void launch_power(const BinTable& br) {
+        const BinTable bc = make_bins(col_len_sorted, 4);
+        hipLaunchKernelGGL(k_pw_init<T>, dim3(grid_for(n)), dim3(kBlock), 0, stream, pw_v, n);
+        hipLaunchKernelGGL(k_sq_partial<T>, dim3(nb_x), dim3(kBlock), 0, stream, pw_v, n, part_dx);
+        hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_dx, nb_x, 1, 0, scal + 1);
+        hipLaunchKernelGGL(k_pw_normalize<T>, dim3(grid_for(n)), dim3(kBlock), 0, stream, pw_v, n, scal + 1);
+        for (int it = 0; it < 40; ++it) {
+            EpiStore<T, T> eu{pw_v, pw_u};
+            launch_rows<kSumProd>(br, rv, eu, part_r, nullptr);
+            hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_r, br.total_blocks, 1, 0,
+                               scal + 0);
+            EpiStore<T, T> ew{pw_u, pw_v};
+            launch_cols<kSumProd>(bc, cv, ew, part_c, nullptr);
+            hipLaunchKernelGGL(k_sum_partials_to, dim3(1), dim3(kBlock), 0, stream, part_c, bc.total_blocks, 1, 0,
+                               scal + 1);
+            hipLaunchKernelGGL(k_pw_normalize<T>, dim3(grid_for(n)), dim3(kBlock), 0, stream, pw_v, n, scal + 1);
+        }
+    }
+
+    // step sizes and primal weight from scal, then A x0 (the starting point
+    // is also the first anchor)
+    void launch_start(const BinTable& br) {
+        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, stream, st, scal + 0, scal + 2, scal + 3);
+        EpiStore<T, double> ea{xp, ax};
+        launch_rows<kSumProd>(br, rv, ea, part_r, nullptr);
+        hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(m)), dim3(kBlock), 0, stream, ax0, ax, static_cast<size_t>(m));
+    }
+
     bo_status solve(bo_pdlp_stats* out) override {
         auto t0 = std::chrono::high_resolution_clock::now();
-        bo_status s0 = prepare();
+        bo_status s0 = prepare(one_wg && small_resident_per_thread() > 0);
         if (s0 != BO_SUCCESS) return s0;
         if (!one_wg) launch_check();  // step 0: KKT of the starting point (k_small runs its own)
         BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));

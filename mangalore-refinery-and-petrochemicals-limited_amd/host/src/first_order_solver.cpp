@@ -255,23 +255,27 @@ FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
         // Precision: LPs up to 2^18 nonzeros run with FP64 iterates (they run
         // in one workgroup, where FP64 costs about the same, and badly scaled
         // models such as Netlib's kb2 or share2b do not converge with FP32
-        // iterates); larger ones run FP32 first and, if that stalls, go on
-        // in FP64 from where it stopped, for the iterations left.
+        // iterates); larger ones run FP32 iterates for at most 50,000
+        // iterations and, if that does not converge, go on in FP64 from
+        // where it stopped, for the iterations left.
         bo_pdlp_params p;
         bo_pdlp_default_params(&p);
         p.max_iterations = static_cast<long long>(options_.max_iterations);
         p.eps_rel_kkt = options_.relative_kkt_tolerance;
         p.fp64_iterates = lp.nnz <= (1LL << 18) ? 1 : 0;
+        const long long budget = p.max_iterations;
+        constexpr long long kFp32Iterations = 50000;
+        if (!p.fp64_iterates && budget > kFp32Iterations) p.max_iterations = kFp32Iterations;
         bo_pdlp* h = nullptr;
         bo_status s = bo_pdlp_create(&lp, &p, &h);
         bo_pdlp_stats st{};
         if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
         if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
         bo_pdlp_destroy(h);
-        if (s == BO_SUCCESS && st.status == 4 && st.iterations < p.max_iterations) {
+        if (s == BO_SUCCESS && !p.fp64_iterates && (st.status == 1 || st.status == 4) && st.iterations < budget) {
             const long long used = st.iterations;
             p.fp64_iterates = 1;
-            p.max_iterations -= used;
+            p.max_iterations = budget - used;
             h = nullptr;
             s = bo_pdlp_create(&lp, &p, &h);
             if (s == BO_SUCCESS) s = bo_pdlp_set_warm_start(h, x.data(), y.data());
