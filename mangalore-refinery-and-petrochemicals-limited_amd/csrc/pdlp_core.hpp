@@ -57,14 +57,24 @@ struct EpiPrimal {
     __device__ const T* vec(int) const { return yg; }
     __device__ bool skip(int j) const { return inst && st[inst[j]].done; }
     __device__ bool wg_dead(int b) const { return wg_live && !wg_live[b]; }
-    __device__ void operator()(int j, const double* acc, double*) const {
+    // Column data the step reads besides A'y, loaded by a reduction kernel
+    // before the partials so the two sets of loads overlap.
+    struct Pre {
+        double xo, x0;
+        T c, lb, ub;
+    };
+    __device__ Pre load(int j) const { return Pre{xk[j], x0[j], c[j], lb[j], ub[j]}; }
+    __device__ void finish(int j, double acc, const Pre& p) const {
         const PdlpState* s = inst ? st + inst[j] : st;
         if (inst && s->done) return;
-        const double xo = xk[j];
-        const T xt = clampT(static_cast<T>(xo - s->tau * (static_cast<double>(c[j]) + acc[0])), lb[j], ub[j]);
+        const T xt = clampT(static_cast<T>(p.xo - s->tau * (static_cast<double>(p.c) + acc)), p.lb, p.ub);
         xp[j] = xt;
         const double w = halpern_w(s, t);
-        xk[j] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[j];
+        xk[j] = w * (2.0 * static_cast<double>(xt) - p.xo) + (1.0 - w) * p.x0;
+    }
+    __device__ void operator()(int j, const double* acc, double*) const {
+        if (inst && st[inst[j]].done) return;
+        finish(j, acc[0], load(j));
     }
 };
 
@@ -90,19 +100,29 @@ struct EpiDual {
     __device__ const T* vec(int) const { return xp; }
     __device__ bool skip(int i) const { return inst && st[inst[i]].done; }
     __device__ bool wg_dead(int b) const { return wg_live && !wg_live[b]; }
-    __device__ void operator()(int i, const double* acc, double*) const {
+    // Row data the step reads besides A x+ (see EpiPrimal::Pre).
+    struct Pre {
+        double axo, yo, y0, ax0;
+        T b;
+        unsigned char eq;
+    };
+    __device__ Pre load(int i) const { return Pre{ax[i], yk[i], y0[i], ax0[i], b[i], eq[i]}; }
+    __device__ void finish(int i, double acc, const Pre& p) const {
         const PdlpState* s = inst ? st + inst[i] : st;
         if (inst && s->done) return;
-        const double axo = ax[i], yo = yk[i];
-        double yn = yo + s->sigma * (2.0 * acc[0] - axo - static_cast<double>(b[i]));
-        if (!eq[i] && yn < 0.0) yn = 0.0;
+        double yn = p.yo + s->sigma * (2.0 * acc - p.axo - static_cast<double>(p.b));
+        if (!p.eq && yn < 0.0) yn = 0.0;
         const T yt = static_cast<T>(yn);
         yp[i] = yt;
         const double w = halpern_w(s, t);
-        const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[i];
+        const double ynew = w * (2.0 * static_cast<double>(yt) - p.yo) + (1.0 - w) * p.y0;
         yk[i] = ynew;
         yg[i] = static_cast<T>(ynew);
-        ax[i] = w * (2.0 * acc[0] - axo) + (1.0 - w) * ax0[i];
+        ax[i] = w * (2.0 * acc - p.axo) + (1.0 - w) * p.ax0;
+    }
+    __device__ void operator()(int i, const double* acc, double*) const {
+        if (inst && st[inst[i]].done) return;
+        finish(i, acc[0], load(i));
     }
 };
 

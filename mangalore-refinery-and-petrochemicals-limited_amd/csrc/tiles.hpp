@@ -419,6 +419,27 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     }
 }
 
+// An epilogue with load(s) / finish(s, acc, pre) has its own data loaded
+// before the partials (the loads overlap); others run as epi(s, acc, part).
+template <class E>
+__device__ inline auto epi_preload(const E& e, int s, int) -> decltype(e.load(s)) {
+    return e.load(s);
+}
+template <class E>
+__device__ inline int epi_preload(const E&, int, long) {
+    return 0;
+}
+template <class E, class P>
+__device__ inline auto epi_finish(const E& e, int s, double acc, const P& p, int)
+    -> decltype(e.finish(s, acc, p)) {
+    e.finish(s, acc, p);
+}
+template <class E, class P>
+__device__ inline void epi_finish(const E& e, int s, double acc, const P&, long) {
+    double r[1] = {acc};
+    e(s, r, nullptr);
+}
+
 // Segment s: its partials in block order, then the pass epilogue.  The slot
 // and partial loads of up to U pairs are issued at once: raw buffer loads at
 // an out-of-range offset read 0, so no load sits under a per-lane branch.
@@ -434,6 +455,7 @@ k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restr
     constexpr unsigned kOut = 0x7FFFFFF0u;  // beyond every range below (ranges are < 2^31 bytes)
     const __amdgpu_buffer_rsrc_t rs = slice_rsrc(slot, nslot * 4);
     const __amdgpu_buffer_rsrc_t rp = slice_rsrc(part, npart * 8);
+    const auto pre = epi_preload(epi, s, 0);
     const int q0 = seg_ptr[s], q1 = seg_ptr[s + 1];
     double acc = 0.0;
     for (int q = q0; q < q1; q += U) {
@@ -449,8 +471,7 @@ k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restr
 #pragma unroll
         for (int u = 0; u < U; ++u) acc += v[u];
     }
-    double r[1] = {acc};
-    epi(s, r, nullptr);
+    epi_finish(epi, s, acc, pre, 0);
 }
 
 // Launch of the reduction with U covering the most pairs a segment has.
