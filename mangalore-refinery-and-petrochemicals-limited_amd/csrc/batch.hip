@@ -20,12 +20,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -38,6 +40,26 @@ namespace bo {
 bo_status ensure_device_public();
 
 namespace {
+
+// f(k) for k in [0, count) on up to 16 host threads (LPs handed out one at a
+// time, so large and small LPs balance).
+template <class F>
+void parallel_for_lps(int count, F&& f) {
+    const int hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    const int nt = std::min(std::min(hw, 16), count);
+    if (nt <= 1) {
+        for (int k = 0; k < count; ++k) f(k);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    pool.reserve(static_cast<size_t>(nt));
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+            for (int k = next++; k < count; k = next++) f(k);
+        });
+    for (auto& th : pool) th.join();
+}
 
 // Up to kBlock consecutive rows or columns of one LP.
 struct Chunk {
@@ -626,15 +648,20 @@ struct Batch final : BatchBase {
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_init), sizeof(PdlpState) * std::max(K, 1)));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&running_host), sizeof(int)));
         std::memset(st_host, 0, sizeof(PdlpState) * std::max(K, 1));
-        // block-diagonal CSR (rows in LP order, columns shifted), its CSC
-        std::vector<int> h_rp(m + 1, 0), h_ci(nnz), h_ir(m), h_ic(n);
-        std::vector<double> h_rv(nnz), hb(m), hc(n), hl(n), hu(n);
+        // block-diagonal CSR (rows in LP order, columns shifted) and its CSC.
+        // The blocks are independent -- LP k's entries fill [zoff[k],
+        // zoff[k+1]) of both orientations -- so host threads build them LP by
+        // LP; the arrays are the same as a serial pass over the whole matrix.
+        std::vector<long long> zoff(K + 1, 0);
+        for (int k = 0; k < K; ++k) zoff[k + 1] = zoff[k] + lps[k].nnz;
+        std::vector<int> h_rp(m + 1, 0), h_ci(nnz), h_ir(m), h_ic(n), h_cp(n + 1, 0), h_ri(nnz);
+        std::vector<double> h_rv(nnz), h_cv(nnz), hb(m), hc(n), hl(n), hu(n);
         std::vector<unsigned char> he(m);
         std::vector<double> bn(K), cn(K);
-        long long o = 0;
-        for (int k = 0; k < K; ++k) {
+        parallel_for_lps(K, [&](int k) {
             const bo_lp& lp = lps[k];
             double b2 = 0.0, c2 = 0.0;
+            long long o = zoff[k];
             for (int i = 0; i < lp.m; ++i) {
                 const int gi = roff[k] + i;
                 for (int q = lp.row_ptr[i]; q < lp.row_ptr[i + 1]; ++q, ++o) {
@@ -657,20 +684,23 @@ struct Batch final : BatchBase {
             }
             bn[k] = std::sqrt(b2);
             cn[k] = std::sqrt(c2);
-        }
-        std::vector<int> h_cp(n + 1, 0), h_ri(nnz);
-        std::vector<double> h_cv(nnz);
-        for (long long q = 0; q < nnz; ++q) h_cp[h_ci[q] + 1]++;
-        for (int j = 0; j < n; ++j) h_cp[j + 1] += h_cp[j];
-        {
-            std::vector<int> pos(h_cp.begin(), h_cp.end() - 1);
-            for (int i = 0; i < m; ++i)
-                for (int q = h_rp[i]; q < h_rp[i + 1]; ++q) {
-                    const int d = pos[h_ci[q]]++;
-                    h_ri[d] = i;
+            // CSC of this block: columns coff[k].., rows ascending
+            std::vector<int> pos(static_cast<size_t>(lp.n) + 1, 0);
+            for (long long q = zoff[k]; q < zoff[k + 1]; ++q) pos[h_ci[q] - coff[k] + 1]++;
+            for (int j = 0; j < lp.n; ++j) pos[j + 1] += pos[j];
+            for (int j = 0; j < lp.n; ++j) h_cp[coff[k] + j + 1] = static_cast<int>(zoff[k] + pos[j + 1]);
+            for (int i = 0; i < lp.m; ++i) {
+                const int gi = roff[k] + i;
+                // the first row's start is the previous LP's end, which another
+                // thread writes: take it from zoff instead
+                const long long q0 = i == 0 ? zoff[k] : h_rp[gi];
+                for (long long q = q0; q < h_rp[gi + 1]; ++q) {
+                    const long long d = zoff[k] + pos[h_ci[q] - coff[k]]++;
+                    h_ri[d] = gi;
                     h_cv[d] = h_rv[q];
                 }
-        }
+            }
+        });
         for (int k = 0; k < K; ++k) {
             PdlpState& s = st_init[k];
             std::memset(&s, 0, sizeof(s));
