@@ -8,6 +8,8 @@ refinery demo.
   infeasible or unbounded never come back OPTIMAL.
 * Reference engine: bit-equal to the reference's own FirstOrderLPSolver on
   the same parsed model, live where oracle/_ref is present.
+* Batched engine: the models as one batch, the badly scaled ones finishing
+  in its FP64 polish.
 """
 import json
 import os
@@ -60,3 +62,26 @@ def test_mps_reference_engine_bitequal(pkg, ref, fname):
     assert b["status"] == a["status"] and b["iterations"] == a["iterations"]
     assert float.hex(b["objective_value"]) == float.hex(a["objective"])
     assert np.array_equal(b["x"], a["x"])
+
+
+def test_mps_models_as_one_batch(pkg, po):
+    # The LP-solvable models as one block-diagonal batch (bo_pdlp_batch_*):
+    # the badly scaled Netlib models stall with FP32 iterates and finish in
+    # the FP64 polish; every LP must reach rel-KKT 1e-6 (by the oracle's
+    # evaluator) and its LP optimum.
+    names = [f for f in FREE if GOLD[f]["highs_status"] == "optimal"]
+    fms = [read(pkg, f) for f in names]
+    lps = [pkg.lpdata.flat_to_std(fm) for fm in fms]
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=1e-6, max_iterations=200000)
+    per, tot = b.solve()
+    print("batch", tot, flush=True)
+    assert tot["solved"] == len(lps) and tot["polished"] >= 1
+    for k, f in enumerate(names):
+        x, y = b.solution(k)
+        kk = po.kkt(lps[k], x, y)
+        sm = -1.0 if fms[k].sense == 1 else 1.0
+        obj = sm * kk["primal_obj"] + fms[k].obj_offset
+        fs = GOLD[f]["highs_objective"]
+        assert per[k]["status"] == 0 and kk["rel_kkt"] <= 1e-6, (f, per[k], kk["rel_kkt"])
+        assert abs(obj - fs) <= 1e-5 * (1 + abs(fs)), (f, obj, fs)
+    b.close()
