@@ -440,8 +440,31 @@ __device__ inline double seg_dot(int b, int e, const int* __restrict__ idx, cons
     return a;
 }
 
+// N block-wide sums at once, each the same as block_sum (xor tree in the
+// wavefront, then the wavefronts in order), with one pair of barriers for all.
+template <int N>
+__device__ inline void block_sums(double (&v)[N], double* shn) {
+    const int lane = static_cast<int>(threadIdx.x) % kWave, wid = static_cast<int>(threadIdx.x) / kWave;
+    const int nw = static_cast<int>(blockDim.x) / kWave;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) shn[wid * N + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += shn[w * N + k];
+        v[k] = t;
+    }
+}
+
 template <class T>
 __device__ inline void small_check(const SmallArgs<T>& a, double* sh, T* ly) {
+    __shared__ double sh14[(kSmallThreads / kWave) * 14];
     const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
     PdlpState* st = a.st;
     double p[14];
@@ -468,10 +491,8 @@ __device__ inline void small_check(const SmallArgs<T>& a, double* sh, T* ly) {
         const double acc[2] = {seg_dot(b, e, a.ri, a.cv64, a.er.yo_cur), seg_dot(b, e, a.ri, a.cv64, a.er.yo_cand)};
         a.ec(j, acc, p + 8);
     }
-    double q[14];
-#pragma unroll
-    for (int k = 0; k < 14; ++k) q[k] = block_sum(p[k], sh);
-    if (tid == 0 && !st->done) decide_core(st, q);
+    block_sums(p, sh14);
+    if (tid == 0 && !st->done) decide_core(st, p);
     __syncthreads();
     if (st->done || !st->restart) return;
     const bool cand = st->restart_to_cand != 0;
@@ -559,8 +580,33 @@ size_t small_resident_bytes(long long m, long long n, long long nnz) {
     return sizeof(T) * static_cast<size_t>(2 * nnz + n + m) + sizeof(int) * static_cast<size_t>(m + n + 2 + 2 * nnz);
 }
 
-template <class T, int R>
+// sum over k = b, b + S, b + 2S, ... < e of val[k] * v[idx[k]], four in flight
+template <int S, class TM, class TV>
+__device__ inline double seg_dot_strided(int b, int e, const int* __restrict__ idx, const TM* __restrict__ val,
+                                         const TV* __restrict__ v) {
+    double a = 0.0;
+    int q = b;
+    for (; q + 3 * S < e; q += 4 * S) {
+        const double w0 = static_cast<double>(val[q]), w1 = static_cast<double>(val[q + S]);
+        const double w2 = static_cast<double>(val[q + 2 * S]), w3 = static_cast<double>(val[q + 3 * S]);
+        const double x0 = static_cast<double>(v[idx[q]]), x1 = static_cast<double>(v[idx[q + S]]);
+        const double x2 = static_cast<double>(v[idx[q + 2 * S]]), x3 = static_cast<double>(v[idx[q + 3 * S]]);
+        a += w0 * x0;
+        a += w1 * x1;
+        a += w2 * x2;
+        a += w3 * x3;
+    }
+    for (; q < e; q += S) a += static_cast<double>(val[q]) * static_cast<double>(v[idx[q]]);
+    return a;
+}
+
+// R: columns (and, with LR = 1, rows) per thread.  LR > 1 (with R = 1): LR
+// consecutive lanes share a row, each summing every LR-th entry, the
+// partial sums added by a fixed xor tree -- a row's dependent chain of LDS
+// loads is LR times shorter, and LR times more wavefronts work on the rows.
+template <class T, int R, int LR>
 __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
+    static_assert(LR == 1 || R == 1, "lanes per row need one row per thread group");
     __shared__ double sh[kSmallThreads / kWave];
     extern __shared__ __attribute__((aligned(16))) unsigned char small_lds[];
     const int m = a.m, n = a.n, nnz = a.nnz;
@@ -647,7 +693,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
         double xk[R], x0[R], yk[R], y0[R], ax[R], ax0[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = tid + r * nt, i = tid + r * nt;
+            const int j = tid + r * nt, i = LR > 1 ? tid / LR : tid + r * nt;
             cc[r] = j < n ? a.ep.c[j] : T(0);
             lo[r] = j < n ? a.ep.lb[j] : T(0);
             hi[r] = j < n ? a.ep.ub[j] : T(0);
@@ -682,9 +728,16 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < R; ++r) {  // row i: y+ = proj(y + sigma (A(2x+ - x) - b)), Halpern update of y, A x
-                const int i = tid + r * nt;
-                if (i < m) {
-                    const double acc = seg_dot(lrp[i], lrp[i + 1], lci, lrv, lx);
+                const int i = LR > 1 ? tid / LR : tid + r * nt;
+                if (i < m) {  // (uniform over the LR lanes of a row)
+                    double acc;
+                    if constexpr (LR == 1) {
+                        acc = seg_dot(lrp[i], lrp[i + 1], lci, lrv, lx);
+                    } else {
+                        acc = seg_dot_strided<LR>(lrp[i] + tid % LR, lrp[i + 1], lci, lrv, lx);
+                        acc += __shfl_xor(acc, 1, kWave);
+                        if constexpr (LR == 4) acc += __shfl_xor(acc, 2, kWave);
+                    }
                     const double axo = ax[r], yo = yk[r];
                     double yn = yo + sigma * (2.0 * acc - axo - static_cast<double>(bb[r]));
                     if (!eq[r] && yn < 0.0) yn = 0.0;
@@ -692,7 +745,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
                     ypr[r] = yt;
                     const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[r];
                     yk[r] = ynew;
-                    ly[i] = static_cast<T>(ynew);
+                    if (LR == 1 || tid % LR == 0) ly[i] = static_cast<T>(ynew);
                     ax[r] = w * (2.0 * acc - axo) + (1.0 - w) * ax0[r];
                 }
             }
@@ -700,12 +753,12 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {  // back to memory for the check
-            const int j = tid + r * nt, i = tid + r * nt;
+            const int j = tid + r * nt, i = LR > 1 ? tid / LR : tid + r * nt;
             if (j < n) {
                 a.ep.xk[j] = xk[r];
                 a.ep.xp[j] = xpr[r];
             }
-            if (i < m) {
+            if (i < m && (LR == 1 || tid % LR == 0)) {
                 a.ed.yk[i] = yk[r];
                 a.ed.yp[i] = ypr[r];
                 a.ed.yg[i] = static_cast<T>(yk[r]);
@@ -1057,6 +1110,7 @@ struct Engine final : EngineBase {
         // target, lanes < 0 selects the stream kernel with LDS budget -lanes,
         // 0 lets the tuner choose
         const bool tune = prm.lanes_rows == 0 || prm.lanes_cols == 0 || prm.hot_columns < 0 || prm.check_every <= 0;
+        const bool tune_cadence = prm.check_every <= 0;
         plan_rows = fixed_plan(0, prm.lanes_rows);
         plan_cols = fixed_plan(1, prm.lanes_cols);
         if (prm.hot_columns > 0 && plan_rows.cap == 0 && plan_rows.tiled == 0) {
@@ -1085,6 +1139,11 @@ struct Engine final : EngineBase {
             bo_status s3 = tune_one_wg();
             if (s3 != BO_SUCCESS) return s3;
         }
+        // A one-workgroup solve checks every 64 steps unless told otherwise:
+        // there a check costs little, and with FP32 iterates longer
+        // intervals have failed to converge on small LPs (C0-sized random
+        // LPs at 128 and 256) where 64 converges and FP64 converges at any.
+        if (one_wg && tune_cadence) prm.check_every = 64;
         return BO_SUCCESS;
     }
 
@@ -1432,7 +1491,7 @@ struct Engine final : EngineBase {
     int small_resident_per_thread() const {
         const long long per = ((m > n ? m : n) + kSmallThreads - 1) / kSmallThreads;
         if (per > kSmallLdsMaxPerThread || nnz >= (1LL << 30)) return 0;
-        if (small_resident_bytes<T>(m, n, nnz) + 1024 > static_cast<size_t>(lds_max)) return 0;
+        if (small_resident_bytes<T>(m, n, nnz) + 4096 > static_cast<size_t>(lds_max)) return 0;  // + static LDS
         return per < 1 ? 1 : static_cast<int>(per);
     }
 
@@ -1442,14 +1501,22 @@ struct Engine final : EngineBase {
             a.power = power_pending ? 1 : 0;
             a.scal = scal;
             const size_t bytes = small_resident_bytes<T>(m, n, nnz);
-            const void* fn = per == 1 ? reinterpret_cast<const void*>(&k_small_lds<T, 1>)
-                                      : reinterpret_cast<const void*>(&k_small_lds<T, 2>);
+            // lanes per row: as many as the workgroup has threads for
+            const int lr = per > 1 ? 1 : (4LL * m <= kSmallThreads ? 4 : (2LL * m <= kSmallThreads ? 2 : 1));
+            const void* fn = per > 1     ? reinterpret_cast<const void*>(&k_small_lds<T, 2, 1>)
+                             : lr == 4 ? reinterpret_cast<const void*>(&k_small_lds<T, 1, 4>)
+                             : lr == 2 ? reinterpret_cast<const void*>(&k_small_lds<T, 1, 2>)
+                                       : reinterpret_cast<const void*>(&k_small_lds<T, 1, 1>);
             if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) ==
                 hipSuccess) {
-                if (per == 1)
-                    hipLaunchKernelGGL((k_small_lds<T, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                if (per > 1)
+                    hipLaunchKernelGGL((k_small_lds<T, 2, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                else if (lr == 4)
+                    hipLaunchKernelGGL((k_small_lds<T, 1, 4>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                else if (lr == 2)
+                    hipLaunchKernelGGL((k_small_lds<T, 1, 2>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
                 else
-                    hipLaunchKernelGGL((k_small_lds<T, 2>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                    hipLaunchKernelGGL((k_small_lds<T, 1, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
                 power_pending = false;
                 return;
             }
