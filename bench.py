@@ -347,8 +347,10 @@ def bench_c4(pkg, args):
     ok_train = len(ok)
     log(f"training data: {len(ok)}/{len(train_ids)} draws solved, {gen_s:.1f}s")
     F = np.array([fam.features(train_ids[i]) for i in ok])
-    net = ws.WarmStartNet(pkg, fam.n, fam.m, width=1024, layers=4, max_batch=args.c4_batch, seed=1)
-    hist = net.fit(F, np.array([tr["x"][i] for i in ok]), np.array([tr["y"][i] for i in ok]),
+    net = ws.WarmStartNet(pkg, fam.n, fam.m, width=1024, layers=4, max_batch=args.c4_batch, seed=1,
+                          primal_weight=True)
+    w_train = np.array([tr["omega"][i] for i in ok])
+    hist = net.fit(F, np.array([tr["x"][i] for i in ok]), np.array([tr["y"][i] for i in ok]), w_train,
                    epochs=args.epochs, lr=args.lr, log=log)
     del F, tr
     log(f"training: val loss {hist['val'][0]:.4f} -> {hist['best_val']:.4f} (epoch {hist['best_epoch']}), "
@@ -362,14 +364,14 @@ def bench_c4(pkg, args):
     log(f"cold (block-diagonal batched engine, {args.block_sample} draws): {cold_bd['solve_s']:.2f}s")
     Fe = np.array([fam.features(k) for k in eval_ids])
     t0 = time.perf_counter()
-    Xn, Yn = net.predict(Fe)
+    Xn, Yn, Wn = net.predict(Fe)
     infer_s = time.perf_counter() - t0
-    warm = fs.solve_warm_with_fallback(eval_ids, (Xn, Yn))
+    warm = fs.solve_warm_with_fallback(eval_ids, (Xn, Yn, Wn))
     warm_s = warm["solve_s"] + infer_s
     log(f"network warm: {warm_s:.2f}s (inference {infer_s:.2f}s) solved {(warm['status'] == 0).sum()} "
         f"fallbacks {len(warm['fallback'])}")
-    Xm, Ym = net.mean_start(len(eval_ids))
-    mean = fs.solve_warm_with_fallback(eval_ids, (Xm, Ym))
+    Xm, Ym, Wm = net.mean_start(len(eval_ids))
+    mean = fs.solve_warm_with_fallback(eval_ids, (Xm, Ym, Wm))
     log(f"mean warm: {mean['solve_s']:.2f}s solved {(mean['status'] == 0).sum()}")
     fs.close()
 
@@ -444,6 +446,9 @@ def bench_c4(pkg, args):
         "iterations_p50_p90_max": {"cold": pct(cold["iterations"]), "network": pct(warm["iterations"]),
                                    "mean": pct(mean["iterations"])},
         "start_distance_median_rel": dist,
+        "primal_weight": {"train_final_median": float(np.median(w_train)),
+                          "predicted_median": float(np.median(Wn)),
+                          "cold_final_median": float(np.median(cold["omega"]))},
         "single_sample": args.single_sample,
         "training": {"draws_solved": ok_train, "generation_s": gen_s, "epochs": args.epochs, "lr": args.lr,
                      "val_loss_untrained": hist["val"][0], "val_loss_best": hist["best_val"],

@@ -299,6 +299,10 @@ bo_status bo_pdlp_family_create(const bo_lp* lp, int capacity, const bo_pdlp_par
 bo_status bo_pdlp_family_set_lps(struct bo_pdlp_family* h, int count, const double* c, const double* b);
 /* Starting points (count x n, count x m, original space); NULL, NULL clears them. */
 bo_status bo_pdlp_family_set_warm_start(struct bo_pdlp_family* h, const double* x, const double* y);
+/* Starting primal weights (count values, finite, >= 0; 0 for an LP means its
+ * own ||c_hat|| / ||b_hat||), e.g. the weights solves of similar LPs ended
+ * with (bo_pdlp_stats.primal_weight); NULL clears them.  Kept until changed. */
+bo_status bo_pdlp_family_set_primal_weights(struct bo_pdlp_family* h, const double* w);
 bo_status bo_pdlp_family_solve(struct bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total);
 /* Solutions of the last solve, original space (count x n, count x m; either may be NULL). */
 bo_status bo_pdlp_family_get_solutions(struct bo_pdlp_family* h, double* x, double* y);

@@ -145,7 +145,8 @@ EXPORTS = [
     "bo_pdlp_batch_set_primal_weights",
     "bo_pdlp_batch_set_objective_rhs", "bo_pdlp_batch_solve", "bo_pdlp_batch_get_solution",
     "bo_pdlp_batch_time_kernels", "bo_pdlp_batch_destroy",
-    "bo_pdlp_family_create", "bo_pdlp_family_set_lps", "bo_pdlp_family_set_warm_start", "bo_pdlp_family_solve",
+    "bo_pdlp_family_create", "bo_pdlp_family_set_lps", "bo_pdlp_family_set_warm_start",
+    "bo_pdlp_family_set_primal_weights", "bo_pdlp_family_solve",
     "bo_pdlp_family_get_solutions", "bo_pdlp_family_time_passes", "bo_pdlp_family_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
     "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_time", "bo_mlp_destroy", "bo_gemm_bf16",
@@ -475,6 +476,15 @@ class PdlpFamily:
         x = np.ascontiguousarray(np.asarray(xs, np.float64).reshape(self.count, self.base.n))
         y = np.ascontiguousarray(np.asarray(ys, np.float64).reshape(self.count, self.base.m))
         _check(lib().bo_pdlp_family_set_warm_start(self.handle, _p(x), _p(y)), "bo_pdlp_family_set_warm_start")
+
+    def set_primal_weights(self, w=None):
+        """Starting primal weights of the next solves (one per LP); None: each
+        LP's own ||c_hat|| / ||b_hat||."""
+        if w is None:
+            _check(lib().bo_pdlp_family_set_primal_weights(self.handle, None), "bo_pdlp_family_set_primal_weights")
+            return
+        a = np.ascontiguousarray(np.asarray(w, np.float64).reshape(self.count))
+        _check(lib().bo_pdlp_family_set_primal_weights(self.handle, _p(a)), "bo_pdlp_family_set_primal_weights")
 
     def solve(self):
         per = (PdlpStats * self.count)()

@@ -1182,6 +1182,11 @@ bo_status bo_pdlp_family_set_warm_start(bo_pdlp_family* h, const double* x, cons
     return h->eng->set_warm_start(x, y);
 }
 
+bo_status bo_pdlp_family_set_primal_weights(bo_pdlp_family* h, const double* w) {
+    if (!h) return BO_INVALID_ARGUMENT;
+    return h->eng->set_primal_weights(w);
+}
+
 bo_status bo_pdlp_family_solve(bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
     if (!h) return BO_INVALID_ARGUMENT;
     auto t0 = std::chrono::high_resolution_clock::now();

@@ -17,10 +17,13 @@ Pipeline (all heavy work on the GPU, through libbo_hip.so):
   engine's FP64 rel-KKT of the unscaled LP decides every result.
 * ``WarmStartNet`` is the BF16 multilayer perceptron of bo_mlp_* (forward,
   backward and Adam kernels on the matrix cores).  It maps the normalised
-  perturbations of a draw (n + m features) to its primal / dual solution,
+  perturbations of a draw (n + m features) to its primal / dual solution
+  and, optionally, the log of the primal weight its solve ended with,
   standardised per coordinate with the training set's mean and spread; the
   output layer starts at zero, so the untrained network predicts the
   training mean.  Training keeps the parameters of the best validation loss.
+  A predicted primal weight starts the solve in place of ||c|| / ||b||: on
+  the staircase family that shortens the slowest solves of a batch.
 """
 from __future__ import annotations
 
@@ -55,8 +58,10 @@ class FamilySolver:
             eng.set_objective_rhs([d.c for d in draws], [d.b for d in draws])
         if warm_rows is None:
             eng.set_warm_start(None, None)
+            eng.set_primal_weights(None)
         else:
-            eng.set_warm_start(*warm_rows)
+            eng.set_warm_start(warm_rows[0], warm_rows[1])
+            eng.set_primal_weights(warm_rows[2] if len(warm_rows) > 2 else None)
         t0 = time.perf_counter()
         p, _ = eng.solve()
         dt = time.perf_counter() - t0
@@ -75,16 +80,17 @@ class FamilySolver:
     def solve(self, ids, warm=None, want_solutions=True):
         """Solve draws `ids` (cold, or from warm = (X, Y) rows aligned with ids).
 
-        Returns dict(per=[stats], x=[...], y=[...], solve_s=wall time of the
-        engine's solves, iterations, status).  Draw generation and the copy of
-        the costs / right-hand sides to the device are outside solve_s."""
+        warm may carry a third member, starting primal weights.  Returns
+        dict(per=[stats], x=[...], y=[...], solve_s=wall time of the engine's
+        solves, iterations, status, omega = the primal weights the solves
+        ended with).  Draw generation and the copy of the costs / right-hand
+        sides to the device are outside solve_s."""
         ids = list(ids)
         per, xs, ys = [], [], []
         solve_s = 0.0
         for s in range(0, len(ids), self.batch):
             chunk = ids[s:s + self.batch]
-            rows = None if warm is None else ([warm[0][s + i] for i in range(len(chunk))],
-                                              [warm[1][s + i] for i in range(len(chunk))])
+            rows = None if warm is None else tuple([part[s + i] for i in range(len(chunk))] for part in warm)
             p, dt, x, y = self._run_chunk(chunk, rows, want_solutions)
             solve_s += dt
             per.extend(p)
@@ -92,7 +98,8 @@ class FamilySolver:
             ys.extend(y)
         return dict(per=per, x=xs, y=ys, solve_s=solve_s,
                     iterations=np.array([q["iterations"] for q in per]),
-                    status=np.array([q["status"] for q in per]))
+                    status=np.array([q["status"] for q in per]),
+                    omega=np.array([q["primal_weight"] for q in per]))
 
     def solve_warm_with_fallback(self, ids, warm, fallback=None):
         """Warm-started solve; draws it leaves unsolved are solved again cold
@@ -121,19 +128,24 @@ class FamilySolver:
 class WarmStartNet:
     """features (n + m) -> standardised (x*, y*) with the bo_mlp_* network."""
 
-    def __init__(self, pkg, n, m, width=1024, layers=4, max_batch=64, seed=1):
+    def __init__(self, pkg, n, m, width=1024, layers=4, max_batch=64, seed=1, primal_weight=False):
         self.pkg, self.n, self.m = pkg, n, m
-        self.net = pkg.Mlp(n + m, width, n + m, layers=layers, max_batch=max_batch, seed=seed)
+        self.with_w = bool(primal_weight)
+        dout = n + m + (1 if self.with_w else 0)
+        self.net = pkg.Mlp(n + m, width, dout, layers=layers, max_batch=max_batch, seed=seed)
         self.max_batch = max_batch
-        self.mu = np.zeros(n + m)
-        self.sd = np.ones(n + m)
+        self.mu = np.zeros(dout)
+        self.sd = np.ones(dout)
         p = self.net.params()
         din, dout = self.net.shapes()[-1]
         p[p.size - din * dout - dout:] = 0.0  # output layer: weights and bias
         self.net.set_params(p)
 
-    def _standardise(self, X, Y):
-        Z = np.hstack([np.asarray(X, np.float64), np.asarray(Y, np.float64)])
+    def _standardise(self, X, Y, W=None):
+        parts = [np.asarray(X, np.float64), np.asarray(Y, np.float64)]
+        if self.with_w:
+            parts.append(np.log(np.asarray(W, np.float64)).reshape(-1, 1))
+        Z = np.hstack(parts)
         self.mu = Z.mean(0)
         sd = Z.std(0)
         pos = sd[sd > 0]
@@ -141,12 +153,14 @@ class WarmStartNet:
         self.sd = np.where(sd > 0, np.maximum(sd, floor), 1.0)
         return ((Z - self.mu) / self.sd).astype(np.float32)
 
-    def fit(self, F, X, Y, epochs=20, lr=1e-4, batch=None, val_frac=0.1, seed=0, log=None):
-        """Adam on the mean squared error of the standardised solutions.
+    def fit(self, F, X, Y, W=None, epochs=20, lr=1e-4, batch=None, val_frac=0.1, seed=0, log=None):
+        """Adam on the mean squared error of the standardised solutions (and
+        log primal weights W, for a network built with primal_weight=True).
         Returns the history (train / validation loss per epoch, seconds)."""
+        assert (W is not None) == self.with_w
         batch = batch or self.max_batch
         F = np.ascontiguousarray(F, np.float32)
-        T = self._standardise(X, Y)
+        T = self._standardise(X, Y, W)
         rng = np.random.default_rng(seed)
         order = rng.permutation(len(F))
         nval = max(1, int(round(val_frac * len(F)))) if len(F) > 1 else 0
@@ -184,16 +198,21 @@ class WarmStartNet:
             cnt += k
         return tot / max(cnt, 1)
 
+    def _split(self, Z):
+        nm = self.n + self.m
+        if self.with_w:
+            return Z[:, :self.n], Z[:, self.n:nm], np.exp(Z[:, nm])
+        return Z[:, :self.n], Z[:, self.n:nm]
+
     def predict(self, F):
-        """Warm starts (X rows, Y rows) for feature rows F."""
+        """Warm starts (X rows, Y rows[, primal weights]) for feature rows F."""
         out = []
         for s in range(0, len(F), self.max_batch):
             out.append(self.net.forward(np.ascontiguousarray(F[s:s + self.max_batch], np.float32)))
-        Z = self.mu + self.sd * np.vstack(out).astype(np.float64)
-        return Z[:, :self.n], Z[:, self.n:]
+        return self._split(self.mu + self.sd * np.vstack(out).astype(np.float64))
 
     def mean_start(self, count):
-        return np.tile(self.mu[:self.n], (count, 1)), np.tile(self.mu[self.n:], (count, 1))
+        return self._split(np.tile(self.mu, (count, 1)))
 
     def close(self):
         self.net.close()

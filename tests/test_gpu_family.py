@@ -112,3 +112,34 @@ def test_family_rejects_bad_input(pkg, family):
     with pytest.raises(pkg.BoError):
         f.set_lps([c], [d.b])
     f.close()
+
+
+def test_family_primal_weights(pkg, po, family):
+    # starting primal weights: each LP solved again from the weight its cold
+    # solve ended with (the learned warm start's second output); still every
+    # LP at rel-KKT 1e-6, and bad weights are refused before anything runs
+    ids = list(range(101, 133))
+    draws = [family.draw(k) for k in ids]
+    f = pkg.PdlpFamily(family.base, capacity=64, eps_rel_kkt=TOL, max_iterations=60000)
+    f.set_lps([d.c for d in draws], [d.b for d in draws])
+    per, _ = f.solve()
+    w = np.array([p["primal_weight"] for p in per])
+    assert np.all(w > 0)
+    f.set_primal_weights(w)
+    per2, _ = f.solve()
+    X, Y = f.solutions()
+    for k, d in enumerate(draws):
+        _check(po, d, X[k], Y[k], per2[k], f"lp {k}")
+    print("cold iters", int(np.max([p["iterations"] for p in per])), "from final weights",
+          int(np.max([p["iterations"] for p in per2])), flush=True)
+    bad = w.copy()
+    bad[3] = -1.0
+    with pytest.raises(pkg.BoError):
+        f.set_primal_weights(bad)
+    bad[3] = np.nan
+    with pytest.raises(pkg.BoError):
+        f.set_primal_weights(bad)
+    f.set_primal_weights(None)
+    per3, _ = f.solve()
+    assert [p["iterations"] for p in per3] == [p["iterations"] for p in per]  # cleared: the cold run again
+    f.close()
