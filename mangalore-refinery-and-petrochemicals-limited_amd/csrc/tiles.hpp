@@ -324,12 +324,11 @@ __device__ inline unsigned run_word(const unsigned* __restrict__ slice_word, int
 }
 
 template <class T, int CH>
-__device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, int nrows, const T* lds, RunState& st,
+__device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* lds, RunState& st,
                                     const WaveDesc& wd, const unsigned* __restrict__ slice_word,
                                     double* __restrict__ part, int lane) {
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-        if (r + u >= nrows) return;
         st.acc += static_cast<double>(c.a[u]) * static_cast<double>(lds[c.g[u]]);
         if (r + u + 1 == st.bound) {  // last row of the current slice (uniform)
             double* row = part + static_cast<size_t>(wd.slice0 + st.sl) * kWave;
@@ -341,10 +340,18 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, int nrows, 
             }
             st.acc = 0.0;
             if (++st.sl < wd.nslice) {
-                if ((st.sl & (kWave - 1)) == 0)
+                // The refill of the word window waits for its own load (and
+                // so for every matrix load in flight) on its own rare path:
+                // read at a join of both paths, the common path would wait
+                // too, at every slice end.
+                unsigned w;
+                if ((st.sl & (kWave - 1)) == 0) {
                     st.words = run_word(slice_word, wd.slice0, wd.nslice, st.sl, lane);
-                const unsigned w =
-                    static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(st.words), st.sl & (kWave - 1)));
+                    w = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(st.words)));
+                } else {
+                    w = static_cast<unsigned>(
+                        __builtin_amdgcn_readlane(static_cast<int>(st.words), st.sl & (kWave - 1)));
+                }
                 st.bound += static_cast<int>(w & ~kWideFlag);
                 st.wide = (w & kWideFlag) != 0;
             }
@@ -405,17 +412,20 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     }
     // Double-buffered walk: the next CH rows are loaded before the current
     // ones are gathered and their finished slices stored, so no store sits
-    // between a load and its use.
+    // between a load and its use.  A counted loop with one exit: with an
+    // exit between the halves, the compiler's wait analysis merged that
+    // path's unconsumed loads into the loop head and waited there for every
+    // load in flight.  Rows past the run load value 0 and index 0 without
+    // touching memory; they add 0 to a sum that is no longer stored (no
+    // slice ends past the run's last row), and every chunk's registers are
+    // consumed on every path, so no load is left pending at the loop head.
     int r = 0;
-    for (;;) {
+    for (int pairs = (nrows + 2 * CH - 1) / (2 * CH); pairs > 0; --pairs) {
         chunk_load(Bc, rv, ri, r + CH, lane);
-        chunk_finish(A, r, nrows, lds, st, wd, slice_word, part, lane);
-        r += CH;
-        if (r >= nrows) break;
-        chunk_load(A, rv, ri, r + CH, lane);
-        chunk_finish(Bc, r, nrows, lds, st, wd, slice_word, part, lane);
-        r += CH;
-        if (r >= nrows) break;
+        chunk_finish(A, r, lds, st, wd, slice_word, part, lane);
+        chunk_load(A, rv, ri, r + 2 * CH, lane);
+        chunk_finish(Bc, r + CH, lds, st, wd, slice_word, part, lane);
+        r += 2 * CH;
     }
 }
 
