@@ -323,13 +323,23 @@ __device__ inline unsigned run_word(const unsigned* __restrict__ slice_word, int
     return q < nslice ? slice_word[first + q] : 0u;
 }
 
-template <class T, int CH>
+template <class T, int CH, bool EARLY>
 __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* lds, RunState& st,
                                     const WaveDesc& wd, const unsigned* __restrict__ slice_word,
                                     double* __restrict__ part, int lane) {
+    // EARLY: the chunk's LDS gathers are all issued before the first is
+    // used, one LDS latency per chunk instead of one per row.  Measured at
+    // C1 it pays in the row pass (27.4 against 28.5 us) and costs in the
+    // column pass (36.0 against 34.7 us, whose slices end every few rows).
+    T g[CH];
+    if constexpr (EARLY) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) g[u] = lds[c.g[u]];
+    }
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-        st.acc += static_cast<double>(c.a[u]) * static_cast<double>(lds[c.g[u]]);
+        const T gv = EARLY ? g[u] : lds[c.g[u]];
+        st.acc += static_cast<double>(c.a[u]) * static_cast<double>(gv);
         if (r + u + 1 == st.bound) {  // last row of the current slice (uniform)
             double* row = part + static_cast<size_t>(wd.slice0 + st.sl) * kWave;
             if (st.wide) {
@@ -360,9 +370,10 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
 }
 
 // One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of the
-// wavefront's run kept in flight per buffer.  PASS only names the
-// instantiation (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of the
-// batched engine), so that profiles tell the passes apart.  active
+// wavefront's run kept in flight per buffer.  PASS names the instantiation
+// (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of the batched
+// engine), so that profiles tell the passes apart; the CSR passes issue a
+// chunk's LDS gathers early (chunk_finish).  active
 // (optional): per-tile flag; a tile whose flag is 0 does nothing.
 template <class T, int CH, int PASS>
 __global__ void __launch_bounds__(kTileThreads)
@@ -422,9 +433,9 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     int r = 0;
     for (int pairs = (nrows + 2 * CH - 1) / (2 * CH); pairs > 0; --pairs) {
         chunk_load(Bc, rv, ri, r + CH, lane);
-        chunk_finish(A, r, lds, st, wd, slice_word, part, lane);
+        chunk_finish<T, CH, PASS % 2 == 0>(A, r, lds, st, wd, slice_word, part, lane);
         chunk_load(A, rv, ri, r + 2 * CH, lane);
-        chunk_finish(Bc, r + CH, lds, st, wd, slice_word, part, lane);
+        chunk_finish<T, CH, PASS % 2 == 0>(Bc, r + CH, lds, st, wd, slice_word, part, lane);
         r += 2 * CH;
     }
 }
