@@ -74,24 +74,41 @@ __device__ inline void seg_accumulate(int k0, int k1, int lane, const int* __res
     for (int k = k0 + lane; k < k1; k += U * L) {
         TV a[U];
         int j[U];
+        // loads from a clamped position (k < k1 here, so k1 - 1 is in the
+        // segment), selected afterwards: no load sits under a branch, where
+        // each would wait for itself before the next is issued
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k + u * L;
-            a[u] = kk < k1 ? val[kk] : TV(0);
-            j[u] = kk < k1 ? idx[kk] : -1;
+            const int kc = kk < k1 ? kk : k1 - 1;
+            const TV av = val[kc];
+            const int jv = idx[kc];
+            a[u] = kk < k1 ? av : TV(0);
+            j[u] = kk < k1 ? jv : -1;
         }
+        if constexpr (HOT) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (j[u] >= 0) {
-                if constexpr (HOT) {
+            for (int u = 0; u < U; ++u) {
+                if (j[u] >= 0) {
                     const TX xv = j[u] < hot.count ? hot.lds[j[u]] : epi.vec(0)[j[u]];
                     acc[0] = seg_combine<MODE>(acc[0], seg_term<MODE>(a[u], xv));
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], epi.vec(v)[j[u]]));
                 }
             }
+        } else {
+            // The gathers are unconditional (entry 0 for lanes past the
+            // segment, whose terms are then not added), so all U of a trip
+            // are in flight together: under a per-entry branch each waited
+            // for its own load before the next was issued.
+            TX xv[NV][U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xv[v][u] = epi.vec(v)[j[u] >= 0 ? j[u] : 0];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    acc[v] = j[u] >= 0 ? seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], xv[v][u])) : acc[v];
         }
     }
 }

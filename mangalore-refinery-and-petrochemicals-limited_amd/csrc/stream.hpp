@@ -47,14 +47,20 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
             TV a[4];
             int j[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 4; ++u) {  // clamped loads, selected afterwards (k < k1 here)
                 const int kk = k + u * kBlock;
-                a[u] = kk < k1 ? val[kk] : TV(0);
-                j[u] = kk < k1 ? idx[kk] : -1;
+                const int kc = kk < k1 ? kk : k1 - 1;
+                const TV av = val[kc];
+                const int jv = idx[kc];
+                a[u] = kk < k1 ? av : TV(0);
+                j[u] = kk < k1 ? jv : -1;
             }
+            TX g[4];  // unconditional gathers (entry 0 past the end): all 4 in flight at once
+#pragma unroll
+            for (int u = 0; u < 4; ++u) g[u] = xv[j[u] >= 0 ? j[u] : 0];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (j[u] >= 0) acc += static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
+                acc = j[u] >= 0 ? acc + static_cast<double>(a[u]) * static_cast<double>(g[u]) : acc;
         }
         acc = group_sum<kWave>(acc);
         if ((tid % kWave) == 0) red[tid / kWave] = acc;
@@ -69,18 +75,25 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
     }
     // phase 1: products of the block's nonzeros into LDS
     constexpr int U = CAP / kBlock;
-    {
+    if (k1 > k0) {  // uniform; clamped loads (k1 - 1 is the block's), selected afterwards
         TV a[U];
         int j[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k0 + tid + u * kBlock;
-            a[u] = kk < k1 ? val[kk] : TV(0);
-            j[u] = kk < k1 ? idx[kk] : -1;
+            const int kc = kk < k1 ? kk : k1 - 1;
+            const TV av = val[kc];
+            const int jv = idx[kc];
+            a[u] = kk < k1 ? av : TV(0);
+            j[u] = kk < k1 ? jv : -1;
         }
+        // unconditional gathers (entry 0 past the block's nonzeros) so that
+        // all U are in flight at once; products past k1 are never read
+        TX g[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (j[u] >= 0) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
+        for (int u = 0; u < U; ++u) g[u] = xv[j[u] >= 0 ? j[u] : 0];
+#pragma unroll
+        for (int u = 0; u < U; ++u) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(g[u]);
     }
     __syncthreads();
     // phase 2: segment sums from LDS, L lanes per segment
