@@ -41,26 +41,41 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
     const int k0 = ptr[s0], k1 = ptr[s1];
     const int tid = threadIdx.x;
     const TX* xv = epi.vec(0);
+    // branch-free loads and gathers for FP32 gathered vectors only (measured
+    // faster there; FP64 keeps the per-entry branches, as in seg_accumulate)
+    constexpr bool kFlat = sizeof(TX) == 4;
     if (k1 - k0 > CAP) {  // one long segment: register accumulation, block reduction
         double acc = 0.0;
         for (int k = k0 + tid; k < k1; k += 4 * kBlock) {
             TV a[4];
             int j[4];
+            if constexpr (kFlat) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {  // clamped loads, selected afterwards (k < k1 here)
-                const int kk = k + u * kBlock;
-                const int kc = kk < k1 ? kk : k1 - 1;
-                const TV av = val[kc];
-                const int jv = idx[kc];
-                a[u] = kk < k1 ? av : TV(0);
-                j[u] = kk < k1 ? jv : -1;
+                for (int u = 0; u < 4; ++u) {  // clamped loads, selected afterwards (k < k1 here)
+                    const int kk = k + u * kBlock;
+                    const int kc = kk < k1 ? kk : k1 - 1;
+                    const TV av = val[kc];
+                    const int jv = idx[kc];
+                    a[u] = kk < k1 ? av : TV(0);
+                    j[u] = kk < k1 ? jv : -1;
+                }
+                TX g[4];  // unconditional gathers (entry 0 past the end): all 4 in flight at once
+#pragma unroll
+                for (int u = 0; u < 4; ++u) g[u] = xv[j[u] >= 0 ? j[u] : 0];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    acc = j[u] >= 0 ? acc + static_cast<double>(a[u]) * static_cast<double>(g[u]) : acc;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k + u * kBlock;
+                    a[u] = kk < k1 ? val[kk] : TV(0);
+                    j[u] = kk < k1 ? idx[kk] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j[u] >= 0) acc += static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
             }
-            TX g[4];  // unconditional gathers (entry 0 past the end): all 4 in flight at once
-#pragma unroll
-            for (int u = 0; u < 4; ++u) g[u] = xv[j[u] >= 0 ? j[u] : 0];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                acc = j[u] >= 0 ? acc + static_cast<double>(a[u]) * static_cast<double>(g[u]) : acc;
         }
         acc = group_sum<kWave>(acc);
         if ((tid % kWave) == 0) red[tid / kWave] = acc;
@@ -75,7 +90,7 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
     }
     // phase 1: products of the block's nonzeros into LDS
     constexpr int U = CAP / kBlock;
-    if (k1 > k0) {  // uniform; clamped loads (k1 - 1 is the block's), selected afterwards
+    if (kFlat && k1 > k0) {  // uniform; clamped loads (k1 - 1 is the block's), selected afterwards
         TV a[U];
         int j[U];
 #pragma unroll
@@ -94,6 +109,18 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
         for (int u = 0; u < U; ++u) g[u] = xv[j[u] >= 0 ? j[u] : 0];
 #pragma unroll
         for (int u = 0; u < U; ++u) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(g[u]);
+    } else if (!kFlat) {
+        TV a[U];
+        int j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k0 + tid + u * kBlock;
+            a[u] = kk < k1 ? val[kk] : TV(0);
+            j[u] = kk < k1 ? idx[kk] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
     }
     __syncthreads();
     // phase 2: segment sums from LDS, L lanes per segment
