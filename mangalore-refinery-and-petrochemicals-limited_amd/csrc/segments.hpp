@@ -72,31 +72,15 @@ __device__ inline void seg_accumulate(int k0, int k1, int lane, const int* __res
                                       HotCache<TX> hot) {
     constexpr int U = unroll<L>();
     for (int k = k0 + lane; k < k1; k += U * L) {
-        TV a[U];
-        int j[U];
-        // FLAT (FP32 gathered vectors): loads from a clamped position (k < k1
-        // here, so k1 - 1 is in the segment), selected afterwards, so no load
-        // sits under a branch, where each would wait for itself before the
-        // next is issued.  Measured: the FP32 column pass of the staircase LP
-        // 11.7 -> 9.2 us; with FP64 vectors (twice the registers per gathered
-        // value) the batched engine's FP64 tail rounds got slower, so they
-        // keep the branches.
-        constexpr bool FLAT = sizeof(TX) == 4 && !HOT;
+        if constexpr (HOT) {
+            TV a[U];
+            int j[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = k + u * L;
-            if constexpr (FLAT) {
-                const int kc = kk < k1 ? kk : k1 - 1;
-                const TV av = val[kc];
-                const int jv = idx[kc];
-                a[u] = kk < k1 ? av : TV(0);
-                j[u] = kk < k1 ? jv : -1;
-            } else {
+            for (int u = 0; u < U; ++u) {
+                const int kk = k + u * L;
                 a[u] = kk < k1 ? val[kk] : TV(0);
                 j[u] = kk < k1 ? idx[kk] : -1;
             }
-        }
-        if constexpr (HOT) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (j[u] >= 0) {
@@ -104,29 +88,40 @@ __device__ inline void seg_accumulate(int k0, int k1, int lane, const int* __res
                     acc[0] = seg_combine<MODE>(acc[0], seg_term<MODE>(a[u], xv));
                 }
             }
-        } else if constexpr (!FLAT) {
+        } else {
+            // Groups of G entries per lane, with no load under a branch:
+            // positions past the segment load entry k1 - 1 (k < k1 here) and
+            // gather entry 0, and their terms are not added, so a group's
+            // loads are all in flight together (under a per-entry branch each
+            // waited for itself).  A group that no lane of the wavefront
+            // reaches ends the trip, so short segments issue no wasted loads
+            // beyond their last group.  Terms are added in entry order.
+            constexpr int G = U < 4 ? U : 4;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (j[u] >= 0) {
+            for (int u0 = 0; u0 < U; u0 += G) {
+                if (!__any(k + u0 * L < k1)) break;  // uniform; later groups start further on
+                TV a[G];
+                int j[G];
+                TX xv[NV][G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const int kk = k + (u0 + g) * L;
+                    const int kc = kk < k1 ? kk : k1 - 1;
+                    const TV av = val[kc];
+                    const int jv = idx[kc];
+                    a[g] = kk < k1 ? av : TV(0);
+                    j[g] = kk < k1 ? jv : -1;
+                }
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xv[v][g] = epi.vec(v)[j[g] >= 0 ? j[g] : 0];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int v = 0; v < NV; ++v)
-                        acc[v] = seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], epi.vec(v)[j[u]]));
-                }
+                        acc[v] = j[g] >= 0 ? seg_combine<MODE>(acc[v], seg_term<MODE>(a[g], xv[v][g])) : acc[v];
             }
-        } else {
-            // FLAT: the gathers are unconditional too (entry 0 for lanes past
-            // the segment, whose terms are then not added), so all U of a
-            // trip are in flight together.
-            TX xv[NV][U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) xv[v][u] = epi.vec(v)[j[u] >= 0 ? j[u] : 0];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    acc[v] = j[u] >= 0 ? seg_combine<MODE>(acc[v], seg_term<MODE>(a[u], xv[v][u])) : acc[v];
         }
     }
 }

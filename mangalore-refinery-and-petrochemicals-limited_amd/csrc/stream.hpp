@@ -42,7 +42,8 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
     const int tid = threadIdx.x;
     const TX* xv = epi.vec(0);
     // branch-free loads and gathers for FP32 gathered vectors only (measured
-    // faster there; FP64 keeps the per-entry branches, as in seg_accumulate)
+    // faster there, in the staircase LP's row pass); the FP64 instantiations
+    // keep the per-entry branches
     constexpr bool kFlat = sizeof(TX) == 4;
     if (k1 - k0 > CAP) {  // one long segment: register accumulation, block reduction
         double acc = 0.0;
