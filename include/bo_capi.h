@@ -153,7 +153,9 @@ typedef struct bo_pdlp_params {
                                   otherwise */
     int fp64_iterates;         /* 1: run the iteration in FP64 (error comparison run) */
     int use_graph;             /* 1: replay each check interval as a hipGraph */
-    double time_limit_s;       /* 0 = none */
+    double time_limit_s;       /* 0 = none; else wall-clock limit of one solve call, tail rounds and FP64
+                                  polish included: LPs still running when it passes stop with status 3.
+                                  A time limit turns the one-workgroup solve off (single_workgroup) */
     int verbose;
     int hot_columns;           /* entries of x+ kept in LDS by the CSR pass: -1 tune, 0 off */
     long long stall_iterations; /* batched and family engines, FP32 iterates: an LP whose FP64 rel-KKT
@@ -167,6 +169,11 @@ typedef struct bo_pdlp_params {
     int single_workgroup;      /* single-LP engine: 1 run the whole solve in one workgroup (no kernel
                                   launch per pass), 0 never, -1 (default) the tuner times both on LPs
                                   of up to 2^18 nonzeros without a time limit */
+    double eps_objective;      /* > 0: an LP also stops only when |c'x - dual objective| <= this
+                                  x (1 + |c'x|); with x and y feasible to eps_rel_kkt the optimum lies
+                                  between the two objectives, so c'x is then within this (relative) of
+                                  it.  0 (default): rel-KKT alone, whose gap term is relative to
+                                  1 + |c'x| + |dual objective| */
 } bo_pdlp_params;
 
 typedef struct bo_pdlp_stats {
@@ -375,6 +382,9 @@ typedef struct bo_fo_options {
     long long check_frequency;
     int engine; /* 0: reference PDHG semantics (FP64, bit-equal); 1: PDLP to rel-KKT */
     double eps_rel_kkt;
+    int cpu_backend; /* 1: CPUFirstOrderSolver, the reference's loop on the host (engine ignored);
+                        0: GPUFirstOrderSolver */
+    double eps_objective; /* PDLP engine: bo_pdlp_params.eps_objective (0: rel-KKT alone) */
 } bo_fo_options;
 
 /* FirstOrderSolverResult / Stats (first_order_solver.hpp:41-60). */
@@ -402,9 +412,12 @@ typedef struct bo_fo_result {
 bo_status bo_presolve_stats(const bo_model* model, int* status, int* reduced_vars,
                             int* reduced_cons, double* obj_offset, long long* reduced_nnz);
 
-/* GPUFirstOrderSolver::solve (first_order_solver.hpp:89-98, src/first_order_solver.cpp:295-306):
- * validate -> presolve -> standard form -> GPU PDHG -> postsolve -> objective.
- * primal_out holds num_vars, dual_out num_cons entries. */
+/* FirstOrderLPSolver::solve (first_order_solver.hpp:103-113, src/first_order_solver.cpp:311-319):
+ * validate -> presolve -> standard form -> PDHG -> postsolve -> objective, the
+ * loop on the GPU (GPUFirstOrderSolver::solve, :295-306) or, with
+ * cpu_backend = 1, on the host (CPUFirstOrderSolver::solve, :14-290).
+ * primal_out holds num_vars, dual_out num_cons entries.  Returns
+ * BO_GPU_UNAVAILABLE when the GPU backend was asked for and no GPU is visible. */
 bo_status bo_first_order_solve(const bo_model* model, const bo_fo_options* opts,
                                bo_fo_result* result, double* primal_out, double* dual_out);
 

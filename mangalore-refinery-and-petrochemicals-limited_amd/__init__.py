@@ -77,7 +77,7 @@ class PdlpParams(C.Structure):
                 ("lanes_rows", C.c_int), ("lanes_cols", C.c_int), ("fp64_iterates", C.c_int),
                 ("use_graph", C.c_int), ("time_limit_s", C.c_double), ("verbose", C.c_int),
                 ("hot_columns", C.c_int), ("stall_iterations", C.c_longlong), ("fp64_polish", C.c_int),
-                ("compact_tail", C.c_int), ("single_workgroup", C.c_int)]
+                ("compact_tail", C.c_int), ("single_workgroup", C.c_int), ("eps_objective", C.c_double)]
 
 
 class PdlpStats(C.Structure):
@@ -123,7 +123,8 @@ class FoOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_longlong), ("feasibility_tolerance", C.c_double),
                 ("optimality_tolerance", C.c_double), ("primal_step_size", C.c_double),
                 ("dual_step_size", C.c_double), ("adaptive_step_size", C.c_int),
-                ("check_frequency", C.c_longlong), ("engine", C.c_int), ("eps_rel_kkt", C.c_double)]
+                ("check_frequency", C.c_longlong), ("engine", C.c_int), ("eps_rel_kkt", C.c_double),
+                ("cpu_backend", C.c_int), ("eps_objective", C.c_double)]
 
 
 class FoResult(C.Structure):
@@ -702,12 +703,18 @@ def presolve_stats(fm: FlatModel):
 
 def first_order_solve(fm: FlatModel, engine=0, max_iterations=50000, feasibility_tolerance=1e-7,
                       optimality_tolerance=1e-7, primal_step_size=0.01, dual_step_size=0.01,
-                      adaptive_step_size=True, check_frequency=100, eps_rel_kkt=1e-6):
-    """GPUFirstOrderSolver::solve on a flattened LPModel (engine 0 = reference
-    PDHG semantics, 1 = PDLP).  Returns a dict of the result fields."""
+                      adaptive_step_size=True, check_frequency=100, eps_rel_kkt=1e-6, backend="gpu",
+                      eps_objective=0.0):
+    """FirstOrderLPSolver::solve on a flattened LPModel.  backend "gpu":
+    GPUFirstOrderSolver (engine 0 = reference PDHG semantics, 1 = PDLP);
+    "cpu": CPUFirstOrderSolver, the reference's loop on the host (needs no
+    GPU).  Returns a dict of the result fields."""
+    if backend not in ("gpu", "cpu"):
+        raise ValueError("backend must be 'gpu' or 'cpu'")
     h = _ModelHolder(fm)
     o = FoOptions(max_iterations, feasibility_tolerance, optimality_tolerance, primal_step_size,
-                  dual_step_size, 1 if adaptive_step_size else 0, check_frequency, engine, eps_rel_kkt)
+                  dual_step_size, 1 if adaptive_step_size else 0, check_frequency, engine, eps_rel_kkt,
+                  1 if backend == "cpu" else 0, eps_objective)
     r = FoResult()
     x = np.zeros(max(fm.num_vars, 1))
     y = np.zeros(max(fm.num_cons * 2, 1))

@@ -510,6 +510,7 @@ struct BatchBase {
     virtual bo_status set_objective_rhs(const double* c, const double* b) = 0;
     virtual void set_iteration_limits(const long long* lim) = 0;
     virtual void set_stop_running(int r) = 0;
+    virtual void set_deadline(const Deadline& d) = 0;
     virtual bo_status time_kernels(int reps, double* ms4, double* bytes4) = 0;
     virtual bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) = 0;
     virtual bo_status get_solution(int k, double* x, double* y) = 0;
@@ -575,6 +576,7 @@ struct Batch final : BatchBase {
     hipGraph_t graph = nullptr;
     int graph_k = 0;
     int stop_running = 0;
+    Deadline deadline;  // unset: time_limit_s counts from the start of solve()
 
     ~Batch() override {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
@@ -707,6 +709,7 @@ struct Batch final : BatchBase {
             s.b_norm = bn[k];
             s.c_norm = cn[k];
             s.eps = prm.eps_rel_kkt;
+            s.eps_obj = prm.eps_objective;
             s.max_iter = prm.max_iterations;
             s.stall_window = sizeof(T) == sizeof(float) ? prm.stall_iterations : 0;
         }
@@ -1034,6 +1037,10 @@ struct Batch final : BatchBase {
     // status 1 with fewer iterations than their budget).
     void set_stop_running(int r) override { stop_running = r; }
 
+    // The absolute time limit of the solve this batch is a round of; LPs
+    // still running when it passes stop with status 3 (TIME_LIMIT).
+    void set_deadline(const Deadline& d) override { deadline = d; }
+
     // Iteration budget of every LP (at most max_iterations of the parameters).
     void set_iteration_limits(const long long* lim) override {
         for (int k = 0; k < K; ++k) st_init[k].max_iter = std::min(lim[k], prm.max_iterations);
@@ -1069,6 +1076,8 @@ struct Batch final : BatchBase {
 
     bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) override {
         auto t0 = std::chrono::high_resolution_clock::now();
+        const Deadline dl = deadline.on ? deadline : Deadline::after(prm.time_limit_s);
+        bool timed_out = false;
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
         launch_check();  // step 0: KKT of the starting points
@@ -1077,6 +1086,10 @@ struct Batch final : BatchBase {
         const int Kc = prm.check_every > 0 ? prm.check_every : 64;
         long long iters = 0;
         while (*running_host > stop_running) {
+            if (dl.passed()) {
+                timed_out = true;
+                break;
+            }
             const long long left = prm.max_iterations - iters;
             const int k = static_cast<int>(std::min<long long>(Kc, left));
             if (k <= 0) break;
@@ -1104,10 +1117,6 @@ struct Batch final : BatchBase {
                 std::fprintf(stderr, "batch: step %lld running %d at %.3f ms\n", iters, *running_host,
                              std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0)
                                  .count());
-            if (prm.time_limit_s > 0.0 &&
-                std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count() >
-                    prm.time_limit_s)
-                break;
         }
         BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState) * K, hipMemcpyDeviceToHost, stream));
         BO_TRY(hipStreamSynchronize(stream));
@@ -1118,7 +1127,7 @@ struct Batch final : BatchBase {
         for (int k = 0; k < K; ++k) {
             const PdlpState& h = st_host[k];
             const int b = h.best;
-            const int status = h.done ? h.status : 1;
+            const int status = h.done ? h.status : (timed_out ? 3 : 1);
             solved += status == 0;
             max_it = std::max(max_it, h.iter);
             sum_it += h.iter;
@@ -1167,20 +1176,6 @@ struct Batch final : BatchBase {
         return BO_SUCCESS;
     }
 };
-
-bool valid_lp(const bo_lp& lp) {
-    if (lp.m < 0 || lp.n < 0 || lp.nnz < 0) return false;
-    if ((lp.m > 0 && (!lp.row_ptr || !lp.b || !lp.is_eq)) || (lp.n > 0 && (!lp.c || !lp.lb || !lp.ub)) ||
-        (lp.nnz > 0 && (!lp.col_idx || !lp.val)))
-        return false;
-    if (lp.m > 0 && (lp.row_ptr[0] != 0 || lp.row_ptr[lp.m] != lp.nnz)) return false;
-    if (lp.m == 0 && lp.nnz != 0) return false;
-    for (int i = 0; i < lp.m; ++i)
-        if (lp.row_ptr[i + 1] < lp.row_ptr[i]) return false;
-    for (long long k = 0; k < lp.nnz; ++k)
-        if (lp.col_idx[k] < 0 || lp.col_idx[k] >= lp.n) return false;
-    return true;
-}
 
 }  // namespace
 }  // namespace bo
@@ -1238,17 +1233,17 @@ extern "C" {
 bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params* params, bo_pdlp_batch** out) {
     if (!out) return BO_INVALID_ARGUMENT;
     *out = nullptr;
-    bo_status s = ensure_device_public();
-    if (s != BO_SUCCESS) return s;
     if (!lps || count <= 0) return BO_INVALID_ARGUMENT;
     long long m = 0, n = 0, nnz = 0;
     for (int k = 0; k < count; ++k) {
-        if (!valid_lp(lps[k])) return BO_INVALID_ARGUMENT;
+        if (!valid_csr_lp(&lps[k])) return BO_INVALID_ARGUMENT;
         m += lps[k].m;
         n += lps[k].n;
         nnz += lps[k].nnz;
     }
     if (m >= (1LL << 31) - 1 || n >= (1LL << 31) - 1 || nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;
+    bo_status s = ensure_device_public();
+    if (s != BO_SUCCESS) return s;
     bo_pdlp_params p;
     if (params) p = *params;
     else bo_pdlp_default_params(&p);
@@ -1312,16 +1307,34 @@ struct TailLp {
     long long used, fp64_used;
     double w;
     std::vector<double> x, y;
+    bo_pdlp_stats last;  // its stats at the end of its last round
 };
 
+// Cut by a round's stop rule with iterations left: status 1 only (an LP the
+// time limit stopped has status 3 and is final).
 bool tail_from_round(const bo_pdlp_stats& s, long long lim) { return s.status == 1 && s.iterations < lim; }
+
+// LPs still queued for a round when the time limit has passed: final, with
+// status 3 (TIME_LIMIT), at the point their last round left.
+void finish_timed_out(bo_pdlp_batch* h, std::vector<TailLp>& queued, std::vector<bo_pdlp_stats>& per) {
+    for (TailLp& t : queued) {
+        per[t.k] = t.last;
+        per[t.k].status = 3;
+        per[t.k].iterations = t.used;
+        per[t.k].fp64_polish_iterations = t.fp64_used;
+        h->fx[t.k] = std::move(t.x);
+        h->fy[t.k] = std::move(t.y);
+        h->in_tail[t.k] = 1;
+    }
+    queued.clear();
+}
 
 // One compact round over `in` (segment kernels, no tile layout): LPs that
 // finish are final (stats and solution kept on the handle), LPs cut by the
 // round's stop rule go to `again`, LPs that stall (FP32 rounds) go to `to64`.
 template <class T>
-bo_status tail_round(bo_pdlp_batch* h, std::vector<TailLp>& in, std::vector<TailLp>& again, std::vector<TailLp>& to64,
-                     std::vector<bo_pdlp_stats>& per) {
+bo_status tail_round(bo_pdlp_batch* h, const Deadline& dl, std::vector<TailLp>& in, std::vector<TailLp>& again,
+                     std::vector<TailLp>& to64, std::vector<bo_pdlp_stats>& per) {
     const bool is64 = sizeof(T) == sizeof(double);
     const int P = static_cast<int>(in.size());
     auto rc0 = std::chrono::high_resolution_clock::now();
@@ -1346,6 +1359,7 @@ bo_status tail_round(bo_pdlp_batch* h, std::vector<TailLp>& in, std::vector<Tail
     if (s != BO_SUCCESS) return s;
     e.set_iteration_limits(lim.data());
     e.set_stop_running(h->compacting() && P >= 16 ? P / 8 : 0);
+    e.set_deadline(dl);
     if ((s = e.set_primal_weights(w.data())) != BO_SUCCESS) return s;
     if ((s = e.set_warm_start(wx.data(), wy.data())) != BO_SUCCESS) return s;
     std::vector<bo_pdlp_stats> per2(P);
@@ -1362,7 +1376,7 @@ bo_status tail_round(bo_pdlp_batch* h, std::vector<TailLp>& in, std::vector<Tail
         std::vector<double> x(h->lps[k].n), y(h->lps[k].m);
         if ((s = e.get_solution(j, x.data(), y.data())) != BO_SUCCESS) return s;
         TailLp t{k, in[j].used + per2[j].iterations, in[j].fp64_used + (is64 ? per2[j].iterations : 0),
-                 per2[j].primal_weight, std::move(x), std::move(y)};
+                 per2[j].primal_weight, std::move(x), std::move(y), per2[j]};
         if (tail_from_round(per2[j], lim[j])) {
             again.push_back(std::move(t));
         } else if (per2[j].status == 4 && !is64 && h->polishing()) {
@@ -1396,6 +1410,8 @@ bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_b
     std::vector<bo_pdlp_stats> per(K);
     bo_pdlp_batch_stats tot{};
     std::fill(h->in_tail.begin(), h->in_tail.end(), 0);
+    const Deadline dl = Deadline::after(h->prm.time_limit_s);
+    h->eng->set_deadline(dl);
     h->eng->set_stop_running(h->compacting() && K >= 16 ? K / 8 : 0);
     bo_status s = h->eng->solve(per.data(), &tot);
     if (s != BO_SUCCESS) return s;
@@ -1405,7 +1421,7 @@ bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_b
         const bool cut = h->compacting() && tail_from_round(per[k], h->prm.max_iterations);
         const bool stalled = per[k].status == 4 && h->polishing();
         if (!cut && !stalled) continue;
-        TailLp t{k, per[k].iterations, main64 ? per[k].iterations : 0, per[k].primal_weight, {}, {}};
+        TailLp t{k, per[k].iterations, main64 ? per[k].iterations : 0, per[k].primal_weight, {}, {}, per[k]};
         t.x.resize(h->lps[k].n);
         t.y.resize(h->lps[k].m);
         if ((s = h->eng->get_solution(k, t.x.data(), t.y.data())) != BO_SUCCESS) return s;
@@ -1416,16 +1432,22 @@ bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_b
         int tail_lps = static_cast<int>(same.size() + p64.size());
         std::vector<TailLp> unused;
         while (!same.empty() || !p64.empty()) {
+            if (dl.passed()) {
+                finish_timed_out(h, same, per);
+                finish_timed_out(h, p64, per);
+                break;
+            }
             if (!same.empty()) {
                 std::vector<TailLp> in;
                 in.swap(same);
-                s = main64 ? tail_round<double>(h, in, same, unused, per) : tail_round<float>(h, in, same, p64, per);
+                s = main64 ? tail_round<double>(h, dl, in, same, unused, per)
+                           : tail_round<float>(h, dl, in, same, p64, per);
                 if (s != BO_SUCCESS) return s;
             }
-            if (!p64.empty()) {
+            if (!p64.empty() && !dl.passed()) {
                 std::vector<TailLp> in;
                 in.swap(p64);
-                if ((s = tail_round<double>(h, in, p64, unused, per)) != BO_SUCCESS) return s;
+                if ((s = tail_round<double>(h, dl, in, p64, unused, per)) != BO_SUCCESS) return s;
             }
         }
         tot.tail_lps = tail_lps;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -41,6 +42,44 @@ void set_last_error(const std::string& msg);
 inline unsigned grid_for(size_t n, int block = kBlock) {
     size_t g = (n + block - 1) / block;
     return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+// An absolute time limit shared by every round of one solve (main batch,
+// tail rounds, FP64 polish), so a solve's wall time stays within its
+// time_limit_s however many rounds it takes.
+struct Deadline {
+    bool on = false;
+    std::chrono::steady_clock::time_point at{};
+    static Deadline after(double seconds) {
+        Deadline d;
+        if (seconds > 0.0) {
+            d.on = true;
+            d.at = std::chrono::steady_clock::now() +
+                   std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                       std::chrono::duration<double>(seconds));
+        }
+        return d;
+    }
+    bool passed() const { return on && std::chrono::steady_clock::now() > at; }
+};
+
+// A bo_lp whose CSR can be read safely: sizes non-negative, every array its
+// sizes require present, row_ptr[0] == 0, row_ptr non-decreasing and ending
+// at nnz (nnz == 0 when there are no rows), every column index in [0, n).
+// Every entry point that takes a bo_lp calls this before it reads the matrix
+// or allocates anything.
+inline bool valid_csr_lp(const bo_lp* lp) {
+    if (!lp || lp->m < 0 || lp->n < 0 || lp->nnz < 0) return false;
+    if ((lp->m > 0 && (!lp->row_ptr || !lp->b || !lp->is_eq)) || (lp->n > 0 && (!lp->c || !lp->lb || !lp->ub)) ||
+        (lp->nnz > 0 && (!lp->col_idx || !lp->val)))
+        return false;
+    if (lp->m == 0) return lp->nnz == 0;
+    if (lp->row_ptr[0] != 0 || lp->row_ptr[lp->m] != lp->nnz) return false;
+    for (int i = 0; i < lp->m; ++i)
+        if (lp->row_ptr[i + 1] < lp->row_ptr[i]) return false;
+    for (long long k = 0; k < lp->nnz; ++k)
+        if (lp->col_idx[k] < 0 || lp->col_idx[k] >= lp->n) return false;
+    return true;
 }
 
 // ---- wave / block reductions (fixed tree order => deterministic) ----------

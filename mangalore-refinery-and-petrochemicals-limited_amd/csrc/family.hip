@@ -634,6 +634,7 @@ struct FamilyBase {
     virtual bo_status set_warm_start(const double* x, const double* y) = 0;
     virtual bo_status set_primal_weights(const double* w) = 0;
     virtual void set_iteration_limits(const long long* lim) = 0;
+    virtual void set_deadline(const Deadline& d) = 0;
     virtual bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) = 0;
     virtual bo_status get_solutions(double* x, double* y) = 0;
     virtual bo_status time_passes(int reps, double* ms2) = 0;
@@ -676,6 +677,7 @@ struct Family final : FamilyBase {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_k = 0;
+    Deadline deadline;  // unset: time_limit_s counts from the start of solve()
 
     ~Family() override {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
@@ -710,6 +712,7 @@ struct Family final : FamilyBase {
         std::memset(st_init, 0, sizeof(PdlpState) * kL);
         for (int k = 0; k < kL; ++k) {
             st_init[k].eps = prm.eps_rel_kkt;
+            st_init[k].eps_obj = prm.eps_objective;
             st_init[k].max_iter = prm.max_iterations;
             st_init[k].stall_window = sizeof(T) == sizeof(float) ? prm.stall_iterations : 0;
         }
@@ -871,6 +874,10 @@ struct Family final : FamilyBase {
         return BO_SUCCESS;
     }
 
+    // The absolute time limit of the solve this engine is a round of; LPs
+    // still running when it passes stop with status 3 (TIME_LIMIT).
+    void set_deadline(const Deadline& d) override { deadline = d; }
+
     void set_iteration_limits(const long long* lim) override {
         for (int k = 0; k < K; ++k) st_init[k].max_iter = std::min(lim[k], prm.max_iterations);
     }
@@ -934,6 +941,8 @@ struct Family final : FamilyBase {
 
     bo_status solve(bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) override {
         auto t0 = std::chrono::high_resolution_clock::now();
+        const Deadline dl = deadline.on ? deadline : Deadline::after(prm.time_limit_s);
+        bool timed_out = false;
         bo_status s0 = prepare();
         if (s0 != BO_SUCCESS) return s0;
         launch_check();  // step 0: KKT of the starting points
@@ -942,6 +951,10 @@ struct Family final : FamilyBase {
         const int Kc = prm.check_every > 0 ? prm.check_every : 64;
         long long iters = 0;
         while (*running_host > 0) {
+            if (dl.passed()) {
+                timed_out = true;
+                break;
+            }
             const long long left = prm.max_iterations - iters;
             const int k = static_cast<int>(std::min<long long>(Kc, left));
             if (k <= 0) break;
@@ -965,10 +978,6 @@ struct Family final : FamilyBase {
             iters += k;
             BO_TRY(hipMemcpyAsync(running_host, running, sizeof(int), hipMemcpyDeviceToHost, stream));
             BO_TRY(hipStreamSynchronize(stream));
-            if (prm.time_limit_s > 0.0 &&
-                std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count() >
-                    prm.time_limit_s)
-                break;
         }
         BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState) * kL, hipMemcpyDeviceToHost, stream));
         BO_TRY(hipStreamSynchronize(stream));
@@ -979,7 +988,7 @@ struct Family final : FamilyBase {
         for (int k = 0; k < K; ++k) {
             const PdlpState& h = st_host[k];
             const int b = h.best;
-            const int status = h.done ? h.status : 1;
+            const int status = h.done ? h.status : (timed_out ? 3 : 1);
             solved += status == 0;
             max_it = std::max(max_it, h.iter);
             sum_it += h.iter;
@@ -1126,19 +1135,16 @@ extern "C" {
 bo_status bo_pdlp_family_create(const bo_lp* lp, int capacity, const bo_pdlp_params* params, bo_pdlp_family** out) {
     if (!out) return BO_INVALID_ARGUMENT;
     *out = nullptr;
-    bo_status s = ensure_device_public();
-    if (s != BO_SUCCESS) return s;
-    if (!lp || capacity <= 0 || capacity > kL || lp->m <= 0 || lp->n <= 0 || lp->nnz <= 0) return BO_INVALID_ARGUMENT;
-    if (!lp->row_ptr || !lp->col_idx || !lp->val || !lp->b || !lp->c || !lp->lb || !lp->ub || !lp->is_eq)
+    if (!valid_csr_lp(lp) || capacity <= 0 || capacity > kL || lp->m <= 0 || lp->n <= 0 || lp->nnz <= 0 ||
+        lp->nnz >= (1LL << 31) - 1)
         return BO_INVALID_ARGUMENT;
-    if (lp->row_ptr[0] != 0 || lp->row_ptr[lp->m] != lp->nnz || lp->nnz >= (1LL << 31) - 1) return BO_INVALID_ARGUMENT;
-    for (int i = 0; i < lp->m; ++i)
-        if (lp->row_ptr[i + 1] < lp->row_ptr[i]) return BO_INVALID_ARGUMENT;
     for (long long q = 0; q < lp->nnz; ++q)
-        if (lp->col_idx[q] < 0 || lp->col_idx[q] >= lp->n || !std::isfinite(lp->val[q])) return BO_INVALID_ARGUMENT;
+        if (!std::isfinite(lp->val[q])) return BO_INVALID_ARGUMENT;
     for (int j = 0; j < lp->n; ++j)
         if (!(lp->lb[j] <= lp->ub[j]) || !std::isfinite(lp->c[j])) return BO_INVALID_ARGUMENT;
     if (static_cast<long long>(std::max(lp->m, lp->n)) * kL >= (1LL << 31)) return BO_INVALID_ARGUMENT;
+    bo_status s = ensure_device_public();
+    if (s != BO_SUCCESS) return s;
     bo_pdlp_params p;
     if (params) p = *params;
     else bo_pdlp_default_params(&p);
@@ -1194,12 +1200,18 @@ bo_status bo_pdlp_family_solve(bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp
     std::vector<bo_pdlp_stats> per(K);
     bo_pdlp_batch_stats tot{};
     h->pol_of.assign(K, -1);
+    const Deadline dl = Deadline::after(h->prm.time_limit_s);
+    h->eng->set_deadline(dl);
     bo_status s = h->eng->solve(per.data(), &tot);
     if (s != BO_SUCCESS) return s;
     std::vector<int> stalled;
     if (h->polishing())
         for (int k = 0; k < K; ++k)
             if (per[k].status == 4 && per[k].iterations < h->prm.max_iterations) stalled.push_back(k);
+    if (!stalled.empty() && dl.passed()) {  // no time left for the polish: stopped by the limit
+        for (int k : stalled) per[k].status = 3;
+        stalled.clear();
+    }
     if (!stalled.empty()) {
         auto tp0 = std::chrono::high_resolution_clock::now();
         const int P = static_cast<int>(stalled.size());
@@ -1229,6 +1241,7 @@ bo_status bo_pdlp_family_solve(bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp
         FamilyBase* f = h->pol.get();
         if ((s = f->set_lps(P, pc.data(), pb.data())) != BO_SUCCESS) return s;
         f->set_iteration_limits(lim.data());
+        f->set_deadline(dl);
         if ((s = f->set_primal_weights(w.data())) != BO_SUCCESS) return s;
         if ((s = f->set_warm_start(px.data(), py.data())) != BO_SUCCESS) return s;
         std::vector<bo_pdlp_stats> per2(P);

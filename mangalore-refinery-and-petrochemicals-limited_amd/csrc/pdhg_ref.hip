@@ -487,18 +487,13 @@ using namespace bo;
 
 extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o, double* x_out,
                                        double* y_out, bo_ref_result* res) {
+    if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
+        return BO_INVALID_ARGUMENT;
+    if (lp->nnz >= (1LL << 31)) return BO_INVALID_ARGUMENT;  // int32 offsets
     bo_status st = ensure_device_public();
     if (st != BO_SUCCESS) return st;
-    if (!lp || !o || !res || lp->m < 0 || lp->n < 0 || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
-        return BO_INVALID_ARGUMENT;
     const int m = lp->m, n = lp->n;
     const long long nnz = lp->nnz;
-    if ((m > 0 && !lp->row_ptr) || (nnz > 0 && (!lp->col_idx || !lp->val)) ||
-        (m > 0 && (!lp->b || !lp->is_eq)) || (n > 0 && (!lp->c || !lp->lb || !lp->ub)))
-        return BO_INVALID_ARGUMENT;
-    if (m > 0 && lp->row_ptr[m] != nnz) return BO_INVALID_ARGUMENT;
-    for (long long k = 0; k < nnz; ++k)
-        if (lp->col_idx[k] < 0 || lp->col_idx[k] >= n) return BO_INVALID_ARGUMENT;
     std::memset(res, 0, sizeof(*res));
 
     auto t_xfer0 = std::chrono::high_resolution_clock::now();

@@ -1133,7 +1133,9 @@ struct Engine final : EngineBase {
         prm.lanes_rows = plan_code(plan_rows);
         prm.lanes_cols = plan_code(plan_cols);
         prm.hot_columns = plan_rows.hot;
-        if (prm.single_workgroup > 0) {
+        if (prm.single_workgroup > 0 && prm.time_limit_s <= 0.0) {
+            // (with a time limit the multi-kernel loop runs: it reads the
+            // clock between check intervals, a one-workgroup solve cannot)
             one_wg = true;
         } else if (prm.single_workgroup < 0 && nnz <= kSmallTuneNnz && prm.time_limit_s <= 0.0) {
             bo_status s3 = tune_one_wg();
@@ -1647,6 +1649,7 @@ struct Engine final : EngineBase {
         st_init->b_norm = b_norm;
         st_init->c_norm = c_norm;
         st_init->eps = prm.eps_rel_kkt;
+        st_init->eps_obj = prm.eps_objective;
         st_init->max_iter = prm.max_iterations;
         BO_TRY(hipMemcpyAsync(st, st_init, sizeof(PdlpState), hipMemcpyHostToDevice, stream));
         power_pending = kernel_power;
@@ -1832,6 +1835,7 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
     std::memset(p, 0, sizeof(*p));
     p->max_iterations = 100000;
     p->eps_rel_kkt = 1e-6;
+    p->eps_objective = 0.0;
     p->ruiz_iterations = 10;
     p->pock_chambolle = 1;
     p->check_every = 0;
@@ -1851,19 +1855,10 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
 bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp** out) {
     if (!out) return BO_INVALID_ARGUMENT;
     *out = nullptr;
+    if (!valid_csr_lp(lp)) return BO_INVALID_ARGUMENT;
+    if (lp->nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;  // int32 offsets with unrolled strides
     bo_status s = ensure_device_public();
     if (s != BO_SUCCESS) return s;
-    if (!lp || lp->m < 0 || lp->n < 0 || lp->nnz < 0) return BO_INVALID_ARGUMENT;
-    if ((lp->m > 0 && (!lp->row_ptr || !lp->b || !lp->is_eq)) || (lp->n > 0 && (!lp->c || !lp->lb || !lp->ub)) ||
-        (lp->nnz > 0 && (!lp->col_idx || !lp->val)))
-        return BO_INVALID_ARGUMENT;
-    if (lp->nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;  // int32 offsets with unrolled strides
-    if (lp->m > 0 && (lp->row_ptr[0] != 0 || lp->row_ptr[lp->m] != lp->nnz)) return BO_INVALID_ARGUMENT;
-    if (lp->m == 0 && lp->nnz != 0) return BO_INVALID_ARGUMENT;
-    for (int i = 0; i < lp->m; ++i)
-        if (lp->row_ptr[i + 1] < lp->row_ptr[i]) return BO_INVALID_ARGUMENT;
-    for (long long k = 0; k < lp->nnz; ++k)
-        if (lp->col_idx[k] < 0 || lp->col_idx[k] >= lp->n) return BO_INVALID_ARGUMENT;
     bo_pdlp_params p;
     if (params) p = *params;
     else bo_pdlp_default_params(&p);

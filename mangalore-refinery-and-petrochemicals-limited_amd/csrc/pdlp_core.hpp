@@ -12,6 +12,7 @@ namespace bo {
 struct PdlpState {
     double eta, omega, tau, sigma;
     double b_norm, c_norm, eps;
+    double eps_obj;     // > 0: also stop only when |c'x - dual obj| <= eps_obj (1 + |c'x|)
     double kkt_last_restart, kkt_prev_candidate;
     // [0] current iterate z, [1] candidate T(z)
     double rel_kkt[2], rel_primal[2], rel_dual[2], rel_gap[2], pobj[2], dobj[2], kkt_w[2];
@@ -155,7 +156,10 @@ __device__ inline void init_state_core(PdlpState* st, double pw_u2, double c2, d
 //   [0,1] c'x  [2,3] ||primal violation||^2  [4,5] the same, scaled  [6,7] b'y
 //   [8,9] ||dual residual||^2  [10,11] the same, scaled  [12,13] sum l plam+ + u plam-
 // for the current iterate [even] and the candidate T(z) [odd]; then the
-// stopping test, the stall test (no halving of the best rel-KKT within
+// stopping test (rel-KKT <= eps and, when eps_obj > 0, the objective gap
+// |c'x - dual obj| <= eps_obj (1 + |c'x|): with x and y feasible to the KKT
+// tolerance the optimum lies between the two objectives, so c'x is then
+// within eps_obj of it relative to 1 + |c'x|), the stall test (no halving of the best rel-KKT within
 // max(stall_window, iter / 2) iterations: restart intervals grow with the
 // iteration count, and so do plateaus of a converging run), the KKT restart
 // rule (beta 0.2 / 0.8 / 0.36) and the bookkeeping.
@@ -187,7 +191,8 @@ __device__ inline void decide_core(PdlpState* st, const double* q) {
         st->status = 2;
         return;
     }
-    if (st->rel_kkt[best] <= st->eps) {
+    if (st->rel_kkt[best] <= st->eps &&
+        (st->eps_obj <= 0.0 || fabs(st->pobj[best] - st->dobj[best]) <= st->eps_obj * (1.0 + fabs(st->pobj[best])))) {
         st->done = 1;
         st->status = 0;
         return;

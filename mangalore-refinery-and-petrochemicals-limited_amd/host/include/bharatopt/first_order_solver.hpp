@@ -40,6 +40,7 @@ struct FirstOrderSolverOptions {
     // --- additions ---
     FirstOrderEngine engine{FirstOrderEngine::REFERENCE_PDHG};
     real_t relative_kkt_tolerance{1e-6};  // PDLP engine stopping tolerance
+    real_t objective_gap_tolerance{0.0};  // PDLP engine: > 0 also requires |c'x - dual obj| <= this (1 + |c'x|)
 };
 
 struct FirstOrderSolverStats {
@@ -73,9 +74,8 @@ public:
     virtual FirstOrderSolverResult solve(const LPModel& model) = 0;
 };
 
-// Declared for source compatibility.  The CPU loop stays in the host
-// application (reference src/first_order_solver.cpp:14-290); this library
-// implements the GPU path only, so solve() reports NUMERICAL_FAILURE.
+// The reference's PDHG loop on the host, FP64, bit-equal to the reference
+// (src/first_order_solver.cpp:14-290); the default backend.
 class CPUFirstOrderSolver : public IFirstOrderLPSolver {
 public:
     explicit CPUFirstOrderSolver(FirstOrderSolverOptions options = {});
@@ -87,6 +87,7 @@ private:
 };
 
 // PDHG on the MI355X: validate -> presolve -> standard form -> GPU -> postsolve.
+// Without a GPU it reports NUMERICAL_FAILURE (no host fallback).
 class GPUFirstOrderSolver : public IFirstOrderLPSolver {
 public:
     explicit GPUFirstOrderSolver(FirstOrderSolverOptions options = {});

@@ -75,9 +75,11 @@ extern "C" bo_status bo_first_order_solve(const bo_model* fm, const bo_fo_option
         opts.dual_step_size = o->dual_step_size;
         opts.adaptive_step_size = o->adaptive_step_size != 0;
         opts.check_frequency = static_cast<size_t>(o->check_frequency);
-        opts.backend_type = FirstOrderBackendType::GPU_FIRST_ORDER;
+        opts.backend_type = o->cpu_backend == 1 ? FirstOrderBackendType::CPU_FIRST_ORDER
+                                                : FirstOrderBackendType::GPU_FIRST_ORDER;
         opts.engine = o->engine == 1 ? FirstOrderEngine::PDLP : FirstOrderEngine::REFERENCE_PDHG;
         if (o->eps_rel_kkt > 0.0) opts.relative_kkt_tolerance = o->eps_rel_kkt;
+        if (o->eps_objective > 0.0) opts.objective_gap_tolerance = o->eps_objective;
         FirstOrderSolverResult res = FirstOrderLPSolver(opts).solve(model);
         r->status = static_cast<int>(res.status);
         r->iterations = static_cast<long long>(res.stats.iterations);
@@ -98,7 +100,8 @@ extern "C" bo_status bo_first_order_solve(const bo_model* fm, const bo_fo_option
             std::memcpy(primal_out, res.primal_solution.data(), res.primal_solution.size() * sizeof(double));
         if (dual_out && !res.dual_solution.empty())
             std::memcpy(dual_out, res.dual_solution.data(), res.dual_solution.size() * sizeof(double));
-        if (!GpuBackend::instance().is_available()) return BO_GPU_UNAVAILABLE;
+        if (opts.backend_type == FirstOrderBackendType::GPU_FIRST_ORDER && !GpuBackend::instance().is_available())
+            return BO_GPU_UNAVAILABLE;
         return BO_SUCCESS;
     } catch (const std::exception& e) {
         set_message(r, e.what());

@@ -1,12 +1,23 @@
-// first_order_solver.cpp -- GPUFirstOrderSolver: the host pipeline around the
-// GPU PDHG engines.  Stage order, messages and result fields follow the
-// reference's CPUFirstOrderSolver::solve (src/first_order_solver.cpp:14-290):
-// validation gate, presolve, bound-only closed form when no rows or columns
-// remain, standard form (<= rows, GE rows negated, EQ rows flagged), the
-// PDHG loop (on the GPU), postsolve, objective on the original model.
+// first_order_solver.cpp -- the reference's first-order solver classes
+// (src/first_order_solver.cpp:14-319) with two back ends:
+//
+//   * CPUFirstOrderSolver runs the reference's PDHG loop on the host, in
+//     FP64, in the reference's operation order: its results are bit-equal to
+//     the reference's for the same model (the default backend of
+//     FirstOrderSolverOptions, so default-constructed solvers keep the
+//     reference's behaviour);
+//   * GPUFirstOrderSolver runs the loop on the MI355X through the C ABI
+//     (bo_pdhg_ref_solve, bit-equal to the same loop, or bo_pdlp_*).
+//
+// Both share the stages around the loop, in the reference's order and with
+// its messages and result fields: validation gate, presolve, bound-only
+// closed form when no rows or columns remain, standard form (<= rows, GE
+// rows negated, EQ rows flagged), the loop, postsolve, objective on the
+// original model.
 //
 // Divergences, all deliberate:
-//   * no GPU -> NUMERICAL_FAILURE with a message, never a CPU fallback;
+//   * GPUFirstOrderSolver without a GPU returns NUMERICAL_FAILURE with a
+//     message; it never falls back to the host loop;
 //   * RANGED rows become two <= rows (the reference keeps only "<= rhs");
 //     their dual is reported as y_upper - y_lower;
 //   * engine PDLP stops on the FP64 relative KKT error instead of the
@@ -17,6 +28,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <limits>
 
 namespace bharatopt {
 
@@ -161,31 +174,129 @@ std::string device_error(bo_status s) {
     return std::string("GPU execution failed (status ") + std::to_string(static_cast<int>(s)) + "): " + (e ? e : "");
 }
 
-}  // namespace
+// What the loop stage hands back to the shared pipeline.
+enum class LoopEnd { CONVERGED, NOT_CONVERGED, EXPLODED, FAILED };
 
-CPUFirstOrderSolver::CPUFirstOrderSolver(FirstOrderSolverOptions options) : options_(options) {}
+// The reference's PDHG loop (src/first_order_solver.cpp:103-279) on the host:
+// diagonal Pock-Chambolle steps, A'y as a row-order scatter, the primal step
+// with the ergodic sum, 2x - x_prev, A x_hat as row dot products, the dual
+// step, and every check_freq iterations the violation of x and of the
+// clamped average, the explosion test and the feasibility + stagnation stop.
+// Every product and sum is one FP64 operation in the reference's order (the
+// file is built without contraction), so the results are the reference's
+// bits.  x_out receives the chosen iterate (x, or the average when it
+// violates less), y the last dual iterate.
+LoopEnd host_reference_loop(const StandardForm& sf, const FirstOrderSolverOptions& o, std::vector<real_t>& x_out,
+                            std::vector<real_t>& y, FirstOrderSolverResult& result) {
+    const size_t m = static_cast<size_t>(sf.m), n = static_cast<size_t>(sf.n);
+    const std::vector<index_t>& rp = sf.A.row_offsets();
+    const std::vector<index_t>& ci = sf.A.col_indices();
+    const std::vector<real_t>& av = sf.A.values();
 
-FirstOrderSolverResult CPUFirstOrderSolver::solve(const LPModel&) {
-    FirstOrderSolverResult r;
-    r.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
-    r.message = "CPUFirstOrderSolver is not part of the MI355X library; use backend GPU_FIRST_ORDER.";
-    return r;
-}
-
-GPUFirstOrderSolver::GPUFirstOrderSolver(FirstOrderSolverOptions options) : options_(options) {}
-
-FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
-    FirstOrderSolverResult result;
-    const auto t0 = clk::now();
-
-    GpuBackend& gpu = GpuBackend::instance();
-    gpu.initialize();
-    if (!gpu.is_available()) {
-        result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
-        result.message = "GPU unavailable: " + gpu.device_info().status_message;
-        return result;
+    std::vector<real_t> col_abs(n, 0.0), row_abs(m, 0.0);
+    for (size_t i = 0; i < m; ++i)
+        for (index_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const real_t a = std::abs(av[static_cast<size_t>(k)]);
+            row_abs[i] += a;
+            col_abs[static_cast<size_t>(ci[static_cast<size_t>(k)])] += a;
+        }
+    std::vector<real_t> tau(n, o.primal_step_size), sigma(m, o.dual_step_size);
+    if (o.adaptive_step_size) {
+        for (size_t j = 0; j < n; ++j) tau[j] = col_abs[j] > 1e-12 ? 0.95 / col_abs[j] : 1.0;
+        for (size_t i = 0; i < m; ++i) sigma[i] = row_abs[i] > 1e-12 ? 0.95 / row_abs[i] : 1.0;
     }
 
+    std::vector<real_t> x(n);
+    for (size_t j = 0; j < n; ++j) {
+        const real_t l = sf.lb[j], u = sf.ub[j];
+        real_t start = 0.0;
+        if (l > -BHARATOPT_INFINITY && u < BHARATOPT_INFINITY) start = 0.5 * (l + u);
+        else if (l > -BHARATOPT_INFINITY) start = l;
+        else if (u < BHARATOPT_INFINITY) start = u;
+        x[j] = clamp(start, l, u);
+    }
+    y.assign(m, 0.0);
+    std::vector<real_t> x_prev = x, x_hat(n, 0.0), aty(n, 0.0), ax(m, 0.0), x_sum(n, 0.0), x_avg(n, 0.0);
+
+    // max over rows of (a_i x - b_i)+ (|a_i x - b_i| on equality rows)
+    auto row_violation = [&](const std::vector<real_t>& v) {
+        sf.A.multiply(v.data(), ax.data());
+        real_t worst = 0.0;
+        for (size_t i = 0; i < m; ++i) {
+            const real_t r = ax[i] - sf.b[i];
+            worst = std::max(worst, sf.is_eq[i] ? std::abs(r) : std::max(0.0, r));
+        }
+        return worst;
+    };
+    auto average = [&](size_t count) {
+        for (size_t j = 0; j < n; ++j)
+            x_avg[j] = clamp(x_sum[j] / static_cast<real_t>(count), sf.lb[j], sf.ub[j]);
+    };
+
+    const size_t check_freq = std::max<size_t>(1, std::min<size_t>(o.check_frequency, 100));
+    size_t iter = 0, count = 0;
+    bool converged = false;
+    while (iter < o.max_iterations) {
+        ++iter;
+        sf.A.multiply_transpose(y.data(), aty.data());
+        real_t dx = 0.0;
+        for (size_t j = 0; j < n; ++j) {
+            const real_t moved = clamp(x[j] - tau[j] * (sf.c[j] + aty[j]), sf.lb[j], sf.ub[j]);
+            dx = std::max(dx, std::abs(moved - x[j]));
+            x[j] = moved;
+            x_sum[j] += moved;
+        }
+        ++count;
+        for (size_t j = 0; j < n; ++j) x_hat[j] = 2.0 * x[j] - x_prev[j];
+        sf.A.multiply(x_hat.data(), ax.data());
+        real_t dy = 0.0;
+        for (size_t i = 0; i < m; ++i) {
+            const real_t step = y[i] + sigma[i] * (ax[i] - sf.b[i]);
+            const real_t moved = sf.is_eq[i] ? step : std::max(0.0, step);
+            dy = std::max(dy, std::abs(moved - y[i]));
+            y[i] = moved;
+        }
+        x_prev = x;
+
+        if (iter % check_freq != 0 && iter != o.max_iterations) continue;
+        const real_t viol_x = row_violation(x);
+        real_t viol_bounds = 0.0;
+        for (size_t j = 0; j < n; ++j) {
+            if (x[j] < sf.lb[j] - 1e-12) viol_bounds = std::max(viol_bounds, sf.lb[j] - x[j]);
+            if (x[j] > sf.ub[j] + 1e-12) viol_bounds = std::max(viol_bounds, x[j] - sf.ub[j]);
+        }
+        average(count);
+        const real_t best = std::min(viol_x, row_violation(x_avg));
+        result.stats.constraint_violation = best;
+        result.stats.bound_violation = viol_bounds;
+        if (std::isnan(best) || best > 1e15) return LoopEnd::EXPLODED;
+        if (best <= o.feasibility_tolerance && viol_bounds <= o.feasibility_tolerance && dx <= 1e-7 && dy <= 1e-7 &&
+            iter >= 100) {
+            converged = true;
+            break;
+        }
+    }
+    result.stats.iterations = iter;
+
+    if (count > 0) average(count);
+    else x_avg = x;
+    const bool use_average = !converged && row_violation(x_avg) <= result.stats.constraint_violation;
+    x_out = use_average ? x_avg : x;
+    return converged ? LoopEnd::CONVERGED : LoopEnd::NOT_CONVERGED;
+}
+
+// A loop stage fills x, y and the result's statistics.  A stage that has a
+// dual objective (PDLP) stores it in the <=-form minimisation sense in
+// `dual_min`; the pipeline converts it to the model's sense.
+using LoopStage = std::function<LoopEnd(const StandardForm&, std::vector<real_t>& x, std::vector<real_t>& y,
+                                        FirstOrderSolverResult&, real_t& dual_min, std::string& failure)>;
+
+// validate -> presolve -> bound-only form or standard form -> loop ->
+// postsolve -> objective, as CPUFirstOrderSolver::solve orders them
+// (src/first_order_solver.cpp:14-290).  solve_time_ms covers everything up
+// to the end of the loop (the reference stops its clock before postsolve).
+FirstOrderSolverResult run_pipeline(const LPModel& model, clk::time_point t0, const LoopStage& loop) {
+    FirstOrderSolverResult result;
     if (!ModelValidator().validate(model).is_valid()) {
         result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
         result.message = "Model validation failed before First-Order LP Solver.";
@@ -221,94 +332,20 @@ FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
     }
 
     const StandardForm sf = build_standard_form(work);
-    const bo_lp lp = as_bo_lp(sf);
     std::vector<real_t> x(static_cast<size_t>(sf.n)), y(static_cast<size_t>(sf.m));
-    bool converged = false;
-
-    if (options_.engine == FirstOrderEngine::REFERENCE_PDHG) {
-        bo_ref_options o{};
-        o.max_iterations = static_cast<long long>(options_.max_iterations);
-        o.feasibility_tolerance = options_.feasibility_tolerance;
-        o.primal_step_size = options_.primal_step_size;
-        o.dual_step_size = options_.dual_step_size;
-        o.adaptive_step_size = options_.adaptive_step_size ? 1 : 0;
-        o.check_frequency = static_cast<long long>(options_.check_frequency);
-        bo_ref_result r{};
-        const bo_status s = bo_pdhg_ref_solve(&lp, &o, x.data(), y.data(), &r);
-        if (s != BO_SUCCESS) {
-            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
-            result.message = device_error(s);
-            return result;
-        }
-        result.stats.constraint_violation = r.constraint_violation;
-        result.stats.bound_violation = r.bound_violation;
-        result.stats.kernel_time_ms = r.kernel_time_ms;
-        result.stats.transfer_time_ms = r.transfer_time_ms;
-        if (r.status == 2) {
-            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
-            result.message = "Numerical failure: exploded iterate detected during PDHG iterations.";
-            return result;
-        }
-        converged = r.status == 0;
-        result.stats.iterations = static_cast<size_t>(r.iterations);
-    } else {
-        // Precision: LPs up to 2^18 nonzeros run with FP64 iterates (they run
-        // in one workgroup, where FP64 costs about the same, and badly scaled
-        // models such as Netlib's kb2 or share2b do not converge with FP32
-        // iterates); larger ones run FP32 iterates for at most 50,000
-        // iterations and, if that does not converge, go on in FP64 from
-        // where it stopped, for the iterations left.
-        bo_pdlp_params p;
-        bo_pdlp_default_params(&p);
-        p.max_iterations = static_cast<long long>(options_.max_iterations);
-        p.eps_rel_kkt = options_.relative_kkt_tolerance;
-        p.fp64_iterates = lp.nnz <= (1LL << 18) ? 1 : 0;
-        const long long budget = p.max_iterations;
-        constexpr long long kFp32Iterations = 50000;
-        if (!p.fp64_iterates && budget > kFp32Iterations) p.max_iterations = kFp32Iterations;
-        bo_pdlp* h = nullptr;
-        bo_status s = bo_pdlp_create(&lp, &p, &h);
-        bo_pdlp_stats st{};
-        if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
-        if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
-        bo_pdlp_destroy(h);
-        if (s == BO_SUCCESS && !p.fp64_iterates && (st.status == 1 || st.status == 4) && st.iterations < budget) {
-            const long long used = st.iterations;
-            p.fp64_iterates = 1;
-            p.max_iterations = budget - used;
-            h = nullptr;
-            s = bo_pdlp_create(&lp, &p, &h);
-            if (s == BO_SUCCESS) s = bo_pdlp_set_warm_start(h, x.data(), y.data());
-            if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
-            if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
-            bo_pdlp_destroy(h);
-            st.iterations += used;
-        }
-        if (s != BO_SUCCESS) {
-            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
-            result.message = device_error(s);
-            return result;
-        }
-        if (st.status == 2) {
-            result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
-            result.message = "Numerical failure: non-finite KKT error during PDLP iterations.";
-            return result;
-        }
-        converged = st.status == 0;
-        result.stats.iterations = static_cast<size_t>(st.iterations);
-        result.stats.relative_kkt = st.rel_kkt;
-        result.stats.restarts = static_cast<size_t>(st.restarts);
-        result.stats.dual_objective = st.dual_obj * sense_mult + work.obj_offset();
-        result.stats.primal_residual = st.rel_primal;
-        result.stats.dual_residual = st.rel_dual;
-        result.stats.kernel_time_ms = st.solve_ms;
-        real_t rv = 0.0, bv = 0.0;
-        max_violation(sf, x, rv, bv);
-        result.stats.constraint_violation = rv;
-        result.stats.bound_violation = bv;
+    std::string failure;
+    real_t dual_min = std::numeric_limits<real_t>::quiet_NaN();
+    const LoopEnd end = loop(sf, x, y, result, dual_min, failure);
+    if (end == LoopEnd::EXPLODED || end == LoopEnd::FAILED) {
+        result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+        result.message = end == LoopEnd::EXPLODED
+                             ? "Numerical failure: exploded iterate detected during PDHG iterations."
+                             : failure;
+        return result;
     }
+    if (!std::isnan(dual_min)) result.stats.dual_objective = dual_min * sense_mult + work.obj_offset();
     result.stats.solve_time_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
-    if (converged) {
+    if (end == LoopEnd::CONVERGED) {
         result.status = FirstOrderSolverStatus::OPTIMAL;
         result.message = "PDHG/PDLP-style first-order solver converged to OPTIMAL solution.";
     } else {
@@ -320,6 +357,120 @@ FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
     result.objective_value = original_objective(model, result.primal_solution);
     result.stats.primal_objective = result.objective_value;
     return result;
+}
+
+// The GPU loop stage: the reference's algorithm (bit-equal) or PDLP.
+LoopEnd gpu_loop(const FirstOrderSolverOptions& options, const StandardForm& sf, std::vector<real_t>& x,
+                 std::vector<real_t>& y, FirstOrderSolverResult& result, real_t& dual_min, std::string& failure) {
+    const bo_lp lp = as_bo_lp(sf);
+    if (options.engine == FirstOrderEngine::REFERENCE_PDHG) {
+        bo_ref_options o{};
+        o.max_iterations = static_cast<long long>(options.max_iterations);
+        o.feasibility_tolerance = options.feasibility_tolerance;
+        o.primal_step_size = options.primal_step_size;
+        o.dual_step_size = options.dual_step_size;
+        o.adaptive_step_size = options.adaptive_step_size ? 1 : 0;
+        o.check_frequency = static_cast<long long>(options.check_frequency);
+        bo_ref_result r{};
+        const bo_status s = bo_pdhg_ref_solve(&lp, &o, x.data(), y.data(), &r);
+        if (s != BO_SUCCESS) {
+            failure = device_error(s);
+            return LoopEnd::FAILED;
+        }
+        result.stats.constraint_violation = r.constraint_violation;
+        result.stats.bound_violation = r.bound_violation;
+        result.stats.kernel_time_ms = r.kernel_time_ms;
+        result.stats.transfer_time_ms = r.transfer_time_ms;
+        if (r.status == 2) return LoopEnd::EXPLODED;
+        result.stats.iterations = static_cast<size_t>(r.iterations);
+        return r.status == 0 ? LoopEnd::CONVERGED : LoopEnd::NOT_CONVERGED;
+    }
+    // Precision: LPs up to 2^18 nonzeros run with FP64 iterates (they run in
+    // one workgroup, where FP64 costs about the same, and badly scaled models
+    // such as Netlib's kb2 or share2b do not converge with FP32 iterates);
+    // larger ones run FP32 iterates for at most 50,000 iterations and, if that
+    // does not converge, go on in FP64 from where it stopped, for the
+    // iterations left.
+    bo_pdlp_params p;
+    bo_pdlp_default_params(&p);
+    p.max_iterations = static_cast<long long>(options.max_iterations);
+    p.eps_rel_kkt = options.relative_kkt_tolerance;
+    p.eps_objective = options.objective_gap_tolerance;
+    p.fp64_iterates = lp.nnz <= (1LL << 18) ? 1 : 0;
+    const long long budget = p.max_iterations;
+    constexpr long long kFp32Iterations = 50000;
+    if (!p.fp64_iterates && budget > kFp32Iterations) p.max_iterations = kFp32Iterations;
+    bo_pdlp* h = nullptr;
+    bo_status s = bo_pdlp_create(&lp, &p, &h);
+    bo_pdlp_stats st{};
+    if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
+    if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
+    bo_pdlp_destroy(h);
+    if (s == BO_SUCCESS && !p.fp64_iterates && (st.status == 1 || st.status == 4) && st.iterations < budget) {
+        const long long used = st.iterations;
+        p.fp64_iterates = 1;
+        p.max_iterations = budget - used;
+        h = nullptr;
+        s = bo_pdlp_create(&lp, &p, &h);
+        if (s == BO_SUCCESS) s = bo_pdlp_set_warm_start(h, x.data(), y.data());
+        if (s == BO_SUCCESS) s = bo_pdlp_solve(h, &st);
+        if (s == BO_SUCCESS) s = bo_pdlp_get_solution(h, x.data(), y.data());
+        bo_pdlp_destroy(h);
+        st.iterations += used;
+    }
+    if (s != BO_SUCCESS) {
+        failure = device_error(s);
+        return LoopEnd::FAILED;
+    }
+    if (st.status == 2) {
+        failure = "Numerical failure: non-finite KKT error during PDLP iterations.";
+        return LoopEnd::FAILED;
+    }
+    result.stats.iterations = static_cast<size_t>(st.iterations);
+    result.stats.relative_kkt = st.rel_kkt;
+    result.stats.restarts = static_cast<size_t>(st.restarts);
+    dual_min = st.dual_obj;
+    result.stats.primal_residual = st.rel_primal;
+    result.stats.dual_residual = st.rel_dual;
+    result.stats.kernel_time_ms = st.solve_ms;
+    real_t rv = 0.0, bv = 0.0;
+    max_violation(sf, x, rv, bv);
+    result.stats.constraint_violation = rv;
+    result.stats.bound_violation = bv;
+    return st.status == 0 ? LoopEnd::CONVERGED : LoopEnd::NOT_CONVERGED;
+}
+
+}  // namespace
+
+CPUFirstOrderSolver::CPUFirstOrderSolver(FirstOrderSolverOptions options) : options_(options) {}
+
+FirstOrderSolverResult CPUFirstOrderSolver::solve(const LPModel& model) {
+    const FirstOrderSolverOptions& o = options_;
+    return run_pipeline(model, clk::now(),
+                        [&o](const StandardForm& sf, std::vector<real_t>& x, std::vector<real_t>& y,
+                             FirstOrderSolverResult& r, real_t&, std::string&) {
+                            return host_reference_loop(sf, o, x, y, r);
+                        });
+}
+
+GPUFirstOrderSolver::GPUFirstOrderSolver(FirstOrderSolverOptions options) : options_(options) {}
+
+FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
+    const auto t0 = clk::now();
+    GpuBackend& gpu = GpuBackend::instance();
+    gpu.initialize();
+    if (!gpu.is_available()) {
+        FirstOrderSolverResult result;
+        result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
+        result.message = "GPU unavailable: " + gpu.device_info().status_message;
+        return result;
+    }
+    const FirstOrderSolverOptions& o = options_;
+    return run_pipeline(model, t0,
+                        [&o](const StandardForm& sf, std::vector<real_t>& x, std::vector<real_t>& y,
+                             FirstOrderSolverResult& r, real_t& dual_min, std::string& failure) {
+                            return gpu_loop(o, sf, x, y, r, dual_min, failure);
+                        });
 }
 
 FirstOrderLPSolver::FirstOrderLPSolver(FirstOrderSolverOptions options) : options_(options) {}

@@ -106,6 +106,7 @@ typedef struct {
     int ruiz_iterations;    /* 10 */
     int pock_chambolle;     /* 1 */
     int threads;            /* OpenMP threads; <= 0: all */
+    double eps_objective;   /* > 0: also stop only when |c'x - dual obj| <= this (1 + |c'x|) */
 } orc_pdlp_opts;
 
 typedef struct {

@@ -45,7 +45,7 @@ static void seg_spmv(int segs, const int* ptr, const int* idx, const double* val
 }
 
 typedef struct {
-    double eta, omega, tau, sigma, b_norm, c_norm, eps;
+    double eta, omega, tau, sigma, b_norm, c_norm, eps, eps_obj;
     double kkt_last_restart, kkt_prev_candidate;
     double rel_kkt[2], rel_primal[2], rel_dual[2], rel_gap[2], pobj[2], dobj[2], kkt_w[2];
     long long iter, iter_last_restart, max_iter;
@@ -75,7 +75,12 @@ static void decide(state_t* st, const double* q) {
     const int best = st->rel_kkt[1] < st->rel_kkt[0] ? 1 : 0;
     st->best = best;
     if (bad) { st->done = 1; st->status = 2; return; }
-    if (st->rel_kkt[best] <= st->eps) { st->done = 1; st->status = 0; return; }
+    if (st->rel_kkt[best] <= st->eps &&
+        (st->eps_obj <= 0.0 || fabs(st->pobj[best] - st->dobj[best]) <= st->eps_obj * (1.0 + fabs(st->pobj[best])))) {
+        st->done = 1;
+        st->status = 0;
+        return;
+    }
     if (st->iter >= st->max_iter) { st->done = 1; st->status = 1; return; }
     if (st->iter == 0) {
         st->kkt_last_restart = st->kkt_w[0];
@@ -138,6 +143,7 @@ int orc_pdlp_omp(int m, int n, const int* rp, const int* ci, const double* v, co
     state_t st;
     memset(&st, 0, sizeof(st));
     st.eps = o->eps_rel_kkt;
+    st.eps_obj = o->eps_objective;
     st.max_iter = o->max_iterations;
     double b2 = 0.0, c2 = 0.0;
     for (int i = 0; i < m; ++i) b2 += b[i] * b[i];

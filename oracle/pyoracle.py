@@ -50,7 +50,8 @@ def oracle_lib():
 
 class OrcPdlpOpts(C.Structure):
     _fields_ = [("max_iterations", C.c_longlong), ("eps_rel_kkt", C.c_double), ("check_every", C.c_int),
-                ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("threads", C.c_int)]
+                ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("threads", C.c_int),
+                ("eps_objective", C.c_double)]
 
 
 class OrcPdlpOut(C.Structure):
@@ -78,12 +79,12 @@ def pdlp_omp_lib():
 
 
 def pdlp_omp(lp, max_iterations=100000, eps_rel_kkt=1e-6, check_every=64, ruiz_iterations=10,
-             pock_chambolle=True, threads=0):
+             pock_chambolle=True, threads=0, eps_objective=0.0):
     """The restarted Halpern PDHG in FP64 on `threads` host cores (0: all).
     Returns (stats dict, x, y)."""
     lib = pdlp_omp_lib()
     o = OrcPdlpOpts(max_iterations, eps_rel_kkt, check_every, ruiz_iterations, 1 if pock_chambolle else 0,
-                    threads)
+                    threads, eps_objective)
     out = OrcPdlpOut()
     x = np.zeros(max(lp.n, 1))
     y = np.zeros(max(lp.m, 1))

@@ -26,9 +26,13 @@ def po():
     return pyoracle
 
 
-@pytest.fixture(scope="session")
-def ref(po):
-    """The reference build (oracle/_ref), skipping when it is not present."""
+@pytest.fixture
+def ref(po, request):
+    """The reference build (oracle/_ref), skipping when it is not present.
+    CPU tests only: GPU tests compare with the committed golden fixtures
+    written from it, so no GPU run loads the reference build."""
+    if request.node.get_closest_marker("gpu") is not None:
+        pytest.fail("a GPU test asked for the reference build; use the golden fixtures")
     if po.ref_lib() is None:
         pytest.skip("oracle/_ref not built (reference sources absent)")
     return po

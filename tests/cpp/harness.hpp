@@ -1,4 +1,5 @@
 // harness.hpp -- minimal self-registering test harness (TEST_CASE / EXPECT_*).
+// A case is a scenario (run once per backend pass), GPU-only or host-only.
 #pragma once
 #include <cmath>
 #include <cstdio>
@@ -7,23 +8,28 @@
 #include <vector>
 
 namespace th {
+enum Kind { SCENARIO, GPU_ONLY, HOST_ONLY };
 struct Case {
     const char* name;
     std::function<void(int&)> fn;
+    Kind kind;
 };
 inline std::vector<Case>& registry() {
     static std::vector<Case> r;
     return r;
 }
 struct Reg {
-    Reg(const char* n, std::function<void(int&)> f) { registry().push_back({n, std::move(f)}); }
+    Reg(const char* n, std::function<void(int&)> f, Kind k) { registry().push_back({n, std::move(f), k}); }
 };
 }  // namespace th
 
-#define TEST_CASE(name)                                    \
+#define TH_CASE(name, kind)                                \
     static void name(int& th_fail);                        \
-    static th::Reg th_reg_##name(#name, name);             \
+    static th::Reg th_reg_##name(#name, name, kind);       \
     static void name(int& th_fail)
+#define TEST_CASE(name) TH_CASE(name, th::SCENARIO)
+#define TEST_CASE_GPU(name) TH_CASE(name, th::GPU_ONLY)
+#define TEST_CASE_HOST(name) TH_CASE(name, th::HOST_ONLY)
 
 #define EXPECT_TRUE(c)                                                              \
     do {                                                                            \
@@ -43,17 +49,21 @@ struct Reg {
         }                                                                                            \
     } while (0)
 
-inline int th_run_all(const char* filter) {
-    int failed = 0, run = 0;
+// Runs the cases whose kind is in `kinds` (and whose name contains
+// `filter`), each reported as "PASS/FAIL [tag] name"; returns the failures.
+inline int th_run(const char* filter, std::vector<th::Kind> kinds, const char* tag, int& run) {
+    int failed = 0;
     for (auto& c : th::registry()) {
+        bool want = false;
+        for (th::Kind k : kinds) want = want || c.kind == k;
+        if (!want) continue;
         if (filter && std::string(c.name).find(filter) == std::string::npos) continue;
         int f = 0;
         c.fn(f);
         ++run;
-        std::printf("%s %s\n", f ? "FAIL" : "PASS", c.name);
+        std::printf("%s [%s] %s\n", f ? "FAIL" : "PASS", tag, c.name);
         std::fflush(stdout);
         failed += f ? 1 : 0;
     }
-    std::printf("%d/%d passed\n", run - failed, run);
-    return failed ? 1 : 0;
+    return failed;
 }

@@ -1,14 +1,18 @@
 // test_dropin.cpp -- the scenarios of the reference's tests/test_first_order_solver.cpp
 // (FirstOrder_01..25) and tests/test_gpu_foundation.cpp (GpuFoundation_01..25),
-// run through the drop-in classes on the GPU.  Models and expected values are
-// the reference's; every solve selects backend GPU_FIRST_ORDER, the path this
-// library implements.
+// run through the drop-in classes.  Models and expected values are the
+// reference's.  Every FirstOrder_* scenario runs twice: first exactly as the
+// reference's tests run it -- a default-constructed FirstOrderLPSolver, whose
+// default backend CPU_FIRST_ORDER is the host loop -- and then, when a GPU is
+// visible, with backend GPU_FIRST_ORDER forced.  Without a GPU only the first
+// pass and the host-only cases run.
 #include <bharatopt/first_order_solver.hpp>
 #include <bharatopt/gpu_backend.hpp>
 #include <bharatopt/solution_verifier.hpp>
 #include <bharatopt/sparse_matrix.hpp>
 
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "harness.hpp"
@@ -16,19 +20,31 @@
 using namespace bharatopt;
 
 namespace {
+bool g_gpu_pass = false;  // the second pass: GPU_FIRST_ORDER forced
+
 FirstOrderSolverOptions gpu_opts(FirstOrderEngine e = FirstOrderEngine::REFERENCE_PDHG) {
     FirstOrderSolverOptions o;
     o.backend_type = FirstOrderBackendType::GPU_FIRST_ORDER;
     o.engine = e;
     return o;
 }
-FirstOrderSolverResult solve(const LPModel& m, FirstOrderSolverOptions o = gpu_opts()) {
-    return FirstOrderLPSolver(o).solve(m);
+// The options a scenario starts from: the defaults, or the GPU backend.
+FirstOrderSolverOptions pass_opts() { return g_gpu_pass ? gpu_opts() : FirstOrderSolverOptions{}; }
+FirstOrderSolverResult solve(const LPModel& m) {
+    if (!g_gpu_pass) {
+        FirstOrderLPSolver solver;  // as the reference's tests construct it
+        return solver.solve(m);
+    }
+    return FirstOrderLPSolver(gpu_opts()).solve(m);
 }
+FirstOrderSolverResult solve(const LPModel& m, FirstOrderSolverOptions o) { return FirstOrderLPSolver(o).solve(m); }
 const double INF = BHARATOPT_INFINITY;
 // primal value j, NaN when the solve returned no solution
 double xs(const FirstOrderSolverResult& r, size_t j) {
     return j < r.primal_solution.size() ? r.primal_solution[j] : std::nan("");
+}
+bool same_bits(const std::vector<real_t>& a, const std::vector<real_t>& b) {
+    return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(real_t)) == 0);
 }
 }  // namespace
 
@@ -236,7 +252,7 @@ TEST_CASE(FirstOrder_19_MandatoryHandLP) {
     EXPECT_NEAR(r.objective_value, 64.0 / 3.0, 1e-2);
 }
 
-TEST_CASE(FirstOrder_19b_HandLP_PDLP) {
+TEST_CASE_GPU(FirstOrder_19b_HandLP_PDLP) {
     LPModel m = hand_lp();
     auto r = solve(m, gpu_opts(FirstOrderEngine::PDLP));
     EXPECT_EQ(r.status, FirstOrderSolverStatus::OPTIMAL);
@@ -272,7 +288,7 @@ TEST_CASE(FirstOrder_22_IndependentVerification) {
     EXPECT_TRUE(SolutionVerifier::verify_solution_feasibility(m, r.primal_solution));
 }
 
-TEST_CASE(FirstOrder_23_ReferenceVsPdlpEngines) {
+TEST_CASE_GPU(FirstOrder_23b_ReferenceVsPdlpEngines) {
     LPModel m("cpu_vs_gpu_pdhg");
     m.set_sense(ObjectiveSense::MAXIMIZE);
     index_t x = m.add_variable("x", 0.0, 10.0, 2.0);
@@ -289,7 +305,7 @@ TEST_CASE(FirstOrder_24_NumericalStability) {
     LPModel m("stability_pdhg");
     index_t x = m.add_variable("x", 0.0, 10.0, 1.0);
     m.add_constraint("c1", {{x, 1.0}}, ConstraintSense::GREATER_EQUAL, 2.0);
-    FirstOrderSolverOptions o = gpu_opts();
+    FirstOrderSolverOptions o = pass_opts();
     o.max_iterations = 1000;
     auto r = solve(m, o);
     EXPECT_TRUE(r.status == FirstOrderSolverStatus::OPTIMAL || r.status == FirstOrderSolverStatus::MAX_ITERATIONS);
@@ -301,14 +317,14 @@ TEST_CASE(FirstOrder_25_IterationLimitTest) {
     index_t x = m.add_variable("x", 0.0, 100.0, 5.0);
     index_t y = m.add_variable("y", 0.0, 100.0, 6.0);
     m.add_constraint("c1", {{x, 1.0}, {y, 1.0}}, ConstraintSense::LESS_EQUAL, 50.0);
-    FirstOrderSolverOptions o = gpu_opts();
+    FirstOrderSolverOptions o = pass_opts();
     o.max_iterations = 5;
     auto r = solve(m, o);
     EXPECT_EQ(r.status, FirstOrderSolverStatus::MAX_ITERATIONS);
     EXPECT_EQ(r.stats.iterations, static_cast<size_t>(5));
 }
 
-TEST_CASE(FirstOrder_26_RangedRowHonoured) {
+TEST_CASE_GPU(FirstOrder_26_RangedRowHonoured) {
     // 2 <= x + y <= 4, min x + 2y: the reference treats the row as x + y <= 2
     // (lower value); this library keeps both sides, optimum x = 2, y = 0.
     LPModel m("ranged");
@@ -321,29 +337,85 @@ TEST_CASE(FirstOrder_26_RangedRowHonoured) {
     EXPECT_TRUE(SolutionVerifier::verify_solution_feasibility(m, r.primal_solution, 1e-5));
 }
 
-TEST_CASE(FirstOrder_27_CpuSolverNotShipped) {
-    LPModel m = hand_lp();
-    FirstOrderSolverOptions o;  // CPU backend by default
-    auto r = FirstOrderLPSolver(o).solve(m);
+TEST_CASE_GPU(FirstOrder_23_CPUvsGPUComparison) {
+    // The reference's test compares the objectives to 1e-2; the GPU engine
+    // runs the same loop bit for bit, so the whole results are compared.
+    LPModel m("cpu_vs_gpu_pdhg");
+    m.set_sense(ObjectiveSense::MAXIMIZE);
+    index_t x = m.add_variable("x", 0.0, 10.0, 2.0);
+    index_t y = m.add_variable("y", 0.0, 10.0, 3.0);
+    m.add_constraint("c1", {{x, 1.0}, {y, 1.0}}, ConstraintSense::LESS_EQUAL, 5.0);
+    FirstOrderSolverOptions oc;
+    oc.backend_type = FirstOrderBackendType::CPU_FIRST_ORDER;
+    auto a = FirstOrderLPSolver(oc).solve(m);
+    auto b = solve(m, gpu_opts());
+    EXPECT_EQ(a.status, FirstOrderSolverStatus::OPTIMAL);
+    EXPECT_EQ(b.status, FirstOrderSolverStatus::OPTIMAL);
+    EXPECT_NEAR(a.objective_value, b.objective_value, 1e-2);
+    EXPECT_EQ(a.stats.iterations, b.stats.iterations);
+    EXPECT_TRUE(same_bits(a.primal_solution, b.primal_solution));
+    EXPECT_TRUE(same_bits(a.dual_solution, b.dual_solution));
+}
+
+TEST_CASE_GPU(FirstOrder_27_HostAndGpuLoopsBitEqual) {
+    // A 60 x 150 LP whose run ends on the iteration limit with the ergodic
+    // average chosen or not: the host loop and the GPU loop agree bit for bit
+    // in status, iterations, violations, x and y.
+    LPModel m("bit_equal");
+    std::vector<index_t> v;
+    for (int j = 0; j < 150; ++j)
+        v.push_back(m.add_variable("x" + std::to_string(j), 0.0, 20.0, std::sin(0.7 * j + 0.3)));
+    for (int i = 0; i < 60; ++i) {
+        std::vector<std::pair<index_t, real_t>> t;
+        for (int k = 0; k < 5; ++k) t.emplace_back(v[(7 * i + 11 * k) % 150], std::cos(1.3 * i + 0.9 * k));
+        m.add_constraint("r" + std::to_string(i), t, i % 3 == 0 ? ConstraintSense::GREATER_EQUAL
+                                                                 : ConstraintSense::LESS_EQUAL, 3.0 + (i % 7));
+    }
+    for (size_t iters : {size_t(1), size_t(37), size_t(2000)}) {
+        FirstOrderSolverOptions oc;
+        oc.max_iterations = iters;
+        oc.check_frequency = 13;
+        FirstOrderSolverOptions og = oc;
+        og.backend_type = FirstOrderBackendType::GPU_FIRST_ORDER;
+        auto a = FirstOrderLPSolver(oc).solve(m);
+        auto b = FirstOrderLPSolver(og).solve(m);
+        EXPECT_EQ(a.status, b.status);
+        EXPECT_EQ(a.stats.iterations, b.stats.iterations);
+        EXPECT_TRUE(a.stats.constraint_violation == b.stats.constraint_violation);
+        EXPECT_TRUE(a.stats.bound_violation == b.stats.bound_violation);
+        EXPECT_TRUE(a.objective_value == b.objective_value);
+        EXPECT_TRUE(same_bits(a.primal_solution, b.primal_solution));
+        EXPECT_TRUE(same_bits(a.dual_solution, b.dual_solution));
+    }
+}
+
+TEST_CASE_HOST(FirstOrder_28_GpuSolverWithoutGpuFailsLoudly) {
+    // GPUFirstOrderSolver never falls back to the host loop: without a GPU it
+    // returns NUMERICAL_FAILURE and no solution.
+    if (GpuBackend::instance().initialize() == GPUStatus::SUCCESS) return;  // only meaningful without a GPU
+    LPModel m("no_gpu");
+    index_t x = m.add_variable("x", 0.0, 10.0, 1.0);
+    m.add_constraint("c1", {{x, 1.0}}, ConstraintSense::GREATER_EQUAL, 2.0);
+    auto r = FirstOrderLPSolver(gpu_opts()).solve(m);
     EXPECT_EQ(r.status, FirstOrderSolverStatus::NUMERICAL_FAILURE);
     EXPECT_TRUE(r.primal_solution.empty());
 }
 
 // ---------------------------------------------------------------- GpuFoundation_*
-TEST_CASE(GpuFoundation_01_GpuAvailability) {
+TEST_CASE_GPU(GpuFoundation_01_GpuAvailability) {
     GPUStatus s = GpuBackend::instance().initialize();
     EXPECT_TRUE(s == GPUStatus::SUCCESS);
     EXPECT_TRUE(GpuBackend::instance().is_available());
 }
 
-TEST_CASE(GpuFoundation_03_DeviceMetadata) {
+TEST_CASE_GPU(GpuFoundation_03_DeviceMetadata) {
     GpuBackend::instance().initialize();
     const auto& info = GpuBackend::instance().device_info();
     EXPECT_TRUE(!info.device_name.empty());
     EXPECT_TRUE(info.total_memory_bytes > 0);
 }
 
-TEST_CASE(GpuFoundation_04_DeviceAllocation) {
+TEST_CASE_GPU(GpuFoundation_04_DeviceAllocation) {
     GpuBuffer<real_t> b;
     EXPECT_EQ(b.size(), static_cast<size_t>(0));
     EXPECT_EQ(b.allocate(100), GPUStatus::SUCCESS);
@@ -353,7 +425,7 @@ TEST_CASE(GpuFoundation_04_DeviceAllocation) {
     EXPECT_EQ(b.size(), static_cast<size_t>(0));
 }
 
-TEST_CASE(GpuFoundation_06_RoundTrip) {
+TEST_CASE_GPU(GpuFoundation_06_RoundTrip) {
     std::vector<real_t> in = {10.0, 20.0, 30.0}, out(3, 0.0);
     GpuBuffer<real_t> d;
     EXPECT_EQ(d.copy_host_to_device(in.data(), 3), GPUStatus::SUCCESS);
@@ -376,13 +448,13 @@ static void check_spmv(int& th_fail, COOMatrix coo, std::vector<real_t> x, std::
     EXPECT_TRUE(t.h2d_time_ms >= 0.0 && t.d2h_time_ms >= 0.0 && t.total_gpu_time_ms >= 0.0);
 }
 
-TEST_CASE(GpuFoundation_07_IdentitySpMV) {
+TEST_CASE_GPU(GpuFoundation_07_IdentitySpMV) {
     COOMatrix c(3, 3);
     for (int i = 0; i < 3; ++i) c.add_entry(i, i, 1.0);
     check_spmv(th_fail, c, {5.0, -3.0, 7.0}, {5.0, -3.0, 7.0});
 }
 
-TEST_CASE(GpuFoundation_09_SmallSparseSpMV) {
+TEST_CASE_GPU(GpuFoundation_09_SmallSparseSpMV) {
     COOMatrix c(2, 3);
     c.add_entry(0, 0, 1.0);
     c.add_entry(0, 2, 2.0);
@@ -390,7 +462,7 @@ TEST_CASE(GpuFoundation_09_SmallSparseSpMV) {
     check_spmv(th_fail, c, {4.0, 5.0, 6.0}, {16.0, 15.0});
 }
 
-TEST_CASE(GpuFoundation_10_RectangularSpMV) {
+TEST_CASE_GPU(GpuFoundation_10_RectangularSpMV) {
     COOMatrix c(4, 2);
     c.add_entry(0, 0, 1.0);
     c.add_entry(1, 1, 2.0);
@@ -399,13 +471,13 @@ TEST_CASE(GpuFoundation_10_RectangularSpMV) {
     check_spmv(th_fail, c, {10.0, 20.0}, {10.0, 40.0, 30.0, 80.0});
 }
 
-TEST_CASE(GpuFoundation_12_LargeSparseSpMV) {
+TEST_CASE_GPU(GpuFoundation_12_LargeSparseSpMV) {
     COOMatrix c(1000, 1000);
     for (int i = 0; i < 1000; ++i) c.add_entry(i, i, 2.0);
     check_spmv(th_fail, c, std::vector<real_t>(1000, 3.0), std::vector<real_t>(1000, 6.0));
 }
 
-TEST_CASE(GpuFoundation_19_NumericalTolerance) {
+TEST_CASE_GPU(GpuFoundation_19_NumericalTolerance) {
     COOMatrix c(1, 1);
     c.add_entry(0, 0, 1e-8);
     CSRMatrix a = CSRMatrix::from_coo(c);
@@ -415,7 +487,7 @@ TEST_CASE(GpuFoundation_19_NumericalTolerance) {
     EXPECT_NEAR(y[0], 1e-16, 1e-18);
 }
 
-TEST_CASE(GpuFoundation_13_VectorOps) {
+TEST_CASE_GPU(GpuFoundation_13_VectorOps) {
     GpuBuffer<real_t> a(3), b(3), r(3);
     std::vector<real_t> ha = {1.0, 2.0, 3.0}, hb = {4.0, 5.0, 6.0}, hr(3);
     a.copy_host_to_device(ha.data(), 3);
@@ -448,7 +520,7 @@ TEST_CASE(GpuFoundation_13_VectorOps) {
     EXPECT_NEAR(nrm, 5.0, 1e-12);
 }
 
-TEST_CASE(GpuFoundation_20_InvalidArguments) {
+TEST_CASE_GPU(GpuFoundation_20_InvalidArguments) {
     GpuBuffer<real_t> empty;
     GpuMatrixCSR none;
     EXPECT_EQ(GpuBackend::instance().spmv(none, empty, empty), GPUStatus::INVALID_ARGUMENT);
@@ -456,7 +528,7 @@ TEST_CASE(GpuFoundation_20_InvalidArguments) {
     EXPECT_EQ(GpuBackend::instance().dot_product(empty, empty, dot, 10), GPUStatus::INVALID_ARGUMENT);
 }
 
-TEST_CASE(GpuFoundation_22_KernelTiming) {
+TEST_CASE_GPU(GpuFoundation_22_KernelTiming) {
     GpuBuffer<real_t> a(10), b(10), r(10);
     std::vector<real_t> h(10, 1.0);
     a.copy_host_to_device(h.data(), 10);
@@ -467,9 +539,15 @@ TEST_CASE(GpuFoundation_22_KernelTiming) {
 }
 
 int main(int argc, char** argv) {
-    if (GpuBackend::instance().initialize() != GPUStatus::SUCCESS) {
-        std::printf("SKIP: no GPU (%s)\n", GpuBackend::instance().device_info().status_message.c_str());
-        return 77;
+    const char* filter = argc > 1 ? argv[1] : nullptr;
+    int run = 0;
+    int failed = th_run(filter, {th::SCENARIO, th::HOST_ONLY}, "default options", run);
+    if (GpuBackend::instance().initialize() == GPUStatus::SUCCESS) {
+        g_gpu_pass = true;
+        failed += th_run(filter, {th::SCENARIO, th::GPU_ONLY}, "GPU_FIRST_ORDER", run);
+    } else {
+        std::printf("no GPU (%s): GPU pass skipped\n", GpuBackend::instance().device_info().status_message.c_str());
     }
-    return th_run_all(argc > 1 ? argv[1] : nullptr);
+    std::printf("%d/%d passed\n", run - failed, run);
+    return failed ? 1 : 0;
 }

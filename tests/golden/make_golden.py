@@ -20,6 +20,7 @@ and scipy's HiGHS are available):
                       objective in the model's sense with its offset).
 
 Usage:  python tests/golden/make_golden.py        (ref_pdhg_kat, lp_optima)
+        python tests/golden/make_golden.py kat    (ref_pdhg_kat only)
         python tests/golden/make_golden.py mps    (mps_golden)
 """
 import hashlib
@@ -81,6 +82,18 @@ def kat_models():
                                 dict(adaptive_step_size=False, max_iterations=1500))
     out["check_frequency_7"] = (lpd.random_lp(40, 120, 4, 12).to_flat(),
                                 dict(check_frequency=7, max_iterations=1500))
+    for seed in range(6):  # LE / EQ / GE rows mixed, both objective senses
+        fm = lpd.random_lp(30, 80, 3, 100 + seed).to_flat()
+        fm.con_sense = np.random.default_rng(1000 + seed).choice([0, 1, 2], size=fm.num_cons).astype(np.int32)
+        fm.sense = int(seed % 2)
+        out[f"mixed_senses_30x80_s{100 + seed}"] = (fm, dict(max_iterations=3000))
+    # MPS models without RANGED rows (the drop-in splits those into two <=
+    # rows where the reference keeps only the lower side)
+    for fname in ("afiro.free.mps", "example1.mps", "bharatopt_refinery_demo.mps", "edge_obj_rhs.free.mps",
+                  "sc50a.free.mps", "blend2.free.mps"):
+        info, fm = pkg.read_mps(text=open(os.path.join(HERE, "mps", fname)).read())
+        assert info["status"] == 0, fname
+        out["mps_" + fname] = (fm, dict(max_iterations=5000))
     return out
 
 
@@ -126,7 +139,7 @@ def highs(lp):
     return float(res.fun)
 
 
-def main():
+def main(kat_only=False):
     if po.ref_lib() is None:
         raise SystemExit("oracle/_ref is not built; run make -C oracle first")
     kat = {}
@@ -139,6 +152,8 @@ def main():
         print(name, r["status"], r["iterations"], r["objective"], flush=True)
     with open(os.path.join(HERE, "ref_pdhg_kat.json"), "w") as f:
         json.dump(kat, f, indent=1, sort_keys=True)
+    if kat_only:
+        return
 
     opt = {}
     for name, lp in pdlp_instances().items():
@@ -222,4 +237,4 @@ if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "mps":
         mps_main()
     else:
-        main()
+        main(kat_only=len(sys.argv) > 1 and sys.argv[1] == "kat")

@@ -93,3 +93,58 @@ def test_presolve_matches_reference(pkg, ref, lpdata):
             assert a["offset"] == b["offset"]
         seen.add(b["status"])
     assert seen == {0, 1, 2, 3}  # every outcome exercised
+
+
+def _malformed_csr_cases(lpdata):
+    """(label, StdLP, nnz override) with a CSR no entry point may read."""
+    base = lpdata.random_lp(6, 9, 2, 3)
+    out = []
+    bad = base.row_ptr.copy()
+    bad[3], bad[4] = bad[4], bad[3] - 1      # decreasing offsets
+    out.append(("non-monotone row_ptr", bad, None))
+    bad = base.row_ptr.copy()
+    bad[0] = -2                              # negative start
+    out.append(("negative row_ptr[0]", bad, None))
+    bad = base.row_ptr.copy()
+    bad[0] = 1                               # does not start at 0
+    out.append(("row_ptr[0] != 0", bad, None))
+    out.append(("nnz larger than row_ptr[m]", base.row_ptr.copy(), base.nnz + 5))
+    out.append(("nnz smaller than row_ptr[m]", base.row_ptr.copy(), base.nnz - 1))
+    out.append(("negative nnz", base.row_ptr.copy(), -1))
+    return base, out
+
+
+def test_malformed_csr_rejected_before_any_device_work(pkg, lpdata):
+    """Every entry point that takes a bo_lp validates its CSR first and
+    returns BO_INVALID_ARGUMENT (4) -- with or without a GPU -- instead of
+    reading col_idx / val through bad offsets."""
+    import ctypes as C
+    L = pkg.lib()
+    base, cases = _malformed_csr_cases(lpdata)
+    for label, rp, nnz in cases:
+        lp = pkg.lpdata.StdLP(base.m, base.n, rp, base.col_idx, base.val, base.b, base.c, base.lb, base.ub,
+                              base.is_eq)
+        h = pkg._LpHolder(lp)
+        h.s.nnz = base.nnz if nnz is None else nnz
+        o = pkg.RefOptions(100, 1e-7, 0.01, 0.01, 1, 100)
+        r = pkg.RefResult()
+        x = np.zeros(base.n)
+        y = np.zeros(base.m)
+        assert L.bo_pdhg_ref_solve(C.byref(h.s), C.byref(o), pkg._p(x), pkg._p(y), C.byref(r)) == 4, label
+        p = pkg.default_pdlp_params()
+        out = C.c_void_p()
+        assert L.bo_pdlp_create(C.byref(h.s), C.byref(p), C.byref(out)) == 4, label
+        assert not out.value
+        arr = (pkg.BoLp * 1)(h.s)
+        assert L.bo_pdlp_batch_create(arr, 1, C.byref(p), C.byref(out)) == 4, label
+        assert L.bo_pdlp_family_create(C.byref(h.s), 4, C.byref(p), C.byref(out)) == 4, label
+    # a column index outside [0, n)
+    bad_ci = base.col_idx.copy()
+    bad_ci[2] = base.n
+    lp = pkg.lpdata.StdLP(base.m, base.n, base.row_ptr, bad_ci, base.val, base.b, base.c, base.lb, base.ub,
+                          base.is_eq)
+    h = pkg._LpHolder(lp)
+    r = pkg.RefResult()
+    o = pkg.RefOptions(100, 1e-7, 0.01, 0.01, 1, 100)
+    assert L.bo_pdhg_ref_solve(C.byref(h.s), C.byref(o), pkg._p(np.zeros(base.n)), pkg._p(np.zeros(base.m)),
+                               C.byref(r)) == 4

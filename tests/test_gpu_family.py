@@ -46,7 +46,7 @@ def test_family_matches_batch_engine(pkg, po, family, fp64):
         if perb[i]["status"] == 0:
             xb, yb = b.solution(i)
             kb = po.kkt(d, xb, yb)
-            assert abs(k["primal_obj"] - kb["primal_obj"]) <= 1e-5 * (1 + abs(kb["primal_obj"]))
+            assert abs(k["primal_obj"] - kb["primal_obj"]) <= TOL * (1 + abs(kb["primal_obj"]))
     its_f = np.array([p["iterations"] for p in per])
     its_b = np.array([p["iterations"] for p in perb])
     print("iterations family median %d batch median %d" % (np.median(its_f), np.median(its_b)), flush=True)
@@ -142,4 +142,23 @@ def test_family_primal_weights(pkg, po, family):
     f.set_primal_weights(None)
     per3, _ = f.solve()
     assert [p["iterations"] for p in per3] == [p["iterations"] for p in per]  # cleared: the cold run again
+    f.close()
+
+
+def test_family_time_limit(pkg, family):
+    # A wall-clock limit far below a solve to 1e-9: every LP ends solved (0)
+    # or stopped by the limit (3), the FP64 polish does not start after it,
+    # and the call returns within the limit plus one check interval.
+    import time
+    ids = list(range(300, 340))
+    draws = [family.draw(k) for k in ids]
+    f = pkg.PdlpFamily(family.base, capacity=64, eps_rel_kkt=1e-9, max_iterations=10**7, time_limit_s=0.05)
+    f.set_lps([d.c for d in draws], [d.b for d in draws])
+    t0 = time.perf_counter()
+    per, tot = f.solve()
+    wall = time.perf_counter() - t0
+    print("family", tot, "wall %.3f s" % wall, flush=True)
+    assert all(p["status"] in (0, 3) for p in per)
+    assert any(p["status"] == 3 for p in per)
+    assert wall <= 0.05 + 1.0
     f.close()

@@ -2,12 +2,15 @@
 bo_first_order_solve): the reference's conformance set, example1 and the
 refinery demo.
 
-* PDLP engine: every model HiGHS solves reaches OPTIMAL with its objective
-  within 1e-5 (relative) of the LP optimum in tests/golden/mps_golden.json
+* PDLP engine, run with the objective-gap stop (eps_objective 1e-6; rel-KKT
+  1e-6 alone leaves errors of 1-2e-6 on adlittle, blend, p0033 and share2b,
+  whose KKT gap term is relative to 1 + |c'x| + |dual objective|): every
+  model HiGHS solves reaches OPTIMAL with its objective within 1e-6
+  (relative) of the LP optimum in tests/golden/mps_golden.json
   (pinned by published figures in tests/test_mps.py); models HiGHS finds
   infeasible or unbounded never come back OPTIMAL.
 * Reference engine: bit-equal to the reference's own FirstOrderLPSolver on
-  the same parsed model, live where oracle/_ref is present.
+  six of these models through the golden KATs (tests/test_gpu_reference_path.py).
 * Batched engine: the models as one batch, the badly scaled ones finishing
   in its FP64 polish.
 """
@@ -36,11 +39,13 @@ def test_mps_pdlp(pkg, fname):
     fm = read(pkg, fname)
     print("case:", fname, g["highs_status"], g.get("highs_objective"), flush=True)
     if g["highs_status"] == "optimal":
-        r = pkg.first_order_solve(fm, engine=1, max_iterations=200000, eps_rel_kkt=1e-6)
+        r = pkg.first_order_solve(fm, engine=1, max_iterations=200000, eps_rel_kkt=1e-6, eps_objective=1e-6)
         f = g["highs_objective"]
         print("status", r["status"], "iterations", r["iterations"], "objective", r["objective_value"], flush=True)
         assert r["status"] == 0, r["message"]
-        assert abs(r["objective_value"] - f) <= 1e-5 * (1 + abs(f))
+        err = abs(r["objective_value"] - f) / (1 + abs(f))
+        print("objective relative error %.2e" % err, flush=True)
+        assert err <= 1e-6
         assert len(r["x"]) == fm.num_vars
     else:
         r = pkg.first_order_solve(fm, engine=1, max_iterations=20000, eps_rel_kkt=1e-6)
@@ -51,19 +56,6 @@ def test_mps_pdlp(pkg, fname):
         assert r["status"] != 0 or "bound optimization" in r["message"]
 
 
-# (no RANGED rows here: the drop-in splits them into two <= rows where the
-# reference keeps only the lower side, a documented divergence)
-@pytest.mark.parametrize("fname", ["afiro.free.mps", "example1.mps", "bharatopt_refinery_demo.mps",
-                                   "edge_obj_rhs.free.mps", "sc50a.free.mps", "blend2.free.mps"])
-def test_mps_reference_engine_bitequal(pkg, ref, fname):
-    fm = read(pkg, fname)
-    a = ref.ref_fo_solve(fm, max_iterations=5000)
-    b = pkg.first_order_solve(fm, engine=0, max_iterations=5000)
-    assert b["status"] == a["status"] and b["iterations"] == a["iterations"]
-    assert float.hex(b["objective_value"]) == float.hex(a["objective"])
-    assert np.array_equal(b["x"], a["x"])
-
-
 def test_mps_models_as_one_batch(pkg, po):
     # The LP-solvable models as one block-diagonal batch (bo_pdlp_batch_*):
     # the badly scaled Netlib models stall with FP32 iterates and finish in
@@ -72,7 +64,7 @@ def test_mps_models_as_one_batch(pkg, po):
     names = [f for f in FREE if GOLD[f]["highs_status"] == "optimal"]
     fms = [read(pkg, f) for f in names]
     lps = [pkg.lpdata.flat_to_std(fm) for fm in fms]
-    b = pkg.PdlpBatch(lps, eps_rel_kkt=1e-6, max_iterations=200000)
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=1e-6, eps_objective=1e-6, max_iterations=200000)
     per, tot = b.solve()
     print("batch", tot, flush=True)
     assert tot["solved"] == len(lps) and tot["polished"] >= 1
@@ -83,5 +75,6 @@ def test_mps_models_as_one_batch(pkg, po):
         obj = sm * kk["primal_obj"] + fms[k].obj_offset
         fs = GOLD[f]["highs_objective"]
         assert per[k]["status"] == 0 and kk["rel_kkt"] <= 1e-6, (f, per[k], kk["rel_kkt"])
-        assert abs(obj - fs) <= 1e-5 * (1 + abs(fs)), (f, obj, fs)
+        print(f, "objective relative error %.2e" % (abs(obj - fs) / (1 + abs(fs))), flush=True)
+        assert abs(obj - fs) <= 1e-6 * (1 + abs(fs)), (f, obj, fs)
     b.close()

@@ -10,7 +10,9 @@ bit-for-bit against the oracle and the reference build, through the C ABI.
   frequencies, iteration limits 0/1/5);
 * the whole drop-in (bo_first_order_solve: validate, presolve, standard form,
   GPU loop, postsolve) equal to the reference's FirstOrderLPSolver on the
-  golden models, and the C++ drop-in test program.
+  golden models (tests/golden/ref_pdhg_kat.json, written from the reference
+  build: seeded random and staircase models, mixed row and objective senses,
+  option variants, six MPS models), and the C++ drop-in test program.
 """
 import hashlib
 import json
@@ -188,20 +190,6 @@ def test_dropin_matches_reference_golden(pkg, models, name):
     assert float.hex(r["objective_value"]) == g["objective"]
     assert float.hex(r["constraint_violation"]) == g["constraint_violation"]
     assert sha(r["x"]) == g["x_sha"] and sha(r["y"]) == g["y_sha"]
-
-
-def test_dropin_matches_reference_live(pkg, ref, lpdata):
-    rng = np.random.default_rng(5)
-    for seed in range(6):
-        lp = lpdata.random_lp(30, 80, 3, 100 + seed)
-        fm = lp.to_flat()
-        fm.con_sense = rng.choice([0, 1, 2], size=fm.num_cons).astype(np.int32)
-        fm.sense = int(seed % 2)
-        a = ref.ref_fo_solve(fm, max_iterations=3000)
-        b = pkg.first_order_solve(fm, engine=0, max_iterations=3000)
-        assert (a["status"], a["iterations"]) == (b["status"], b["iterations"]), seed
-        assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
-        assert a["objective"] == b["objective_value"]
 
 
 def test_dropin_validation_and_infeasible_presolve(pkg, lpdata):

@@ -263,3 +263,23 @@ def test_pdlp_omp_iteration_limit_and_threads(po, lpdata):
     # reductions combine in another order with more threads; the path barely moves
     assert abs(a["iterations"] - b["iterations"]) <= 256
     assert np.max(np.abs(xa - xb)) <= 1e-3 * (1 + np.max(np.abs(xa)))
+
+
+def test_pdlp_omp_objective_gap_stop(po, lpdata):
+    """The objective-gap stop (eps_objective): a run that stops on rel-KKT
+    1e-6 alone can leave |f - f*| / (1 + |f*|) above 1e-6 (its gap term is
+    relative to 1 + |c'x| + |dual obj|); with eps_objective = 1e-6 the stop
+    also needs |c'x - dual obj| <= 1e-6 (1 + |c'x|), and the objective is then
+    within 1e-6 of the exact optimum on every golden LP."""
+    import sys
+    sys.path.insert(0, HERE)
+    from golden import make_golden
+    inst = make_golden.pdlp_instances()
+    for name in sorted(OPT):
+        st, x, y = po.pdlp_omp(inst[name], threads=2, eps_objective=1e-6)
+        assert st["status"] == 0, (name, st)
+        k = po.kkt(inst[name], x, y)
+        assert k["rel_kkt"] <= 1e-6
+        assert abs(k["primal_obj"] - k["dual_obj"]) <= 1e-6 * (1 + abs(k["primal_obj"])) * (1 + 1e-3), name
+        f = OPT[name]["highs"]
+        assert abs(k["primal_obj"] - f) / (1 + abs(f)) <= 1e-6, (name, k["primal_obj"], f)
