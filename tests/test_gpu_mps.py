@@ -2,11 +2,15 @@
 bo_first_order_solve): the reference's conformance set, example1 and the
 refinery demo.
 
-* PDLP engine, run with the objective-gap stop (eps_objective 1e-6; rel-KKT
-  1e-6 alone leaves errors of 1-2e-6 on adlittle, blend, p0033 and share2b,
-  whose KKT gap term is relative to 1 + |c'x| + |dual objective|): every
-  model HiGHS solves reaches OPTIMAL with its objective within 1e-6
-  (relative) of the LP optimum in tests/golden/mps_golden.json
+* PDLP engine, run as the benchmark runs it: rel-KKT 1e-6 plus the
+  objective-gap stop at 5e-7.  rel-KKT 1e-6 alone leaves objective errors of
+  1-2e-6 on adlittle, blend, p0033 and share2b (its gap term is relative to
+  1 + |c'x| + |dual objective|); the gap stop at 1e-6 still leaves 1.26e-6 on
+  sc50a in the FP32 batch, where x's infeasibility (within the KKT tolerance)
+  moves c'x past the optimum by more than the gap; at 5e-7 the worst is
+  5.1e-7, for 0.5% more iterations.  Every model HiGHS solves reaches OPTIMAL
+  with its objective within 1e-6 (relative) of the LP optimum in
+  tests/golden/mps_golden.json
   (pinned by published figures in tests/test_mps.py); models HiGHS finds
   infeasible or unbounded never come back OPTIMAL.
 * Reference engine: bit-equal to the reference's own FirstOrderLPSolver on
@@ -39,7 +43,7 @@ def test_mps_pdlp(pkg, fname):
     fm = read(pkg, fname)
     print("case:", fname, g["highs_status"], g.get("highs_objective"), flush=True)
     if g["highs_status"] == "optimal":
-        r = pkg.first_order_solve(fm, engine=1, max_iterations=200000, eps_rel_kkt=1e-6, eps_objective=1e-6)
+        r = pkg.first_order_solve(fm, engine=1, max_iterations=200000, eps_rel_kkt=1e-6, eps_objective=5e-7)
         f = g["highs_objective"]
         print("status", r["status"], "iterations", r["iterations"], "objective", r["objective_value"], flush=True)
         assert r["status"] == 0, r["message"]
@@ -64,7 +68,7 @@ def test_mps_models_as_one_batch(pkg, po):
     names = [f for f in FREE if GOLD[f]["highs_status"] == "optimal"]
     fms = [read(pkg, f) for f in names]
     lps = [pkg.lpdata.flat_to_std(fm) for fm in fms]
-    b = pkg.PdlpBatch(lps, eps_rel_kkt=1e-6, eps_objective=1e-6, max_iterations=200000)
+    b = pkg.PdlpBatch(lps, eps_rel_kkt=1e-6, eps_objective=5e-7, max_iterations=200000)
     per, tot = b.solve()
     print("batch", tot, flush=True)
     assert tot["solved"] == len(lps) and tot["polished"] >= 1
