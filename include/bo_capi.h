@@ -133,6 +133,19 @@ typedef struct bo_ref_result {
  * multiply-adds: results are bit-equal to the reference.  x_out[n], y_out[m]. */
 bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* opts, double* x_out,
                             double* y_out, bo_ref_result* result);
+/* No counterpart in the reference.  For callers that cannot allocate memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): bo_pdhg_ref_solve with the same arguments and bit-identical
+ * results, calling no hipMalloc / hipFree or other allocating routine.  Every
+ * device array of the solve comes from one fixed array per GPU (about 2 MiB,
+ * held until the process ends) and the host-side CSC from fixed static host
+ * arrays.  Limit: m + n <= 7680 and nnz <= 65536 (the LPs that run in one
+ * workgroup); a larger LP returns BO_INVALID_ARGUMENT with bo_last_error()
+ * saying so.  Calls are serialized by a process-wide lock: on one GPU they
+ * share that array, which keeps the last call's contents (each call clears
+ * what must start at zero), so overlapping calls would otherwise race. */
+bo_status bo_pdhg_ref_solve_noalloc(const bo_lp* lp, const bo_ref_options* opts, double* x_out,
+                                    double* y_out, bo_ref_result* result);
 
 /* ---- PDLP engine (FP32 iterates, FP64 checks) ----------------------------- */
 typedef struct bo_pdlp_params {

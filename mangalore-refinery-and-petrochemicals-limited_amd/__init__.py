@@ -139,7 +139,8 @@ class FoResult(C.Structure):
 EXPORTS = [
     "bo_last_error", "bo_device_init", "bo_buf_alloc", "bo_buf_free", "bo_buf_h2d", "bo_buf_d2h",
     "bo_spmv_csr_f64", "bo_vec_add_f64", "bo_vec_sub_f64", "bo_vec_scale_f64", "bo_axpy_f64",
-    "bo_dot_f64", "bo_norm2_f64", "bo_pdhg_ref_solve", "bo_pdlp_default_params", "bo_pdlp_create",
+    "bo_dot_f64", "bo_norm2_f64", "bo_pdhg_ref_solve", "bo_pdhg_ref_solve_noalloc", "bo_pdlp_default_params",
+    "bo_pdlp_create",
     "bo_pdlp_set_warm_start", "bo_pdlp_solve", "bo_pdlp_get_solution", "bo_pdlp_destroy",
     "bo_pdlp_time_spmv", "bo_presolve_stats", "bo_first_order_solve", "bo_copy_kernel_calibrate",
     "bo_pdlp_get_pass_info", "bo_pdlp_time_kernels", "bo_pdlp_batch_create", "bo_pdlp_batch_set_warm_start",
@@ -177,6 +178,7 @@ def lib():
         L.bo_dot_f64.argtypes = [P, P, P, C.c_size_t, P]
         L.bo_norm2_f64.argtypes = [P, P, C.c_size_t, P]
         L.bo_pdhg_ref_solve.argtypes = [P, P, P, P, P]
+        L.bo_pdhg_ref_solve_noalloc.argtypes = [P, P, P, P, P]
         L.bo_pdlp_default_params.argtypes = [P]
         L.bo_pdlp_default_params.restype = None
         L.bo_pdlp_create.argtypes = [P, P, C.POINTER(P)]
@@ -300,16 +302,17 @@ class RefSolve:
 
 def pdhg_ref_solve(lp: StdLP, max_iterations=50000, feasibility_tolerance=1e-7,
                    primal_step_size=0.01, dual_step_size=0.01, adaptive_step_size=True,
-                   check_frequency=100) -> RefSolve:
-    """The reference's PDHG loop on the GPU (FP64, bit-equal)."""
+                   check_frequency=100, noalloc=False) -> RefSolve:
+    """The reference's PDHG loop on the GPU (FP64, bit-equal).  noalloc:
+    through bo_pdhg_ref_solve_noalloc (no allocation; small LPs only)."""
     h = _LpHolder(lp)
     o = RefOptions(max_iterations, feasibility_tolerance, primal_step_size, dual_step_size,
                    1 if adaptive_step_size else 0, check_frequency)
     r = RefResult()
     x = np.zeros(max(lp.n, 1))
     y = np.zeros(max(lp.m, 1))
-    _check(lib().bo_pdhg_ref_solve(C.byref(h.s), C.byref(o), _p(x), _p(y), C.byref(r)),
-           "bo_pdhg_ref_solve")
+    name = "bo_pdhg_ref_solve_noalloc" if noalloc else "bo_pdhg_ref_solve"
+    _check(getattr(lib(), name)(C.byref(h.s), C.byref(o), _p(x), _p(y), C.byref(r)), name)
     return RefSolve(r.status, r.iterations, r.loop_iterations, r.constraint_violation,
                     r.bound_violation, bool(r.used_average), r.kernel_time_ms, x[:lp.n], y[:lp.m], r.loop)
 

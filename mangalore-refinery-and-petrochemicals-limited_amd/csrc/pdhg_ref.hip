@@ -16,6 +16,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -477,104 +479,85 @@ inline double u64_to_double(unsigned long long u) {
     return d;
 }
 
-}  // namespace
+__global__ void k_fill_zero(double* __restrict__ p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0.0;
+}
 
-bo_status ensure_device_public();
+// Device arrays of one solve of the reference loop.
+struct RefDev {
+    int *rp, *ci, *cp, *ri;
+    double *rv, *cv, *b, *c, *lb, *ub, *tau, *sigma, *x, *y, *xsum, *xhat, *xavg, *out;
+    unsigned char* eq;
+    unsigned long long* red;  // [0] pmax [1] dmax [2] viol [3] avg viol [4] bound viol
+};
 
-}  // namespace bo
-
-using namespace bo;
-
-extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o, double* x_out,
-                                       double* y_out, bo_ref_result* res) {
-    if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
-        return BO_INVALID_ARGUMENT;
-    if (lp->nnz >= (1LL << 31)) return BO_INVALID_ARGUMENT;  // int32 offsets
-    bo_status st = ensure_device_public();
-    if (st != BO_SUCCESS) return st;
+// Host side of a solve, into caller-provided arrays: the CSC (scanning rows
+// in order keeps each column's rows ascending, the order the reference's
+// row-order scatter adds into each A'y entry; pos holds n counters) and the
+// starting point (src/first_order_solver.cpp:131-142).
+void ref_host_prep(const bo_lp* lp, int* cp, int* ri, double* cv, int* pos, double* x0) {
     const int m = lp->m, n = lp->n;
     const long long nnz = lp->nnz;
-    std::memset(res, 0, sizeof(*res));
-
-    auto t_xfer0 = std::chrono::high_resolution_clock::now();
-    // CSC from CSR: scanning rows in order keeps each column's rows ascending.
-    std::vector<int> cp(static_cast<size_t>(n) + 1, 0), ri(static_cast<size_t>(nnz));
-    std::vector<double> cv(static_cast<size_t>(nnz));
+    for (int j = 0; j <= n; ++j) cp[j] = 0;
     for (long long k = 0; k < nnz; ++k) cp[lp->col_idx[k] + 1]++;
     for (int j = 0; j < n; ++j) cp[j + 1] += cp[j];
-    {
-        std::vector<int> pos(cp.begin(), cp.end() - 1);
-        for (int i = 0; i < m; ++i)
-            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
-                int p = pos[lp->col_idx[k]]++;
-                ri[p] = i;
-                cv[p] = lp->val[k];
-            }
-    }
-    // Initial point (src/first_order_solver.cpp:131-142).
-    std::vector<double> x0(static_cast<size_t>(n));
+    for (int j = 0; j < n; ++j) pos[j] = cp[j];
+    for (int i = 0; i < m; ++i)
+        for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
+            const int q = pos[lp->col_idx[k]]++;
+            ri[q] = i;
+            cv[q] = lp->val[k];
+        }
     for (int j = 0; j < n; ++j) {
-        double l = lp->lb[j], u = lp->ub[j], target = 0.0;
+        const double l = lp->lb[j], u = lp->ub[j];
+        double target = 0.0;
         if (l > -INFINITY && u < INFINITY) target = 0.5 * (l + u);
         else if (l > -INFINITY) target = l;
         else if (u < INFINITY) target = u;
         x0[j] = (target < l) ? l : ((u < target) ? u : target);
     }
+}
 
-    DevBufs mem;
-    int *d_rp, *d_ci, *d_cp, *d_ri;
-    double *d_rv, *d_cv, *d_b, *d_c, *d_lb, *d_ub, *d_tau, *d_sigma, *d_x, *d_y, *d_xsum, *d_xhat, *d_xavg;
-    unsigned char* d_eq;
-    unsigned long long* d_red;  // [0] pmax [1] dmax [2] viol [3] avg viol [4] bound viol
-    BO_TRY(mem.alloc(&d_rp, m + 1));
-    BO_TRY(mem.alloc(&d_ci, nnz));
-    BO_TRY(mem.alloc(&d_rv, nnz));
-    BO_TRY(mem.alloc(&d_cp, n + 1));
-    BO_TRY(mem.alloc(&d_ri, nnz));
-    BO_TRY(mem.alloc(&d_cv, nnz));
-    BO_TRY(mem.alloc(&d_b, m));
-    BO_TRY(mem.alloc(&d_eq, m));
-    BO_TRY(mem.alloc(&d_sigma, m));
-    BO_TRY(mem.alloc(&d_y, m));
-    BO_TRY(mem.alloc(&d_c, n));
-    BO_TRY(mem.alloc(&d_lb, n));
-    BO_TRY(mem.alloc(&d_ub, n));
-    BO_TRY(mem.alloc(&d_tau, n));
-    BO_TRY(mem.alloc(&d_x, n));
-    BO_TRY(mem.alloc(&d_xsum, n));
-    BO_TRY(mem.alloc(&d_xhat, n));
-    BO_TRY(mem.alloc(&d_xavg, n));
-    BO_TRY(mem.alloc(&d_red, 8));
-    auto h2d = [](void* d, const void* h, size_t bytes) {
-        return bytes ? hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) : hipSuccess;
+// Uploads the LP, its CSC and starting point into d, clears y, the ergodic
+// sum and x_hat, runs the loop (one workgroup when the LP fits, else a
+// kernel per pass with the checks on the host), chooses the iterate and
+// downloads x and y.  Allocates nothing: every array is in d.  All work is
+// on the null stream; kernel_time_ms is the host's clock from the first
+// launch to the last download.
+bo_status ref_run(const bo_lp* lp, const RefDev& d, const int* cp, const int* ri, const double* cv,
+                  const double* x0, const bo_ref_options* o, double* x_out, double* y_out, bo_ref_result* res) {
+    const int m = lp->m, n = lp->n;
+    const long long nnz = lp->nnz;
+    std::memset(res, 0, sizeof(*res));
+    const auto t_xfer0 = std::chrono::high_resolution_clock::now();
+    auto h2d = [](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, 0) : hipSuccess;
     };
-    if (m > 0) BO_TRY(h2d(d_rp, lp->row_ptr, sizeof(int) * (m + 1)));
-    BO_TRY(h2d(d_ci, lp->col_idx, sizeof(int) * nnz));
-    BO_TRY(h2d(d_rv, lp->val, sizeof(double) * nnz));
-    BO_TRY(h2d(d_cp, cp.data(), sizeof(int) * (n + 1)));
-    BO_TRY(h2d(d_ri, ri.data(), sizeof(int) * nnz));
-    BO_TRY(h2d(d_cv, cv.data(), sizeof(double) * nnz));
-    BO_TRY(h2d(d_b, lp->b, sizeof(double) * m));
-    BO_TRY(h2d(d_eq, lp->is_eq, m));
-    BO_TRY(h2d(d_c, lp->c, sizeof(double) * n));
-    BO_TRY(h2d(d_lb, lp->lb, sizeof(double) * n));
-    BO_TRY(h2d(d_ub, lp->ub, sizeof(double) * n));
-    BO_TRY(h2d(d_x, x0.data(), sizeof(double) * n));
-    BO_TRY(hipMemset(d_y, 0, sizeof(double) * (m ? m : 1)));
-    BO_TRY(hipMemset(d_xsum, 0, sizeof(double) * (n ? n : 1)));
-    BO_TRY(hipMemset(d_xhat, 0, sizeof(double) * (n ? n : 1)));
-    BO_TRY(hipDeviceSynchronize());
-    auto t_xfer1 = std::chrono::high_resolution_clock::now();
-
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, 0);
+    if (m > 0) BO_TRY(h2d(d.rp, lp->row_ptr, sizeof(int) * (m + 1)));
+    BO_TRY(h2d(d.ci, lp->col_idx, sizeof(int) * nnz));
+    BO_TRY(h2d(d.rv, lp->val, sizeof(double) * nnz));
+    BO_TRY(h2d(d.cp, cp, sizeof(int) * (n + 1)));
+    BO_TRY(h2d(d.ri, ri, sizeof(int) * nnz));
+    BO_TRY(h2d(d.cv, cv, sizeof(double) * nnz));
+    BO_TRY(h2d(d.b, lp->b, sizeof(double) * m));
+    BO_TRY(h2d(d.eq, lp->is_eq, m));
+    BO_TRY(h2d(d.c, lp->c, sizeof(double) * n));
+    BO_TRY(h2d(d.lb, lp->lb, sizeof(double) * n));
+    BO_TRY(h2d(d.ub, lp->ub, sizeof(double) * n));
+    BO_TRY(h2d(d.x, x0, sizeof(double) * n));
+    if (m > 0) hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(m)), dim3(kBlock), 0, 0, d.y, m);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n)), dim3(kBlock), 0, 0, d.xsum, n);
+        hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n)), dim3(kBlock), 0, 0, d.xhat, n);
+    }
+    BO_TRY(hipGetLastError());
+    BO_TRY(hipStreamSynchronize(0));
+    const auto t_xfer1 = std::chrono::high_resolution_clock::now();
 
     const unsigned gm = grid_for(m), gn = grid_for(n), gmax = grid_for(m > n ? m : n);
-    hipLaunchKernelGGL(k_ref_steps, dim3(gmax), dim3(kBlock), 0, 0, m, n, d_rp, d_rv, d_cp, d_cv,
-                       o->adaptive_step_size ? 1 : 0, o->primal_step_size, o->dual_step_size, d_tau,
-                       d_sigma);
+    hipLaunchKernelGGL(k_ref_steps, dim3(gmax), dim3(kBlock), 0, 0, m, n, d.rp, d.rv, d.cp, d.cv,
+                       o->adaptive_step_size ? 1 : 0, o->primal_step_size, o->dual_step_size, d.tau, d.sigma);
 
     long long check_freq = o->check_frequency < 100 ? o->check_frequency : 100;
     if (check_freq < 1) check_freq = 1;  // the reference divides by zero here
@@ -594,11 +577,9 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
     const bool one_wg = lds_fit || (static_cast<long long>(n) + m <= kRefSmallMaxEntries && nnz <= kRefSmallMaxNnz);
     res->loop = lds_fit ? 2 : (one_wg ? 1 : 0);
     if (one_wg && o->max_iterations > 0) {
-        double* d_out = nullptr;
-        BO_TRY(mem.alloc(&d_out, 8));
-        RefSmallArgs a{m, n, static_cast<int>(nnz), d_rp, d_ci, d_cp, d_ri, d_rv, d_cv, d_b, d_c, d_lb, d_ub,
-                       d_tau, d_sigma, d_eq, d_x, d_y, d_xsum, d_xhat, d_xavg, o->max_iterations, check_freq,
-                       o->feasibility_tolerance, d_out};
+        RefSmallArgs a{m, n, static_cast<int>(nnz), d.rp, d.ci, d.cp, d.ri, d.rv, d.cv, d.b, d.c, d.lb, d.ub,
+                       d.tau, d.sigma, d.eq, d.x, d.y, d.xsum, d.xhat, d.xavg, o->max_iterations, check_freq,
+                       o->feasibility_tolerance, d.out};
         if (lds_fit) {
             const void* fn = per_thread <= 1 ? reinterpret_cast<const void*>(&k_ref_lds<1>)
                                              : reinterpret_cast<const void*>(&k_ref_lds<2>);
@@ -612,7 +593,8 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
         }
         BO_TRY(hipGetLastError());
         double out[8];
-        BO_TRY(hipMemcpy(out, d_out, sizeof(out), hipMemcpyDeviceToHost));
+        BO_TRY(hipMemcpyAsync(out, d.out, sizeof(out), hipMemcpyDeviceToHost, 0));
+        BO_TRY(hipStreamSynchronize(0));
         iter = static_cast<long long>(out[0]);
         avg_count = static_cast<long long>(out[5]);
         const int status = static_cast<int>(out[2]);
@@ -624,19 +606,20 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
     while (!one_wg && iter < o->max_iterations) {
         iter++;
         const int track = (iter % check_freq == 0 || iter == o->max_iterations) ? 1 : 0;
-        if (track) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d_red, 8);
-        hipLaunchKernelGGL(k_ref_primal, dim3(gn), dim3(kBlock), 0, 0, n, d_cp, d_ri, d_cv, d_y, d_c,
-                           d_tau, d_lb, d_ub, d_x, d_xsum, d_xhat, d_red + 0, track);
+        if (track) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d.red, 8);
+        hipLaunchKernelGGL(k_ref_primal, dim3(gn), dim3(kBlock), 0, 0, n, d.cp, d.ri, d.cv, d.y, d.c,
+                           d.tau, d.lb, d.ub, d.x, d.xsum, d.xhat, d.red + 0, track);
         avg_count++;
-        hipLaunchKernelGGL(k_ref_dual, dim3(gm), dim3(kBlock), 0, 0, m, d_rp, d_ci, d_rv, d_xhat, d_b,
-                           d_sigma, d_eq, d_y, d_red + 1, track);
+        hipLaunchKernelGGL(k_ref_dual, dim3(gm), dim3(kBlock), 0, 0, m, d.rp, d.ci, d.rv, d.xhat, d.b,
+                           d.sigma, d.eq, d.y, d.red + 1, track);
         if (track) {
-            hipLaunchKernelGGL(k_ref_avg, dim3(gn), dim3(kBlock), 0, 0, n, d_x, d_xsum,
-                               static_cast<double>(avg_count), d_lb, d_ub, d_xavg, d_red + 4, 0);
-            hipLaunchKernelGGL(k_ref_viol, dim3(gm), dim3(kBlock), 0, 0, m, d_rp, d_ci, d_rv, d_x, d_xavg,
-                               d_b, d_eq, d_red + 2, d_red + 3, 1);
+            hipLaunchKernelGGL(k_ref_avg, dim3(gn), dim3(kBlock), 0, 0, n, d.x, d.xsum,
+                               static_cast<double>(avg_count), d.lb, d.ub, d.xavg, d.red + 4, 0);
+            hipLaunchKernelGGL(k_ref_viol, dim3(gm), dim3(kBlock), 0, 0, m, d.rp, d.ci, d.rv, d.x, d.xavg,
+                               d.b, d.eq, d.red + 2, d.red + 3, 1);
             BO_TRY(hipGetLastError());
-            BO_TRY(hipMemcpy(red, d_red, sizeof(red), hipMemcpyDeviceToHost));
+            BO_TRY(hipMemcpyAsync(red, d.red, sizeof(red), hipMemcpyDeviceToHost, 0));
+            BO_TRY(hipStreamSynchronize(0));
             const double pmax = u64_to_double(red[0]), dmax = u64_to_double(red[1]);
             const double max_viol = u64_to_double(red[2]), avg_viol = u64_to_double(red[3]);
             const double bviol = u64_to_double(red[4]);
@@ -663,28 +646,185 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
         res->status = converged ? 0 : 1;
         res->iterations = iter;
         // Iterate selection (src/first_order_solver.cpp:261-279).
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d_red, 8);
-        hipLaunchKernelGGL(k_ref_avg, dim3(gn), dim3(kBlock), 0, 0, n, d_x, d_xsum,
-                           static_cast<double>(avg_count), d_lb, d_ub, d_xavg, d_red + 4,
-                           avg_count > 0 ? 0 : 1);
-        hipLaunchKernelGGL(k_ref_viol, dim3(gm), dim3(kBlock), 0, 0, m, d_rp, d_ci, d_rv, d_x, d_xavg,
-                           d_b, d_eq, d_red + 2, d_red + 3, 0);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, 0, d.red, 8);
+        hipLaunchKernelGGL(k_ref_avg, dim3(gn), dim3(kBlock), 0, 0, n, d.x, d.xsum,
+                           static_cast<double>(avg_count), d.lb, d.ub, d.xavg, d.red + 4, avg_count > 0 ? 0 : 1);
+        hipLaunchKernelGGL(k_ref_viol, dim3(gm), dim3(kBlock), 0, 0, m, d.rp, d.ci, d.rv, d.x, d.xavg,
+                           d.b, d.eq, d.red + 2, d.red + 3, 0);
         BO_TRY(hipGetLastError());
-        BO_TRY(hipMemcpy(red, d_red, sizeof(red), hipMemcpyDeviceToHost));
+        BO_TRY(hipMemcpyAsync(red, d.red, sizeof(red), hipMemcpyDeviceToHost, 0));
+        BO_TRY(hipStreamSynchronize(0));
         const double avg_viol = u64_to_double(red[3]);
         const bool use_avg = !converged && (avg_viol <= res->constraint_violation);
         res->used_average = use_avg ? 1 : 0;
-        if (n > 0) BO_TRY(hipMemcpy(x_out, use_avg ? d_xavg : d_x, sizeof(double) * n, hipMemcpyDeviceToHost));
-        if (m > 0) BO_TRY(hipMemcpy(y_out, d_y, sizeof(double) * m, hipMemcpyDeviceToHost));
+        if (n > 0)
+            BO_TRY(hipMemcpyAsync(x_out, use_avg ? d.xavg : d.x, sizeof(double) * n, hipMemcpyDeviceToHost, 0));
+        if (m > 0) BO_TRY(hipMemcpyAsync(y_out, d.y, sizeof(double) * m, hipMemcpyDeviceToHost, 0));
+        BO_TRY(hipStreamSynchronize(0));
     }
-    hipEventRecord(e1, 0);
-    hipEventSynchronize(e1);
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     BO_TRY(hipGetLastError());
-    res->kernel_time_ms = ms;
+    const auto t_end = std::chrono::high_resolution_clock::now();
+    res->kernel_time_ms = std::chrono::duration<double, std::milli>(t_end - t_xfer1).count();
     res->transfer_time_ms = std::chrono::duration<double, std::milli>(t_xfer1 - t_xfer0).count();
     return BO_SUCCESS;
+}
+
+// ---- the allocation-free variant (bo_pdhg_ref_solve_noalloc) ---------------
+// LPs that run in one workgroup: m + n <= 7,680 and nnz <= 65,536.  Every
+// device array of a solve is carved from `scratch`, a fixed array in each
+// GPU's memory (kNoAllocScratchBytes, held until the process ends); the
+// host-side CSC and starting point go to fixed static host arrays.
+constexpr int kNoAllocMaxEntries = kRefSmallMaxEntries;           // m + n
+constexpr long long kNoAllocMaxNnz = kRefSmallMaxNnz;               // nonzeros
+constexpr size_t kScratchAlign = 256;
+constexpr size_t scratch_round(size_t b) { return (b + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+// int arrays: rp (m+1), ci, cp (n+1), ri; double arrays: rv, cv, b, sigma,
+// y (m each), c, lb, ub, tau, x, xsum, xhat, xavg (n each), out (8); eq (m);
+// red (8 x u64); each rounded up to the alignment.
+constexpr size_t kNoAllocScratchBytes =
+    4 * (kNoAllocMaxEntries + 2 + 2 * kNoAllocMaxNnz) + 8 * (2 * kNoAllocMaxNnz + 8 * kNoAllocMaxEntries + 8) +
+    kNoAllocMaxEntries + 64 + 21 * kScratchAlign;
+
+}  // namespace
+
+bo_status ensure_device_public();
+
+}  // namespace bo
+
+// One per GPU (each device holds its own copy at its own address).  Host
+// code never uses this name as a device address: it looks the address up
+// with hipGetSymbolAddress and passes that to the kernels.
+static __device__ __align__(256) unsigned char scratch[bo::kNoAllocScratchBytes];
+
+using namespace bo;
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_lock;               // one call at a time: the host staging arrays are shared
+void* g_scratch_ptr[kMaxDevices] = {};   // scratch's address on each device, looked up once
+int g_h_cp[kNoAllocMaxEntries + 1];
+int g_h_pos[kNoAllocMaxEntries];
+int g_h_ri[kNoAllocMaxNnz];
+double g_h_cv[kNoAllocMaxNnz];
+double g_h_x0[kNoAllocMaxEntries];
+
+// scratch's device address on the current device (looked up with that device
+// current, the first time it is needed there).  Called under g_noalloc_lock.
+bo_status scratch_address(void** out) {
+    int dev = 0;
+    BO_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_last_error("bo_pdhg_ref_solve_noalloc: device index beyond the library's table");
+        return BO_INVALID_ARGUMENT;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        BO_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_last_error("bo_pdhg_ref_solve_noalloc: hipGetSymbolAddress returned no address");
+            return BO_EXECUTION_FAILED;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BO_SUCCESS;
+}
+
+// Carves the solve's arrays from the scratch address `base`, in a fixed order.
+RefDev carve(unsigned char* base, int m, int n, long long nnz, size_t* used) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += scratch_round(bytes ? bytes : 1);
+        return p;
+    };
+    RefDev d{};
+    d.rp = reinterpret_cast<int*>(take(sizeof(int) * (m + 1)));
+    d.ci = reinterpret_cast<int*>(take(sizeof(int) * nnz));
+    d.cp = reinterpret_cast<int*>(take(sizeof(int) * (n + 1)));
+    d.ri = reinterpret_cast<int*>(take(sizeof(int) * nnz));
+    d.rv = reinterpret_cast<double*>(take(sizeof(double) * nnz));
+    d.cv = reinterpret_cast<double*>(take(sizeof(double) * nnz));
+    d.b = reinterpret_cast<double*>(take(sizeof(double) * m));
+    d.sigma = reinterpret_cast<double*>(take(sizeof(double) * m));
+    d.y = reinterpret_cast<double*>(take(sizeof(double) * m));
+    d.c = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.lb = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.ub = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.tau = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.x = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.xsum = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.xhat = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.xavg = reinterpret_cast<double*>(take(sizeof(double) * n));
+    d.out = reinterpret_cast<double*>(take(sizeof(double) * 8));
+    d.eq = take(m);
+    d.red = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 8));
+    *used = off;
+    return d;
+}
+
+}  // namespace
+
+extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o, double* x_out,
+                                       double* y_out, bo_ref_result* res) {
+    if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
+        return BO_INVALID_ARGUMENT;
+    if (lp->nnz >= (1LL << 31)) return BO_INVALID_ARGUMENT;  // int32 offsets
+    bo_status st = ensure_device_public();
+    if (st != BO_SUCCESS) return st;
+    const int m = lp->m, n = lp->n;
+    const long long nnz = lp->nnz;
+    std::vector<int> cp(static_cast<size_t>(n) + 1), ri(static_cast<size_t>(nnz)), pos(static_cast<size_t>(n) + 1);
+    std::vector<double> cv(static_cast<size_t>(nnz)), x0(static_cast<size_t>(n) + 1);
+    ref_host_prep(lp, cp.data(), ri.data(), cv.data(), pos.data(), x0.data());
+    DevBufs mem;
+    RefDev d{};
+    BO_TRY(mem.alloc(&d.rp, m + 1));
+    BO_TRY(mem.alloc(&d.ci, nnz));
+    BO_TRY(mem.alloc(&d.rv, nnz));
+    BO_TRY(mem.alloc(&d.cp, n + 1));
+    BO_TRY(mem.alloc(&d.ri, nnz));
+    BO_TRY(mem.alloc(&d.cv, nnz));
+    BO_TRY(mem.alloc(&d.b, m));
+    BO_TRY(mem.alloc(&d.eq, m));
+    BO_TRY(mem.alloc(&d.sigma, m));
+    BO_TRY(mem.alloc(&d.y, m));
+    BO_TRY(mem.alloc(&d.c, n));
+    BO_TRY(mem.alloc(&d.lb, n));
+    BO_TRY(mem.alloc(&d.ub, n));
+    BO_TRY(mem.alloc(&d.tau, n));
+    BO_TRY(mem.alloc(&d.x, n));
+    BO_TRY(mem.alloc(&d.xsum, n));
+    BO_TRY(mem.alloc(&d.xhat, n));
+    BO_TRY(mem.alloc(&d.xavg, n));
+    BO_TRY(mem.alloc(&d.out, 8));
+    BO_TRY(mem.alloc(&d.red, 8));
+    return ref_run(lp, d, cp.data(), ri.data(), cv.data(), x0.data(), o, x_out, y_out, res);
+}
+
+extern "C" bo_status bo_pdhg_ref_solve_noalloc(const bo_lp* lp, const bo_ref_options* o, double* x_out,
+                                               double* y_out, bo_ref_result* res) {
+    if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
+        return BO_INVALID_ARGUMENT;
+    if (static_cast<long long>(lp->m) + lp->n > kNoAllocMaxEntries || lp->nnz > kNoAllocMaxNnz) {
+        set_last_error("bo_pdhg_ref_solve_noalloc: the LP exceeds the fixed scratch (needs m + n <= " +
+                       std::to_string(kNoAllocMaxEntries) + " and nnz <= " + std::to_string(kNoAllocMaxNnz) +
+                       "; got m + n = " + std::to_string(static_cast<long long>(lp->m) + lp->n) +
+                       ", nnz = " + std::to_string(lp->nnz) + "); use bo_pdhg_ref_solve");
+        return BO_INVALID_ARGUMENT;
+    }
+    bo_status st = ensure_device_public();
+    if (st != BO_SUCCESS) return st;
+    std::lock_guard<std::mutex> hold(g_noalloc_lock);
+    void* base = nullptr;
+    if ((st = scratch_address(&base)) != BO_SUCCESS) return st;
+    size_t used = 0;
+    const RefDev d = carve(static_cast<unsigned char*>(base), lp->m, lp->n, lp->nnz, &used);
+    if (used > kNoAllocScratchBytes) {  // cannot happen within the limits above
+        set_last_error("bo_pdhg_ref_solve_noalloc: scratch layout exceeds its size");
+        return BO_EXECUTION_FAILED;
+    }
+    ref_host_prep(lp, g_h_cp, g_h_ri, g_h_cv, g_h_pos, g_h_x0);
+    return ref_run(lp, d, g_h_cp, g_h_ri, g_h_cv, g_h_x0, o, x_out, y_out, res);
 }

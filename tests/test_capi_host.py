@@ -148,3 +148,12 @@ def test_malformed_csr_rejected_before_any_device_work(pkg, lpdata):
     o = pkg.RefOptions(100, 1e-7, 0.01, 0.01, 1, 100)
     assert L.bo_pdhg_ref_solve(C.byref(h.s), C.byref(o), pkg._p(np.zeros(base.n)), pkg._p(np.zeros(base.m)),
                                C.byref(r)) == 4
+
+
+def test_noalloc_variant_refuses_oversized_lp_before_device_work(pkg, lpdata):
+    """bo_pdhg_ref_solve_noalloc checks its fixed limits (m + n <= 7680,
+    nnz <= 65536) before anything touches a device, with or without a GPU."""
+    lp = lpdata.random_lp_sampled(2000, 6000, 5, 11)
+    with pytest.raises(pkg.BoError) as e:
+        pkg.pdhg_ref_solve(lp, max_iterations=10, noalloc=True)
+    assert e.value.status == 4 and "m + n <= 7680" in str(e.value)
