@@ -417,6 +417,12 @@ typedef struct bo_fo_result {
     long long primal_len;
     long long dual_len;
     char message[256];
+    /* SolutionVerifier::verify of the returned primal solution against the
+     * original model with the reported objective, at the options'
+     * feasibility tolerance (1e-7 by default); status -1 when there is no
+     * solution to verify, else bo_verify_result.status */
+    int verify_status;
+    double verify_max_violation; /* largest constraint or bound violation the verifier found */
 } bo_fo_result;
 
 /* PresolveEngine::presolve (include/bharatopt/presolve.hpp:96-102), host only:
@@ -433,6 +439,48 @@ bo_status bo_presolve_stats(const bo_model* model, int* status, int* reduced_var
  * BO_GPU_UNAVAILABLE when the GPU backend was asked for and no GPU is visible. */
 bo_status bo_first_order_solve(const bo_model* model, const bo_fo_options* opts,
                                bo_fo_result* result, double* primal_out, double* dual_out);
+
+/* ---- solution verifier ----------------------------------------------------- */
+/* VerificationOptions (include/bharatopt/solution_verifier.hpp:13-18). */
+typedef struct bo_verify_options {
+    double feasibility_tolerance; /* default 1e-7 */
+    double integrality_tolerance; /* default 1e-5 */
+    double objective_tolerance;   /* default 1e-4 */
+    int check_objective;          /* default 1 */
+} bo_verify_options;
+
+/* VerificationResult (solution_verifier.hpp:20-40) without its texts.
+ * status: 0 verified, 1 failed verification, 2 objective mismatch (the
+ * reported objective differs), 3 dimension mismatch; diagnostics: the number
+ * of diagnostic messages the verifier produced. */
+typedef struct bo_verify_result {
+    int verified;
+    int has_reported_objective;
+    double objective_reported;
+    double objective_recomputed;
+    double objective_difference;
+    double maximum_constraint_violation;
+    double maximum_lower_bound_violation;
+    double maximum_upper_bound_violation;
+    double maximum_integrality_violation;
+    long long violated_constraint_count;
+    long long violated_bound_count;
+    long long violated_integrality_count;
+    long long variable_count;
+    long long constraint_count;
+    int status;
+    int diagnostics;
+} bo_verify_result;
+
+/* SolutionVerifier::verify (solution_verifier.hpp:46-52, src/solution_verifier.cpp:42-215),
+ * host FP64: bounds, integrality, row activities (LE / GE / EQ / RANGED) and
+ * the objective of candidate x (x_len entries; a length other than
+ * model->num_vars is the verifier's dimension failure) against the original
+ * model.  reported_objective != NULL: the overload that also compares a
+ * reported objective.  opts NULL: the defaults.  Needs no GPU. */
+bo_status bo_verify_solution(const bo_model* model, const double* x, long long x_len,
+                             const double* reported_objective, const bo_verify_options* opts,
+                             bo_verify_result* result);
 
 /* ---- MPS reader ------------------------------------------------------------ */
 /* MpsParser::parse_string / parse_file (include/bharatopt/mps_parser.hpp:40-49,

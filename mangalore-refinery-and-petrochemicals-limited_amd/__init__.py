@@ -109,6 +109,19 @@ class BoModel(C.Structure):
                 ("term_var", P), ("term_val", P)]
 
 
+class VerifyOptions(C.Structure):
+    _fields_ = [("feasibility_tolerance", C.c_double), ("integrality_tolerance", C.c_double),
+                ("objective_tolerance", C.c_double), ("check_objective", C.c_int)]
+
+
+class VerifyResult(C.Structure):
+    _fields_ = [("verified", C.c_int), ("has_reported_objective", C.c_int)] + [(k, C.c_double) for k in (
+        "objective_reported", "objective_recomputed", "objective_difference", "maximum_constraint_violation",
+        "maximum_lower_bound_violation", "maximum_upper_bound_violation", "maximum_integrality_violation")] + [
+        (k, C.c_longlong) for k in ("violated_constraint_count", "violated_bound_count", "violated_integrality_count",
+                                    "variable_count", "constraint_count")] + [("status", C.c_int), ("diagnostics", C.c_int)]
+
+
 class MpsInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("num_vars", C.c_int), ("num_cons", C.c_int), ("nonzeros", C.c_longlong),
                 ("integer_vars", C.c_int), ("binary_vars", C.c_int), ("parse_time_ms", C.c_double),
@@ -132,7 +145,8 @@ class FoResult(C.Structure):
         "objective_value", "primal_objective", "dual_objective", "primal_residual", "dual_residual",
         "constraint_violation", "bound_violation", "solve_time_ms", "transfer_time_ms",
         "kernel_time_ms")] + [("primal_len", C.c_longlong), ("dual_len", C.c_longlong),
-                              ("message", C.c_char * 256)]
+                              ("message", C.c_char * 256), ("verify_status", C.c_int),
+                              ("verify_max_violation", C.c_double)]
 
 
 # every symbol include/bo_capi.h declares
@@ -152,7 +166,7 @@ EXPORTS = [
     "bo_pdlp_family_get_solutions", "bo_pdlp_family_time_passes", "bo_pdlp_family_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
     "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_time", "bo_mlp_destroy", "bo_gemm_bf16",
-    "bo_mps_read", "bo_mps_model", "bo_mps_destroy",
+    "bo_mps_read", "bo_mps_model", "bo_mps_destroy", "bo_verify_solution",
 ]
 
 
@@ -195,6 +209,7 @@ def lib():
         L.bo_mps_destroy.restype = None
         L.bo_presolve_stats.argtypes = [P, P, P, P, P, P]
         L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
+        L.bo_verify_solution.argtypes = [P, P, C.c_longlong, P, P, P]
         for name in EXPORTS:
             if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params",
                             "bo_pdlp_batch_destroy", "bo_pdlp_family_destroy", "bo_mlp_destroy",
@@ -646,6 +661,22 @@ class _ModelHolder:
         self.s = BoModel(fm.num_vars, fm.num_cons, int(fm.sense), float(fm.obj_offset),
                          _p(a["lb"]), _p(a["ub"]), _p(a["obj"]), _p(a["vt"]), _p(a["cs"]),
                          _p(a["rhs"]), _p(a["ru"]), _p(a["tp"]), _p(a["tv"]), _p(a["tw"]))
+
+
+def verify_solution(fm: FlatModel, x, reported_objective=None, feasibility_tolerance=1e-7,
+                    integrality_tolerance=1e-5, objective_tolerance=1e-4, check_objective=True):
+    """SolutionVerifier::verify on the host (no GPU): a dict of the
+    VerificationResult fields (status 0 verified, 1 failed, 2 objective
+    mismatch, 3 dimension mismatch)."""
+    h = _ModelHolder(fm)
+    xx = np.ascontiguousarray(x, np.float64)
+    o = VerifyOptions(feasibility_tolerance, integrality_tolerance, objective_tolerance, 1 if check_objective else 0)
+    r = VerifyResult()
+    rep = None if reported_objective is None else C.c_double(reported_objective)
+    _check(lib().bo_verify_solution(C.byref(h.s), _p(xx) if xx.size else None, int(xx.size),
+                                    None if rep is None else C.byref(rep), C.byref(o), C.byref(r)),
+           "bo_verify_solution")
+    return {k: getattr(r, k) for k, _ in VerifyResult._fields_}
 
 
 def copy_kernel_ms(nbytes, reps=20):

@@ -13,6 +13,7 @@
 #include <bharatopt/config.hpp>
 #include <bharatopt/gpu_backend.hpp>
 #include <bharatopt/lp_model.hpp>
+#include <bharatopt/solution_verifier.hpp>
 #include <bharatopt/sparse_matrix.hpp>
 
 namespace bharatopt {
@@ -41,6 +42,7 @@ struct FirstOrderSolverOptions {
     FirstOrderEngine engine{FirstOrderEngine::REFERENCE_PDHG};
     real_t relative_kkt_tolerance{1e-6};  // PDLP engine stopping tolerance
     real_t objective_gap_tolerance{0.0};  // PDLP engine: > 0 also requires |c'x - dual obj| <= this (1 + |c'x|)
+    bool verify_solution{true};  // run SolutionVerifier on every returned primal solution (result.verification)
 };
 
 struct FirstOrderSolverStats {
@@ -66,6 +68,12 @@ struct FirstOrderSolverResult {
     real_t objective_value{0.0};
     FirstOrderSolverStats stats;
     std::string message;
+    // --- additions ---
+    // SolutionVerifier::verify of primal_solution against the original model
+    // with its reported objective (feasibility tolerance of the options);
+    // informational: it does not change status.  variable_count == 0 when no
+    // solution was returned or verify_solution is off.
+    VerificationResult verification;
 };
 
 class IFirstOrderLPSolver {

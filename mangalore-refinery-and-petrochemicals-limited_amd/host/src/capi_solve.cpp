@@ -3,12 +3,18 @@
 // callers that bind C rather than C++ (ctypes, cgo, JNI).
 #include <bharatopt/first_order_solver.hpp>
 #include <bharatopt/presolve.hpp>
+#include <bharatopt/solution_verifier.hpp>
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <string>
 
 using namespace bharatopt;
+
+namespace bo {
+void set_last_error(const std::string& msg);  // bo_last_error() text (backend.hip)
+}
 
 namespace {
 
@@ -35,6 +41,13 @@ LPModel model_from_flat(const bo_model* fm) {
     return model;
 }
 
+// 0 verified, 1 failed, 2 objective mismatch, 3 dimension mismatch.
+int verification_status(const VerificationResult& vr) {
+    if (vr.status_message.find("Dimension mismatch") != std::string::npos) return 3;
+    if (vr.status_message.find("objective mismatch") != std::string::npos) return 2;
+    return vr.verified ? 0 : 1;
+}
+
 void set_message(bo_fo_result* r, const std::string& m) {
     std::strncpy(r->message, m.c_str(), sizeof(r->message) - 1);
     r->message[sizeof(r->message) - 1] = 0;
@@ -54,6 +67,49 @@ extern "C" bo_status bo_presolve_stats(const bo_model* fm, int* status, int* red
         if (reduced_nnz) *reduced_nnz = static_cast<long long>(pr.reduced_model.num_nonzeros());
         return BO_SUCCESS;
     } catch (...) {
+        return BO_INVALID_ARGUMENT;
+    }
+}
+
+extern "C" bo_status bo_verify_solution(const bo_model* fm, const double* x, long long x_len,
+                                        const double* reported, const bo_verify_options* opts,
+                                        bo_verify_result* out) {
+    if (!fm || !out || fm->num_vars < 0 || fm->num_cons < 0 || x_len < 0 || (x_len > 0 && !x)) return BO_INVALID_ARGUMENT;
+    if ((fm->num_vars > 0 && (!fm->lb || !fm->ub || !fm->obj)) ||
+        (fm->num_cons > 0 && (!fm->con_sense || !fm->rhs || !fm->term_ptr)))
+        return BO_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    try {
+        const LPModel model = model_from_flat(fm);
+        VerificationOptions vo;
+        if (opts) {
+            vo.feasibility_tolerance = opts->feasibility_tolerance;
+            vo.integrality_tolerance = opts->integrality_tolerance;
+            vo.objective_tolerance = opts->objective_tolerance;
+            vo.check_objective = opts->check_objective != 0;
+        }
+        const std::vector<real_t> cand(x, x + x_len);
+        const SolutionVerifier sv(vo);
+        const VerificationResult vr = reported ? sv.verify(model, cand, *reported, vo) : sv.verify(model, cand, vo);
+        out->verified = vr.verified ? 1 : 0;
+        out->has_reported_objective = vr.has_reported_objective ? 1 : 0;
+        out->objective_reported = vr.objective_reported;
+        out->objective_recomputed = vr.objective_recomputed;
+        out->objective_difference = vr.objective_difference;
+        out->maximum_constraint_violation = vr.maximum_constraint_violation;
+        out->maximum_lower_bound_violation = vr.maximum_lower_bound_violation;
+        out->maximum_upper_bound_violation = vr.maximum_upper_bound_violation;
+        out->maximum_integrality_violation = vr.maximum_integrality_violation;
+        out->violated_constraint_count = static_cast<long long>(vr.violated_constraint_count);
+        out->violated_bound_count = static_cast<long long>(vr.violated_bound_count);
+        out->violated_integrality_count = static_cast<long long>(vr.violated_integrality_count);
+        out->variable_count = static_cast<long long>(vr.variable_count);
+        out->constraint_count = static_cast<long long>(vr.constraint_count);
+        out->status = verification_status(vr);
+        out->diagnostics = static_cast<int>(vr.diagnostic_messages.size());
+        return BO_SUCCESS;
+    } catch (const std::exception& e) {
+        bo::set_last_error(std::string("bo_verify_solution: ") + e.what());
         return BO_INVALID_ARGUMENT;
     }
 }
@@ -96,6 +152,12 @@ extern "C" bo_status bo_first_order_solve(const bo_model* fm, const bo_fo_option
         r->primal_len = static_cast<long long>(res.primal_solution.size());
         r->dual_len = static_cast<long long>(res.dual_solution.size());
         set_message(r, res.message);
+        r->verify_status = res.verification.variable_count > 0 || res.verification.status_message.size()
+                               ? verification_status(res.verification)
+                               : -1;
+        r->verify_max_violation = std::max({res.verification.maximum_constraint_violation,
+                                            res.verification.maximum_lower_bound_violation,
+                                            res.verification.maximum_upper_bound_violation});
         if (primal_out && !res.primal_solution.empty())
             std::memcpy(primal_out, res.primal_solution.data(), res.primal_solution.size() * sizeof(double));
         if (dual_out && !res.dual_solution.empty())

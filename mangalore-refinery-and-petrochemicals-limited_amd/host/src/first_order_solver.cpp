@@ -295,7 +295,16 @@ using LoopStage = std::function<LoopEnd(const StandardForm&, std::vector<real_t>
 // postsolve -> objective, as CPUFirstOrderSolver::solve orders them
 // (src/first_order_solver.cpp:14-290).  solve_time_ms covers everything up
 // to the end of the loop (the reference stops its clock before postsolve).
-FirstOrderSolverResult run_pipeline(const LPModel& model, clk::time_point t0, const LoopStage& loop) {
+// The verifier's judgement of a returned solution (see FirstOrderSolverResult::verification).
+void verify_result(const LPModel& model, const FirstOrderSolverOptions& o, FirstOrderSolverResult& r) {
+    if (!o.verify_solution || r.primal_solution.empty()) return;
+    VerificationOptions vo;
+    vo.feasibility_tolerance = o.feasibility_tolerance;
+    r.verification = SolutionVerifier(vo).verify(model, r.primal_solution, r.objective_value, vo);
+}
+
+FirstOrderSolverResult run_pipeline(const LPModel& model, const FirstOrderSolverOptions& o, clk::time_point t0,
+                                    const LoopStage& loop) {
     FirstOrderSolverResult result;
     if (!ModelValidator().validate(model).is_valid()) {
         result.status = FirstOrderSolverStatus::NUMERICAL_FAILURE;
@@ -328,6 +337,7 @@ FirstOrderSolverResult run_pipeline(const LPModel& model, clk::time_point t0, co
         result.objective_value = original_objective(model, result.primal_solution);
         result.stats.primal_objective = result.objective_value;
         result.message = "LP model solved via bound optimization & presolve postsolve.";
+        verify_result(model, o, result);
         return result;
     }
 
@@ -356,6 +366,7 @@ FirstOrderSolverResult run_pipeline(const LPModel& model, clk::time_point t0, co
     result.dual_solution = fold_duals(sf, y);
     result.objective_value = original_objective(model, result.primal_solution);
     result.stats.primal_objective = result.objective_value;
+    verify_result(model, o, result);
     return result;
 }
 
@@ -446,7 +457,7 @@ CPUFirstOrderSolver::CPUFirstOrderSolver(FirstOrderSolverOptions options) : opti
 
 FirstOrderSolverResult CPUFirstOrderSolver::solve(const LPModel& model) {
     const FirstOrderSolverOptions& o = options_;
-    return run_pipeline(model, clk::now(),
+    return run_pipeline(model, o, clk::now(),
                         [&o](const StandardForm& sf, std::vector<real_t>& x, std::vector<real_t>& y,
                              FirstOrderSolverResult& r, real_t&, std::string&) {
                             return host_reference_loop(sf, o, x, y, r);
@@ -466,7 +477,7 @@ FirstOrderSolverResult GPUFirstOrderSolver::solve(const LPModel& model) {
         return result;
     }
     const FirstOrderSolverOptions& o = options_;
-    return run_pipeline(model, t0,
+    return run_pipeline(model, o, t0,
                         [&o](const StandardForm& sf, std::vector<real_t>& x, std::vector<real_t>& y,
                              FirstOrderSolverResult& r, real_t& dual_min, std::string& failure) {
                             return gpu_loop(o, sf, x, y, r, dual_min, failure);

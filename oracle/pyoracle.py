@@ -139,6 +139,8 @@ def ref_lib():
         lib.ref_simplex_solve.argtypes = [P, P, P, P, P]
         lib.ref_verify.restype = C.c_int
         lib.ref_verify.argtypes = [P, P, C.c_double, P, P, P, P]
+        lib.ref_verify_full.restype = C.c_int
+        lib.ref_verify_full.argtypes = [P, P, C.c_longlong, P, C.c_double, C.c_double, C.c_double, C.c_int, P]
         lib.ref_csr_spmv.restype = C.c_longlong
         lib.ref_csr_spmv.argtypes = [C.c_int, C.c_int, C.c_longlong, P, P, P, P, P, P, P]
         lib.ref_mps_parse.restype = C.c_int
@@ -345,6 +347,34 @@ def ref_verify(fm, x, tol=1e-7):
     ok = lib.ref_verify(C.byref(h.s), _p(xx), tol, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
     return dict(verified=ok == 1, max_cons_viol=a.value, max_lb_viol=b.value,
                 max_ub_viol=c.value, objective=d.value)
+
+
+class RefVerifyOut(C.Structure):
+    _fields_ = [("verified", C.c_int), ("has_reported_objective", C.c_int)] + [(k, C.c_double) for k in (
+        "objective_reported", "objective_recomputed", "objective_difference", "maximum_constraint_violation",
+        "maximum_lower_bound_violation", "maximum_upper_bound_violation", "maximum_integrality_violation")] + [
+        (k, C.c_longlong) for k in ("violated_constraint_count", "violated_bound_count", "violated_integrality_count",
+                                    "variable_count", "constraint_count")] + [("status", C.c_int), ("diagnostics", C.c_int)]
+
+
+def ref_verify_full(fm, x, reported_objective=None, feasibility_tolerance=1e-7, integrality_tolerance=1e-5,
+                    objective_tolerance=1e-4, check_objective=True):
+    """The reference's SolutionVerifier::verify, every VerificationResult
+    field (same keys as the package's verify_solution); None without
+    oracle/_ref."""
+    lib = ref_lib()
+    if lib is None:
+        return None
+    h = _RefModelHolder(fm)
+    xx = _f64(x)
+    out = RefVerifyOut()
+    rep = None if reported_objective is None else C.c_double(reported_objective)
+    rc = lib.ref_verify_full(C.byref(h.s), _p(xx) if xx.size else None, int(xx.size),
+                             None if rep is None else C.byref(rep), feasibility_tolerance, integrality_tolerance,
+                             objective_tolerance, 1 if check_objective else 0, C.byref(out))
+    if rc != 0:
+        raise RuntimeError("ref_verify_full failed")
+    return {k: getattr(out, k) for k, _ in RefVerifyOut._fields_}
 
 
 def ref_spmv(rows, cols, ri, ci, v, x=None, xt=None):

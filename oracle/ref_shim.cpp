@@ -216,6 +216,58 @@ int ref_verify(const ref_model* fm, const double* x, double tol, double* max_con
     }
 }
 
+// SolutionVerifier::verify with every field of VerificationResult; the
+// candidate has x_len entries (a length other than num_vars exercises the
+// dimension check); reported != NULL uses the overload with a reported
+// objective.  status: 0 verified, 1 failed verification, 2 objective
+// mismatch, 3 dimension mismatch (from status_message).
+struct ref_verify_out {
+    int verified, has_reported_objective;
+    double objective_reported, objective_recomputed, objective_difference;
+    double max_constraint_violation, max_lower_bound_violation, max_upper_bound_violation,
+        max_integrality_violation;
+    long long violated_constraint_count, violated_bound_count, violated_integrality_count, variable_count,
+        constraint_count;
+    int status, diagnostics;
+};
+
+int ref_verify_full(const ref_model* fm, const double* x, long long x_len, const double* reported, double feas_tol,
+                    double int_tol, double obj_tol, int check_objective, ref_verify_out* o) {
+    try {
+        LPModel model = build_model(fm);
+        VerificationOptions vo;
+        vo.feasibility_tolerance = feas_tol;
+        vo.integrality_tolerance = int_tol;
+        vo.objective_tolerance = obj_tol;
+        vo.check_objective = check_objective != 0;
+        SolutionVerifier sv(vo);
+        std::vector<real_t> xs(x, x + x_len);
+        VerificationResult vr = reported ? sv.verify(model, xs, *reported, vo) : sv.verify(model, xs, vo);
+        o->verified = vr.verified ? 1 : 0;
+        o->has_reported_objective = vr.has_reported_objective ? 1 : 0;
+        o->objective_reported = vr.objective_reported;
+        o->objective_recomputed = vr.objective_recomputed;
+        o->objective_difference = vr.objective_difference;
+        o->max_constraint_violation = vr.maximum_constraint_violation;
+        o->max_lower_bound_violation = vr.maximum_lower_bound_violation;
+        o->max_upper_bound_violation = vr.maximum_upper_bound_violation;
+        o->max_integrality_violation = vr.maximum_integrality_violation;
+        o->violated_constraint_count = static_cast<long long>(vr.violated_constraint_count);
+        o->violated_bound_count = static_cast<long long>(vr.violated_bound_count);
+        o->violated_integrality_count = static_cast<long long>(vr.violated_integrality_count);
+        o->variable_count = static_cast<long long>(vr.variable_count);
+        o->constraint_count = static_cast<long long>(vr.constraint_count);
+        const std::string& m = vr.status_message;
+        o->status = m.find("Dimension mismatch") != std::string::npos ? 3
+                    : m.find("objective mismatch") != std::string::npos ? 2
+                    : vr.verified ? 0 : 1;
+        o->diagnostics = static_cast<int>(vr.diagnostic_messages.size());
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 // CSRMatrix::from_coo + multiply / multiply_transpose.  Returns nnz after
 // cleaning, -1 on exceptions.  y has rows entries, yt has cols entries.
 long long ref_csr_spmv(int rows, int cols, long long nnz, const int* ri, const int* ci,

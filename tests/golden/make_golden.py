@@ -21,6 +21,8 @@ and scipy's HiGHS are available):
 
 Usage:  python tests/golden/make_golden.py        (ref_pdhg_kat, lp_optima)
         python tests/golden/make_golden.py kat    (ref_pdhg_kat only)
+        python tests/golden/make_golden.py verifier  (verifier_kat: the reference SolutionVerifier's
+                                                      results on candidates that break each rule)
         python tests/golden/make_golden.py mps    (mps_golden)
 """
 import hashlib
@@ -114,6 +116,102 @@ def presolve_mix_model():
     mb.add_constraint([(free, -1.0), (v[1], 1.0)], lpd.LE, 5.0)
     _ = empty
     return mb.build()
+
+
+def verifier_model():
+    """Every variable kind and row sense: max x0 - 2x1 + 3x2 + 0.5x3 + x5 + 2.5,
+    x0 in [0, 10], x1 >= -5, x2 integer in [0, 8], x3 binary, x4 free,
+    x5 <= 3; rows x0 + 2x1 <= 10, x0 - x2 >= -2, x1 + x3 + x4 = 4,
+    1 <= x2 + x5 <= 6 (ranged)."""
+    mb = lpd.ModelBuilder(sense=1)
+    mb.offset = 2.5
+    x0 = mb.add_variable(0.0, 10.0, 1.0)
+    x1 = mb.add_variable(-5.0, np.inf, -2.0)
+    x2 = mb.add_variable(0.0, 8.0, 3.0, vtype=1)
+    x3 = mb.add_variable(0.0, 1.0, 0.5, vtype=2)
+    x4 = mb.add_variable(-np.inf, np.inf, 0.0)
+    x5 = mb.add_variable(-np.inf, 3.0, 1.0)
+    mb.add_constraint([(x0, 1.0), (x1, 2.0)], lpd.LE, 10.0)
+    mb.add_constraint([(x0, 1.0), (x2, -1.0)], lpd.GE, -2.0)
+    mb.add_constraint([(x1, 1.0), (x3, 1.0), (x4, 1.0)], lpd.EQ, 4.0)
+    mb.add_constraint([(x2, 1.0), (x5, 1.0)], lpd.RANGED, 1.0, 6.0)
+    return mb.build()
+
+
+def verifier_cases():
+    """name -> (FlatModel, candidate x, kwargs of verify_solution)."""
+    fm = verifier_model()
+    good = np.array([2.0, 1.0, 3.0, 1.0, 2.0, 1.0])  # feasible; objective 2 - 2 + 9 + 0.5 + 1 + 2.5 = 13
+
+    def at(**kv):
+        x = good.copy()
+        for k, v in kv.items():
+            x[int(k[1:])] = v
+        return x
+
+    out = {
+        "feasible": (fm, good, {}),
+        "lower_bound": (fm, at(x0=-0.5), {}),
+        "upper_bound": (fm, at(x0=10.2), {}),
+        "upper_bound_within_tol": (fm, at(x0=10.0 + 5e-8), {}),
+        "integer": (fm, at(x2=3.4), {}),
+        "integer_within_tol": (fm, at(x2=3.0 + 5e-6), {}),
+        "binary": (fm, at(x3=0.5), {}),
+        "nan": (fm, at(x4=np.nan), {}),
+        "plus_inf": (fm, at(x1=np.inf), {}),
+        "minus_inf": (fm, at(x5=-np.inf), {}),
+        "le_and_eq_rows": (fm, at(x1=5.0), {}),
+        "ge_row": (fm, at(x2=5.0, x5=0.5), {}),
+        "eq_row": (fm, at(x4=2.5), {}),
+        "ranged_low": (fm, at(x5=-3.0), {}),
+        "ranged_high": (fm, at(x2=4.0, x5=3.0), {}),
+        "several_rules": (fm, at(x0=-1.0, x2=2.5, x4=7.0), {}),
+        "short_candidate": (fm, good[:5], {}),
+        "long_candidate": (fm, np.append(good, 0.0), {}),
+        "empty_candidate": (fm, good[:0], {}),
+        "reported_exact": (fm, good, dict(reported_objective=13.0)),
+        "reported_off_1e-3": (fm, good, dict(reported_objective=13.001)),
+        "reported_off_1e-5": (fm, good, dict(reported_objective=13.00001)),
+        "reported_off_unchecked": (fm, good, dict(reported_objective=14.0, check_objective=False)),
+        "reported_with_violation": (fm, at(x0=-0.5), dict(reported_objective=1.0)),
+        "loose_feasibility": (fm, at(x4=2.005), dict(feasibility_tolerance=1e-2)),
+        "loose_integrality": (fm, at(x2=3.2), dict(integrality_tolerance=0.25)),
+    }
+    rng = np.random.default_rng(77)
+    for t in range(40):  # seeded random models: every sense, both objective senses, mixed types
+        m_, n_ = int(rng.integers(3, 12)), int(rng.integers(3, 12))
+        mb = lpd.ModelBuilder(sense=int(t % 2))
+        mb.offset = float(rng.normal())
+        for j in range(n_):
+            lo = -np.inf if rng.random() < 0.2 else float(rng.integers(-5, 3))
+            hi = np.inf if rng.random() < 0.2 else max(lo, 0.0) + float(rng.integers(0, 8))
+            mb.add_variable(lo, hi, float(rng.normal()), vtype=int(rng.integers(0, 3)) if rng.random() < 0.3 else 0)
+        for i in range(m_):
+            k = int(rng.integers(1, n_ + 1))
+            cols = rng.choice(n_, size=k, replace=False)
+            terms = [(int(j), float(np.round(rng.normal() * 3, 3))) for j in cols]
+            sense = int(rng.integers(0, 4))
+            rhs = float(rng.integers(-4, 8))
+            mb.add_constraint(terms, sense, rhs, rhs + float(rng.integers(0, 6)))
+        fmr = mb.build()
+        x = rng.integers(-3, 6, size=n_).astype(float) + np.where(rng.random(n_) < 0.3, rng.normal(size=n_) * 1e-6, 0.0)
+        kw = dict(reported_objective=float(rng.normal() * 10)) if t % 3 == 0 else {}
+        out[f"random_{t}"] = (fmr, x, kw)
+    return out
+
+
+def verifier_main():
+    """verifier_kat.json: the reference SolutionVerifier's full result on every case."""
+    if po.ref_lib() is None:
+        raise SystemExit("oracle/_ref is not built; run make -C oracle first")
+    out = {}
+    for name, (fm, x, kw) in verifier_cases().items():
+        r = po.ref_verify_full(fm, x, **kw)
+        out[name] = {k: (float.hex(float(v)) if isinstance(v, float) else v) for k, v in r.items()}
+        print(name, r["status"], r["verified"], r["violated_constraint_count"], r["violated_bound_count"],
+              r["violated_integrality_count"], flush=True)
+    with open(os.path.join(HERE, "verifier_kat.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
 
 
 def pdlp_instances():
@@ -236,5 +334,7 @@ def mps_main():
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "mps":
         mps_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "verifier":
+        verifier_main()
     else:
         main(kat_only=len(sys.argv) > 1 and sys.argv[1] == "kat")

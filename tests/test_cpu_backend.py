@@ -94,3 +94,22 @@ def test_cpp_dropin_default_options_pass():
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     passed = [ln for ln in p.stdout.splitlines() if ln.startswith("PASS [default options] FirstOrder_")]
     assert len(passed) >= 24
+
+
+def test_dropin_results_carry_the_verifiers_judgement(pkg, lpdata, models):
+    """Every returned solution is judged by SolutionVerifier against the
+    original model with the reported objective (verify_status: 0 verified,
+    1 failed, 2 objective mismatch; -1 no solution); the verdict equals the
+    verifier's own on the same point."""
+    for name in ("hand_lp_64_3", "benchmark10", "presolve_mix", "iteration_limit_5", "mps_afiro.free.mps"):
+        fm, kw = models[name]
+        r = pkg.first_order_solve(fm, backend="cpu", **kw)
+        v = pkg.verify_solution(fm, r["x"], reported_objective=r["objective_value"])
+        assert r["verify_status"] == v["status"], name
+        assert r["verify_max_violation"] == max(v["maximum_constraint_violation"], v["maximum_lower_bound_violation"],
+                                                 v["maximum_upper_bound_violation"])
+        if r["status"] == 0 and name != "iteration_limit_5":
+            assert r["verify_status"] == 0, (name, r["verify_max_violation"])
+    mb = lpdata.ModelBuilder()
+    mb.add_variable(5.0, 2.0, 1.0)  # validation failure: no solution, nothing to verify
+    assert pkg.first_order_solve(mb.build(), backend="cpu")["verify_status"] == -1

@@ -190,6 +190,9 @@ def test_dropin_matches_reference_golden(pkg, models, name):
     assert float.hex(r["objective_value"]) == g["objective"]
     assert float.hex(r["constraint_violation"]) == g["constraint_violation"]
     assert sha(r["x"]) == g["x_sha"] and sha(r["y"]) == g["y_sha"]
+    # the drop-in's own verifier verdict on the returned point
+    v = pkg.verify_solution(fm, r["x"], reported_objective=r["objective_value"])
+    assert r["verify_status"] == v["status"]
 
 
 def test_dropin_validation_and_infeasible_presolve(pkg, lpdata):
