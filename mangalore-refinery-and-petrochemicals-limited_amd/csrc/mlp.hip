@@ -7,12 +7,11 @@
 // weights are BF16 in the products; every product accumulates in FP32.
 //
 // The product kernel (k_gemm) computes C[M x N] = op(A)[M x K] op(B)[K x N]
-// with v_mfma_f32_32x32x8_bf16: a 256-thread workgroup owns a 64 x 64 tile of
-// C, each wavefront a 32 x 32 quarter; K advances 32 at a time through LDS
-// (A and B tiles of 64 x 32 / 32 x 64 BF16).  Operand layout of the
-// instruction (lane l, 0..63): A row l % 32, k = 4 (l / 32) + 0..3; B column
-// l % 32, same k; result register r of lane l is C[8 (r / 4) + 4 (l / 32) +
-// r % 4][l % 32].  op(A) / op(B) read the stored matrix directly or
+// with gfx950's v_mfma_f32_32x32x16_bf16: a 256-thread workgroup owns a 64 x 64
+// tile of C, each wavefront a 32 x 32 quarter; K advances 64 at a time
+// through LDS, 16 per instruction.  Operand layout of the instruction (lane
+// l, 0..63): A row l % 32, k = 8 (l / 32) + 0..7; B column l % 32, same k;
+// result register r of lane l is C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  op(A) / op(B) read the stored matrix directly or
 // transposed (strides), so forward (X W), input gradients (dZ W') and weight
 // gradients (H' dZ) use the one kernel.  The epilogue adds a bias, applies
 // ReLU and writes FP32 and / or BF16.
@@ -36,7 +35,7 @@ bo_status ensure_device_public();
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // float -> bfloat16, round to nearest even (NaN kept quiet)
@@ -73,7 +72,7 @@ typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 // in flight while the current tile is multiplied.
 //
 // LDS tiles hold both operands as [row of C or column of C][k], k
-// contiguous, so each lane reads its 4 k-values with one 8-byte LDS load.
+// contiguous, so each lane reads its 8 k-values with one 16-byte LDS load.
 template <int L>
 __device__ inline void tile_load(us8* v, const unsigned short* __restrict__ G, long long s_r, long long s_k, int r0,
                                  int R, int k0, int kend, bool vec) {
@@ -153,11 +152,11 @@ k_gemm(int M, int N, int K, int kc, const unsigned short* __restrict__ A, long l
             tile_load<LB>(rb, B, sbn, sbk, bn, N, k0 + kTk, ke, vb);
         }
 #pragma unroll
-        for (int kk = 0; kk < kTk; kk += 8) {
-            const int kq = kk + 4 * (lane / 32);
-            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + lane % 32][kq]);
-            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + lane % 32][kq]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        for (int kk = 0; kk < kTk; kk += 16) {
+            const int kq = kk + 8 * (lane / 32);
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm + lane % 32][kq]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn + lane % 32][kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
     }

@@ -74,3 +74,21 @@ def gradients(params, shapes, x, t, quant=False):
         dh = dz @ _q(W, quant).T
         dz = _q(np.where(hs[l] > 0.0, dh, 0.0), quant)
     return loss, pack(grads)
+
+
+def adam_steps(params, grads, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """Adam (Kingma & Ba 2015) in FP64 from zero moments: for t = 1, 2, ...
+    with gradient grads[t-1], m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+    w -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps).  Returns the
+    parameters after every step (a list)."""
+    w = np.asarray(params, np.float64).copy()
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    out = []
+    for t, g in enumerate(grads, start=1):
+        g = np.asarray(g, np.float64)
+        m = beta1 * m + (1.0 - beta1) * g
+        v = beta2 * v + (1.0 - beta2) * g * g
+        w = w - lr * (m / (1.0 - beta1 ** t)) / (np.sqrt(v / (1.0 - beta2 ** t)) + eps)
+        out.append(w.copy())
+    return out

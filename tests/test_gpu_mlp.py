@@ -83,3 +83,32 @@ def test_training_reduces_loss(pkg):
     final, _ = net.gradients(x[:64], t[:64])
     print("loss %.5f -> %.5f" % (first, final), flush=True)
     assert final < 0.1 * first
+
+
+def test_adam_steps_match_oracle(pkg):
+    """k_adam (FP32 master weights) against the FP64 Adam of oracle/mlp_ref.py
+    (pinned by hand-worked steps in tests/test_mlp_oracle.py): three
+    training steps from set parameters, each step's gradient taken with
+    bo_mlp_gradients at the parameters the previous step left; the
+    parameters after each step must match the oracle's to FP32 rounding of
+    the update."""
+    rng = np.random.default_rng(21)
+    n_in, width, n_out, B = 30, 64, 20, 8
+    net = pkg.Mlp(n_in, width, n_out, layers=3, max_batch=16, seed=4)
+    p0 = (rng.standard_normal(net.nparam) * 0.2).astype(np.float32)
+    net.set_params(p0)  # also resets Adam's moments and step count
+    x = rng.standard_normal((B, n_in)).astype(np.float32)
+    t = rng.standard_normal((B, n_out)).astype(np.float32)
+    lr = 1e-3
+    grads, got = [], []
+    for _ in range(3):
+        _, g = net.gradients(x, t)  # gradient at the current parameters (the one train_step uses)
+        grads.append(g.astype(np.float64))
+        net.train_step(x, t, lr)
+        got.append(net.params().astype(np.float64))
+    want = mlp_ref.adam_steps(p0.astype(np.float64), grads, lr)
+    for step, (a, b) in enumerate(zip(got, want), start=1):
+        move = np.abs(b - p0)
+        err = np.max(np.abs(a - b) / (move + 1e-7 * np.abs(b) + 1e-12))
+        print("adam step", step, "max relative error of the move %.2e" % err, flush=True)
+        assert np.all(np.abs(a - b) <= 1e-3 * move + 2e-7 * np.abs(b) + 1e-9), step

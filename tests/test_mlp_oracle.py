@@ -50,3 +50,25 @@ def test_bf16_rounding_ties_to_even():
     assert mlp_ref.bf16_round(np.float32(1.0 + 2.0 ** -8)) == 1.0
     assert mlp_ref.bf16_round(np.float32(1.0 + 3 * 2.0 ** -8)) == 1.0 + 2.0 ** -6
     assert mlp_ref.bf16_round(np.float32(-2.5)) == -2.5
+
+
+def test_adam_oracle_hand_worked_steps():
+    """Adam pinned by hand: at step 1 the bias-corrected moments are g and
+    g^2, so the move is lr g / (|g| + eps) (lr sign(g) up to eps); at step 2
+    with gradients g1, g2: m = 0.09 g1 + 0.1 g2, v = 0.000999 g1^2 + 0.001 g2^2,
+    m_hat = m / 0.19, v_hat = v / 0.001999."""
+    from oracle import mlp_ref
+    w0 = np.array([1.0, -2.0, 0.5, 0.0])
+    g1 = np.array([0.3, -4.0, 1e-3, 0.0])
+    g2 = np.array([-0.6, 2.0, 1e-3, 5.0])
+    lr = 0.01
+    s1, s2 = mlp_ref.adam_steps(w0, [g1, g2], lr)
+    exp1 = w0 - lr * g1 / (np.abs(g1) + 1e-8)
+    assert np.allclose(s1, exp1, rtol=0, atol=1e-15)
+    m = 0.09 * g1 + 0.1 * g2
+    v = 0.000999 * g1 ** 2 + 0.001 * g2 ** 2
+    exp2 = exp1 - lr * (m / 0.19) / (np.sqrt(v / 0.001999) + 1e-8)
+    assert np.allclose(s2, exp2, rtol=0, atol=1e-14)
+    # a planted mistake (no bias correction) must differ
+    wrong = w0 - lr * (0.1 * g1) / (np.sqrt(0.001 * g1 ** 2) + 1e-8)
+    assert not np.allclose(s1, wrong, rtol=0, atol=1e-6)
