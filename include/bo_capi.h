@@ -247,6 +247,7 @@ typedef struct bo_pdlp_pass_info {
     long long padded;    /* tiled: stored entries including slice padding */
     double gather_bytes; /* algorithmic bytes of one launch of the (gather) kernel */
     double reduce_bytes; /* ... of the reduce + epilogue kernel; 0 when fused */
+    int reduce_loads;    /* tiled: partials loaded per round of the reduction (tuned) */
 } bo_pdlp_pass_info;
 bo_status bo_pdlp_get_pass_info(struct bo_pdlp* h, int pass, bo_pdlp_pass_info* out);
 /* Per-launch averages over `reps` alternating iterations, HIP events on the
@@ -439,6 +440,26 @@ bo_status bo_presolve_stats(const bo_model* model, int* status, int* reduced_var
  * BO_GPU_UNAVAILABLE when the GPU backend was asked for and no GPU is visible. */
 bo_status bo_first_order_solve(const bo_model* model, const bo_fo_options* opts,
                                bo_fo_result* result, double* primal_out, double* dual_out);
+
+/* ---- routing cost model ---------------------------------------------------- */
+/* Predicted wall time of one GPU first-order solve on the MI355X. */
+typedef struct bo_cost_estimate {
+    double total_ms;          /* create_ms + fixed_ms + iterations x per_iteration_ms */
+    double create_ms;         /* upload and layout (PDLP: handle creation with schedule tuning) */
+    double fixed_ms;          /* per-solve device set-up */
+    double per_iteration_ms;
+    long long iterations;     /* iterations assumed */
+    int loop;                 /* reference engine: its loop tier (bo_ref_result.loop); PDLP: -1 */
+    int in_domain;            /* 1: nnz within the measured range (1,500 .. 17.1M) */
+} bo_cost_estimate;
+
+/* Replaces the GPU estimate of AnalyticalCostEstimator::estimate
+ * (src/cost_estimator.cpp:188-197, the analytic 24.5 + 0.0008 nnz ms) with
+ * costs measured on an MI355X (calibrate_cost.py), interpolated in log(nnz).
+ * engine 0: reference PDHG (bo_pdhg_ref_solve), 1: PDLP (bo_pdlp_*);
+ * iterations 0: 50,000 (reference) or 10,000 (PDLP).  Host arithmetic only. */
+bo_status bo_pdhg_cost_estimate(int m, int n, long long nnz, int engine, long long iterations,
+                                bo_cost_estimate* out);
 
 /* ---- solution verifier ----------------------------------------------------- */
 /* VerificationOptions (include/bharatopt/solution_verifier.hpp:13-18). */

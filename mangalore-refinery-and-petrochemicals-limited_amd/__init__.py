@@ -93,7 +93,7 @@ class PdlpStats(C.Structure):
 class PdlpPassInfo(C.Structure):
     _fields_ = [("schedule", C.c_int), ("kernels", C.c_int), ("blocks", C.c_int), ("block_entries", C.c_int),
                 ("tiles", C.c_int), ("slices", C.c_int), ("pairs", C.c_longlong), ("padded", C.c_longlong),
-                ("gather_bytes", C.c_double), ("reduce_bytes", C.c_double)]
+                ("gather_bytes", C.c_double), ("reduce_bytes", C.c_double), ("reduce_loads", C.c_int)]
 
 
 class PdlpBatchStats(C.Structure):
@@ -120,6 +120,12 @@ class VerifyResult(C.Structure):
         "maximum_lower_bound_violation", "maximum_upper_bound_violation", "maximum_integrality_violation")] + [
         (k, C.c_longlong) for k in ("violated_constraint_count", "violated_bound_count", "violated_integrality_count",
                                     "variable_count", "constraint_count")] + [("status", C.c_int), ("diagnostics", C.c_int)]
+
+
+class CostEstimate(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("create_ms", C.c_double), ("fixed_ms", C.c_double),
+                ("per_iteration_ms", C.c_double), ("iterations", C.c_longlong), ("loop", C.c_int),
+                ("in_domain", C.c_int)]
 
 
 class MpsInfo(C.Structure):
@@ -166,7 +172,7 @@ EXPORTS = [
     "bo_pdlp_family_get_solutions", "bo_pdlp_family_time_passes", "bo_pdlp_family_destroy",
     "bo_mlp_create", "bo_mlp_num_params", "bo_mlp_get_params", "bo_mlp_set_params", "bo_mlp_forward",
     "bo_mlp_gradients", "bo_mlp_train_step", "bo_mlp_time", "bo_mlp_destroy", "bo_gemm_bf16",
-    "bo_mps_read", "bo_mps_model", "bo_mps_destroy", "bo_verify_solution",
+    "bo_mps_read", "bo_mps_model", "bo_mps_destroy", "bo_verify_solution", "bo_pdhg_cost_estimate",
 ]
 
 
@@ -210,6 +216,7 @@ def lib():
         L.bo_presolve_stats.argtypes = [P, P, P, P, P, P]
         L.bo_copy_kernel_calibrate.argtypes = [C.c_size_t, C.c_int, P]
         L.bo_verify_solution.argtypes = [P, P, C.c_longlong, P, P, P]
+        L.bo_pdhg_cost_estimate.argtypes = [C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_longlong, P]
         for name in EXPORTS:
             if name not in ("bo_last_error", "bo_buf_free", "bo_pdlp_destroy", "bo_pdlp_default_params",
                             "bo_pdlp_batch_destroy", "bo_pdlp_family_destroy", "bo_mlp_destroy",
@@ -313,6 +320,7 @@ class RefSolve:
     x: np.ndarray
     y: np.ndarray
     loop: int = 0  # 0 kernel per pass, 1 one workgroup, 2 one workgroup with the matrix in LDS
+    transfer_time_ms: float = 0.0  # host-side CSC build and upload
 
 
 def pdhg_ref_solve(lp: StdLP, max_iterations=50000, feasibility_tolerance=1e-7,
@@ -329,7 +337,8 @@ def pdhg_ref_solve(lp: StdLP, max_iterations=50000, feasibility_tolerance=1e-7,
     name = "bo_pdhg_ref_solve_noalloc" if noalloc else "bo_pdhg_ref_solve"
     _check(getattr(lib(), name)(C.byref(h.s), C.byref(o), _p(x), _p(y), C.byref(r)), name)
     return RefSolve(r.status, r.iterations, r.loop_iterations, r.constraint_violation,
-                    r.bound_violation, bool(r.used_average), r.kernel_time_ms, x[:lp.n], y[:lp.m], r.loop)
+                    r.bound_violation, bool(r.used_average), r.kernel_time_ms, x[:lp.n], y[:lp.m], r.loop,
+                    r.transfer_time_ms)
 
 
 def default_pdlp_params(**kw) -> PdlpParams:
@@ -677,6 +686,14 @@ def verify_solution(fm: FlatModel, x, reported_objective=None, feasibility_toler
                                     None if rep is None else C.byref(rep), C.byref(o), C.byref(r)),
            "bo_verify_solution")
     return {k: getattr(r, k) for k, _ in VerifyResult._fields_}
+
+
+def cost_estimate(m, n, nnz, engine=1, iterations=0):
+    """Predicted ms of a GPU first-order solve (bo_pdhg_cost_estimate; host only)."""
+    out = CostEstimate()
+    _check(lib().bo_pdhg_cost_estimate(int(m), int(n), int(nnz), int(engine), int(iterations), C.byref(out)),
+           "bo_pdhg_cost_estimate")
+    return {k: getattr(out, k) for k, _ in CostEstimate._fields_}
 
 
 def copy_kernel_ms(nbytes, reps=20):

@@ -523,14 +523,15 @@ void ref_host_prep(const bo_lp* lp, int* cp, int* ri, double* cv, int* pos, doub
 // sum and x_hat, runs the loop (one workgroup when the LP fits, else a
 // kernel per pass with the checks on the host), chooses the iterate and
 // downloads x and y.  Allocates nothing: every array is in d.  All work is
-// on the null stream; kernel_time_ms is the host's clock from the first
-// launch to the last download.
+// on the null stream; transfer_time_ms runs from t_xfer0 (the caller's start
+// of the host preparation) to the end of the uploads, kernel_time_ms is the
+// host's clock from the first launch to the last download.
 bo_status ref_run(const bo_lp* lp, const RefDev& d, const int* cp, const int* ri, const double* cv,
-                  const double* x0, const bo_ref_options* o, double* x_out, double* y_out, bo_ref_result* res) {
+                  const double* x0, const bo_ref_options* o, double* x_out, double* y_out, bo_ref_result* res,
+                  std::chrono::high_resolution_clock::time_point t_xfer0) {
     const int m = lp->m, n = lp->n;
     const long long nnz = lp->nnz;
     std::memset(res, 0, sizeof(*res));
-    const auto t_xfer0 = std::chrono::high_resolution_clock::now();
     auto h2d = [](void* dst, const void* src, size_t bytes) {
         return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, 0) : hipSuccess;
     };
@@ -775,6 +776,7 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
     if (st != BO_SUCCESS) return st;
     const int m = lp->m, n = lp->n;
     const long long nnz = lp->nnz;
+    const auto t0 = std::chrono::high_resolution_clock::now();
     std::vector<int> cp(static_cast<size_t>(n) + 1), ri(static_cast<size_t>(nnz)), pos(static_cast<size_t>(n) + 1);
     std::vector<double> cv(static_cast<size_t>(nnz)), x0(static_cast<size_t>(n) + 1);
     ref_host_prep(lp, cp.data(), ri.data(), cv.data(), pos.data(), x0.data());
@@ -800,7 +802,7 @@ extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o,
     BO_TRY(mem.alloc(&d.xavg, n));
     BO_TRY(mem.alloc(&d.out, 8));
     BO_TRY(mem.alloc(&d.red, 8));
-    return ref_run(lp, d, cp.data(), ri.data(), cv.data(), x0.data(), o, x_out, y_out, res);
+    return ref_run(lp, d, cp.data(), ri.data(), cv.data(), x0.data(), o, x_out, y_out, res, t0);
 }
 
 extern "C" bo_status bo_pdhg_ref_solve_noalloc(const bo_lp* lp, const bo_ref_options* o, double* x_out,
@@ -825,6 +827,7 @@ extern "C" bo_status bo_pdhg_ref_solve_noalloc(const bo_lp* lp, const bo_ref_opt
         set_last_error("bo_pdhg_ref_solve_noalloc: scratch layout exceeds its size");
         return BO_EXECUTION_FAILED;
     }
+    const auto t0 = std::chrono::high_resolution_clock::now();
     ref_host_prep(lp, g_h_cp, g_h_ri, g_h_cv, g_h_pos, g_h_x0);
-    return ref_run(lp, d, g_h_cp, g_h_ri, g_h_cv, g_h_x0, o, x_out, y_out, res);
+    return ref_run(lp, d, g_h_cp, g_h_ri, g_h_cv, g_h_x0, o, x_out, y_out, res, t0);
 }

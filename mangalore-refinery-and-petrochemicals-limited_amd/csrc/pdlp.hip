@@ -927,6 +927,7 @@ struct Engine final : EngineBase {
         unsigned lds_bytes = 0;  // dynamic LDS of the gather kernel
         long long pairs = 0;
         int max_pairs = 0;  // most partials a segment adds
+        int reduce_u = 0;   // loads in flight per round of the reduction (0: cover max_pairs)
         size_t padded = 0;
         TileDesc* td = nullptr;
         unsigned* words = nullptr;  // slice words
@@ -1272,7 +1273,7 @@ struct Engine final : EngineBase {
                                t.td, t.words, t.val, t.idx, gvec, ng, t.W, t.part, nullptr);
         if (mid) hipEventRecord(mid, stream);
         launch_tile_reduce(stream, t.max_pairs, nseg, t.seg_ptr, t.slot, static_cast<int>(t.pairs), t.part,
-                           t.nslices * kWave, e, done);
+                           t.nslices * kWave, e, done, t.reduce_u);
     }
 
     void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
@@ -1426,6 +1427,31 @@ struct Engine final : EngineBase {
         }
         plan_rows = pr;
         plan_cols = pc;
+        // Loads in flight per round of each tiled reduction: the fewest
+        // that cover a segment's partials wastes address-unit work when most
+        // segments have far fewer partials than the longest (C1's column
+        // pass: at most 5, mostly 1); timed on the real layout, kept when
+        // 3% faster.
+        for (int pass = 0; pass < 2; ++pass) {
+            TilePass& t = tp[pass];
+            if ((pass == 0 ? plan_rows.tiled : plan_cols.tiled) <= 0 || t.max_pairs <= 1) continue;
+            int cover = 1;
+            while (cover < t.max_pairs && cover < 16) cover *= 2;
+            double best = 1e30;
+            int best_u = 0;
+            for (int u = cover; u >= 1; u /= 2) {
+                t.reduce_u = u;
+                double ms4[4];
+                bo_status s = time_pair(plan_rows, plan_cols, 4, nullptr, nullptr, ms4);
+                if (s != BO_SUCCESS) return s;
+                const double red = pass == 0 ? ms4[3] : ms4[1];
+                if (best_u == 0 || red < 0.97 * best) {
+                    best = red;
+                    best_u = u;
+                }
+            }
+            t.reduce_u = best_u;
+        }
         if (prm.check_every <= 0) {
             // check cadence: 128 steps when one FP64 check costs at least
             // four iteration steps (it then stays under 3% of a solve), else 64
@@ -1811,6 +1837,9 @@ struct Engine final : EngineBase {
             out->gather_bytes = (tb + 2.0) * nnz + tb * ng + 4.0 * t.nslices +
                                 static_cast<double>(sizeof(TileDesc)) * t.ntiles + 8.0 * t.pairs;
             out->reduce_bytes = 4.0 * (nseg + 1) + 12.0 * t.pairs + epi * nseg;
+            int cover = 1;
+            while (cover < t.max_pairs && cover < 16) cover *= 2;
+            out->reduce_loads = t.reduce_u > 0 ? t.reduce_u : cover;
         } else {
             out->kernels = 1;
             out->gather_bytes = (tb + 4.0) * nnz + 4.0 * (nseg + 1) + tb * ng + epi * nseg;

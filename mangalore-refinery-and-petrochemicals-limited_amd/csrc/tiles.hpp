@@ -289,6 +289,29 @@ __device__ inline unsigned short buf_idx(__amdgpu_buffer_rsrc_t r, int byte_off)
     return __builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0);
 }
 
+// A double moved between lanes by a DPP control (both 32-bit halves); lanes
+// whose source is outside the pattern, or whose row is masked off, get 0.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes, in lane 63 (other lanes hold partial sums): pairs
+// and quads by quad permutes, the 16 lanes of a row by shifts of 4 and 8,
+// then row 0 into row 1 and rows 0-1 into rows 2-3 by row broadcasts.  Lane
+// moves run on the vector ALU, with no LDS round trip between the steps.
+__device__ inline double wave_total_dpp(double v) {
+    v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0x114, 0xF>(v);  // row_shr:4
+    v += dpp_move<0x118, 0xF>(v);  // row_shr:8
+    v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // CH rows of a wavefront's run, starting at row r (rows past the run read 0).
 template <class T, int CH>
 struct RowChunk {
@@ -343,8 +366,8 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
         if (r + u + 1 == st.bound) {  // last row of the current slice (uniform)
             double* row = part + static_cast<size_t>(wd.slice0 + st.sl) * kWave;
             if (st.wide) {
-                const double t = wave_sum(st.acc);
-                if (lane == 0) row[0] = t;
+                const double t = wave_total_dpp(st.acc);
+                if (lane == kWave - 1) row[0] = t;
             } else {
                 row[lane] = st.acc;
             }
@@ -495,11 +518,15 @@ k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restr
     epi_finish(epi, s, acc, pre, 0);
 }
 
-// Launch of the reduction with U covering the most pairs a segment has.
+// Launch of the reduction with U loads in flight per round: `u` when given
+// (a tuned value: a segment's partials are still added in order, so any U
+// gives the same sums), else the smallest U covering the most pairs a
+// segment has.
 template <class Epi>
 inline void launch_tile_reduce(hipStream_t stream, int max_pairs, int nseg, const int* seg_ptr, const unsigned* slot,
-                               int nslot, const double* part, int npart, const Epi& e, const int* done) {
+                               int nslot, const double* part, int npart, const Epi& e, const int* done, int u = 0) {
     const dim3 grid(grid_for(static_cast<size_t>(nseg))), block(kBlock);
+    if (u > 0) max_pairs = u;
     if (max_pairs <= 1)
         hipLaunchKernelGGL((k_tile_reduce<1, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
                            e, done);
