@@ -8,9 +8,12 @@
 //
 // Layout of a tile's pairs (built on the host):
 //   * pairs sorted by length; 64 of them form a SELL slice (entry j of lane l
-//     at slice start + 64 j + l, padded to the slice's longest pair), and a
-//     long pair forms a "wide" slice of its own instead (entry j at slice
-//     start + j: the 64 lanes stride over it and add with a fixed xor tree);
+//     at slice start + 64 j + l, padded to the slice's longest pair).  A long
+//     pair (128 or more nonzeros) gets L lanes instead, L = 8 .. 64 so that
+//     it spans 16 to 31 rows: 64 / L such pairs share a slice, entry j of
+//     member p at row j / L, lane p L + j % L, and at the slice's end each
+//     group of L lanes adds its values with lane moves (DPP) into its last
+//     lane;
 //   * the slices are dealt to the 16 wavefronts longest-first (LPT) and each
 //     wavefront's slices are stored back to back, so a wavefront streams ONE
 //     contiguous run of 64-entry rows: FP32 values and 16-bit block-local
@@ -34,8 +37,10 @@ namespace bo {
 
 constexpr int kTileThreads = 1024;           // one workgroup per CU: the block fills most of the LDS
 constexpr int kTileWaves = kTileThreads / kWave;
-constexpr int kWideMin = 128;                // pair length from which a pair gets a wide slice
-constexpr unsigned kWideFlag = 0x80000000u;  // slice word: wide slice (low bits: rows)
+constexpr int kWideMin = 128;                // pair length from which a pair gets L > 1 lanes
+constexpr int kLaneRows = 16;                // such a pair spans kLaneRows .. 2 kLaneRows - 1 rows
+constexpr unsigned kLgShift = 28;            // slice word: rows | log2(lanes per pair) << kLgShift
+constexpr unsigned kRowsMask = (1u << kLgShift) - 1u;
 constexpr int kFillTrips = 10;               // 16-byte loads per thread covering a 160 KB block
 constexpr int kTileRows = 8;                 // rows of a wavefront's run in flight per buffer
 
@@ -59,7 +64,7 @@ struct HostTiles {
     int B = 0;  // blocks
     int W = 0;  // entries per block
     std::vector<TileDesc> tiles;
-    std::vector<unsigned> slice_word;   // per slice: rows | kWideFlag for a wide slice
+    std::vector<unsigned> slice_word;   // per slice: rows | log2(lanes per pair) << kLgShift
     std::vector<unsigned short> idx;    // padded entries: block-local gathered index
     std::vector<int> src;               // padded entries: source nonzero, -1 for padding
     std::vector<int> seg_ptr;           // per segment: range of its partial slots in `slot`
@@ -73,7 +78,7 @@ struct HostTiles {
 // Builds the tiled layout of a segment matrix (ptr over nseg segments, gidx
 // the gathered index of every nonzero, ng gathered entries).  wmax: largest
 // block (a multiple of 64); target_tiles: about how many tiles to cut;
-// wide_min: pair length from which a pair gets a wide slice.
+// wide_min: pair length from which a pair gets L > 1 lanes.
 // Returns false when the padded layout does not fit 32-bit offsets.
 inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int wmax, int target_tiles, HostTiles& h,
                         int wide_min = kWideMin) {
@@ -172,25 +177,35 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
     std::vector<unsigned> pair_slot;
     std::vector<size_t> order;
     struct Slice {
-        std::vector<size_t> members;  // pair indices (into pairs[c]); one for a wide slice
+        std::vector<size_t> members;  // pair indices (into pairs[c])
         int rows;
-        bool wide;
+        int lg;  // log2 of the lanes each member gets
+    };
+    // lanes of a pair of `len` nonzeros: 1 below wide_min, else the power of
+    // two (8 .. 64) that makes it span kLaneRows .. 2 kLaneRows - 1 rows
+    auto lanes_lg = [wide_min](int len) {
+        if (len < wide_min) return 0;
+        int lg = 3;
+        while (lg < 6 && len >= (2 << lg) * kLaneRows) ++lg;
+        return lg;
     };
     for (const Span& sp : spans) {
         const std::vector<Pair>& P = pairs[sp.c];
         order.resize(sp.p1 - sp.p0);
         std::iota(order.begin(), order.end(), sp.p0);
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return P[a].len > P[b].len; });
-        // slices, longest pairs first: a wide slice per pair of wide_min or
-        // more nonzeros, then SELL slices of 64 pairs
+        // slices, longest pairs first: pairs of one lane count together, 64 / L
+        // per slice (L lanes each), padded to the slice's longest member
         std::vector<Slice> sl;
-        size_t q = 0;
-        for (; q < order.size() && P[order[q]].len >= wide_min; ++q)
-            sl.push_back(Slice{{order[q]}, (P[order[q]].len + kWave - 1) / kWave, true});
-        for (; q < order.size(); q += kWave) {
-            Slice s{{}, P[order[q]].len, false};
-            for (size_t r = q; r < std::min(order.size(), q + kWave); ++r) s.members.push_back(order[r]);
+        for (size_t q = 0; q < order.size();) {
+            const int lg = lanes_lg(P[order[q]].len);
+            const int lanes = 1 << lg;
+            Slice s{{}, (P[order[q]].len + lanes - 1) / lanes, lg};
+            size_t r = q;
+            for (; r < order.size() && r < q + static_cast<size_t>(kWave / lanes) && lanes_lg(P[order[r]].len) == lg; ++r)
+                s.members.push_back(order[r]);
             sl.push_back(std::move(s));
+            q = r;
         }
         // deal the slices to the wavefronts, longest first, to the least loaded
         std::vector<int> owner(sl.size());
@@ -220,21 +235,25 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
                 if (off + ent_n > 0xFFFFFFFFull) return false;
                 const unsigned gslice = static_cast<unsigned>(h.slice_word.size());
                 if (gslice >= (1u << 26)) return false;
-                h.slice_word.push_back(static_cast<unsigned>(s.rows) | (s.wide ? kWideFlag : 0u));
+                if (static_cast<unsigned>(s.rows) > kRowsMask) return false;
+                h.slice_word.push_back(static_cast<unsigned>(s.rows) | (static_cast<unsigned>(s.lg) << kLgShift));
                 h.padded += ent_n;
                 h.idx.resize(h.padded, 0);
                 h.src.resize(h.padded, -1);
-                for (size_t lane = 0; lane < s.members.size(); ++lane) {
-                    const Pair& p = P[s.members[lane]];
+                const int lanes = 1 << s.lg;
+                for (size_t mi = 0; mi < s.members.size(); ++mi) {
+                    const Pair& p = P[s.members[mi]];
+                    const size_t lane0 = mi * static_cast<size_t>(lanes);
                     for (int j = 0; j < p.len; ++j) {
                         const int k = ent[p.start + j];
-                        const size_t e = s.wide ? off + j : off + static_cast<size_t>(j) * kWave + lane;
+                        const size_t e = off + static_cast<size_t>(j / lanes) * kWave + lane0 + (j % lanes);
                         h.idx[e] = static_cast<unsigned short>(gidx[k] - sp.c * W);
                         h.src[e] = k;
                     }
                     pair_seg.push_back(p.s);
                     pair_blk.push_back(sp.c);
-                    pair_slot.push_back(gslice * kWave + static_cast<unsigned>(s.wide ? 0 : lane));
+                    // the member's sum ends in the last lane of its group
+                    pair_slot.push_back(gslice * kWave + static_cast<unsigned>(lane0 + lanes - 1));
                 }
                 wd.nrows += s.rows;
                 wd.nslice++;
@@ -298,17 +317,19 @@ __device__ inline double dpp_move(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// Sum over the 64 lanes, in lane 63 (other lanes hold partial sums): pairs
-// and quads by quad permutes, the 16 lanes of a row by shifts of 4 and 8,
-// then row 0 into row 1 and rows 0-1 into rows 2-3 by row broadcasts.  Lane
-// moves run on the vector ALU, with no LDS round trip between the steps.
-__device__ inline double wave_total_dpp(double v) {
-    v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-    v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-    v += dpp_move<0x114, 0xF>(v);  // row_shr:4
-    v += dpp_move<0x118, 0xF>(v);  // row_shr:8
-    v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+// Sum over each group of 2^lg consecutive lanes, in the group's last lane
+// (the other lanes hold partial sums): pairs and quads by quad permutes, 8
+// and 16 lanes by row shifts of 4 and 8, 32 lanes by broadcasting row 0 into
+// row 1 and row 2 into row 3, all 64 by broadcasting lane 31 into rows 2-3.
+// lg is uniform across the wavefront.  Lane moves run on the vector ALU,
+// with no LDS round trip between the steps.
+__device__ inline double lane_group_sum(double v, int lg) {
+    if (lg >= 1) v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    if (lg >= 2) v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    if (lg >= 3) v += dpp_move<0x114, 0xF>(v);  // row_shr:4
+    if (lg >= 4) v += dpp_move<0x118, 0xF>(v);  // row_shr:8
+    if (lg >= 5) v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    if (lg >= 6) v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -336,7 +357,7 @@ struct RunState {
     double acc;
     int sl;     // current slice within the run
     int bound;  // row after the current slice's last row
-    int wide;
+    int lg;     // log2 of the lanes per member of the current slice
     unsigned words;
 };
 
@@ -365,12 +386,8 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
         st.acc += static_cast<double>(c.a[u]) * static_cast<double>(gv);
         if (r + u + 1 == st.bound) {  // last row of the current slice (uniform)
             double* row = part + static_cast<size_t>(wd.slice0 + st.sl) * kWave;
-            if (st.wide) {
-                const double t = wave_total_dpp(st.acc);
-                if (lane == kWave - 1) row[0] = t;
-            } else {
-                row[lane] = st.acc;
-            }
+            // one coalesced row per slice; a member's sum is in its group's last lane
+            row[lane] = st.lg ? lane_group_sum(st.acc, st.lg) : st.acc;
             st.acc = 0.0;
             if (++st.sl < wd.nslice) {
                 // The refill of the word window waits for its own load (and
@@ -385,8 +402,8 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
                     w = static_cast<unsigned>(
                         __builtin_amdgcn_readlane(static_cast<int>(st.words), st.sl & (kWave - 1)));
                 }
-                st.bound += static_cast<int>(w & ~kWideFlag);
-                st.wide = (w & kWideFlag) != 0;
+                st.bound += static_cast<int>(w & kRowsMask);
+                st.lg = static_cast<int>(w >> kLgShift);
             }
         }
     }
@@ -441,8 +458,8 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     if (nrows == 0) return;
     {
         const unsigned w0 = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(st.words), 0));
-        st.bound = static_cast<int>(w0 & ~kWideFlag);
-        st.wide = (w0 & kWideFlag) != 0;
+        st.bound = static_cast<int>(w0 & kRowsMask);
+        st.lg = static_cast<int>(w0 >> kLgShift);
     }
     // Double-buffered walk: the next CH rows are loaded before the current
     // ones are gathered and their finished slices stored, so no store sits
