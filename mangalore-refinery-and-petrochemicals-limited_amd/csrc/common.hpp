@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 #include "../../include/bo_capi.h"
 
@@ -92,6 +94,34 @@ __device__ inline auto wg_dead(const E& e, int block, int) -> decltype(e.wg_dead
 template <class E>
 __device__ inline bool wg_dead(const E&, int, long) {
     return false;
+}
+
+// Split epilogues: an epilogue with load(s) / finish(s, acc, pre) has the
+// data of segment s it reads besides the segment's sum loaded by the kernel
+// before that sum exists, so those loads overlap the pass's own; others run
+// as epi(s, acc, part) once the sum is known.
+template <class E, class = void>
+struct EpiHasLoad : std::false_type {};
+template <class E>
+struct EpiHasLoad<E, std::void_t<decltype(std::declval<const E&>().load(0))>> : std::true_type {};
+
+template <class E>
+__device__ inline auto epi_preload(const E& e, int s, int) -> decltype(e.load(s)) {
+    return e.load(s);
+}
+template <class E>
+__device__ inline int epi_preload(const E&, int, long) {
+    return 0;
+}
+template <class E, class P>
+__device__ inline auto epi_finish(const E& e, int s, double acc, const P& p, int)
+    -> decltype(e.finish(s, acc, p)) {
+    e.finish(s, acc, p);
+}
+template <class E, class P>
+__device__ inline void epi_finish(const E& e, int s, double acc, const P&, long) {
+    double r[1] = {acc};
+    e(s, r, nullptr);
 }
 
 __device__ inline double wave_sum(double v) {

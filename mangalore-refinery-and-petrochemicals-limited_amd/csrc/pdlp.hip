@@ -918,7 +918,7 @@ struct Engine final : EngineBase {
     bool one_wg = false;                // the whole solve in one workgroup (k_small)
     bool small_lds_set = false;         // k_small's dynamic LDS limit raised
     bool power_pending = false;         // prepare() left the power iteration and the start to k_small_lds
-    int* sblk[2][3] = {};               // stream blocks per pass (rows, cols) and LDS budget
+    int4* sblk[2][3] = {};              // stream block descriptors per pass (rows, cols) and LDS budget
     int nsblk[2][3] = {};
     // tiled layout per pass (0: CSR pass gathering x+, 1: CSC pass gathering y)
     struct TilePass {
@@ -1089,11 +1089,17 @@ struct Engine final : EngineBase {
         // stream-kernel blocks for every LDS budget the plans may use
         for (int pass = 0; pass < 2; ++pass) {
             const std::vector<int>& len = pass == 0 ? row_len_sorted : col_len_sorted;
+            std::vector<int> first(len.size() + 1, 0);  // first nonzero of every segment
+            for (size_t q = 0; q < len.size(); ++q) first[q + 1] = first[q] + len[q];
             for (int c = 0; c < 3; ++c) {
                 const std::vector<int> blk = stream_blocks(len, kStreamCaps[c]);
-                nsblk[pass][c] = static_cast<int>(blk.size()) - 1;
-                BO_TRY(mem.alloc(&sblk[pass][c], blk.size()));
-                BO_TRY(up(sblk[pass][c], blk.data(), sizeof(int) * blk.size()));
+                std::vector<int4> desc(blk.size() - 1);
+                for (size_t b = 0; b + 1 < blk.size(); ++b)
+                    desc[b] = make_int4(blk[b], blk[b + 1], first[blk[b]], first[blk[b + 1]]);
+                nsblk[pass][c] = static_cast<int>(desc.size());
+                BO_TRY(mem.alloc(&sblk[pass][c], desc.size()));
+                BO_TRY(up(sblk[pass][c], desc.data(), sizeof(int4) * desc.size()));
+                BO_TRY(hipStreamSynchronize(stream));  // the host staging goes out of scope
             }
         }
         BO_TRY(hipStreamSynchronize(stream));
@@ -1241,15 +1247,15 @@ struct Engine final : EngineBase {
         const int nb = nsblk[pass][c];
         switch (c) {
             case 0:
-                hipLaunchKernelGGL((k_stream<1024, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c], nb,
+                hipLaunchKernelGGL((k_stream<1024, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c],
                                    ptr, idx, vals, e, done);
                 break;
             case 1:
-                hipLaunchKernelGGL((k_stream<2048, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c], nb,
+                hipLaunchKernelGGL((k_stream<2048, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c],
                                    ptr, idx, vals, e, done);
                 break;
             default:
-                hipLaunchKernelGGL((k_stream<4096, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c], nb,
+                hipLaunchKernelGGL((k_stream<4096, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c],
                                    ptr, idx, vals, e, done);
                 break;
         }

@@ -138,12 +138,19 @@ __device__ inline double group_reduce(double v) {
 //   static constexpr int Q;         number of per-block partial sums (0..8)
 //   const TX* vec(int v) const;     gathered vector v
 //   __device__ void operator()(int seg, const double* acc, double* part) const;
+// A split epilogue (load / finish, NV == 1, Q == 0, sums of products) has
+// segment seg's own data loaded by the group's first lane together with the
+// segment's bounds, before the entries; done (optional) is read at the same
+// time.  Returns false when done is set (nothing has been written).
 template <int MODE, int L, class TV, class TX, class Epi, bool HOT = false>
-__device__ inline void seg_process(const BinTable& bt, int bin, int vblock, const int* __restrict__ ptr,
+__device__ inline bool seg_process(const BinTable& bt, int bin, int vblock, const int* __restrict__ ptr,
                                    const int* __restrict__ idx, const TV* __restrict__ val,
-                                   const Epi& epi, double* part, HotCache<TX> hot = {nullptr, 0}) {
+                                   const Epi& epi, double* part, const int* __restrict__ done,
+                                   HotCache<TX> hot = {nullptr, 0}) {
     constexpr int NV = Epi::NV;
+    constexpr bool kSplit = EpiHasLoad<Epi>::value && NV == 1 && Epi::Q == 0 && MODE == kSumProd;
     const int seg_end = bt.seg_start[bin + 1];
+    const int dn = done ? *done : 0;
     if constexpr (L <= kWave) {
         constexpr int per_block = kBlock / L;
         const int seg = bt.seg_start[bin] + (vblock - bt.block_start[bin]) * per_block +
@@ -153,8 +160,17 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
         const bool live = seg < seg_end;
+        int k0 = 0, k1 = 0;
+        decltype(epi_preload(epi, 0, 0)) pre{};
         if (live) {
-            const int k0 = ptr[seg], k1 = ptr[seg + 1];
+            k0 = ptr[seg];
+            k1 = ptr[seg + 1];
+            if constexpr (kSplit) {
+                if (lane == 0) pre = epi_preload(epi, seg, 0);
+            }
+        }
+        if (dn) return false;
+        if (live) {
             if constexpr (MODE == kWrite) {
                 for (int k = k0 + lane; k < k1; k += L) epi.write(seg, k, val[k], idx[k]);
             } else {
@@ -164,7 +180,10 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
         if constexpr (MODE != kWrite) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[v] = group_reduce<MODE, L>(acc[v]);
-            if (live && lane == 0) epi(seg, acc, part);
+            if (live && lane == 0) {
+                if constexpr (kSplit) epi_finish(epi, seg, acc[0], pre, 0);
+                else epi(seg, acc, part);
+            }
         }
     } else {
         // one segment per workgroup
@@ -174,9 +193,14 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
         const int k0 = ptr[seg], k1 = ptr[seg + 1];
+        decltype(epi_preload(epi, 0, 0)) pre{};
+        if constexpr (kSplit) {
+            if (threadIdx.x == 0) pre = epi_preload(epi, seg, 0);
+        }
+        if (dn) return false;
         if constexpr (MODE == kWrite) {
             for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kBlock) epi.write(seg, k, val[k], idx[k]);
-            return;
+            return true;
         } else {
             seg_accumulate<MODE, kBlock, NV, HOT>(k0, k1, static_cast<int>(threadIdx.x), idx, val, epi, acc, hot);
         }
@@ -196,9 +220,11 @@ __device__ inline void seg_process(const BinTable& bt, int bin, int vblock, cons
                 acc[v] = t;
             }
             (void)seg_end;
-            epi(seg, acc, part);
+            if constexpr (kSplit) epi_finish(epi, seg, acc[0], pre, 0);
+            else epi(seg, acc, part);
         }
     }
+    return true;
 }
 
 template <int MODE, class TV, class TX, class Epi>
@@ -206,7 +232,6 @@ __global__ void __launch_bounds__(kBlock)
 k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx,
            const TV* __restrict__ val, Epi epi, double* __restrict__ partials,
            const int* __restrict__ done) {
-    if (done && *done) return;
     if (wg_dead(epi, static_cast<int>(blockIdx.x), 0)) return;
     constexpr int Q = Epi::Q;
     const int vblock = static_cast<int>(blockIdx.x);
@@ -215,17 +240,19 @@ k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx
     double part[Q > 0 ? Q : 1];
 #pragma unroll
     for (int q = 0; q < (Q > 0 ? Q : 1); ++q) part[q] = 0.0;
+    bool go = true;
     switch (bt.lanes[bin]) {
-        case 1: seg_process<MODE, 1, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        case 2: seg_process<MODE, 2, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        case 4: seg_process<MODE, 4, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        case 8: seg_process<MODE, 8, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        case 16: seg_process<MODE, 16, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        case 32: seg_process<MODE, 32, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        case 64: seg_process<MODE, 64, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
-        default: seg_process<MODE, 256, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part); break;
+        case 1: go = seg_process<MODE, 1, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        case 2: go = seg_process<MODE, 2, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        case 4: go = seg_process<MODE, 4, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        case 8: go = seg_process<MODE, 8, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        case 16: go = seg_process<MODE, 16, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        case 32: go = seg_process<MODE, 32, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        case 64: go = seg_process<MODE, 64, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
+        default: go = seg_process<MODE, 256, TV, TX>(bt, bin, vblock, ptr, idx, val, epi, part, done); break;
     }
     if constexpr (Q > 0) {
+        if (!go) return;  // uniform: done is one flag
         __shared__ double s[kBlock / kWave];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -233,6 +260,7 @@ k_segments(BinTable bt, const int* __restrict__ ptr, const int* __restrict__ idx
             if (threadIdx.x == 0) partials[static_cast<size_t>(blockIdx.x) * Q + q] = t;
         }
     }
+    (void)go;
 }
 
 // Persistent variant with an LDS copy of the hottest `hot_count` entries of
@@ -257,15 +285,15 @@ k_segments_hot(BinTable bt, const int* __restrict__ ptr, const int* __restrict__
     for (int g = blockIdx.x; g < bt.total_blocks; g += gridDim.x) {
         while (bin + 1 < bt.nbins && g >= bt.block_start[bin + 1]) ++bin;
         switch (bt.lanes[bin]) {
-            case 1: seg_process<MODE, 1, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
-            case 2: seg_process<MODE, 2, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
-            case 4: seg_process<MODE, 4, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
-            case 8: seg_process<MODE, 8, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
-            case 16: seg_process<MODE, 16, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
-            case 32: seg_process<MODE, 32, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
-            case 64: seg_process<MODE, 64, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot); break;
+            case 1: seg_process<MODE, 1, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
+            case 2: seg_process<MODE, 2, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
+            case 4: seg_process<MODE, 4, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
+            case 8: seg_process<MODE, 8, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
+            case 16: seg_process<MODE, 16, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
+            case 32: seg_process<MODE, 32, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
+            case 64: seg_process<MODE, 64, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot); break;
             default:
-                seg_process<MODE, 256, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, hot);
+                seg_process<MODE, 256, TV, TX, Epi, true>(bt, bin, g, ptr, idx, val, epi, part, nullptr, hot);
                 __syncthreads();  // the one-segment path reuses its LDS reduction slots on its next trip
                 break;
         }

@@ -28,17 +28,22 @@ __device__ inline double group_sum(double v) {
     return v;
 }
 
+// Block b is blk[b] = {first segment, end segment, first nonzero, end
+// nonzero}.  Before anything waits, a workgroup has in flight its
+// descriptor and the done flag, then -- once the descriptor is in -- the
+// matrix entries, the bounds of the segment it reduces in phase 2 and that
+// segment's epilogue data (split epilogues), so a pass costs three dependent
+// memory round trips (descriptor, entries, gathers) before its stores.
 template <int CAP, class TV, class TX, class Epi>
 __global__ void __launch_bounds__(kBlock)
-k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, const int* __restrict__ idx,
+k_stream(const int4* __restrict__ blk, const int* __restrict__ ptr, const int* __restrict__ idx,
          const TV* __restrict__ val, Epi epi, const int* __restrict__ done) {
     static_assert(Epi::NV == 1 && Epi::Q == 0, "stream passes gather one vector and keep no partials");
-    if (done && *done) return;
     __shared__ double prod[CAP];
     __shared__ double red[kBlock / kWave];
-    const int b = blockIdx.x;
-    const int s0 = blk[b], s1 = blk[b + 1];
-    const int k0 = ptr[s0], k1 = ptr[s1];
+    const int4 d = blk[blockIdx.x];
+    const int dn = done ? *done : 0;
+    const int s0 = d.x, s1 = d.y, k0 = d.z, k1 = d.w;
     const int tid = threadIdx.x;
     const TX* xv = epi.vec(0);
     // branch-free loads and gathers for FP32 gathered vectors only (measured
@@ -46,6 +51,9 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
     // keep the per-entry branches
     constexpr bool kFlat = sizeof(TX) == 4;
     if (k1 - k0 > CAP) {  // one long segment: register accumulation, block reduction
+        decltype(epi_preload(epi, 0, 0)) pre{};
+        if (tid == 0) pre = epi_preload(epi, s0, 0);
+        if (dn) return;
         double acc = 0.0;
         for (int k = k0 + tid; k < k1; k += 4 * kBlock) {
             TV a[4];
@@ -84,10 +92,22 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
         if (tid == 0) {
             double t = 0.0;
             for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
-            double r[1] = {t};
-            epi(s0, r, nullptr);
+            epi_finish(epi, s0, t, pre, 0);
         }
         return;
+    }
+    // phase-2 set-up, in flight with the entries: the segment this thread
+    // group reduces, its bounds, and (lane 0) its epilogue data
+    const int nseg = s1 - s0;
+    const int L = stream_group(nseg);
+    const int seg = s0 + tid / L;
+    const int lane = tid % L;
+    int a0 = 0, a1 = 0;
+    decltype(epi_preload(epi, 0, 0)) pre{};
+    if (seg < s1) {
+        a0 = ptr[seg] - k0;
+        a1 = ptr[seg + 1] - k0;
+        if (lane == 0) pre = epi_preload(epi, seg, 0);
     }
     // phase 1: products of the block's nonzeros into LDS
     constexpr int U = CAP / kBlock;
@@ -103,6 +123,7 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
             a[u] = kk < k1 ? av : TV(0);
             j[u] = kk < k1 ? jv : -1;
         }
+        if (dn) return;  // uniform; nothing has been written
         // unconditional gathers (entry 0 past the block's nonzeros) so that
         // all U are in flight at once; products past k1 are never read
         TX g[U];
@@ -111,6 +132,7 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
 #pragma unroll
         for (int u = 0; u < U; ++u) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(g[u]);
     } else if (!kFlat) {
+        if (dn) return;
         TV a[U];
         int j[U];
 #pragma unroll
@@ -122,18 +144,13 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (j[u] >= 0) prod[tid + u * kBlock] = static_cast<double>(a[u]) * static_cast<double>(xv[j[u]]);
+    } else if (dn) {
+        return;
     }
     __syncthreads();
     // phase 2: segment sums from LDS, L lanes per segment
-    const int nseg = s1 - s0;
-    const int L = stream_group(nseg);
-    const int seg = s0 + tid / L;
-    const int lane = tid % L;
     double acc = 0.0;
-    if (seg < s1) {
-        const int a0 = ptr[seg] - k0, a1 = ptr[seg + 1] - k0;
-        for (int q = a0 + lane; q < a1; q += L) acc += prod[q];
-    }
+    for (int q = a0 + lane; q < a1; q += L) acc += prod[q];
     switch (L) {  // uniform across the block
         case 64: acc = group_sum<64>(acc); break;
         case 32: acc = group_sum<32>(acc); break;
@@ -143,10 +160,7 @@ k_stream(const int* __restrict__ blk, int nblk, const int* __restrict__ ptr, con
         case 2: acc = group_sum<2>(acc); break;
         default: break;
     }
-    if (seg < s1 && lane == 0) {
-        double r[1] = {acc};
-        epi(seg, r, nullptr);
-    }
+    if (seg < s1 && lane == 0) epi_finish(epi, seg, acc, pre, 0);
 }
 
 }  // namespace bo

@@ -480,27 +480,6 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     }
 }
 
-// An epilogue with load(s) / finish(s, acc, pre) has its own data loaded
-// before the partials (the loads overlap); others run as epi(s, acc, part).
-template <class E>
-__device__ inline auto epi_preload(const E& e, int s, int) -> decltype(e.load(s)) {
-    return e.load(s);
-}
-template <class E>
-__device__ inline int epi_preload(const E&, int, long) {
-    return 0;
-}
-template <class E, class P>
-__device__ inline auto epi_finish(const E& e, int s, double acc, const P& p, int)
-    -> decltype(e.finish(s, acc, p)) {
-    e.finish(s, acc, p);
-}
-template <class E, class P>
-__device__ inline void epi_finish(const E& e, int s, double acc, const P&, long) {
-    double r[1] = {acc};
-    e(s, r, nullptr);
-}
-
 // Segment s: its partials in block order, then the pass epilogue.  The slot
 // and partial loads of up to U pairs are issued at once: raw buffer loads at
 // an out-of-range offset read 0, so no load sits under a per-lane branch.
