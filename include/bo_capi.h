@@ -173,7 +173,9 @@ typedef struct bo_pdlp_params {
     int hot_columns;           /* entries of x+ kept in LDS by the CSR pass: -1 tune, 0 off */
     long long stall_iterations; /* batched and family engines, FP32 iterates: an LP whose FP64 rel-KKT
                                    has not halved within max(this, iterations / 2) iterations stops
-                                   as STALLED (4); 0 = never (default 4096) */
+                                   as STALLED (4); 0 = never (default 16384: on the 512-LP ragged
+                                   batch 4096 stalled 146 LPs that FP32 would have finished, at 8%
+                                   more batch time) */
     int fp64_polish;           /* batched and family engines: finish every STALLED LP with FP64 iterates from
                                   its FP32 point, within its remaining iteration budget (default 1) */
     int compact_tail;          /* batched engine: once at most 1/8 of a batch's LPs still run, continue

@@ -1881,7 +1881,7 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
     p->time_limit_s = 0.0;
     p->verbose = 0;
     p->hot_columns = -1;
-    p->stall_iterations = 4096;
+    p->stall_iterations = 16384;
     p->fp64_polish = 1;
     p->compact_tail = 1;
     p->single_workgroup = -1;

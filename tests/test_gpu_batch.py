@@ -142,19 +142,20 @@ def test_batch_rejects_bad_lp(pkg, lpdata):
 
 def test_batch_fp64_polish_of_stalled_lps(pkg, po):
     # Draws of a staircase family whose FP32 iterates stop short of 1e-6
-    # (their rel-KKT stops halving): with the polish off they end STALLED
+    # (their rel-KKT stops halving within 4096 iterations, the window these
+    # draws are known to stall in): with the polish off they end STALLED
     # (status 4); with it on, the FP64 engine finishes them from their FP32
     # points within the same iteration budget, and the oracle accepts them.
     fam = pkg.lpdata.StaircaseFamily(periods=12, blk_rows=100, blk_cols=250, inv_per_link=10, seed=7)
     lps = [fam.draw(k) for k in (31, 143, 185, 1, 2)]
     print("case: fp64 polish, staircase family draws 31 143 185 1 2", flush=True)
-    off = pkg.PdlpBatch(lps, eps_rel_kkt=TOL, max_iterations=40000, fp64_polish=0)
+    off = pkg.PdlpBatch(lps, eps_rel_kkt=TOL, max_iterations=40000, fp64_polish=0, stall_iterations=4096)
     per0, tot0 = off.solve()
     print("polish off", [(p["status"], p["iterations"]) for p in per0], flush=True)
     assert any(p["status"] == 4 for p in per0)
     assert all(p["status"] in (0, 1, 4) for p in per0)
     off.close()
-    on = pkg.PdlpBatch(lps, eps_rel_kkt=TOL, max_iterations=40000)
+    on = pkg.PdlpBatch(lps, eps_rel_kkt=TOL, max_iterations=40000, stall_iterations=4096)
     per, tot = on.solve()
     print("polish on", [(p["status"], p["iterations"], p["fp64_polish_iterations"]) for p in per], flush=True)
     assert tot["solved"] == len(lps)

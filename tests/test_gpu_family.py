@@ -88,7 +88,8 @@ def test_family_fp64_polish(pkg, po, family):
     # polish test): the family engine hands them to its FP64 engine
     ids = [31, 143, 185, 1, 2]
     draws = [family.draw(k) for k in ids]
-    f = pkg.PdlpFamily(family.base, capacity=len(ids), eps_rel_kkt=TOL, max_iterations=40000)
+    f = pkg.PdlpFamily(family.base, capacity=len(ids), eps_rel_kkt=TOL, max_iterations=40000,
+                       stall_iterations=4096)
     f.set_lps([d.c for d in draws], [d.b for d in draws])
     per, tot = f.solve()
     print("polish", [(p["status"], p["iterations"], p["fp64_polish_iterations"]) for p in per], flush=True)
