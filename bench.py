@@ -448,7 +448,7 @@ def bench_c4(pkg, args):
             log(f"family passes (ms): primal {fam_ms[0]:.4f} dual {fam_ms[1]:.4f}")
 
     t0 = time.perf_counter()
-    tr = fs.solve(train_ids)
+    tr = fs.solve(train_ids, progress=log)
     gen_s = time.perf_counter() - t0
     ok = [i for i, st in enumerate(tr["status"]) if st == 0]
     ok_train = len(ok)
@@ -480,6 +480,15 @@ def bench_c4(pkg, args):
     Xm, Ym, Wm = net.mean_start(len(eval_ids))
     mean = fs.solve_warm_with_fallback(eval_ids, (Xm, Ym, Wm))
     log(f"mean warm: {mean['solve_s']:.2f}s solved {(mean['status'] == 0).sum()}")
+    sens = None
+    if not args.no_sensitivity:
+        # how close a start must be to save iterations: the first batch of
+        # held-out draws from their own (cold) solutions, disturbed
+        sid = [i for i in range(min(args.c4_batch, len(eval_ids))) if cold["status"][i] == 0]
+        sens = {"draws": len(sid), "cold": float(np.mean([cold["iterations"][i] for i in sid]))}
+        sens.update(ws.start_sensitivity(fs, [eval_ids[i] for i in sid], [cold["x"][i] for i in sid],
+                                         [cold["y"][i] for i in sid], fam.base.lb, fam.base.ub))
+        log(f"start sensitivity (mean iterations): {sens}")
     fs.close()
 
     # plain cold single-LP engine on a sample of the held-out draws
@@ -553,6 +562,7 @@ def bench_c4(pkg, args):
         "iterations_p50_p90_max": {"cold": pct(cold["iterations"]), "network": pct(warm["iterations"]),
                                    "mean": pct(mean["iterations"])},
         "start_distance_median_rel": dist,
+        "start_sensitivity_mean_iterations": sens,
         "primal_weight": {"train_final_median": float(np.median(w_train)),
                           "predicted_median": float(np.median(Wn)),
                           "cold_final_median": float(np.median(cold["omega"]))},
@@ -608,6 +618,7 @@ def main():
                     help="c4: held-out draws also solved one at a time by the single-LP engine")
     ap.add_argument("--max-iterations", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sensitivity", action="store_true", help="c4: skip the start-sensitivity table")
     ap.add_argument("--no-warm-cold", action="store_true", help="c1: skip the compact warm-vs-cold leg")
     ap.add_argument("--no-fp64-run", action="store_true", help="c1/c2: skip the FP64-everywhere run")
     ap.add_argument("--wc-train-draws", type=int, default=512, help="warm-vs-cold leg: training draws")
@@ -826,7 +837,7 @@ def warm_vs_cold_leg(pkg, args):
     eval_ids = list(range(20001, 20001 + args.wc_eval_draws))
     fs.solve(eval_ids[:64], want_solutions=False)  # builds the engine, warms the kernels
     t0 = time.perf_counter()
-    tr = fs.solve(train_ids)
+    tr = fs.solve(train_ids, progress=log)
     gen_s = time.perf_counter() - t0
     ok = [i for i, st in enumerate(tr["status"]) if st == 0]
     F = np.array([fam.features(train_ids[i]) for i in ok])

@@ -77,10 +77,11 @@ class FamilySolver:
                     ys.append(y)
         return p, dt, xs, ys
 
-    def solve(self, ids, warm=None, want_solutions=True):
+    def solve(self, ids, warm=None, want_solutions=True, progress=None):
         """Solve draws `ids` (cold, or from warm = (X, Y) rows aligned with ids).
 
-        warm may carry a third member, starting primal weights.  Returns
+        warm may carry a third member, starting primal weights.  progress
+        (optional): called with a line of text about every 64 batches.  Returns
         dict(per=[stats], x=[...], y=[...], solve_s=wall time of the engine's
         solves, iterations, status, omega = the primal weights the solves
         ended with).  Draw generation and the copy of the costs / right-hand
@@ -96,6 +97,8 @@ class FamilySolver:
             per.extend(p)
             xs.extend(x)
             ys.extend(y)
+            if progress and (s // self.batch) % 64 == 63:
+                progress(f"{len(per)}/{len(ids)} draws solved, {solve_s:.1f}s of solves")
         return dict(per=per, x=xs, y=ys, solve_s=solve_s,
                     iterations=np.array([q["iterations"] for q in per]),
                     status=np.array([q["status"] for q in per]),
@@ -216,6 +219,38 @@ class WarmStartNet:
 
     def close(self):
         self.net.close()
+
+
+def start_sensitivity(solver, ids, X, Y, lb, ub, seed=0):
+    """Iterations of warm-started solves of draws `ids` from their own
+    solutions (X, Y rows, e.g. from a cold solve), disturbed two ways: every
+    entry by a relative error r (x (1 + r N(0,1)): zeros and bound-active
+    entries at 0 stay put, each entry keeps r of its own precision), and
+    every entry by an absolute error r times the vector's root-mean-square
+    entry (small entries and zeros are lost); plus the mean solution with
+    the true active set (x at a bound, zero duals) kept.  How close, and in
+    which sense, a predicted start must be before it saves iterations.
+    Returns {label: mean iterations}."""
+    rng = np.random.default_rng(seed)
+    X, Y = np.asarray(X, np.float64), np.asarray(Y, np.float64)
+
+    def run(WX, WY):
+        return float(solver.solve(ids, (WX, WY), want_solutions=False)["iterations"].mean())
+
+    out = {"exact": run(X, Y)}
+    for r in (1e-3, 1e-2, 1e-1):
+        out[f"relative {r:g} per entry"] = run(np.clip(X * (1 + r * rng.standard_normal(X.shape)), lb, ub),
+                                               Y * (1 + r * rng.standard_normal(Y.shape)))
+    nx = np.linalg.norm(X, axis=1, keepdims=True) / np.sqrt(X.shape[1])
+    ny = np.linalg.norm(Y, axis=1, keepdims=True) / np.sqrt(Y.shape[1])
+    for r in (1e-4, 1e-2):
+        out[f"absolute {r:g} x rms"] = run(X + r * nx * rng.standard_normal(X.shape),
+                                            Y + r * ny * rng.standard_normal(Y.shape))
+    atb = (np.abs(X - lb) <= 1e-9 * (1 + np.abs(lb))) | (np.abs(X - ub) <= 1e-9 * (1 + np.abs(ub)))
+    yz = np.abs(Y) <= 1e-12
+    out["mean values, true active set"] = run(np.where(atb, X, np.clip(np.tile(X.mean(0), (len(X), 1)), lb, ub)),
+                                              np.where(yz, 0.0, np.tile(Y.mean(0), (len(Y), 1))))
+    return out
 
 
 def relative_distance(Xp, Yp, X, Y):
