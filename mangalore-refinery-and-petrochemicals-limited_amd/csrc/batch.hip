@@ -905,9 +905,9 @@ struct Batch final : BatchBase {
     }
 
     template <class Epi>
-    void launch_reduce(const TilePass& tl, int nseg, const Epi& e) {
+    void launch_reduce(const TilePass& tl, int nseg, const Epi& e, const hipEvent_t* ev = nullptr) {
         launch_tile_reduce(stream, tl.max_pairs, nseg, tl.seg_ptr, tl.slot, static_cast<int>(tl.pairs), tl.part,
-                           tl.nslices * kWave, e, nullptr);
+                           tl.nslices * kWave, e, nullptr, 0, ev);
     }
 
     void launch_check() {
@@ -988,29 +988,30 @@ struct Batch final : BatchBase {
         };
         BO_TRY(hipMemcpyAsync(st, live.data(), sizeof(PdlpState) * K, hipMemcpyHostToDevice, stream));
         flags();
-        hipEvent_t ev[5];
+        // each kernel's own duration: events bound to its dispatch (launch_ev)
+        hipEvent_t ev[8];
         for (hipEvent_t& e : ev) BO_TRY(hipEventCreate(&e));
         double acc[4] = {0, 0, 0, 0};
         for (int r = 0; r < reps; ++r) {
             const TilePass& tc = tp[1];
             const TilePass& tr = tp[0];
-            BO_TRY(hipEventRecord(ev[0], stream));
-            hipLaunchKernelGGL((k_tile<T, kTileRows, 3>), dim3(tc.ntiles), dim3(kTileThreads),
-                               static_cast<unsigned>(tc.W * sizeof(T)), stream, tc.td, tc.words, tc.val, tc.idx, yg, m,
-                               tc.W, tc.part, tc.active);
-            BO_TRY(hipEventRecord(ev[1], stream));
-            launch_reduce(tc, n, EpiPrimal<T>{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, 0, inst_c, live_red_c});
-            BO_TRY(hipEventRecord(ev[2], stream));
-            hipLaunchKernelGGL((k_tile<T, kTileRows, 2>), dim3(tr.ntiles), dim3(kTileThreads),
-                               static_cast<unsigned>(tr.W * sizeof(T)), stream, tr.td, tr.words, tr.val, tr.idx, xp, n,
-                               tr.W, tr.part, tr.active);
-            BO_TRY(hipEventRecord(ev[3], stream));
-            launch_reduce(tr, m, EpiDual<T>{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, 0, inst_r, live_red_r});
-            BO_TRY(hipEventRecord(ev[4], stream));
-            BO_TRY(hipEventSynchronize(ev[4]));
+            launch_ev(k_tile<T, kTileRows, 3>, dim3(tc.ntiles), dim3(kTileThreads),
+                      static_cast<unsigned>(tc.W * sizeof(T)), stream, &ev[0], static_cast<const TileDesc*>(tc.td),
+                      static_cast<const unsigned*>(tc.words), static_cast<const T*>(tc.val),
+                      static_cast<const unsigned short*>(tc.idx), static_cast<const T*>(yg), m, tc.W, tc.part,
+                      static_cast<const unsigned char*>(tc.active));
+            launch_reduce(tc, n, EpiPrimal<T>{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, 0, inst_c, live_red_c}, &ev[2]);
+            launch_ev(k_tile<T, kTileRows, 2>, dim3(tr.ntiles), dim3(kTileThreads),
+                      static_cast<unsigned>(tr.W * sizeof(T)), stream, &ev[4], static_cast<const TileDesc*>(tr.td),
+                      static_cast<const unsigned*>(tr.words), static_cast<const T*>(tr.val),
+                      static_cast<const unsigned short*>(tr.idx), static_cast<const T*>(xp), n, tr.W, tr.part,
+                      static_cast<const unsigned char*>(tr.active));
+            launch_reduce(tr, m, EpiDual<T>{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, 0, inst_r, live_red_r},
+                          &ev[6]);
+            BO_TRY(hipStreamSynchronize(stream));
             for (int q = 0; q < 4; ++q) {
                 float t = 0.0f;
-                BO_TRY(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+                BO_TRY(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
                 acc[q] += t;
             }
         }

@@ -1,6 +1,7 @@
 // common.hpp -- shared HIP helpers for the MI355X (gfx950) PDHG engine.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -44,6 +45,18 @@ void set_last_error(const std::string& msg);
 inline unsigned grid_for(size_t n, int block = kBlock) {
     size_t g = (n + block - 1) / block;
     return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+// A kernel launch, optionally timed by a pair of events bound to the
+// dispatch itself (ev[0] when it starts, ev[1] when it ends: the kernel's own
+// duration, without the launch gap an event recorded before it would add).
+template <class F, class... A>
+inline void launch_ev(F kernel, dim3 grid, dim3 block, unsigned lds, hipStream_t stream, const hipEvent_t* ev,
+                      A... args) {
+    if (ev)
+        hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, ev[0], ev[1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
 }
 
 // An absolute time limit shared by every round of one solve (main batch,

@@ -1227,36 +1227,40 @@ struct Engine final : EngineBase {
     static int plan_code(const PassPlan& p) { return p.tiled > 0 ? -p.tiled : (p.cap > 0 ? -p.cap : p.kp); }
 
     // ---- launch helpers
+    // ev (optional): events timing the launch (launch_ev)
     template <int MODE, class TV, class Epi>
-    void launch_rows(const BinTable& bt, const TV* vals, const Epi& epi, double* partials, const int* done) {
-        hipLaunchKernelGGL((k_segments<MODE, TV, typename Epi::TXv, Epi>), dim3(bt.total_blocks), dim3(kBlock), 0,
-                           stream, bt, rp, ci, vals, epi, partials, done);
+    void launch_rows(const BinTable& bt, const TV* vals, const Epi& epi, double* partials, const int* done,
+                     const hipEvent_t* ev = nullptr) {
+        launch_ev(k_segments<MODE, TV, typename Epi::TXv, Epi>, dim3(bt.total_blocks), dim3(kBlock), 0, stream, ev,
+                  bt, static_cast<const int*>(rp), static_cast<const int*>(ci), vals, epi, partials, done);
     }
     template <int MODE, class TV, class Epi>
-    void launch_cols(const BinTable& bt, const TV* vals, const Epi& epi, double* partials, const int* done) {
-        hipLaunchKernelGGL((k_segments<MODE, TV, typename Epi::TXv, Epi>), dim3(bt.total_blocks), dim3(kBlock), 0,
-                           stream, bt, cp, ri, vals, epi, partials, done);
+    void launch_cols(const BinTable& bt, const TV* vals, const Epi& epi, double* partials, const int* done,
+                     const hipEvent_t* ev = nullptr) {
+        launch_ev(k_segments<MODE, TV, typename Epi::TXv, Epi>, dim3(bt.total_blocks), dim3(kBlock), 0, stream, ev,
+                  bt, static_cast<const int*>(cp), static_cast<const int*>(ri), vals, epi, partials, done);
     }
 
     const int* done_flag() const { return &st->done; }
 
     template <class Epi>
     void launch_stream(int pass, int cap, const int* ptr, const int* idx, const T* vals, const Epi& e,
-                       const int* done) {
+                       const int* done, const hipEvent_t* ev = nullptr) {
         const int c = cap_index(cap);
         const int nb = nsblk[pass][c];
+        const int4* blk = sblk[pass][c];
         switch (c) {
             case 0:
-                hipLaunchKernelGGL((k_stream<1024, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c],
-                                   ptr, idx, vals, e, done);
+                launch_ev(k_stream<1024, T, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
+                          done);
                 break;
             case 1:
-                hipLaunchKernelGGL((k_stream<2048, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c],
-                                   ptr, idx, vals, e, done);
+                launch_ev(k_stream<2048, T, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
+                          done);
                 break;
             default:
-                hipLaunchKernelGGL((k_stream<4096, T, T, Epi>), dim3(nb), dim3(kBlock), 0, stream, sblk[pass][c],
-                                   ptr, idx, vals, e, done);
+                launch_ev(k_stream<4096, T, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
+                          done);
                 break;
         }
     }
@@ -1270,42 +1274,50 @@ struct Engine final : EngineBase {
     // Tiled pass: gather kernel into the partials, `mid` (optional event),
     // reduction + epilogue kernel.  Only the reduction reads the done flag:
     // partials computed after convergence are never used.
+    // ev4 (optional): start / end events of the gather and of the reduction
     template <int PASS, class Epi>
-    void launch_tiled(const T* gvec, const Epi& e, const int* done, hipEvent_t mid) {
+    void launch_tiled(const T* gvec, const Epi& e, const int* done, hipEvent_t mid, const hipEvent_t* ev4 = nullptr) {
         const TilePass& t = tp[PASS];
         const int nseg = PASS == 0 ? m : n, ng = PASS == 0 ? n : m;
         if (t.ntiles > 0)
-            hipLaunchKernelGGL((k_tile<T, kTileRows, PASS>), dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream,
-                               t.td, t.words, t.val, t.idx, gvec, ng, t.W, t.part, nullptr);
+            launch_ev(k_tile<T, kTileRows, PASS>, dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream, ev4,
+                      static_cast<const TileDesc*>(t.td), static_cast<const unsigned*>(t.words),
+                      static_cast<const T*>(t.val), static_cast<const unsigned short*>(t.idx), gvec, ng, t.W, t.part,
+                      static_cast<const unsigned char*>(nullptr));
         if (mid) hipEventRecord(mid, stream);
         launch_tile_reduce(stream, t.max_pairs, nseg, t.seg_ptr, t.slot, static_cast<int>(t.pairs), t.part,
-                           t.nslices * kWave, e, done, t.reduce_u);
+                           t.nslices * kWave, e, done, t.reduce_u, ev4 ? ev4 + 2 : nullptr);
     }
 
-    void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
+    // ev4 (optional): start / end events of the pass's kernels (a fused pass
+    // uses the first pair only)
+    void launch_primal(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr,
+                       const hipEvent_t* ev4 = nullptr) {
         EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, nullptr};
         if (p.tiled > 0) {
-            launch_tiled<1>(yg, e, done, mid);
+            launch_tiled<1>(yg, e, done, mid, ev4);
             return;
         }
-        if (p.cap > 0) launch_stream(1, p.cap, cp, ri, cv, e, done);
-        else launch_cols<kSumProd>(p.bt, cv, e, nullptr, done);
+        if (p.cap > 0) launch_stream(1, p.cap, cp, ri, cv, e, done, ev4);
+        else launch_cols<kSumProd>(p.bt, cv, e, nullptr, done, ev4);
         if (mid) hipEventRecord(mid, stream);
     }
-    void launch_dual(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr) {
+    void launch_dual(const PassPlan& p, const int* done, int t, hipEvent_t mid = nullptr,
+                     const hipEvent_t* ev4 = nullptr) {
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, nullptr};
         if (p.tiled > 0) {
-            launch_tiled<0>(xp, e, done, mid);
+            launch_tiled<0>(xp, e, done, mid, ev4);
             return;
         }
         if (p.cap > 0) {
-            launch_stream(0, p.cap, rp, ci, rv, e, done);
+            launch_stream(0, p.cap, rp, ci, rv, e, done, ev4);
         } else if (p.hot > 0) {
-            hipLaunchKernelGGL((k_segments_hot<kSumProd, T, T, EpiDual<T>>),
-                               dim3(std::min(p.hot_grid, p.bt.total_blocks)), dim3(kBlock),
-                               static_cast<unsigned>(p.hot * sizeof(T)), stream, p.bt, rp, ci, rv, e, p.hot, done);
+            launch_ev(k_segments_hot<kSumProd, T, T, EpiDual<T>>, dim3(std::min(p.hot_grid, p.bt.total_blocks)),
+                      dim3(kBlock), static_cast<unsigned>(p.hot * sizeof(T)), stream, ev4, p.bt,
+                      static_cast<const int*>(rp), static_cast<const int*>(ci), static_cast<const T*>(rv), e, p.hot,
+                      done);
         } else {
-            launch_rows<kSumProd>(p.bt, rv, e, nullptr, done);
+            launch_rows<kSumProd>(p.bt, rv, e, nullptr, done, ev4);
         }
         if (mid) hipEventRecord(mid, stream);
     }
@@ -1816,8 +1828,34 @@ struct Engine final : EngineBase {
         return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms);
     }
 
+    // Each iteration kernel's own duration (events bound to its dispatch,
+    // launch_ev), passes alternating as in a solve: CSC gather, CSC
+    // reduction, CSR gather, CSR reduction (a fused pass: its kernel in the
+    // gather entry, 0 for the reduction).
     bo_status time_kernels(int reps, double* ms4) override {
-        return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, nullptr, nullptr, ms4);
+        reps = reps < 1 ? 1 : reps;
+        std::vector<hipEvent_t> ev(8 * static_cast<size_t>(reps));
+        for (auto& e : ev) BO_TRY(hipEventCreate(&e));
+        launch_primal(plan_cols, nullptr, 0);
+        launch_dual(plan_rows, nullptr, 0);
+        for (int r = 0; r < reps; ++r) {
+            launch_primal(plan_cols, nullptr, r, nullptr, &ev[8 * r]);
+            launch_dual(plan_rows, nullptr, r, nullptr, &ev[8 * r + 4]);
+        }
+        BO_TRY(hipStreamSynchronize(stream));
+        BO_TRY(hipGetLastError());
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const bool two[2] = {plan_cols.tiled > 0, plan_rows.tiled > 0};
+        for (int r = 0; r < reps; ++r)
+            for (int q = 0; q < 4; ++q) {
+                if (q % 2 == 1 && !two[q / 2]) continue;  // a fused pass: no reduction kernel
+                float a = 0.f;
+                BO_TRY(hipEventElapsedTime(&a, ev[8 * r + 2 * q], ev[8 * r + 2 * q + 1]));
+                acc[q] += a;
+            }
+        for (auto& e : ev) hipEventDestroy(e);
+        for (int q = 0; q < 4; ++q) ms4[q] = acc[q] / reps;
+        return BO_SUCCESS;
     }
 
     // Schedule of a pass and the algorithmic bytes of its kernels: what each

@@ -517,27 +517,24 @@ k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restr
 // Launch of the reduction with U loads in flight per round: `u` when given
 // (a tuned value: a segment's partials are still added in order, so any U
 // gives the same sums), else the smallest U covering the most pairs a
-// segment has.
+// segment has.  ev (optional): events timing the launch (launch_ev).
 template <class Epi>
 inline void launch_tile_reduce(hipStream_t stream, int max_pairs, int nseg, const int* seg_ptr, const unsigned* slot,
-                               int nslot, const double* part, int npart, const Epi& e, const int* done, int u = 0) {
+                               int nslot, const double* part, int npart, const Epi& e, const int* done, int u = 0,
+                               const hipEvent_t* ev = nullptr) {
     const dim3 grid(grid_for(static_cast<size_t>(nseg))), block(kBlock);
     if (u > 0) max_pairs = u;
     if (max_pairs <= 1)
-        hipLaunchKernelGGL((k_tile_reduce<1, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
-                           e, done);
+        launch_ev(k_tile_reduce<1, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
     else if (max_pairs <= 2)
-        hipLaunchKernelGGL((k_tile_reduce<2, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
-                           e, done);
+        launch_ev(k_tile_reduce<2, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
     else if (max_pairs <= 4)
-        hipLaunchKernelGGL((k_tile_reduce<4, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
-                           e, done);
+        launch_ev(k_tile_reduce<4, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
     else if (max_pairs <= 8)
-        hipLaunchKernelGGL((k_tile_reduce<8, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
-                           e, done);
+        launch_ev(k_tile_reduce<8, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
     else
-        hipLaunchKernelGGL((k_tile_reduce<16, Epi>), grid, block, 0, stream, nseg, seg_ptr, slot, nslot, part, npart,
-                           e, done);
+        launch_ev(k_tile_reduce<16, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e,
+                  done);
 }
 
 }  // namespace bo
