@@ -5,6 +5,9 @@ FP64 oracle".  Run in the build container (scipy's HiGHS is the independent
 solver; it never ships); the GPU tests and bench.py only read the JSON.
 
 Workloads (generators of the package's lpdata module, seeds as bench.py):
+  c0  random_lp(300, 1000, 5 nonzeros per column, seed 42), the reference's
+      CPU-sized LP: HiGHS dual simplex, and oracle/pdlp_omp.c to 1e-9
+      (the reference's own simplex chain agrees: tests/golden/lp_optima.json)
   c1  zipf_lp(200000, 500000, s=1.5, cap=2000, seed=42) -- 17.1M nonzeros,
       beyond HiGHS here: oracle/pdlp_omp.c (FP64) run to rel-KKT 1e-9 with
       the objective-gap stop at 1e-9, certified by the independent
@@ -20,7 +23,7 @@ Each entry records the SHA-256 of the instance's arrays (row_ptr, col_idx,
 val, b, c, lb, ub, is_eq), checked before a fixture is used, and the SHA-256
 of the generator source (lpdata.py) it was written with.
 
-Usage:  python tests/golden/make_objectives.py [--threads N] [--procs P] c1 c2 c3 c4
+Usage:  python tests/golden/make_objectives.py [--threads N] [--procs P] c0 c1 c2 c3 c4
         (updates the named entries; N OpenMP threads for the FP64 restatement,
         P processes for the C4 draws; several runs may update the file at once)
 """
@@ -129,7 +132,9 @@ def main(names):
     data = {}
     for name in names:
         print(name, flush=True)
-        if name == "c1":
+        if name == "c0":
+            data["c0"] = entry(lpd.random_lp(300, 1000, 5, 42), ["highs", "pdlp"])
+        elif name == "c1":
             data["c1"] = entry(lpd.zipf_lp(200_000, 500_000, 1.5, 2000, seed=42), ["pdlp"])
         elif name == "c2":
             data["c2"] = entry(lpd.staircase_lp(24, 2000, 5000, 200, seed=42), ["highs-ipm", "pdlp"])
@@ -161,4 +166,4 @@ if __name__ == "__main__":
         else:
             raise SystemExit("unknown option " + args[0])
         args = args[2:]
-    main(args or ["c3", "c4", "c2", "c1"])
+    main(args or ["c0", "c3", "c4", "c2", "c1"])

@@ -3,6 +3,7 @@ against an FP64 oracle (tests/golden/objectives.json, written in the build
 container: HiGHS where it is tractable, the FP64 restatement
 oracle/pdlp_omp.c to rel-KKT 1e-9 beyond):
 
+* C0  the reference's CPU-sized random LP (300 x 1,000), single-LP engine;
 * C1  the 200,000 x 500,000 Zipf LP (17.1M nonzeros), single-LP engine;
 * C2  the 24-period staircase LP, single-LP engine;
 * C3  8 LPs of the 512-LP ragged batch (the largest among them), batched engine;
@@ -46,12 +47,18 @@ def check(po, lp, x, y, entry, name):
     assert err <= TOL, (name, err)
 
 
-@pytest.mark.parametrize("workload", ["c1", "c2"])
+SINGLE = {
+    "c0": lambda lpd: lpd.random_lp(300, 1000, 5, 42),
+    "c1": lambda lpd: lpd.zipf_lp(200_000, 500_000, 1.5, 2000, seed=42),
+    "c2": lambda lpd: lpd.staircase_lp(24, 2000, 5000, 200, seed=42),
+}
+
+
+@pytest.mark.parametrize("workload", sorted(SINGLE))
 def test_single_lp_workloads_full_size(pkg, po, lpdata, workload):
     if workload not in FX:
         pytest.skip(f"no objective fixture for {workload}")
-    lp = (lpdata.zipf_lp(200_000, 500_000, 1.5, 2000, seed=42) if workload == "c1"
-          else lpdata.staircase_lp(24, 2000, 5000, 200, seed=42))
+    lp = SINGLE[workload](lpdata)
     print("case:", workload, lp.m, lp.n, lp.nnz, flush=True)
     h = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=200000)
     s = h.solve()
