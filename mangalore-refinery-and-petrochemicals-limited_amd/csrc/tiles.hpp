@@ -372,9 +372,12 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
                                     const WaveDesc& wd, const unsigned* __restrict__ slice_word,
                                     double* __restrict__ part, int lane) {
     // EARLY: the chunk's LDS gathers are all issued before the first is
-    // used, one LDS latency per chunk instead of one per row.  Measured at
-    // C1 it pays in the row pass (27.4 against 28.5 us) and costs in the
-    // column pass (36.0 against 34.7 us, whose slices end every few rows).
+    // used, one LDS latency per chunk instead of one per row.  It pays when
+    // slices span several chunks: on the 200,000 x 500,000 Zipf LP the row
+    // pass gains 1 us, and the column pass, once long columns took 8-64
+    // lanes (slices of 11 rows on average instead of 5.6), 1.6 us (27.9
+    // against 29.5 us); with the short slices it had cost the column pass
+    // 1.3 us.
     T g[CH];
     if constexpr (EARLY) {
 #pragma unroll
@@ -412,7 +415,7 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
 // One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of the
 // wavefront's run kept in flight per buffer.  PASS names the instantiation
 // (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of the batched
-// engine), so that profiles tell the passes apart; the CSR passes issue a
+// engine), so that profiles tell the passes apart; every pass issues a
 // chunk's LDS gathers early (chunk_finish).  active
 // (optional): per-tile flag; a tile whose flag is 0 does nothing.
 template <class T, int CH, int PASS>
@@ -473,9 +476,9 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     int r = 0;
     for (int pairs = (nrows + 2 * CH - 1) / (2 * CH); pairs > 0; --pairs) {
         chunk_load(Bc, rv, ri, r + CH, lane);
-        chunk_finish<T, CH, PASS % 2 == 0>(A, r, lds, st, wd, slice_word, part, lane);
+        chunk_finish<T, CH, true>(A, r, lds, st, wd, slice_word, part, lane);
         chunk_load(A, rv, ri, r + 2 * CH, lane);
-        chunk_finish<T, CH, PASS % 2 == 0>(Bc, r + CH, lds, st, wd, slice_word, part, lane);
+        chunk_finish<T, CH, true>(Bc, r + CH, lds, st, wd, slice_word, part, lane);
         r += 2 * CH;
     }
 }
