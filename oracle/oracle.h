@@ -107,6 +107,13 @@ typedef struct {
     int pock_chambolle;     /* 1 */
     int threads;            /* OpenMP threads; <= 0: all */
     double eps_objective;   /* > 0: also stop only when |c'x - dual obj| <= this (1 + |c'x|) */
+    const double* x_start;  /* optional, n: starting primal point (unscaled; projected on the bounds) */
+    const double* y_start;  /* optional, m: starting duals (unscaled; inequality duals clipped at 0) */
+    double primal_weight_start; /* > 0: starting primal weight instead of ||c_hat|| / ||b_hat|| */
+    const char* checkpoint_path;  /* optional: the iteration state is written there every
+                                     checkpoint_every iterations (at a check), and a run that finds a
+                                     matching file there continues from it instead of starting */
+    long long checkpoint_every;
 } orc_pdlp_opts;
 
 typedef struct {

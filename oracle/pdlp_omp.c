@@ -26,6 +26,7 @@
 #include <math.h>
 #include <omp.h>
 #include <stdlib.h>
+#include <stdio.h>
 #include <string.h>
 
 #define NEW(T, k) ((T*)calloc((size_t)((k) > 0 ? (k) : 1), sizeof(T)))
@@ -51,6 +52,45 @@ typedef struct {
     long long iter, iter_last_restart, max_iter;
     int done, status, restart, restart_to_cand, best, restarts;
 } state_t;
+
+/* Checkpoint of the iteration state at a check: the state record and the
+ * carried vectors (scaled space).  The scaled problem itself is rebuilt by
+ * the same deterministic set-up on resume.  Written to path.tmp, then renamed. */
+static const char kCkptMagic[8] = {'B', 'O', 'P', 'D', 'L', 'P', 'C', '1'};
+
+static int ckpt_write(const char* path, int m, int n, const state_t* st, double* const* xs, double* const* ys) {
+    char tmp[4096];
+    if (snprintf(tmp, sizeof(tmp), "%s.tmp", path) >= (int)sizeof(tmp)) return -1;
+    FILE* f = fopen(tmp, "wb");
+    if (!f) return -1;
+    const long long dims[3] = {m, n, (long long)sizeof(state_t)};
+    int ok = fwrite(kCkptMagic, 1, 8, f) == 8 && fwrite(dims, sizeof(dims), 1, f) == 1 &&
+             fwrite(st, sizeof(state_t), 1, f) == 1;
+    for (int a = 0; ok && a < 3; ++a) ok = fwrite(xs[a], sizeof(double), (size_t)n, f) == (size_t)n;
+    for (int a = 0; ok && a < 5; ++a) ok = fwrite(ys[a], sizeof(double), (size_t)m, f) == (size_t)m;
+    if (fclose(f) != 0) ok = 0;
+    if (!ok || rename(tmp, path) != 0) {
+        remove(tmp);
+        return -1;
+    }
+    return 0;
+}
+
+static int ckpt_read(const char* path, int m, int n, state_t* st, double* const* xs, double* const* ys) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return -1;
+    char magic[8];
+    long long dims[3];
+    state_t s;
+    int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(dims, sizeof(dims), 1, f) == 1 &&
+             dims[0] == m && dims[1] == n && dims[2] == (long long)sizeof(state_t) && fread(&s, sizeof(s), 1, f) == 1;
+    for (int a = 0; ok && a < 3; ++a) ok = fread(xs[a], sizeof(double), (size_t)n, f) == (size_t)n;
+    for (int a = 0; ok && a < 5; ++a) ok = fread(ys[a], sizeof(double), (size_t)m, f) == (size_t)m;
+    fclose(f);
+    if (!ok) return -1;
+    *st = s;
+    return 0;
+}
 
 /* q: [0,1] c'x [2,3] ||viol||^2 [4,5] scaled [6,7] b'y [8,9] ||dual res||^2
  * [10,11] scaled [12,13] sum l plam+ + u plam-, for z [even] and T(z) [odd]. */
@@ -156,6 +196,7 @@ int orc_pdlp_omp(int m, int n, const int* rp, const int* ci, const double* v, co
     double q8 = 0, q9 = 0, q10 = 0, q11 = 0, q12 = 0, q13 = 0;
     double chat2 = 0.0, bhat2 = 0.0, u2 = 0.0, w2 = 0.0, dx2 = 0.0, dy2 = 0.0;
     long long iters_run = 0;
+    long long resumed_iter = 0;  /* iteration a checkpoint put the run at (0: a fresh start) */
     double t_setup = 0.0;
 
 #pragma omp parallel num_threads(threads)
@@ -198,19 +239,21 @@ int orc_pdlp_omp(int m, int n, const int* rp, const int* ci, const double* v, co
 #pragma omp for
         for (int j = 0; j < n; ++j)
             for (int k = cp[j]; k < cp[j + 1]; ++k) cvh[k] = cs[j] * cv[k] * rs[ri[k]];
-        /* scaled data and the starting point (projection of 0) */
+        /* scaled data and the starting point (projection of 0, or of x_start / y_start) */
 #pragma omp for reduction(+ : chat2)
         for (int j = 0; j < n; ++j) {
             ch[j] = c[j] * cs[j];
             lh[j] = lb[j] / cs[j];
             uh[j] = ub[j] / cs[j];
-            xk[j] = x0[j] = xp[j] = clampd(0.0, lh[j], uh[j]);
+            xk[j] = x0[j] = xp[j] = clampd(o->x_start ? o->x_start[j] / cs[j] : 0.0, lh[j], uh[j]);
             chat2 += ch[j] * ch[j];
         }
 #pragma omp for reduction(+ : bhat2)
         for (int i = 0; i < m; ++i) {
             bh[i] = b[i] * rs[i];
-            yk[i] = y0[i] = yp[i] = 0.0;
+            double ys = o->y_start ? o->y_start[i] / rs[i] : 0.0;
+            if (!is_eq[i] && ys < 0.0) ys = 0.0;
+            yk[i] = y0[i] = yp[i] = ys;
             bhat2 += bh[i] * bh[i];
         }
         /* power iteration for ||A_hat||_2 */
@@ -246,7 +289,8 @@ int orc_pdlp_omp(int m, int n, const int* rp, const int* ci, const double* v, co
         {
             const double smax = sqrt(u2), cn = sqrt(chat2), bn = sqrt(bhat2);
             st.eta = smax > 0.0 ? 0.95 / smax : 1.0;
-            st.omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+            st.omega = o->primal_weight_start > 0.0 ? o->primal_weight_start
+                                                    : ((cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0);
             st.tau = st.eta / st.omega;
             st.sigma = st.eta * st.omega;
             st.kkt_prev_candidate = INFINITY;
@@ -255,9 +299,30 @@ int orc_pdlp_omp(int m, int n, const int* rp, const int* ci, const double* v, co
 #pragma omp for
         for (int i = 0; i < m; ++i) ax0[i] = ax[i];
 #pragma omp single
-        t_setup = now_s() - t0;
+        {
+            t_setup = now_s() - t0;
+            if (o->checkpoint_path) {  /* continue a checkpointed run of this LP */
+                double* const xs[3] = {xk, x0, xp};
+                double* const ys[5] = {yk, y0, yp, ax, ax0};
+                state_t saved;
+                if (ckpt_read(o->checkpoint_path, m, n, &saved, xs, ys) == 0) {
+                    st = saved;
+                    st.max_iter = o->max_iterations;
+                    st.eps = o->eps_rel_kkt;
+                    st.eps_obj = o->eps_objective;
+                    resumed_iter = st.iter;
+                }
+            }
+        }
 
         for (;;) {
+#pragma omp single
+            if (o->checkpoint_path && o->checkpoint_every > 0 && st.iter > resumed_iter &&
+                st.iter % o->checkpoint_every == 0) {
+                double* const xs[3] = {xk, x0, xp};
+                double* const ys[5] = {yk, y0, yp, ax, ax0};
+                ckpt_write(o->checkpoint_path, m, n, &st, xs, ys);
+            }
             /* ---- the FP64 check of z and T(z) on the unscaled problem ---- */
 #pragma omp single
             q0 = q1 = q2 = q3 = q4 = q5 = q6 = q7 = q8 = q9 = q10 = q11 = q12 = q13 = 0.0;

@@ -51,7 +51,8 @@ def oracle_lib():
 class OrcPdlpOpts(C.Structure):
     _fields_ = [("max_iterations", C.c_longlong), ("eps_rel_kkt", C.c_double), ("check_every", C.c_int),
                 ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int), ("threads", C.c_int),
-                ("eps_objective", C.c_double)]
+                ("eps_objective", C.c_double), ("x_start", P), ("y_start", P), ("primal_weight_start", C.c_double),
+                ("checkpoint_path", C.c_char_p), ("checkpoint_every", C.c_longlong)]
 
 
 class OrcPdlpOut(C.Structure):
@@ -79,12 +80,20 @@ def pdlp_omp_lib():
 
 
 def pdlp_omp(lp, max_iterations=100000, eps_rel_kkt=1e-6, check_every=64, ruiz_iterations=10,
-             pock_chambolle=True, threads=0, eps_objective=0.0):
-    """The restarted Halpern PDHG in FP64 on `threads` host cores (0: all).
-    Returns (stats dict, x, y)."""
+             pock_chambolle=True, threads=0, eps_objective=0.0, x_start=None, y_start=None, primal_weight_start=0.0,
+             checkpoint_path=None, checkpoint_every=0):
+    """The restarted Halpern PDHG in FP64 on `threads` host cores (0: all),
+    from zero or from (x_start, y_start) with an optional starting primal
+    weight.  checkpoint_path / checkpoint_every: the iteration state is
+    written there every so many iterations, and a call that finds a matching
+    file continues that run.  Returns (stats dict, x, y)."""
     lib = pdlp_omp_lib()
+    xs = None if x_start is None else _f64(x_start)
+    ys = None if y_start is None else _f64(y_start)
     o = OrcPdlpOpts(max_iterations, eps_rel_kkt, check_every, ruiz_iterations, 1 if pock_chambolle else 0,
-                    threads, eps_objective)
+                    threads, eps_objective, _p(xs) if xs is not None else None, _p(ys) if ys is not None else None,
+                    float(primal_weight_start), checkpoint_path.encode() if checkpoint_path else None,
+                    int(checkpoint_every))
     out = OrcPdlpOut()
     x = np.zeros(max(lp.n, 1))
     y = np.zeros(max(lp.m, 1))

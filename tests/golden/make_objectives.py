@@ -32,6 +32,7 @@ import hashlib
 import json
 import os
 import sys
+import tempfile
 import time
 from multiprocessing import Pool
 
@@ -89,8 +90,14 @@ def highs(lp, method):
 
 
 def pdlp_fp64(lp, eps=1e-9):
+    """The FP64 restatement to rel-KKT eps.  It checkpoints its iteration
+    state every 8,192 iterations to a file keyed by the instance's SHA-256 in
+    the temporary directory: rerunning the same command after an
+    interruption continues the run where the checkpoint left it."""
+    ckpt = os.path.join(tempfile.gettempdir(), f"bo_objectives_{instance_sha(lp)[:16]}_{eps:g}.ckpt")
     t0 = time.perf_counter()
-    st, x, y = po.pdlp_omp(lp, max_iterations=5_000_000, eps_rel_kkt=eps, eps_objective=eps, threads=THREADS)
+    st, x, y = po.pdlp_omp(lp, max_iterations=5_000_000, eps_rel_kkt=eps, eps_objective=eps, threads=THREADS,
+                           checkpoint_path=ckpt, checkpoint_every=8192)
     k = po.kkt(lp, x, y)
     assert st["status"] == 0 and k["rel_kkt"] <= eps * 1.001, (st, k)
     return dict(objective=float(k["primal_obj"]), dual_objective=float(k["dual_obj"]), rel_kkt=float(k["rel_kkt"]),

@@ -283,3 +283,35 @@ def test_pdlp_omp_objective_gap_stop(po, lpdata):
         assert abs(k["primal_obj"] - k["dual_obj"]) <= 1e-6 * (1 + abs(k["primal_obj"])) * (1 + 1e-3), name
         f = OPT[name]["highs"]
         assert abs(k["primal_obj"] - f) / (1 + abs(f)) <= 1e-6, (name, k["primal_obj"], f)
+
+
+def test_pdlp_omp_warm_start_and_checkpoint_resume(po, lpdata, tmp_path):
+    """The FP64 restatement's optional starting point and checkpoints (what
+    tests/golden/make_objectives.py uses to survive interruptions of its
+    multi-hour runs): started at its own optimum it stops at the first check
+    (0 iterations); a run cut into 30 calls of growing iteration caps, each
+    continuing the checkpoint, ends on the uninterrupted run's iteration
+    count and objective, and a checkpoint of another LP's dimensions is
+    ignored."""
+    lp = lpdata.staircase_lp(3, 200, 500, 20, seed=3)
+    st0, x0, y0 = po.pdlp_omp(lp, max_iterations=500000, eps_rel_kkt=1e-9, eps_objective=1e-9, threads=1)
+    assert st0["status"] == 0
+    st1, _, _ = po.pdlp_omp(lp, max_iterations=500000, eps_rel_kkt=1e-9, eps_objective=1e-9, threads=1,
+                            x_start=x0, y_start=y0, primal_weight_start=st0["primal_weight"])
+    assert st1["status"] == 0 and st1["iterations"] == 0
+    ck = str(tmp_path / "run.ckpt")
+    cap, calls = 1024, 0
+    while True:
+        st, x, y = po.pdlp_omp(lp, max_iterations=cap, eps_rel_kkt=1e-9, eps_objective=1e-9, threads=1,
+                               checkpoint_path=ck, checkpoint_every=512)
+        calls += 1
+        if st["status"] == 0 or cap >= 200000:
+            break
+        cap += 1024
+    assert calls > 10 and st["status"] == 0
+    assert st["iterations"] == st0["iterations"] and st["primal_obj"] == st0["primal_obj"]
+    other = lpdata.random_lp(60, 200, 3, 7)  # other dimensions: the file is not used
+    st2, _, _ = po.pdlp_omp(other, max_iterations=100000, eps_rel_kkt=1e-6, threads=1, checkpoint_path=ck,
+                            checkpoint_every=0)
+    ref, _, _ = po.pdlp_omp(other, max_iterations=100000, eps_rel_kkt=1e-6, threads=1)
+    assert st2["status"] == 0 and st2["iterations"] == ref["iterations"]
