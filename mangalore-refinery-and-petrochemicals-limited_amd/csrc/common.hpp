@@ -118,6 +118,14 @@ struct EpiHasLoad : std::false_type {};
 template <class E>
 struct EpiHasLoad<E, std::void_t<decltype(std::declval<const E&>().load(0))>> : std::true_type {};
 
+// An epilogue whose two gathered vectors are interleaved (entry j of both
+// in one 16-byte double2, vec2()) declares kPair = true: one gather per
+// entry instead of two.
+template <class E, class = void>
+struct EpiPair : std::false_type {};
+template <class E>
+struct EpiPair<E, std::void_t<decltype(E::kPair)>> : std::bool_constant<E::kPair> {};
+
 template <class E>
 __device__ inline auto epi_preload(const E& e, int s, int) -> decltype(e.load(s)) {
     return e.load(s);

@@ -167,6 +167,61 @@ struct EpiCheckCols {
     }
 };
 
+// The same two check passes with the current point and the candidate
+// interleaved (entry j of both in one double2): the multi-kernel loop's
+// checks gather one 16 bytes per nonzero instead of two 8 bytes, so the
+// passes issue half the gather requests.  Same arithmetic, same sums.
+template <class T>
+struct EpiCheckRows2 {
+    using TXv = double;
+    static constexpr int NV = 2;
+    static constexpr int Q = 6;
+    static constexpr bool kPair = true;
+    const double2* xo2;  // unscaled x (current, candidate)
+    const double* b;
+    const unsigned char* eq;
+    const double* rs;
+    const double* yk;
+    const T* yp;
+    double2* yo2;     // unscaled y (current, candidate), gathered by the column pass
+    double* ax_cand;  // scaled A x_cand, taken over when restarting to the candidate
+    __device__ const double2* vec2() const { return xo2; }
+    __device__ void operator()(int i, const double* acc, double* part) const {
+        const double bi = b[i], ri = rs[i];
+        const bool e = eq[i] != 0;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double r = acc[p] - bi;
+            const double v = e ? fabs(r) : (r > 0.0 ? r : 0.0);
+            part[p] += v * v;
+            part[2 + p] += (ri * v) * (ri * v);
+        }
+        ax_cand[i] = ri * acc[1];
+        double yc = ri * yk[i];
+        if (!e && yc < 0.0) yc = 0.0;  // the reported dual is sign-feasible
+        const double ya = ri * static_cast<double>(yp[i]);
+        yo2[i] = make_double2(yc, ya);
+        part[4] += bi * yc;
+        part[5] += bi * ya;
+    }
+};
+
+struct EpiCheckCols2 {
+    using TXv = double;
+    static constexpr int NV = 2;
+    static constexpr int Q = 6;
+    static constexpr bool kPair = true;
+    const double2* yo2;
+    const double* c;
+    const double* lb;
+    const double* ub;
+    const double* cs;
+    __device__ const double2* vec2() const { return yo2; }
+    __device__ void operator()(int j, const double* acc, double* part) const {
+        EpiCheckCols{nullptr, nullptr, c, lb, ub, cs}(j, acc, part);
+    }
+};
+
 // ---------------------------------------------------------------- elementwise
 __global__ void k_fill_f64(double* p, double v, size_t n) {
     size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -299,6 +354,59 @@ __global__ void k_check_x(int n, const PdlpState* st, const double* xk, const T*
     }
 }
 
+// k_check_x with the two points interleaved (EpiCheckRows2 gathers them).
+template <class T>
+__global__ void k_check_x2(int n, const PdlpState* st, const double* xk, const T* xp, const double* s,
+                           const double* c, const double* lb, const double* ub, double2* xo2, double* partials) {
+    if (st->done) return;
+    __shared__ double sh[kBlock / kWave];
+    double p0 = 0.0, p1 = 0.0;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const double sj = s[j], l = lb[j], u = ub[j];
+        double xc = sj * xk[j];
+        xc = xc < l ? l : (xc > u ? u : xc);
+        double xa = sj * static_cast<double>(xp[j]);
+        xa = xa < l ? l : (xa > u ? u : xa);
+        xo2[j] = make_double2(xc, xa);
+        p0 += c[j] * xc;
+        p1 += c[j] * xa;
+    }
+    double t0 = block_sum(p0, sh);
+    double t1 = block_sum(p1, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x * 2 + 0] = t0;
+        partials[blockIdx.x * 2 + 1] = t1;
+    }
+}
+
+// out[i] = component `which` (0: current point, 1: candidate) of v2[i].
+__global__ void k_pick2(size_t n, const double2* v2, int which, double* out) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = which ? v2[i].y : v2[i].x;
+}
+
+// N block-wide sums at once, each the same as block_sum (xor tree in the
+// wavefront, then the wavefronts in order), with one pair of barriers for all.
+template <int N>
+__device__ inline void block_sums(double (&v)[N], double* shn) {
+    const int lane = static_cast<int>(threadIdx.x) % kWave, wid = static_cast<int>(threadIdx.x) / kWave;
+    const int nw = static_cast<int>(blockDim.x) / kWave;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) shn[wid * N + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += shn[w * N + k];
+        v[k] = t;
+    }
+}
+
 struct CheckDims {
     int nb_x, nb_rows, nb_cols;
 };
@@ -307,32 +415,46 @@ struct CheckDims {
 // stopping test, the KKT restart rule and the bookkeeping.
 constexpr int kDecideThreads = 1024;
 
+// acc[k] += p[b * Q + k] over the blocks b = tid, tid + blockDim.x, ... in
+// order, the loads of four blocks in flight at once.
+template <int Q>
+__device__ inline void add_block_partials(const double* __restrict__ p, int nb, double* acc) {
+    constexpr int kIn = 4;
+    const int stride = static_cast<int>(blockDim.x);
+    for (int b0 = static_cast<int>(threadIdx.x); b0 < nb; b0 += kIn * stride) {
+        double v[kIn][Q];
+#pragma unroll
+        for (int u = 0; u < kIn; ++u) {
+            const int b = b0 + u * stride;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) v[u][k] = b < nb ? p[static_cast<size_t>(b) * Q + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kIn; ++u) {
+            if (b0 + u * stride < nb) {
+#pragma unroll
+                for (int k = 0; k < Q; ++k) acc[k] += v[u][k];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kDecideThreads)
 k_decide(PdlpState* st, CheckDims d, const double* px, const double* pr, const double* pc) {
     if (st->done) return;
-    __shared__ double sh[kDecideThreads / kWave];
+    __shared__ double sh[(kDecideThreads / kWave) * 14];
     double a[14];
 #pragma unroll
     for (int k = 0; k < 14; ++k) a[k] = 0.0;
-    for (int b = threadIdx.x; b < d.nb_x; b += blockDim.x) {
-        a[0] += px[b * 2 + 0];
-        a[1] += px[b * 2 + 1];
-    }
-    for (int b = threadIdx.x; b < d.nb_rows; b += blockDim.x) {
-        const double* p = pr + static_cast<size_t>(b) * 6;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) a[2 + k] += p[k];
-    }
-    for (int b = threadIdx.x; b < d.nb_cols; b += blockDim.x) {
-        const double* p = pc + static_cast<size_t>(b) * 6;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) a[8 + k] += p[k];
-    }
-    double q[14];
-#pragma unroll
-    for (int k = 0; k < 14; ++k) q[k] = block_sum(a[k], sh);
+    // Each thread adds blocks tid, tid + 1024, ... in that order; the loads
+    // of 4 of them are issued before the first is added (a check's row and
+    // column passes leave tens of thousands of blocks at C1 sizes).
+    add_block_partials<2>(px, d.nb_x, a);
+    add_block_partials<6>(pr, d.nb_rows, a + 2);
+    add_block_partials<6>(pc, d.nb_cols, a + 8);
+    block_sums(a, sh);  // the 14 sums of block_sum, with one pair of barriers
     if (threadIdx.x != 0) return;
-    decide_core(st, q);
+    decide_core(st, a);
 }
 
 // Restart: move to the chosen point, make it the new anchor, measure the
@@ -438,28 +560,6 @@ __device__ inline double seg_dot(int b, int e, const int* __restrict__ idx, cons
     }
     for (; q < e; ++q) a += static_cast<double>(val[q]) * static_cast<double>(v[idx[q]]);
     return a;
-}
-
-// N block-wide sums at once, each the same as block_sum (xor tree in the
-// wavefront, then the wavefronts in order), with one pair of barriers for all.
-template <int N>
-__device__ inline void block_sums(double (&v)[N], double* shn) {
-    const int lane = static_cast<int>(threadIdx.x) % kWave, wid = static_cast<int>(threadIdx.x) / kWave;
-    const int nw = static_cast<int>(blockDim.x) / kWave;
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) shn[wid * N + k] = v[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        double t = 0.0;
-        for (int w = 0; w < nw; ++w) t += shn[w * N + k];
-        v[k] = t;
-    }
 }
 
 template <class T>
@@ -907,6 +1007,7 @@ struct Engine final : EngineBase {
     T *pw_v = nullptr, *pw_u = nullptr;
     // check buffers
     double *xo_cur = nullptr, *xo_cand = nullptr, *yo_cur = nullptr, *yo_cand = nullptr, *ax_cand = nullptr;
+    double2 *xo2 = nullptr, *yo2 = nullptr;  // the multi-kernel checks' points, interleaved
     double *wx = nullptr, *wy = nullptr;
     int has_warm = 0;
     double *part_x = nullptr, *part_r = nullptr, *part_c = nullptr, *part_dx = nullptr, *part_dy = nullptr;
@@ -1047,6 +1148,7 @@ struct Engine final : EngineBase {
         BO_TRY(mem.alloc(&pw_u, m));
         BO_TRY(mem.alloc(&yo_cur, m));
         BO_TRY(mem.alloc(&yo_cand, m));
+        BO_TRY(mem.alloc(&yo2, m));
         BO_TRY(mem.alloc(&wy, m));
         BO_TRY(mem.alloc(&c64, n));
         BO_TRY(mem.alloc(&lb64, n));
@@ -1062,6 +1164,7 @@ struct Engine final : EngineBase {
         BO_TRY(mem.alloc(&pw_v, n));
         BO_TRY(mem.alloc(&xo_cur, n));
         BO_TRY(mem.alloc(&xo_cand, n));
+        BO_TRY(mem.alloc(&xo2, n));
         BO_TRY(mem.alloc(&wx, n));
         BO_TRY(mem.alloc(&scal, 8));
         BO_TRY(mem.alloc(&st, 1));
@@ -1333,11 +1436,11 @@ struct Engine final : EngineBase {
     }
 
     void launch_check() {
-        hipLaunchKernelGGL(k_check_x<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, xk, xp, cs, c64, lb64, ub64,
-                           xo_cur, xo_cand, part_x);
-        EpiCheckRows<T> er{xo_cur, xo_cand, b64, eq, rs, yk, yp, yo_cur, yo_cand, ax_cand};
+        hipLaunchKernelGGL(k_check_x2<T>, dim3(nb_x), dim3(kBlock), 0, stream, n, st, xk, xp, cs, c64, lb64, ub64,
+                           xo2, part_x);
+        EpiCheckRows2<T> er{xo2, b64, eq, rs, yk, yp, yo2, ax_cand};
         launch_rows<kSumProd>(plan_rows.bt, rv64, er, part_r, done_flag());
-        EpiCheckCols ec{yo_cur, yo_cand, c64, lb64, ub64, cs};
+        EpiCheckCols2 ec{yo2, c64, lb64, ub64, cs};
         launch_cols<kSumProd>(plan_cols.bt, cv64, ec, part_c, done_flag());
         CheckDims d{nb_x, plan_rows.bt.total_blocks, plan_cols.bt.total_blocks};
         hipLaunchKernelGGL(k_decide, dim3(1), dim3(kDecideThreads), 0, stream, st, d, part_x, part_r, part_c);
@@ -1811,6 +1914,13 @@ struct Engine final : EngineBase {
     bo_status get_solution(double* hx, double* hy) override {
         const int b = st_host->best;
         std::vector<double> tx(n), ty(m);
+        if (!one_wg) {  // the multi-kernel checks keep both points interleaved
+            if (n) hipLaunchKernelGGL(k_pick2, dim3(grid_for(n)), dim3(kBlock), 0, stream, static_cast<size_t>(n),
+                                      static_cast<const double2*>(xo2), b, b ? xo_cand : xo_cur);
+            if (m) hipLaunchKernelGGL(k_pick2, dim3(grid_for(m)), dim3(kBlock), 0, stream, static_cast<size_t>(m),
+                                      static_cast<const double2*>(yo2), b, b ? yo_cand : yo_cur);
+            BO_TRY(hipGetLastError());
+        }
         if (n) BO_TRY(hipMemcpyAsync(tx.data(), b ? xo_cand : xo_cur, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
         if (m) BO_TRY(hipMemcpyAsync(ty.data(), b ? yo_cand : yo_cur, sizeof(double) * m, hipMemcpyDeviceToHost, stream));
         BO_TRY(hipStreamSynchronize(stream));

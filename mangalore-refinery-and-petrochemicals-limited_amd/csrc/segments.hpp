@@ -112,10 +112,19 @@ __device__ inline void seg_accumulate(int k0, int k1, int lane, const int* __res
                     a[g] = kk < k1 ? av : TV(0);
                     j[g] = kk < k1 ? jv : -1;
                 }
+                if constexpr (EpiPair<Epi>::value) {
 #pragma unroll
-                for (int g = 0; g < G; ++g)
+                    for (int g = 0; g < G; ++g) {
+                        const double2 xp2 = epi.vec2()[j[g] >= 0 ? j[g] : 0];
+                        xv[0][g] = xp2.x;
+                        xv[1][g] = xp2.y;
+                    }
+                } else {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) xv[v][g] = epi.vec(v)[j[g] >= 0 ? j[g] : 0];
+                    for (int g = 0; g < G; ++g)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) xv[v][g] = epi.vec(v)[j[g] >= 0 ? j[g] : 0];
+                }
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
