@@ -4,13 +4,16 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "../../include/bo_capi.h"
 
@@ -57,6 +60,27 @@ inline void launch_ev(F kernel, dim3 grid, dim3 block, unsigned lds, hipStream_t
         hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, ev[0], ev[1], 0, args...);
     else
         hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
+}
+
+// Host threads for set-up work: the hardware's, at most 16 (the host share
+// of one GPU on an 8-GPU node), at most `cap`.
+inline int host_thread_count(long long cap) {
+    const long long hw = std::max(1u, std::thread::hardware_concurrency());
+    return static_cast<int>(std::max(1LL, std::min(std::min(hw, 16LL), cap)));
+}
+
+// f(t, nt) on nt threads, t = 0 .. nt - 1 (the caller's thread runs t = 0).
+template <class F>
+inline void host_parallel(int nt, F&& f) {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back([&f, t, nt] { f(t, nt); });
+    f(0, nt);
+    for (auto& th : pool) th.join();
+}
+
+// [begin, end) of part t of nt equal parts of [0, count).
+inline std::pair<long long, long long> host_part(long long count, int t, int nt) {
+    return {count * t / nt, count * (t + 1) / nt};
 }
 
 // An absolute time limit shared by every round of one solve (main batch,

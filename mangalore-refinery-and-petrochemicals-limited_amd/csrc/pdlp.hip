@@ -31,10 +31,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <numeric>
 #include <vector>
 
@@ -1055,15 +1057,35 @@ struct Engine final : EngineBase {
         if (stream) hipStreamDestroy(stream);
     }
 
+    // verbose >= 1: the wall time of each set-up phase on stderr
+    void phase(const char* what, std::chrono::steady_clock::time_point& t) const {
+        const auto now = std::chrono::steady_clock::now();
+        if (prm.verbose >= 1)
+            std::fprintf(stderr, "bo_pdlp_create: %-34s %8.1f ms\n", what,
+                         std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+
     bo_status create(const bo_lp* lp, const bo_pdlp_params* p) {
         prm = *p;
         m = lp->m;
         n = lp->n;
         nnz = lp->nnz;
+        auto tph = std::chrono::steady_clock::now();
         BO_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PdlpState)));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_init), sizeof(PdlpState)));
         std::memset(st_host, 0, sizeof(PdlpState));
+        int dev = 0;
+        BO_TRY(hipGetDevice(&dev));
+        BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+        BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        // tiled layouts: for an explicit request, or for the tuner (2 tiles per CU)
+        auto tiles_for = [](int code) {
+            return code == 0 ? 2 : (code < 0 && code >= -kMaxTilesPerCu ? -code : 0);
+        };
+        const int tiles_rows = tiles_for(prm.lanes_rows), tiles_cols = tiles_for(prm.lanes_cols);
+        const int nth = host_thread_count(1 + nnz / (1 << 20));
         // ---- permutations: rows and columns by descending length (stable)
         std::vector<int> rlen(m), clen(n, 0);
         for (int i = 0; i < m; ++i) rlen[i] = lp->row_ptr[i + 1] - lp->row_ptr[i];
@@ -1080,33 +1102,92 @@ struct Engine final : EngineBase {
         col_len_sorted.resize(n);
         for (int p2 = 0; p2 < m; ++p2) row_len_sorted[p2] = rlen[row_of[p2]];
         for (int p2 = 0; p2 < n; ++p2) col_len_sorted[p2] = clen[col_of[p2]];
-        // permuted CSR
+        // permuted CSR, rows filled on nth host threads
         std::vector<int> h_rp(m + 1), h_ci(nnz);
         std::vector<double> h_rv(nnz);
         h_rp[0] = 0;
-        for (int p2 = 0; p2 < m; ++p2) {
-            const int i = row_of[p2];
-            int o = h_rp[p2];
-            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k, ++o) {
-                h_ci[o] = col_pos[lp->col_idx[k]];
-                h_rv[o] = lp->val[k];
-            }
-            h_rp[p2 + 1] = o;
-        }
-        // permuted CSC (transpose of the permuted CSR)
-        std::vector<int> h_cp(n + 1, 0), h_ri(nnz);
-        std::vector<double> h_cv(nnz);
-        for (long long k = 0; k < nnz; ++k) h_cp[h_ci[k] + 1]++;
-        for (int j = 0; j < n; ++j) h_cp[j + 1] += h_cp[j];
-        {
-            std::vector<int> pos(h_cp.begin(), h_cp.end() - 1);
-            for (int p2 = 0; p2 < m; ++p2)
-                for (int k = h_rp[p2]; k < h_rp[p2 + 1]; ++k) {
-                    const int q = pos[h_ci[k]]++;
-                    h_ri[q] = p2;
-                    h_cv[q] = h_rv[k];
+        for (int p2 = 0; p2 < m; ++p2) h_rp[p2 + 1] = h_rp[p2] + row_len_sorted[p2];
+        host_parallel(nth, [&](int t, int nt) {
+            const auto r = host_part(m, t, nt);
+            for (long long p2 = r.first; p2 < r.second; ++p2) {
+                const int i = row_of[p2];
+                int o = h_rp[p2];
+                for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k, ++o) {
+                    h_ci[o] = col_pos[lp->col_idx[k]];
+                    h_rv[o] = lp->val[k];
                 }
+            }
+        });
+        std::vector<int> h_cp(n + 1, 0), h_ri(nnz);  // the permuted CSC, below
+        std::vector<double> h_cv(nnz);
+        // The host layouts of the tiled passes are cut on threads of their
+        // own while the rest of the set-up goes on; `joiner` waits for them on
+        // every way out of this function (they read the host arrays above).
+        HostTiles ht[2];
+        bool ht_built[2] = {false, false};
+        std::atomic<bool> ht_oom{false};
+        std::thread tile_thread[2];
+        struct Joiner {
+            std::thread* t;
+            ~Joiner() {
+                for (int q = 0; q < 2; ++q)
+                    if (t[q].joinable()) t[q].join();
+            }
+        } joiner{tile_thread};
+        auto start_tiles = [&](int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx) {
+            if (k <= 0) return;
+            tile_thread[pass] = std::thread([this, pass, k, &ptr, &gidx, &ht, &ht_built, &ht_oom] {
+                try {
+                    ht_built[pass] = host_tiles(pass, k, ptr, gidx, ht[pass]);
+                } catch (const std::bad_alloc&) {
+                    ht_oom = true;
+                }
+            });
+        };
+        start_tiles(0, tiles_rows, h_rp, h_ci);
+        // permuted CSC (transpose of the permuted CSR): thread t counts its
+        // rows' entries per column, then writes them after those of threads
+        // 0 .. t-1, so every column keeps its rows in ascending order
+        {
+            std::vector<int> cnt(static_cast<size_t>(nth) * n, 0);
+            host_parallel(nth, [&](int t, int nt) {
+                int* c = cnt.data() + static_cast<size_t>(t) * n;
+                const auto r = host_part(m, t, nt);
+                for (int k = h_rp[r.first]; k < h_rp[r.second]; ++k) c[h_ci[k]]++;
+            });
+            host_parallel(nth, [&](int t, int nt) {  // column ranges: totals, then each thread's start
+                const auto r = host_part(n, t, nt);
+                for (long long j = r.first; j < r.second; ++j) {
+                    int tot = 0;
+                    for (int q = 0; q < nth; ++q) tot += cnt[static_cast<size_t>(q) * n + j];
+                    h_cp[j + 1] = tot;
+                }
+            });
+            for (int j = 0; j < n; ++j) h_cp[j + 1] += h_cp[j];
+            host_parallel(nth, [&](int t, int nt) {
+                const auto r = host_part(n, t, nt);
+                for (long long j = r.first; j < r.second; ++j) {
+                    int o = h_cp[j];
+                    for (int q = 0; q < nth; ++q) {
+                        int& c = cnt[static_cast<size_t>(q) * n + j];
+                        const int v = c;
+                        c = o;
+                        o += v;
+                    }
+                }
+            });
+            host_parallel(nth, [&](int t, int nt) {
+                int* pos = cnt.data() + static_cast<size_t>(t) * n;
+                const auto r = host_part(m, t, nt);
+                for (long long p2 = r.first; p2 < r.second; ++p2)
+                    for (int k = h_rp[p2]; k < h_rp[p2 + 1]; ++k) {
+                        const int q = pos[h_ci[k]]++;
+                        h_ri[q] = static_cast<int>(p2);
+                        h_cv[q] = h_rv[k];
+                    }
+            });
         }
+        start_tiles(1, tiles_cols, h_cp, h_ri);
         std::vector<double> hb(m), hc(n), hl(n), hu(n);
         std::vector<unsigned char> he(m);
         double b2 = 0.0, c2 = 0.0;
@@ -1124,6 +1205,7 @@ struct Engine final : EngineBase {
         }
         b_norm = std::sqrt(b2);
         c_norm = std::sqrt(c2);
+        phase("renumbering, CSR and CSC (host)", tph);
         // ---- device memory
         BO_TRY(mem.alloc(&rp, m + 1));
         BO_TRY(mem.alloc(&ci, nnz));
@@ -1183,6 +1265,7 @@ struct Engine final : EngineBase {
         BO_TRY(up(lb64, hl.data(), sizeof(double) * n));
         BO_TRY(up(ub64, hu.data(), sizeof(double) * n));
         BO_TRY(hipStreamSynchronize(stream));
+        phase("allocation and upload", tph);
         // ---- launch geometry
         nb_x = static_cast<int>(std::min<unsigned>(grid_for(n), 1024));
         nb_y = static_cast<int>(std::min<unsigned>(grid_for(m), 1024));
@@ -1206,16 +1289,17 @@ struct Engine final : EngineBase {
             }
         }
         BO_TRY(hipStreamSynchronize(stream));
-        int dev = 0;
-        BO_TRY(hipGetDevice(&dev));
-        BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
-        BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        // tiled layouts: for an explicit request, or for the tuner (2 tiles per CU)
-        auto tiles_for = [](int code) {
-            return code == 0 ? 2 : (code < 0 && code >= -kMaxTilesPerCu ? -code : 0);
-        };
-        if (int k = tiles_for(prm.lanes_rows)) BO_TRY(build_tile_pass(0, k, h_rp, h_ci));
-        if (int k = tiles_for(prm.lanes_cols)) BO_TRY(build_tile_pass(1, k, h_cp, h_ri));
+        phase("stream-kernel blocks", tph);
+        for (int pass = 0; pass < 2; ++pass) {
+            if (tile_thread[pass].joinable()) tile_thread[pass].join();
+            if (ht_oom) {
+                set_last_error("out of host memory while cutting the tiled layout");
+                return BO_OUT_OF_MEMORY;
+            }
+            if (ht_built[pass]) BO_TRY(upload_tile_pass(pass, pass == 0 ? tiles_rows : tiles_cols, ht[pass]));
+            ht[pass] = HostTiles{};
+            phase(pass == 0 ? "tiled layout, CSR pass (wait, upload)" : "tiled layout, CSC pass (wait, upload)", tph);
+        }
         // launch plans: lanes > 0 fixes the segment kernel's nonzeros-per-lane
         // target, lanes < 0 selects the stream kernel with LDS budget -lanes,
         // 0 lets the tuner choose
@@ -1236,10 +1320,12 @@ struct Engine final : EngineBase {
         }
         BO_TRY(mem.alloc(&part_r, max_r * 6));
         BO_TRY(mem.alloc(&part_c, max_c * 6));
+        phase("bin tables", tph);
         if (tune) {
             bo_status s2 = autotune();
             if (s2 != BO_SUCCESS) return s2;
         }
+        phase("schedule tuning", tph);
         prm.lanes_rows = plan_code(plan_rows);
         prm.lanes_cols = plan_code(plan_cols);
         prm.hot_columns = plan_rows.hot;
@@ -1256,20 +1342,25 @@ struct Engine final : EngineBase {
         // intervals have failed to converge on small LPs (C0-sized random
         // LPs at 128 and 256) where 64 converges and FP64 converges at any.
         if (one_wg && tune_cadence) prm.check_every = 64;
+        phase("one-workgroup tuning", tph);
         return BO_SUCCESS;
     }
 
-    // Cuts and uploads the tiled layout of one pass; leaves it unbuilt when the
-    // LDS is too small or the padded layout would not fit 32-bit offsets.
-    hipError_t build_tile_pass(int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx) {
-        TilePass& t = tp[pass];
+    // Host part of the tiled layout of one pass (no device call: it runs on a
+    // host thread of its own); false when the LDS is too small or the padded
+    // layout would not fit 32-bit offsets (the pass then stays unbuilt).
+    bool host_tiles(int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx, HostTiles& h) const {
         const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
-        if (nseg == 0) return hipSuccess;
+        if (nseg == 0) return false;
         // LDS holds the vector block: the largest block that fits
         const int wmax = std::min(65536, lds_max / static_cast<int>(sizeof(T))) / 64 * 64;
-        if (wmax < 64) return hipSuccess;
-        HostTiles h;
-        if (!build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, k * num_cus, h)) return hipSuccess;
+        if (wmax < 64) return false;
+        return build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, k * num_cus, h);
+    }
+
+    // Uploads the host layout of one pass (cut for k tiles per CU).
+    hipError_t upload_tile_pass(int pass, int k, const HostTiles& h) {
+        TilePass& t = tp[pass];
         // buffer ranges of the kernels are 32-bit byte counts below 2^31
         if (h.pairs * 4 >= (1LL << 31) - 64 || static_cast<long long>(h.slice_word.size()) * kWave * 8 >= (1LL << 31) - 64)
             return hipSuccess;

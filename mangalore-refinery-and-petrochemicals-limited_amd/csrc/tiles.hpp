@@ -28,6 +28,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <vector>
 
@@ -80,6 +81,8 @@ struct HostTiles {
 // block (a multiple of 64); target_tiles: about how many tiles to cut;
 // wide_min: pair length from which a pair gets L > 1 lanes.
 // Returns false when the padded layout does not fit 32-bit offsets.
+// The work runs on up to 16 host threads (segment ranges for the pairs,
+// tiles for the slices); the layout is the same for any thread count.
 inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int wmax, int target_tiles, HostTiles& h,
                         int wide_min = kWideMin) {
     h = HostTiles{};
@@ -93,33 +96,69 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
     h.B = B;
     h.W = W;
     const long long nnz = ptr[nseg];
-    // nonzeros grouped by block, segment-ascending inside a block; runs of one
-    // segment inside a block are its (block, segment) pairs
-    std::vector<long long> bstart(B + 1, 0);
-    for (long long k = 0; k < nnz; ++k) bstart[gidx[k] / W + 1]++;
-    for (int c = 0; c < B; ++c) bstart[c + 1] += bstart[c];
-    std::vector<int> ent(static_cast<size_t>(nnz));
+    const int nth = host_thread_count(1 + nnz / (1 << 19));
+    // Nonzeros grouped by block, segment-ascending inside a block; runs of one
+    // segment inside a block are its (block, segment) pairs.  Thread t takes
+    // a range of segments: it counts its nonzeros and pairs per block, and
+    // writes them after those of threads 0 .. t-1 in every block.
     struct Pair {
         int s;
         int len;
         long long start;  // into ent
     };
-    std::vector<std::vector<Pair>> pairs(B);
-    {
-        std::vector<long long> pos(bstart.begin(), bstart.end() - 1);
+    // segment ranges of equal nonzero counts (segments may be sorted by length)
+    std::vector<int> sbound(static_cast<size_t>(nth) + 1, nseg);
+    sbound[0] = 0;
+    for (int t = 1; t < nth; ++t)
+        sbound[t] = static_cast<int>(std::lower_bound(ptr, ptr + nseg, static_cast<int>(nnz * t / nth)) - ptr);
+    auto seg_range = [&](int t) { return std::pair<long long, long long>(sbound[t], sbound[t + 1]); };
+    std::vector<long long> zc(static_cast<size_t>(nth) * B, 0), pc(static_cast<size_t>(nth) * B, 0);
+    host_parallel(nth, [&](int t, int) {
+        const auto r = seg_range(t);
+        long long* z = zc.data() + static_cast<size_t>(t) * B;
+        long long* q = pc.data() + static_cast<size_t>(t) * B;
         std::vector<int> last(B, -1);
-        for (int s = 0; s < nseg; ++s) {
+        for (long long s = r.first; s < r.second; ++s)
+            for (int k = ptr[s]; k < ptr[s + 1]; ++k) {
+                const int c = gidx[k] / W;
+                z[c]++;
+                if (last[c] != s) {
+                    q[c]++;
+                    last[c] = static_cast<int>(s);
+                }
+            }
+    });
+    std::vector<long long> bstart(B + 1, 0);
+    std::vector<std::vector<Pair>> pairs(B);
+    for (int c = 0; c < B; ++c) {  // per (thread, block): first entry and first pair
+        long long zs = bstart[c], ps = 0;
+        for (int t = 0; t < nth; ++t) {
+            const long long z = zc[static_cast<size_t>(t) * B + c], q = pc[static_cast<size_t>(t) * B + c];
+            zc[static_cast<size_t>(t) * B + c] = zs;
+            pc[static_cast<size_t>(t) * B + c] = ps;
+            zs += z;
+            ps += q;
+        }
+        bstart[c + 1] = zs;
+        pairs[c].resize(static_cast<size_t>(ps));
+    }
+    std::vector<int> ent(static_cast<size_t>(nnz));
+    host_parallel(nth, [&](int t, int) {
+        const auto r = seg_range(t);
+        long long* zpos = zc.data() + static_cast<size_t>(t) * B;
+        long long* ppos = pc.data() + static_cast<size_t>(t) * B;
+        std::vector<int> last(B, -1);
+        for (long long s = r.first; s < r.second; ++s)
             for (int k = ptr[s]; k < ptr[s + 1]; ++k) {
                 const int c = gidx[k] / W;
                 if (last[c] != s) {
-                    pairs[c].push_back(Pair{s, 0, pos[c]});
-                    last[c] = s;
+                    pairs[c][ppos[c]++] = Pair{static_cast<int>(s), 0, zpos[c]};
+                    last[c] = static_cast<int>(s);
                 }
-                pairs[c].back().len++;
-                ent[pos[c]++] = k;
+                pairs[c][ppos[c] - 1].len++;
+                ent[zpos[c]++] = k;
             }
-        }
-    }
+    });
     // Tiles: consecutive pairs of one block with equal work (nonzeros plus a
     // per-pair cost for its partial).  Blocks get tile counts in proportion
     // to their work (largest remainders), so the layout has target_tiles
@@ -172,15 +211,6 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
     // heaviest tiles first (longest-processing-time-first dispatch)
     std::stable_sort(spans.begin(), spans.end(), [](const Span& a, const Span& b) { return a.work > b.work; });
 
-    // (segment, block, slot) of every pair, gathered into seg_ptr / slot below
-    std::vector<int> pair_seg, pair_blk;
-    std::vector<unsigned> pair_slot;
-    std::vector<size_t> order;
-    struct Slice {
-        std::vector<size_t> members;  // pair indices (into pairs[c])
-        int rows;
-        int lg;  // log2 of the lanes each member gets
-    };
     // lanes of a pair of `len` nonzeros: 1 below wide_min, else the power of
     // two (8 .. 64) that makes it span kLaneRows .. 2 kLaneRows - 1 rows
     auto lanes_lg = [wide_min](int len) {
@@ -189,92 +219,157 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
         while (lg < 6 && len >= (2 << lg) * kLaneRows) ++lg;
         return lg;
     };
-    for (const Span& sp : spans) {
-        const std::vector<Pair>& P = pairs[sp.c];
-        order.resize(sp.p1 - sp.p0);
-        std::iota(order.begin(), order.end(), sp.p0);
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return P[a].len > P[b].len; });
-        // slices, longest pairs first: pairs of one lane count together, 64 / L
-        // per slice (L lanes each), padded to the slice's longest member
-        std::vector<Slice> sl;
-        for (size_t q = 0; q < order.size();) {
-            const int lg = lanes_lg(P[order[q]].len);
-            const int lanes = 1 << lg;
-            Slice s{{}, (P[order[q]].len + lanes - 1) / lanes, lg};
-            size_t r = q;
-            for (; r < order.size() && r < q + static_cast<size_t>(kWave / lanes) && lanes_lg(P[order[r]].len) == lg; ++r)
-                s.members.push_back(order[r]);
-            sl.push_back(std::move(s));
-            q = r;
-        }
-        // deal the slices to the wavefronts, longest first, to the least loaded
-        std::vector<int> owner(sl.size());
-        std::vector<long long> load(kTileWaves, 0);
-        std::vector<size_t> by_rows(sl.size());
-        std::iota(by_rows.begin(), by_rows.end(), 0);
-        std::stable_sort(by_rows.begin(), by_rows.end(), [&](size_t a, size_t b) { return sl[a].rows > sl[b].rows; });
-        for (size_t i : by_rows) {
-            const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-            owner[i] = w;
-            load[w] += sl[i].rows + 1;
-        }
+    // Each tile on its own (tiles on the threads), with offsets local to the
+    // tile: its descriptor, slice words, padded entries and pairs.
+    struct Local {
         TileDesc td{};
-        td.block = sp.c;
-        td.seg_lo = P[sp.p0].s;
-        td.seg_hi = P[sp.p1 - 1].s;
-        for (int w = 0; w < kTileWaves; ++w) {
-            WaveDesc& wd = td.wave[w];
-            if (h.padded > 0xFFFFFFFFull) return false;
-            wd.off = static_cast<unsigned>(h.padded);
-            wd.slice0 = static_cast<int>(h.slice_word.size());
-            for (size_t i : by_rows) {
-                if (owner[i] != w) continue;
-                const Slice& s = sl[i];
-                const size_t off = h.padded;
-                const size_t ent_n = static_cast<size_t>(s.rows) * kWave;
-                if (off + ent_n > 0xFFFFFFFFull) return false;
-                const unsigned gslice = static_cast<unsigned>(h.slice_word.size());
-                if (gslice >= (1u << 26)) return false;
-                if (static_cast<unsigned>(s.rows) > kRowsMask) return false;
-                h.slice_word.push_back(static_cast<unsigned>(s.rows) | (static_cast<unsigned>(s.lg) << kLgShift));
-                h.padded += ent_n;
-                h.idx.resize(h.padded, 0);
-                h.src.resize(h.padded, -1);
-                const int lanes = 1 << s.lg;
-                for (size_t mi = 0; mi < s.members.size(); ++mi) {
-                    const Pair& p = P[s.members[mi]];
-                    const size_t lane0 = mi * static_cast<size_t>(lanes);
-                    for (int j = 0; j < p.len; ++j) {
-                        const int k = ent[p.start + j];
-                        const size_t e = off + static_cast<size_t>(j / lanes) * kWave + lane0 + (j % lanes);
-                        h.idx[e] = static_cast<unsigned short>(gidx[k] - sp.c * W);
-                        h.src[e] = k;
-                    }
-                    pair_seg.push_back(p.s);
-                    pair_blk.push_back(sp.c);
-                    // the member's sum ends in the last lane of its group
-                    pair_slot.push_back(gslice * kWave + static_cast<unsigned>(lane0 + lanes - 1));
-                }
-                wd.nrows += s.rows;
-                wd.nslice++;
+        std::vector<unsigned> words;
+        std::vector<unsigned short> idx;
+        std::vector<int> src;
+        std::vector<int> pseg;       // pairs: segment,
+        std::vector<unsigned> pslot;  // and slot (64 local slice + lane)
+        int max_wave_slices = 0;
+        bool fits = true;
+    };
+    const int nspan = static_cast<int>(spans.size());
+    std::vector<Local> loc(static_cast<size_t>(nspan));
+    std::atomic<int> next{0};
+    host_parallel(host_thread_count(nspan), [&](int, int) {
+        struct Slice {
+            std::vector<size_t> members;  // pair indices (into pairs[c])
+            int rows;
+            int lg;  // log2 of the lanes each member gets
+        };
+        std::vector<size_t> order;
+        for (int si = next++; si < nspan; si = next++) {
+            const Span& sp = spans[static_cast<size_t>(si)];
+            Local& L = loc[static_cast<size_t>(si)];
+            const std::vector<Pair>& P = pairs[sp.c];
+            order.resize(sp.p1 - sp.p0);
+            std::iota(order.begin(), order.end(), sp.p0);
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return P[a].len > P[b].len; });
+            // slices, longest pairs first: pairs of one lane count together, 64 / L
+            // per slice (L lanes each), padded to the slice's longest member
+            std::vector<Slice> sl;
+            for (size_t q = 0; q < order.size();) {
+                const int lg = lanes_lg(P[order[q]].len);
+                const int lanes = 1 << lg;
+                Slice s{{}, (P[order[q]].len + lanes - 1) / lanes, lg};
+                size_t r = q;
+                for (; r < order.size() && r < q + static_cast<size_t>(kWave / lanes) && lanes_lg(P[order[r]].len) == lg;
+                     ++r)
+                    s.members.push_back(order[r]);
+                sl.push_back(std::move(s));
+                q = r;
             }
-            td.nslice += wd.nslice;
-            h.max_wave_slices = std::max(h.max_wave_slices, wd.nslice);
+            // deal the slices to the wavefronts, longest first, to the least loaded
+            std::vector<int> owner(sl.size());
+            std::vector<long long> load(kTileWaves, 0);
+            std::vector<size_t> by_rows(sl.size());
+            std::iota(by_rows.begin(), by_rows.end(), 0);
+            std::stable_sort(by_rows.begin(), by_rows.end(),
+                             [&](size_t a, size_t b) { return sl[a].rows > sl[b].rows; });
+            for (size_t i : by_rows) {
+                const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+                owner[i] = w;
+                load[w] += sl[i].rows + 1;
+            }
+            size_t padded = 0;
+            for (const Slice& s : sl) padded += static_cast<size_t>(s.rows) * kWave;
+            L.idx.assign(padded, 0);
+            L.src.assign(padded, -1);
+            L.td.block = sp.c;
+            L.td.seg_lo = P[sp.p0].s;
+            L.td.seg_hi = P[sp.p1 - 1].s;
+            size_t off = 0;
+            for (int w = 0; w < kTileWaves; ++w) {
+                WaveDesc& wd = L.td.wave[w];
+                wd.off = static_cast<unsigned>(off);  // local; made global below
+                wd.slice0 = static_cast<int>(L.words.size());
+                for (size_t i : by_rows) {
+                    if (owner[i] != w) continue;
+                    const Slice& s = sl[i];
+                    const unsigned lslice = static_cast<unsigned>(L.words.size());
+                    if (static_cast<unsigned>(s.rows) > kRowsMask) L.fits = false;
+                    L.words.push_back(static_cast<unsigned>(s.rows) | (static_cast<unsigned>(s.lg) << kLgShift));
+                    const int lanes = 1 << s.lg;
+                    for (size_t mi = 0; mi < s.members.size(); ++mi) {
+                        const Pair& p = P[s.members[mi]];
+                        const size_t lane0 = mi * static_cast<size_t>(lanes);
+                        for (int j = 0; j < p.len; ++j) {
+                            const int k = ent[p.start + j];
+                            const size_t e = off + static_cast<size_t>(j / lanes) * kWave + lane0 + (j % lanes);
+                            L.idx[e] = static_cast<unsigned short>(gidx[k] - sp.c * W);
+                            L.src[e] = k;
+                        }
+                        L.pseg.push_back(p.s);
+                        // the member's sum ends in the last lane of its group
+                        L.pslot.push_back(lslice * kWave + static_cast<unsigned>(lane0 + lanes - 1));
+                    }
+                    off += static_cast<size_t>(s.rows) * kWave;
+                    wd.nrows += s.rows;
+                    wd.nslice++;
+                }
+                L.td.nslice += wd.nslice;
+                L.max_wave_slices = std::max(L.max_wave_slices, wd.nslice);
+            }
         }
-        h.tiles.push_back(td);
+    });
+    // Global offsets in tile order: padded entries, slices, pairs.
+    std::vector<size_t> poff(static_cast<size_t>(nspan) + 1, 0), soff(static_cast<size_t>(nspan) + 1, 0),
+        qoff(static_cast<size_t>(nspan) + 1, 0);
+    for (int si = 0; si < nspan; ++si) {
+        const Local& L = loc[static_cast<size_t>(si)];
+        if (!L.fits) return false;
+        poff[si + 1] = poff[si] + L.idx.size();
+        soff[si + 1] = soff[si] + L.words.size();
+        qoff[si + 1] = qoff[si] + L.pseg.size();
+        h.max_wave_slices = std::max(h.max_wave_slices, L.max_wave_slices);
     }
-    // segment -> partial slots, in block order
-    h.pairs = static_cast<long long>(pair_seg.size());
+    if (poff[nspan] > 0xFFFFFFFFull || soff[nspan] > (1u << 26)) return false;
+    h.padded = poff[nspan];
+    h.pairs = static_cast<long long>(qoff[nspan]);
+    h.tiles.resize(static_cast<size_t>(nspan));
+    h.slice_word.resize(soff[nspan]);
+    h.idx.resize(h.padded);
+    h.src.resize(h.padded);
+    std::vector<int> pair_seg(qoff[nspan]);
+    std::vector<unsigned> pair_slot(qoff[nspan]);
+    host_parallel(host_thread_count(nspan), [&](int t, int nt) {
+        const auto r = host_part(nspan, t, nt);
+        for (long long si = r.first; si < r.second; ++si) {
+            const Local& L = loc[static_cast<size_t>(si)];
+            TileDesc td = L.td;
+            for (int w = 0; w < kTileWaves; ++w) {
+                td.wave[w].off += static_cast<unsigned>(poff[si]);
+                td.wave[w].slice0 += static_cast<int>(soff[si]);
+            }
+            h.tiles[static_cast<size_t>(si)] = td;
+            std::copy(L.words.begin(), L.words.end(), h.slice_word.begin() + static_cast<long long>(soff[si]));
+            std::copy(L.idx.begin(), L.idx.end(), h.idx.begin() + static_cast<long long>(poff[si]));
+            std::copy(L.src.begin(), L.src.end(), h.src.begin() + static_cast<long long>(poff[si]));
+            for (size_t q = 0; q < L.pseg.size(); ++q) {
+                pair_seg[qoff[si] + q] = L.pseg[q];
+                pair_slot[qoff[si] + q] = L.pslot[q] + static_cast<unsigned>(soff[si]) * kWave;
+            }
+        }
+    });
+    loc.clear();
+    // segment -> partial slots, in block order: a counting sort by segment
+    // over the pairs taken block by block (a segment has at most one pair
+    // per block)
     h.seg_ptr.assign(static_cast<size_t>(nseg) + 1, 0);
     for (int s : pair_seg) h.seg_ptr[s + 1]++;
     for (int s = 0; s < nseg; ++s) h.seg_ptr[s + 1] += h.seg_ptr[s];
-    std::vector<size_t> pidx(pair_seg.size());
-    std::iota(pidx.begin(), pidx.end(), 0);
-    std::stable_sort(pidx.begin(), pidx.end(), [&](size_t a, size_t b) {
-        return pair_seg[a] != pair_seg[b] ? pair_seg[a] < pair_seg[b] : pair_blk[a] < pair_blk[b];
-    });
-    h.slot.resize(pidx.size());
-    for (size_t i = 0; i < pidx.size(); ++i) h.slot[i] = pair_slot[pidx[i]];
+    h.slot.resize(pair_seg.size());
+    {
+        std::vector<int> pos(h.seg_ptr.begin(), h.seg_ptr.end() - 1);
+        for (int c = 0; c < B; ++c)
+            for (int si = 0; si < nspan; ++si) {
+                if (spans[static_cast<size_t>(si)].c != c) continue;
+                for (size_t q = qoff[si]; q < qoff[si + 1]; ++q) h.slot[pos[pair_seg[q]]++] = pair_slot[q];
+            }
+    }
     for (int s = 0; s < nseg; ++s) h.max_seg_pairs = std::max(h.max_seg_pairs, h.seg_ptr[s + 1] - h.seg_ptr[s]);
     return true;
 }
