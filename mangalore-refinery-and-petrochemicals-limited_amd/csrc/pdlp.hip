@@ -1057,6 +1057,19 @@ struct Engine final : EngineBase {
         if (stream) hipStreamDestroy(stream);
     }
 
+    // Indices by descending length, equal lengths in index order (a stable
+    // counting sort).
+    static std::vector<int> order_by_length(const std::vector<int>& len) {
+        int top = 0;
+        for (int l : len) top = std::max(top, l);
+        std::vector<int> start(static_cast<size_t>(top) + 2, 0);
+        for (int l : len) start[static_cast<size_t>(top - l) + 1]++;
+        for (int v = 0; v <= top; ++v) start[v + 1] += start[v];
+        std::vector<int> order(len.size());
+        for (size_t i = 0; i < len.size(); ++i) order[start[static_cast<size_t>(top - len[i])]++] = static_cast<int>(i);
+        return order;
+    }
+
     // verbose >= 1: the wall time of each set-up phase on stderr
     void phase(const char* what, std::chrono::steady_clock::time_point& t) const {
         const auto now = std::chrono::steady_clock::now();
@@ -1089,13 +1102,21 @@ struct Engine final : EngineBase {
         // ---- permutations: rows and columns by descending length (stable)
         std::vector<int> rlen(m), clen(n, 0);
         for (int i = 0; i < m; ++i) rlen[i] = lp->row_ptr[i + 1] - lp->row_ptr[i];
-        for (long long k = 0; k < nnz; ++k) clen[lp->col_idx[k]]++;
-        row_of.resize(m);
-        col_of.resize(n);
-        std::iota(row_of.begin(), row_of.end(), 0);
-        std::iota(col_of.begin(), col_of.end(), 0);
-        std::stable_sort(row_of.begin(), row_of.end(), [&](int a, int b) { return rlen[a] > rlen[b]; });
-        std::stable_sort(col_of.begin(), col_of.end(), [&](int a, int b) { return clen[a] > clen[b]; });
+        {  // column lengths: per-thread counts over row ranges, then added
+            std::vector<int> cc(static_cast<size_t>(nth - 1) * n, 0);
+            host_parallel(nth, [&](int t, int nt) {
+                int* c = t == 0 ? clen.data() : cc.data() + static_cast<size_t>(t - 1) * n;
+                const auto r = host_part(m, t, nt);
+                for (int k = lp->row_ptr[r.first]; k < lp->row_ptr[r.second]; ++k) c[lp->col_idx[k]]++;
+            });
+            host_parallel(nth, [&](int t, int nt) {
+                const auto r = host_part(n, t, nt);
+                for (int q = 0; q + 1 < nth; ++q)
+                    for (long long j = r.first; j < r.second; ++j) clen[j] += cc[static_cast<size_t>(q) * n + j];
+            });
+        }
+        row_of = order_by_length(rlen);
+        col_of = order_by_length(clen);
         std::vector<int> col_pos(n);
         for (int p2 = 0; p2 < n; ++p2) col_pos[col_of[p2]] = p2;
         row_len_sorted.resize(m);
