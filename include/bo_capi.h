@@ -169,7 +169,9 @@ typedef struct bo_pdlp_params {
     double time_limit_s;       /* 0 = none; else wall-clock limit of one solve call, tail rounds and FP64
                                   polish included: LPs still running when it passes stop with status 3.
                                   A time limit turns the one-workgroup solve off (single_workgroup) */
-    int verbose;
+    int verbose;               /* diagnostics on stderr: >= 1 the single-LP engine's set-up phases and
+                                  their wall times at create, >= 2 also the batched engine's progress
+                                  per check interval and per tail round; 0 (default) silent */
     int hot_columns;           /* entries of x+ kept in LDS by the CSR pass: -1 tune, 0 off */
     long long stall_iterations; /* batched and family engines, FP32 iterates: an LP whose FP64 rel-KKT
                                    has not halved within max(this, iterations / 2) iterations stops
