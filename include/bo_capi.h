@@ -153,9 +153,7 @@ typedef struct bo_pdlp_params {
     double eps_rel_kkt;        /* stop when FP64 rel-KKT <= this (default 1e-6) */
     int ruiz_iterations;       /* default 10 */
     int pock_chambolle;        /* 1: l1 (alpha=1) rescaling after Ruiz */
-    int check_every;           /* iterations between FP64 KKT / restart checks; 0 (default): the
-                                  single-LP engine's tuner picks 64 or 128, the batched and family
-                                  engines use 64 */
+    int check_every;           /* iterations between FP64 KKT / restart checks; 0 (default): 64 */
     int lanes_rows;            /* CSR pass schedule: k > 0 segment kernel with k nonzeros per lane,
                                   -k (1..64) tiled gather (LDS-resident vector block, SELL-64
                                   slices) + reduce kernels, cut for k tiles per CU,
