@@ -417,30 +417,6 @@ struct CheckDims {
 // stopping test, the KKT restart rule and the bookkeeping.
 constexpr int kDecideThreads = 1024;
 
-// acc[k] += p[b * Q + k] over the blocks b = tid, tid + blockDim.x, ... in
-// order, the loads of four blocks in flight at once.
-template <int Q>
-__device__ inline void add_block_partials(const double* __restrict__ p, int nb, double* acc) {
-    constexpr int kIn = 4;
-    const int stride = static_cast<int>(blockDim.x);
-    for (int b0 = static_cast<int>(threadIdx.x); b0 < nb; b0 += kIn * stride) {
-        double v[kIn][Q];
-#pragma unroll
-        for (int u = 0; u < kIn; ++u) {
-            const int b = b0 + u * stride;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) v[u][k] = b < nb ? p[static_cast<size_t>(b) * Q + k] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kIn; ++u) {
-            if (b0 + u * stride < nb) {
-#pragma unroll
-                for (int k = 0; k < Q; ++k) acc[k] += v[u][k];
-            }
-        }
-    }
-}
-
 __global__ void __launch_bounds__(kDecideThreads)
 k_decide(PdlpState* st, CheckDims d, const double* px, const double* pr, const double* pc) {
     if (st->done) return;
@@ -448,12 +424,20 @@ k_decide(PdlpState* st, CheckDims d, const double* px, const double* pr, const d
     double a[14];
 #pragma unroll
     for (int k = 0; k < 14; ++k) a[k] = 0.0;
-    // Each thread adds blocks tid, tid + 1024, ... in that order; the loads
-    // of 4 of them are issued before the first is added (a check's row and
-    // column passes leave tens of thousands of blocks at C1 sizes).
-    add_block_partials<2>(px, d.nb_x, a);
-    add_block_partials<6>(pr, d.nb_rows, a + 2);
-    add_block_partials<6>(pc, d.nb_cols, a + 8);
+    for (int b = threadIdx.x; b < d.nb_x; b += blockDim.x) {
+        a[0] += px[b * 2 + 0];
+        a[1] += px[b * 2 + 1];
+    }
+    for (int b = threadIdx.x; b < d.nb_rows; b += blockDim.x) {
+        const double* p = pr + static_cast<size_t>(b) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a[2 + k] += p[k];
+    }
+    for (int b = threadIdx.x; b < d.nb_cols; b += blockDim.x) {
+        const double* p = pc + static_cast<size_t>(b) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a[8 + k] += p[k];
+    }
     block_sums(a, sh);  // the 14 sums of block_sum, with one pair of barriers
     if (threadIdx.x != 0) return;
     decide_core(st, a);
