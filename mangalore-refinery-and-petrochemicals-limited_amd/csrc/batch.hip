@@ -1232,69 +1232,77 @@ using namespace bo;
 extern "C" {
 
 bo_status bo_pdlp_batch_create(const bo_lp* lps, int count, const bo_pdlp_params* params, bo_pdlp_batch** out) {
-    if (!out) return BO_INVALID_ARGUMENT;
-    *out = nullptr;
-    if (!lps || count <= 0) return BO_INVALID_ARGUMENT;
-    long long m = 0, n = 0, nnz = 0;
-    for (int k = 0; k < count; ++k) {
-        if (!valid_csr_lp(&lps[k])) return BO_INVALID_ARGUMENT;
-        m += lps[k].m;
-        n += lps[k].n;
-        nnz += lps[k].nnz;
-    }
-    if (m >= (1LL << 31) - 1 || n >= (1LL << 31) - 1 || nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;
-    bo_status s = ensure_device_public();
-    if (s != BO_SUCCESS) return s;
-    bo_pdlp_params p;
-    if (params) p = *params;
-    else bo_pdlp_default_params(&p);
-    auto h = std::make_unique<bo_pdlp_batch>();
-    bo_status st;
-    if (p.fp64_iterates) {
-        auto e = std::make_unique<Batch<double>>();
-        st = e->create(lps, count, &p);
-        h->eng = std::move(e);
-    } else {
-        auto e = std::make_unique<Batch<float>>();
-        st = e->create(lps, count, &p);
-        h->eng = std::move(e);
-    }
-    if (st != BO_SUCCESS) return st;
-    h->prm = p;
-    h->count = count;
-    h->in_tail.assign(count, 0);
-    h->fx.resize(count);
-    h->fy.resize(count);
-    if (h->keep_lps()) {
-        h->lps.reserve(count);
-        for (int k = 0; k < count; ++k) h->lps.emplace_back(lps[k]);
-    }
-    *out = h.release();
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!out) return BO_INVALID_ARGUMENT;
+        *out = nullptr;
+        if (!lps || count <= 0) return BO_INVALID_ARGUMENT;
+        long long m = 0, n = 0, nnz = 0;
+        for (int k = 0; k < count; ++k) {
+            if (!valid_csr_lp(&lps[k])) return BO_INVALID_ARGUMENT;
+            m += lps[k].m;
+            n += lps[k].n;
+            nnz += lps[k].nnz;
+        }
+        if (m >= (1LL << 31) - 1 || n >= (1LL << 31) - 1 || nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;
+        bo_status s = ensure_device_public();
+        if (s != BO_SUCCESS) return s;
+        bo_pdlp_params p;
+        if (params) p = *params;
+        else bo_pdlp_default_params(&p);
+        auto h = std::make_unique<bo_pdlp_batch>();
+        bo_status st;
+        if (p.fp64_iterates) {
+            auto e = std::make_unique<Batch<double>>();
+            st = e->create(lps, count, &p);
+            h->eng = std::move(e);
+        } else {
+            auto e = std::make_unique<Batch<float>>();
+            st = e->create(lps, count, &p);
+            h->eng = std::move(e);
+        }
+        if (st != BO_SUCCESS) return st;
+        h->prm = p;
+        h->count = count;
+        h->in_tail.assign(count, 0);
+        h->fx.resize(count);
+        h->fy.resize(count);
+        if (h->keep_lps()) {
+            h->lps.reserve(count);
+            for (int k = 0; k < count; ++k) h->lps.emplace_back(lps[k]);
+        }
+        *out = h.release();
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_batch_set_warm_start(bo_pdlp_batch* h, const double* x, const double* y) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->set_warm_start(x, y);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->set_warm_start(x, y);
+    });
 }
 
 bo_status bo_pdlp_batch_set_primal_weights(bo_pdlp_batch* h, const double* w) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->set_primal_weights(w);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->set_primal_weights(w);
+    });
 }
 
 bo_status bo_pdlp_batch_set_objective_rhs(bo_pdlp_batch* h, const double* c, const double* b) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    bo_status s = h->eng->set_objective_rhs(c, b);
-    if (s != BO_SUCCESS) return s;
-    size_t oc = 0, ob = 0;
-    for (HostLp& lp : h->lps) {
-        if (c) std::copy(c + oc, c + oc + lp.n, lp.c.begin());
-        if (b) std::copy(b + ob, b + ob + lp.m, lp.b.begin());
-        oc += lp.n;
-        ob += lp.m;
-    }
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        bo_status s = h->eng->set_objective_rhs(c, b);
+        if (s != BO_SUCCESS) return s;
+        size_t oc = 0, ob = 0;
+        for (HostLp& lp : h->lps) {
+            if (c) std::copy(c + oc, c + oc + lp.n, lp.c.begin());
+            if (b) std::copy(b + ob, b + ob + lp.m, lp.b.begin());
+            oc += lp.n;
+            ob += lp.m;
+        }
+        return BO_SUCCESS;
+    });
 }
 
 }  // extern "C"
@@ -1405,86 +1413,92 @@ extern "C" {
 // iterates stall continues in an FP64 batch.  Every LP carries over its
 // point, its primal weight and the iterations it has used.
 bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    auto t0 = std::chrono::high_resolution_clock::now();
-    const int K = h->count;
-    std::vector<bo_pdlp_stats> per(K);
-    bo_pdlp_batch_stats tot{};
-    std::fill(h->in_tail.begin(), h->in_tail.end(), 0);
-    const Deadline dl = Deadline::after(h->prm.time_limit_s);
-    h->eng->set_deadline(dl);
-    h->eng->set_stop_running(h->compacting() && K >= 16 ? K / 8 : 0);
-    bo_status s = h->eng->solve(per.data(), &tot);
-    if (s != BO_SUCCESS) return s;
-    std::vector<TailLp> same, p64;
-    const bool main64 = h->prm.fp64_iterates != 0;
-    for (int k = 0; k < K; ++k) {
-        const bool cut = h->compacting() && tail_from_round(per[k], h->prm.max_iterations);
-        const bool stalled = per[k].status == 4 && h->polishing();
-        if (!cut && !stalled) continue;
-        TailLp t{k, per[k].iterations, main64 ? per[k].iterations : 0, per[k].primal_weight, {}, {}, per[k]};
-        t.x.resize(h->lps[k].n);
-        t.y.resize(h->lps[k].m);
-        if ((s = h->eng->get_solution(k, t.x.data(), t.y.data())) != BO_SUCCESS) return s;
-        (stalled ? p64 : same).push_back(std::move(t));
-    }
-    if (!same.empty() || !p64.empty()) {
-        auto tp0 = std::chrono::high_resolution_clock::now();
-        int tail_lps = static_cast<int>(same.size() + p64.size());
-        std::vector<TailLp> unused;
-        while (!same.empty() || !p64.empty()) {
-            if (dl.passed()) {
-                finish_timed_out(h, same, per);
-                finish_timed_out(h, p64, per);
-                break;
-            }
-            if (!same.empty()) {
-                std::vector<TailLp> in;
-                in.swap(same);
-                s = main64 ? tail_round<double>(h, dl, in, same, unused, per)
-                           : tail_round<float>(h, dl, in, same, p64, per);
-                if (s != BO_SUCCESS) return s;
-            }
-            if (!p64.empty() && !dl.passed()) {
-                std::vector<TailLp> in;
-                in.swap(p64);
-                if ((s = tail_round<double>(h, dl, in, p64, unused, per)) != BO_SUCCESS) return s;
-            }
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        auto t0 = std::chrono::high_resolution_clock::now();
+        const int K = h->count;
+        std::vector<bo_pdlp_stats> per(K);
+        bo_pdlp_batch_stats tot{};
+        std::fill(h->in_tail.begin(), h->in_tail.end(), 0);
+        const Deadline dl = Deadline::after(h->prm.time_limit_s);
+        h->eng->set_deadline(dl);
+        h->eng->set_stop_running(h->compacting() && K >= 16 ? K / 8 : 0);
+        bo_status s = h->eng->solve(per.data(), &tot);
+        if (s != BO_SUCCESS) return s;
+        std::vector<TailLp> same, p64;
+        const bool main64 = h->prm.fp64_iterates != 0;
+        for (int k = 0; k < K; ++k) {
+            const bool cut = h->compacting() && tail_from_round(per[k], h->prm.max_iterations);
+            const bool stalled = per[k].status == 4 && h->polishing();
+            if (!cut && !stalled) continue;
+            TailLp t{k, per[k].iterations, main64 ? per[k].iterations : 0, per[k].primal_weight, {}, {}, per[k]};
+            t.x.resize(h->lps[k].n);
+            t.y.resize(h->lps[k].m);
+            if ((s = h->eng->get_solution(k, t.x.data(), t.y.data())) != BO_SUCCESS) return s;
+            (stalled ? p64 : same).push_back(std::move(t));
         }
-        tot.tail_lps = tail_lps;
-        tot.tail_ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tp0).count();
-    }
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
-    tot.solved = 0;
-    tot.max_iterations = 0;
-    tot.sum_iterations = 0;
-    tot.polished = 0;
-    for (int k = 0; k < K; ++k) {
-        per[k].solve_ms = ms;
-        tot.solved += per[k].status == 0;
-        tot.max_iterations = std::max(tot.max_iterations, per[k].iterations);
-        tot.sum_iterations += per[k].iterations;
-        tot.polished += (!main64 && per[k].fp64_polish_iterations > 0) ? 1 : 0;
-    }
-    tot.solve_ms = ms;
-    if (per_lp) std::copy(per.begin(), per.end(), per_lp);
-    if (total) *total = tot;
-    return BO_SUCCESS;
+        if (!same.empty() || !p64.empty()) {
+            auto tp0 = std::chrono::high_resolution_clock::now();
+            int tail_lps = static_cast<int>(same.size() + p64.size());
+            std::vector<TailLp> unused;
+            while (!same.empty() || !p64.empty()) {
+                if (dl.passed()) {
+                    finish_timed_out(h, same, per);
+                    finish_timed_out(h, p64, per);
+                    break;
+                }
+                if (!same.empty()) {
+                    std::vector<TailLp> in;
+                    in.swap(same);
+                    s = main64 ? tail_round<double>(h, dl, in, same, unused, per)
+                               : tail_round<float>(h, dl, in, same, p64, per);
+                    if (s != BO_SUCCESS) return s;
+                }
+                if (!p64.empty() && !dl.passed()) {
+                    std::vector<TailLp> in;
+                    in.swap(p64);
+                    if ((s = tail_round<double>(h, dl, in, p64, unused, per)) != BO_SUCCESS) return s;
+                }
+            }
+            tot.tail_lps = tail_lps;
+            tot.tail_ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tp0).count();
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+        tot.solved = 0;
+        tot.max_iterations = 0;
+        tot.sum_iterations = 0;
+        tot.polished = 0;
+        for (int k = 0; k < K; ++k) {
+            per[k].solve_ms = ms;
+            tot.solved += per[k].status == 0;
+            tot.max_iterations = std::max(tot.max_iterations, per[k].iterations);
+            tot.sum_iterations += per[k].iterations;
+            tot.polished += (!main64 && per[k].fp64_polish_iterations > 0) ? 1 : 0;
+        }
+        tot.solve_ms = ms;
+        if (per_lp) std::copy(per.begin(), per.end(), per_lp);
+        if (total) *total = tot;
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_batch_time_kernels(bo_pdlp_batch* h, int reps, double* ms4, double* bytes4) {
-    if (!h || !ms4 || !bytes4) return BO_INVALID_ARGUMENT;
-    return h->eng->time_kernels(reps, ms4, bytes4);
+    return host_guard([&]() -> bo_status {
+        if (!h || !ms4 || !bytes4) return BO_INVALID_ARGUMENT;
+        return h->eng->time_kernels(reps, ms4, bytes4);
+    });
 }
 
 bo_status bo_pdlp_batch_get_solution(bo_pdlp_batch* h, int k, double* x, double* y) {
-    if (!h || k < 0 || k >= h->count) return BO_INVALID_ARGUMENT;
-    if (h->in_tail[k]) {
-        if (x) std::copy(h->fx[k].begin(), h->fx[k].end(), x);
-        if (y) std::copy(h->fy[k].begin(), h->fy[k].end(), y);
-        return BO_SUCCESS;
-    }
-    return h->eng->get_solution(k, x, y);
+    return host_guard([&]() -> bo_status {
+        if (!h || k < 0 || k >= h->count) return BO_INVALID_ARGUMENT;
+        if (h->in_tail[k]) {
+            if (x) std::copy(h->fx[k].begin(), h->fx[k].end(), x);
+            if (y) std::copy(h->fy[k].begin(), h->fy[k].end(), y);
+            return BO_SUCCESS;
+        }
+        return h->eng->get_solution(k, x, y);
+    });
 }
 
 void bo_pdlp_batch_destroy(bo_pdlp_batch* h) { delete h; }

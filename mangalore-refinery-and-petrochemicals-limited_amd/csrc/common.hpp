@@ -9,6 +9,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -62,6 +64,25 @@ inline void launch_ev(F kernel, dim3 grid, dim3 block, unsigned lds, hipStream_t
         hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
 }
 
+// Runs an entry point's body so that no C++ exception leaves the C ABI: a
+// host allocation that fails returns BO_OUT_OF_MEMORY, any other exception
+// BO_EXECUTION_FAILED, each with its message in the last-error text.
+template <class F>
+inline bo_status host_guard(F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        set_last_error("out of host memory");
+        return BO_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(std::string("host exception: ") + e.what());
+        return BO_EXECUTION_FAILED;
+    } catch (...) {
+        set_last_error("host exception");
+        return BO_EXECUTION_FAILED;
+    }
+}
+
 // Host threads for set-up work: the hardware's, at most 16 (the host share
 // of one GPU on an 8-GPU node), at most `cap`.
 inline int host_thread_count(long long cap) {
@@ -70,12 +91,24 @@ inline int host_thread_count(long long cap) {
 }
 
 // f(t, nt) on nt threads, t = 0 .. nt - 1 (the caller's thread runs t = 0).
+// An exception thrown by any part (std::bad_alloc, say) is rethrown in the
+// caller once every thread has finished.
 template <class F>
 inline void host_parallel(int nt, F&& f) {
+    std::vector<std::exception_ptr> err(static_cast<size_t>(nt));
+    auto run = [&f, &err, nt](int t) {
+        try {
+            f(t, nt);
+        } catch (...) {
+            err[static_cast<size_t>(t)] = std::current_exception();
+        }
+    };
     std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back([&f, t, nt] { f(t, nt); });
-    f(0, nt);
+    for (int t = 1; t < nt; ++t) pool.emplace_back(run, t);
+    run(0);
     for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // [begin, end) of part t of nt equal parts of [0, count).

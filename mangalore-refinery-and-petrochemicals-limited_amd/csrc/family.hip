@@ -1133,174 +1133,188 @@ bo_status make_family(const bo_lp& lp, int capacity, const bo_pdlp_params& p, st
 extern "C" {
 
 bo_status bo_pdlp_family_create(const bo_lp* lp, int capacity, const bo_pdlp_params* params, bo_pdlp_family** out) {
-    if (!out) return BO_INVALID_ARGUMENT;
-    *out = nullptr;
-    if (!valid_csr_lp(lp) || capacity <= 0 || capacity > kL || lp->m <= 0 || lp->n <= 0 || lp->nnz <= 0 ||
-        lp->nnz >= (1LL << 31) - 1)
-        return BO_INVALID_ARGUMENT;
-    for (long long q = 0; q < lp->nnz; ++q)
-        if (!std::isfinite(lp->val[q])) return BO_INVALID_ARGUMENT;
-    for (int j = 0; j < lp->n; ++j)
-        if (!(lp->lb[j] <= lp->ub[j]) || !std::isfinite(lp->c[j])) return BO_INVALID_ARGUMENT;
-    if (static_cast<long long>(std::max(lp->m, lp->n)) * kL >= (1LL << 31)) return BO_INVALID_ARGUMENT;
-    bo_status s = ensure_device_public();
-    if (s != BO_SUCCESS) return s;
-    bo_pdlp_params p;
-    if (params) p = *params;
-    else bo_pdlp_default_params(&p);
-    auto h = std::make_unique<bo_pdlp_family>();
-    s = p.fp64_iterates ? make_family<double>(*lp, capacity, p, h->eng) : make_family<float>(*lp, capacity, p, h->eng);
-    if (s != BO_SUCCESS) return s;
-    h->prm = p;
-    h->m = lp->m;
-    h->n = lp->n;
-    h->rp.assign(lp->row_ptr, lp->row_ptr + lp->m + 1);
-    h->ci.assign(lp->col_idx, lp->col_idx + lp->nnz);
-    h->val.assign(lp->val, lp->val + lp->nnz);
-    h->lb.assign(lp->lb, lp->lb + lp->n);
-    h->ub.assign(lp->ub, lp->ub + lp->n);
-    h->eq.assign(lp->is_eq, lp->is_eq + lp->m);
-    h->c.clear();
-    h->b.clear();
-    for (int k = 0; k < capacity; ++k) {
-        h->c.insert(h->c.end(), lp->c, lp->c + lp->n);
-        h->b.insert(h->b.end(), lp->b, lp->b + lp->m);
-    }
-    h->pol_of.assign(capacity, -1);
-    *out = h.release();
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!out) return BO_INVALID_ARGUMENT;
+        *out = nullptr;
+        if (!valid_csr_lp(lp) || capacity <= 0 || capacity > kL || lp->m <= 0 || lp->n <= 0 || lp->nnz <= 0 ||
+            lp->nnz >= (1LL << 31) - 1)
+            return BO_INVALID_ARGUMENT;
+        for (long long q = 0; q < lp->nnz; ++q)
+            if (!std::isfinite(lp->val[q])) return BO_INVALID_ARGUMENT;
+        for (int j = 0; j < lp->n; ++j)
+            if (!(lp->lb[j] <= lp->ub[j]) || !std::isfinite(lp->c[j])) return BO_INVALID_ARGUMENT;
+        if (static_cast<long long>(std::max(lp->m, lp->n)) * kL >= (1LL << 31)) return BO_INVALID_ARGUMENT;
+        bo_status s = ensure_device_public();
+        if (s != BO_SUCCESS) return s;
+        bo_pdlp_params p;
+        if (params) p = *params;
+        else bo_pdlp_default_params(&p);
+        auto h = std::make_unique<bo_pdlp_family>();
+        s = p.fp64_iterates ? make_family<double>(*lp, capacity, p, h->eng) : make_family<float>(*lp, capacity, p, h->eng);
+        if (s != BO_SUCCESS) return s;
+        h->prm = p;
+        h->m = lp->m;
+        h->n = lp->n;
+        h->rp.assign(lp->row_ptr, lp->row_ptr + lp->m + 1);
+        h->ci.assign(lp->col_idx, lp->col_idx + lp->nnz);
+        h->val.assign(lp->val, lp->val + lp->nnz);
+        h->lb.assign(lp->lb, lp->lb + lp->n);
+        h->ub.assign(lp->ub, lp->ub + lp->n);
+        h->eq.assign(lp->is_eq, lp->is_eq + lp->m);
+        h->c.clear();
+        h->b.clear();
+        for (int k = 0; k < capacity; ++k) {
+            h->c.insert(h->c.end(), lp->c, lp->c + lp->n);
+            h->b.insert(h->b.end(), lp->b, lp->b + lp->m);
+        }
+        h->pol_of.assign(capacity, -1);
+        *out = h.release();
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_family_set_lps(bo_pdlp_family* h, int count, const double* c, const double* b) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    bo_status s = h->eng->set_lps(count, c, b);
-    if (s != BO_SUCCESS) return s;
-    if (c) {
-        h->c.assign(c, c + static_cast<size_t>(count) * h->n);
-        h->b.assign(b, b + static_cast<size_t>(count) * h->m);
-    }
-    h->pol_of.assign(count, -1);
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        bo_status s = h->eng->set_lps(count, c, b);
+        if (s != BO_SUCCESS) return s;
+        if (c) {
+            h->c.assign(c, c + static_cast<size_t>(count) * h->n);
+            h->b.assign(b, b + static_cast<size_t>(count) * h->m);
+        }
+        h->pol_of.assign(count, -1);
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_family_set_warm_start(bo_pdlp_family* h, const double* x, const double* y) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->set_warm_start(x, y);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->set_warm_start(x, y);
+    });
 }
 
 bo_status bo_pdlp_family_set_primal_weights(bo_pdlp_family* h, const double* w) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->set_primal_weights(w);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->set_primal_weights(w);
+    });
 }
 
 bo_status bo_pdlp_family_solve(bo_pdlp_family* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    auto t0 = std::chrono::high_resolution_clock::now();
-    const int K = h->eng->count();
-    std::vector<bo_pdlp_stats> per(K);
-    bo_pdlp_batch_stats tot{};
-    h->pol_of.assign(K, -1);
-    const Deadline dl = Deadline::after(h->prm.time_limit_s);
-    h->eng->set_deadline(dl);
-    bo_status s = h->eng->solve(per.data(), &tot);
-    if (s != BO_SUCCESS) return s;
-    std::vector<int> stalled;
-    if (h->polishing())
-        for (int k = 0; k < K; ++k)
-            if (per[k].status == 4 && per[k].iterations < h->prm.max_iterations) stalled.push_back(k);
-    if (!stalled.empty() && dl.passed()) {  // no time left for the polish: stopped by the limit
-        for (int k : stalled) per[k].status = 3;
-        stalled.clear();
-    }
-    if (!stalled.empty()) {
-        auto tp0 = std::chrono::high_resolution_clock::now();
-        const int P = static_cast<int>(stalled.size());
-        if (!h->pol) {
-            bo_pdlp_params p = h->prm;
-            p.fp64_iterates = 1;
-            std::vector<double> c0(h->c.begin(), h->c.begin() + h->n), b0(h->b.begin(), h->b.begin() + h->m);
-            bo_lp v = h->base();
-            v.c = c0.data();
-            v.b = b0.data();
-            if ((s = make_family<double>(v, kL, p, h->pol)) != BO_SUCCESS) return s;
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        auto t0 = std::chrono::high_resolution_clock::now();
+        const int K = h->eng->count();
+        std::vector<bo_pdlp_stats> per(K);
+        bo_pdlp_batch_stats tot{};
+        h->pol_of.assign(K, -1);
+        const Deadline dl = Deadline::after(h->prm.time_limit_s);
+        h->eng->set_deadline(dl);
+        bo_status s = h->eng->solve(per.data(), &tot);
+        if (s != BO_SUCCESS) return s;
+        std::vector<int> stalled;
+        if (h->polishing())
+            for (int k = 0; k < K; ++k)
+                if (per[k].status == 4 && per[k].iterations < h->prm.max_iterations) stalled.push_back(k);
+        if (!stalled.empty() && dl.passed()) {  // no time left for the polish: stopped by the limit
+            for (int k : stalled) per[k].status = 3;
+            stalled.clear();
         }
-        std::vector<double> X(static_cast<size_t>(K) * h->n), Y(static_cast<size_t>(K) * h->m);
-        if ((s = h->eng->get_solutions(X.data(), Y.data())) != BO_SUCCESS) return s;
-        std::vector<double> pc, pb, px, py, w(P);
-        std::vector<long long> lim(P);
-        for (int j = 0; j < P; ++j) {
-            const int k = stalled[j];
-            const size_t on = static_cast<size_t>(k) * h->n, om = static_cast<size_t>(k) * h->m;
-            pc.insert(pc.end(), h->c.begin() + on, h->c.begin() + on + h->n);
-            pb.insert(pb.end(), h->b.begin() + om, h->b.begin() + om + h->m);
-            px.insert(px.end(), X.begin() + on, X.begin() + on + h->n);
-            py.insert(py.end(), Y.begin() + om, Y.begin() + om + h->m);
-            w[j] = per[k].primal_weight;
-            lim[j] = h->prm.max_iterations - per[k].iterations;
+        if (!stalled.empty()) {
+            auto tp0 = std::chrono::high_resolution_clock::now();
+            const int P = static_cast<int>(stalled.size());
+            if (!h->pol) {
+                bo_pdlp_params p = h->prm;
+                p.fp64_iterates = 1;
+                std::vector<double> c0(h->c.begin(), h->c.begin() + h->n), b0(h->b.begin(), h->b.begin() + h->m);
+                bo_lp v = h->base();
+                v.c = c0.data();
+                v.b = b0.data();
+                if ((s = make_family<double>(v, kL, p, h->pol)) != BO_SUCCESS) return s;
+            }
+            std::vector<double> X(static_cast<size_t>(K) * h->n), Y(static_cast<size_t>(K) * h->m);
+            if ((s = h->eng->get_solutions(X.data(), Y.data())) != BO_SUCCESS) return s;
+            std::vector<double> pc, pb, px, py, w(P);
+            std::vector<long long> lim(P);
+            for (int j = 0; j < P; ++j) {
+                const int k = stalled[j];
+                const size_t on = static_cast<size_t>(k) * h->n, om = static_cast<size_t>(k) * h->m;
+                pc.insert(pc.end(), h->c.begin() + on, h->c.begin() + on + h->n);
+                pb.insert(pb.end(), h->b.begin() + om, h->b.begin() + om + h->m);
+                px.insert(px.end(), X.begin() + on, X.begin() + on + h->n);
+                py.insert(py.end(), Y.begin() + om, Y.begin() + om + h->m);
+                w[j] = per[k].primal_weight;
+                lim[j] = h->prm.max_iterations - per[k].iterations;
+            }
+            FamilyBase* f = h->pol.get();
+            if ((s = f->set_lps(P, pc.data(), pb.data())) != BO_SUCCESS) return s;
+            f->set_iteration_limits(lim.data());
+            f->set_deadline(dl);
+            if ((s = f->set_primal_weights(w.data())) != BO_SUCCESS) return s;
+            if ((s = f->set_warm_start(px.data(), py.data())) != BO_SUCCESS) return s;
+            std::vector<bo_pdlp_stats> per2(P);
+            bo_pdlp_batch_stats tot2{};
+            if ((s = f->solve(per2.data(), &tot2)) != BO_SUCCESS) return s;
+            for (int j = 0; j < P; ++j) {
+                const int k = stalled[j];
+                const long long fp32_its = per[k].iterations;
+                per[k] = per2[j];
+                per[k].iterations += fp32_its;
+                per[k].fp64_polish_iterations = per2[j].iterations;
+                h->pol_of[k] = j;
+            }
+            tot.polished = P;
+            tot.tail_lps = P;
+            tot.tail_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tp0).count();
         }
-        FamilyBase* f = h->pol.get();
-        if ((s = f->set_lps(P, pc.data(), pb.data())) != BO_SUCCESS) return s;
-        f->set_iteration_limits(lim.data());
-        f->set_deadline(dl);
-        if ((s = f->set_primal_weights(w.data())) != BO_SUCCESS) return s;
-        if ((s = f->set_warm_start(px.data(), py.data())) != BO_SUCCESS) return s;
-        std::vector<bo_pdlp_stats> per2(P);
-        bo_pdlp_batch_stats tot2{};
-        if ((s = f->solve(per2.data(), &tot2)) != BO_SUCCESS) return s;
-        for (int j = 0; j < P; ++j) {
-            const int k = stalled[j];
-            const long long fp32_its = per[k].iterations;
-            per[k] = per2[j];
-            per[k].iterations += fp32_its;
-            per[k].fp64_polish_iterations = per2[j].iterations;
-            h->pol_of[k] = j;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+        tot.solved = 0;
+        tot.max_iterations = 0;
+        tot.sum_iterations = 0;
+        for (int k = 0; k < K; ++k) {
+            per[k].solve_ms = ms;
+            tot.solved += per[k].status == 0;
+            tot.max_iterations = std::max(tot.max_iterations, per[k].iterations);
+            tot.sum_iterations += per[k].iterations;
         }
-        tot.polished = P;
-        tot.tail_lps = P;
-        tot.tail_ms =
-            std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - tp0).count();
-    }
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
-    tot.solved = 0;
-    tot.max_iterations = 0;
-    tot.sum_iterations = 0;
-    for (int k = 0; k < K; ++k) {
-        per[k].solve_ms = ms;
-        tot.solved += per[k].status == 0;
-        tot.max_iterations = std::max(tot.max_iterations, per[k].iterations);
-        tot.sum_iterations += per[k].iterations;
-    }
-    tot.solve_ms = ms;
-    if (per_lp) std::copy(per.begin(), per.end(), per_lp);
-    if (total) *total = tot;
-    return BO_SUCCESS;
+        tot.solve_ms = ms;
+        if (per_lp) std::copy(per.begin(), per.end(), per_lp);
+        if (total) *total = tot;
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_family_get_solutions(bo_pdlp_family* h, double* x, double* y) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    const int K = h->eng->count();
-    bo_status s = h->eng->get_solutions(x, y);
-    if (s != BO_SUCCESS) return s;
-    bool any = false;
-    for (int k = 0; k < K; ++k) any = any || h->pol_of[k] >= 0;
-    if (!any || !h->pol) return BO_SUCCESS;
-    const int P = h->pol->count();
-    std::vector<double> X(static_cast<size_t>(P) * h->n), Y(static_cast<size_t>(P) * h->m);
-    if ((s = h->pol->get_solutions(x ? X.data() : nullptr, y ? Y.data() : nullptr)) != BO_SUCCESS) return s;
-    for (int k = 0; k < K; ++k) {
-        const int j = h->pol_of[k];
-        if (j < 0) continue;
-        if (x) std::copy(X.begin() + static_cast<size_t>(j) * h->n, X.begin() + static_cast<size_t>(j + 1) * h->n,
-                         x + static_cast<size_t>(k) * h->n);
-        if (y) std::copy(Y.begin() + static_cast<size_t>(j) * h->m, Y.begin() + static_cast<size_t>(j + 1) * h->m,
-                         y + static_cast<size_t>(k) * h->m);
-    }
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        const int K = h->eng->count();
+        bo_status s = h->eng->get_solutions(x, y);
+        if (s != BO_SUCCESS) return s;
+        bool any = false;
+        for (int k = 0; k < K; ++k) any = any || h->pol_of[k] >= 0;
+        if (!any || !h->pol) return BO_SUCCESS;
+        const int P = h->pol->count();
+        std::vector<double> X(static_cast<size_t>(P) * h->n), Y(static_cast<size_t>(P) * h->m);
+        if ((s = h->pol->get_solutions(x ? X.data() : nullptr, y ? Y.data() : nullptr)) != BO_SUCCESS) return s;
+        for (int k = 0; k < K; ++k) {
+            const int j = h->pol_of[k];
+            if (j < 0) continue;
+            if (x) std::copy(X.begin() + static_cast<size_t>(j) * h->n, X.begin() + static_cast<size_t>(j + 1) * h->n,
+                             x + static_cast<size_t>(k) * h->n);
+            if (y) std::copy(Y.begin() + static_cast<size_t>(j) * h->m, Y.begin() + static_cast<size_t>(j + 1) * h->m,
+                             y + static_cast<size_t>(k) * h->m);
+        }
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_family_time_passes(bo_pdlp_family* h, int reps, double* ms2) {
-    if (!h || reps <= 0 || !ms2) return BO_INVALID_ARGUMENT;
-    return h->eng->time_passes(reps, ms2);
+    return host_guard([&]() -> bo_status {
+        if (!h || reps <= 0 || !ms2) return BO_INVALID_ARGUMENT;
+        return h->eng->time_passes(reps, ms2);
+    });
 }
 
 void bo_pdlp_family_destroy(bo_pdlp_family* h) { delete h; }

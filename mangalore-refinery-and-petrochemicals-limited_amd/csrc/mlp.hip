@@ -498,103 +498,117 @@ extern "C" {
 
 bo_status bo_mlp_create(int n_in, int width, int n_out, int layers, int max_batch, unsigned long long seed,
                         bo_mlp** out) {
-    if (!out) return BO_INVALID_ARGUMENT;
-    *out = nullptr;
-    bo_status s = ensure_device_public();
-    if (s != BO_SUCCESS) return s;
-    if (n_in <= 0 || width <= 0 || n_out <= 0 || layers < 2 || max_batch <= 0) return BO_INVALID_ARGUMENT;
-    if (static_cast<long long>(max_batch) * std::max(std::max(n_in, n_out), width) >= (1LL << 31))
-        return BO_INVALID_ARGUMENT;
-    auto h = std::make_unique<bo_mlp>();
-    s = h->create(n_in, width, n_out, layers, max_batch, seed);
-    if (s != BO_SUCCESS) return s;
-    *out = h.release();
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!out) return BO_INVALID_ARGUMENT;
+        *out = nullptr;
+        bo_status s = ensure_device_public();
+        if (s != BO_SUCCESS) return s;
+        if (n_in <= 0 || width <= 0 || n_out <= 0 || layers < 2 || max_batch <= 0) return BO_INVALID_ARGUMENT;
+        if (static_cast<long long>(max_batch) * std::max(std::max(n_in, n_out), width) >= (1LL << 31))
+            return BO_INVALID_ARGUMENT;
+        auto h = std::make_unique<bo_mlp>();
+        s = h->create(n_in, width, n_out, layers, max_batch, seed);
+        if (s != BO_SUCCESS) return s;
+        *out = h.release();
+        return BO_SUCCESS;
+    });
 }
 
 long long bo_mlp_num_params(const bo_mlp* h) { return h ? static_cast<long long>(h->nparam) : 0; }
 
 bo_status bo_mlp_get_params(bo_mlp* h, float* params) {
-    if (!h || !params) return BO_INVALID_ARGUMENT;
-    BO_TRY(hipMemcpyAsync(params, h->w, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
-    BO_TRY(hipStreamSynchronize(h->stream));
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h || !params) return BO_INVALID_ARGUMENT;
+        BO_TRY(hipMemcpyAsync(params, h->w, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
+        BO_TRY(hipStreamSynchronize(h->stream));
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_mlp_set_params(bo_mlp* h, const float* params) {
-    if (!h || !params) return BO_INVALID_ARGUMENT;
-    return h->set_params(params);
+    return host_guard([&]() -> bo_status {
+        if (!h || !params) return BO_INVALID_ARGUMENT;
+        return h->set_params(params);
+    });
 }
 
 bo_status bo_mlp_forward(bo_mlp* h, const float* x, int batch, float* y) {
-    if (!h || !x || !y || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
-    bo_status s = h->upload_x(x, batch);
-    if (s != BO_SUCCESS) return s;
-    h->err = hipSuccess;
-    h->forward(batch);
-    BO_TRY(h->take_error());
-    BO_TRY(hipMemcpyAsync(y, h->y, sizeof(float) * batch * h->n_out, hipMemcpyDeviceToHost, h->stream));
-    BO_TRY(hipStreamSynchronize(h->stream));
-    BO_TRY(hipGetLastError());
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h || !x || !y || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
+        bo_status s = h->upload_x(x, batch);
+        if (s != BO_SUCCESS) return s;
+        h->err = hipSuccess;
+        h->forward(batch);
+        BO_TRY(h->take_error());
+        BO_TRY(hipMemcpyAsync(y, h->y, sizeof(float) * batch * h->n_out, hipMemcpyDeviceToHost, h->stream));
+        BO_TRY(hipStreamSynchronize(h->stream));
+        BO_TRY(hipGetLastError());
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_mlp_gradients(bo_mlp* h, const float* x, const float* target, int batch, double* loss, float* grads) {
-    if (!h || !x || !target || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
-    bo_status s = h->upload_x(x, batch);
-    if (s != BO_SUCCESS) return s;
-    BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
-    h->err = hipSuccess;
-    h->forward(batch);
-    h->backward(batch);
-    BO_TRY(h->take_error());
-    if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (grads) BO_TRY(hipMemcpyAsync(grads, h->g, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
-    BO_TRY(hipStreamSynchronize(h->stream));
-    BO_TRY(hipGetLastError());
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h || !x || !target || batch <= 0 || batch > h->max_batch) return BO_INVALID_ARGUMENT;
+        bo_status s = h->upload_x(x, batch);
+        if (s != BO_SUCCESS) return s;
+        BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
+        h->err = hipSuccess;
+        h->forward(batch);
+        h->backward(batch);
+        BO_TRY(h->take_error());
+        if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (grads) BO_TRY(hipMemcpyAsync(grads, h->g, sizeof(float) * h->nparam, hipMemcpyDeviceToHost, h->stream));
+        BO_TRY(hipStreamSynchronize(h->stream));
+        BO_TRY(hipGetLastError());
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_mlp_train_step(bo_mlp* h, const float* x, const float* target, int batch, float lr, double* loss) {
-    if (!h || !x || !target || batch <= 0 || batch > h->max_batch || !(lr > 0.0f)) return BO_INVALID_ARGUMENT;
-    bo_status s = h->upload_x(x, batch);
-    if (s != BO_SUCCESS) return s;
-    BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
-    h->err = hipSuccess;
-    h->forward(batch);
-    h->backward(batch);
-    BO_TRY(h->take_error());
-    h->adam(lr);
-    if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    BO_TRY(hipStreamSynchronize(h->stream));
-    BO_TRY(hipGetLastError());
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!h || !x || !target || batch <= 0 || batch > h->max_batch || !(lr > 0.0f)) return BO_INVALID_ARGUMENT;
+        bo_status s = h->upload_x(x, batch);
+        if (s != BO_SUCCESS) return s;
+        BO_TRY(hipMemcpyAsync(h->t, target, sizeof(float) * batch * h->n_out, hipMemcpyHostToDevice, h->stream));
+        h->err = hipSuccess;
+        h->forward(batch);
+        h->backward(batch);
+        BO_TRY(h->take_error());
+        h->adam(lr);
+        if (loss) BO_TRY(hipMemcpyAsync(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        BO_TRY(hipStreamSynchronize(h->stream));
+        BO_TRY(hipGetLastError());
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_mlp_time(bo_mlp* h, int what, int batch, int reps, double* ms) {
-    if (!h || !ms || (what != 0 && what != 1) || batch <= 0 || batch > h->max_batch || reps <= 0)
-        return BO_INVALID_ARGUMENT;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    BO_TRY(hipEventCreate(&e0));
-    hipError_t e = hipEventCreate(&e1);
-    if (e == hipSuccess) {
-        h->err = hipSuccess;
-        for (int r = 0; r <= reps; ++r) {  // one untimed pass first
-            if (r == 1) e = hipEventRecord(e0, h->stream);
-            h->forward(batch);
-            if (what == 1) h->backward(batch);
+    return host_guard([&]() -> bo_status {
+        if (!h || !ms || (what != 0 && what != 1) || batch <= 0 || batch > h->max_batch || reps <= 0)
+            return BO_INVALID_ARGUMENT;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        BO_TRY(hipEventCreate(&e0));
+        hipError_t e = hipEventCreate(&e1);
+        if (e == hipSuccess) {
+            h->err = hipSuccess;
+            for (int r = 0; r <= reps; ++r) {  // one untimed pass first
+                if (r == 1) e = hipEventRecord(e0, h->stream);
+                h->forward(batch);
+                if (what == 1) h->backward(batch);
+            }
+            if (e == hipSuccess) e = h->take_error();
+            if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            *ms = static_cast<double>(t) / reps;
         }
-        if (e == hipSuccess) e = h->take_error();
-        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float t = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-        *ms = static_cast<double>(t) / reps;
-    }
-    hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    BO_TRY(e);
-    return BO_SUCCESS;
+        hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        BO_TRY(e);
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_gemm_bf16(int M, int N, int K, const unsigned short* A, long long sam, long long sak,

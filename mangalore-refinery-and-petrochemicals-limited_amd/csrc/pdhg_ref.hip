@@ -769,65 +769,69 @@ RefDev carve(unsigned char* base, int m, int n, long long nnz, size_t* used) {
 
 extern "C" bo_status bo_pdhg_ref_solve(const bo_lp* lp, const bo_ref_options* o, double* x_out,
                                        double* y_out, bo_ref_result* res) {
-    if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
-        return BO_INVALID_ARGUMENT;
-    if (lp->nnz >= (1LL << 31)) return BO_INVALID_ARGUMENT;  // int32 offsets
-    bo_status st = ensure_device_public();
-    if (st != BO_SUCCESS) return st;
-    const int m = lp->m, n = lp->n;
-    const long long nnz = lp->nnz;
-    const auto t0 = std::chrono::high_resolution_clock::now();
-    std::vector<int> cp(static_cast<size_t>(n) + 1), ri(static_cast<size_t>(nnz)), pos(static_cast<size_t>(n) + 1);
-    std::vector<double> cv(static_cast<size_t>(nnz)), x0(static_cast<size_t>(n) + 1);
-    ref_host_prep(lp, cp.data(), ri.data(), cv.data(), pos.data(), x0.data());
-    DevBufs mem;
-    RefDev d{};
-    BO_TRY(mem.alloc(&d.rp, m + 1));
-    BO_TRY(mem.alloc(&d.ci, nnz));
-    BO_TRY(mem.alloc(&d.rv, nnz));
-    BO_TRY(mem.alloc(&d.cp, n + 1));
-    BO_TRY(mem.alloc(&d.ri, nnz));
-    BO_TRY(mem.alloc(&d.cv, nnz));
-    BO_TRY(mem.alloc(&d.b, m));
-    BO_TRY(mem.alloc(&d.eq, m));
-    BO_TRY(mem.alloc(&d.sigma, m));
-    BO_TRY(mem.alloc(&d.y, m));
-    BO_TRY(mem.alloc(&d.c, n));
-    BO_TRY(mem.alloc(&d.lb, n));
-    BO_TRY(mem.alloc(&d.ub, n));
-    BO_TRY(mem.alloc(&d.tau, n));
-    BO_TRY(mem.alloc(&d.x, n));
-    BO_TRY(mem.alloc(&d.xsum, n));
-    BO_TRY(mem.alloc(&d.xhat, n));
-    BO_TRY(mem.alloc(&d.xavg, n));
-    BO_TRY(mem.alloc(&d.out, 8));
-    BO_TRY(mem.alloc(&d.red, 8));
-    return ref_run(lp, d, cp.data(), ri.data(), cv.data(), x0.data(), o, x_out, y_out, res, t0);
+    return host_guard([&]() -> bo_status {
+        if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
+            return BO_INVALID_ARGUMENT;
+        if (lp->nnz >= (1LL << 31)) return BO_INVALID_ARGUMENT;  // int32 offsets
+        bo_status st = ensure_device_public();
+        if (st != BO_SUCCESS) return st;
+        const int m = lp->m, n = lp->n;
+        const long long nnz = lp->nnz;
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        std::vector<int> cp(static_cast<size_t>(n) + 1), ri(static_cast<size_t>(nnz)), pos(static_cast<size_t>(n) + 1);
+        std::vector<double> cv(static_cast<size_t>(nnz)), x0(static_cast<size_t>(n) + 1);
+        ref_host_prep(lp, cp.data(), ri.data(), cv.data(), pos.data(), x0.data());
+        DevBufs mem;
+        RefDev d{};
+        BO_TRY(mem.alloc(&d.rp, m + 1));
+        BO_TRY(mem.alloc(&d.ci, nnz));
+        BO_TRY(mem.alloc(&d.rv, nnz));
+        BO_TRY(mem.alloc(&d.cp, n + 1));
+        BO_TRY(mem.alloc(&d.ri, nnz));
+        BO_TRY(mem.alloc(&d.cv, nnz));
+        BO_TRY(mem.alloc(&d.b, m));
+        BO_TRY(mem.alloc(&d.eq, m));
+        BO_TRY(mem.alloc(&d.sigma, m));
+        BO_TRY(mem.alloc(&d.y, m));
+        BO_TRY(mem.alloc(&d.c, n));
+        BO_TRY(mem.alloc(&d.lb, n));
+        BO_TRY(mem.alloc(&d.ub, n));
+        BO_TRY(mem.alloc(&d.tau, n));
+        BO_TRY(mem.alloc(&d.x, n));
+        BO_TRY(mem.alloc(&d.xsum, n));
+        BO_TRY(mem.alloc(&d.xhat, n));
+        BO_TRY(mem.alloc(&d.xavg, n));
+        BO_TRY(mem.alloc(&d.out, 8));
+        BO_TRY(mem.alloc(&d.red, 8));
+        return ref_run(lp, d, cp.data(), ri.data(), cv.data(), x0.data(), o, x_out, y_out, res, t0);
+    });
 }
 
 extern "C" bo_status bo_pdhg_ref_solve_noalloc(const bo_lp* lp, const bo_ref_options* o, double* x_out,
                                                double* y_out, bo_ref_result* res) {
-    if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
-        return BO_INVALID_ARGUMENT;
-    if (static_cast<long long>(lp->m) + lp->n > kNoAllocMaxEntries || lp->nnz > kNoAllocMaxNnz) {
-        set_last_error("bo_pdhg_ref_solve_noalloc: the LP exceeds the fixed scratch (needs m + n <= " +
-                       std::to_string(kNoAllocMaxEntries) + " and nnz <= " + std::to_string(kNoAllocMaxNnz) +
-                       "; got m + n = " + std::to_string(static_cast<long long>(lp->m) + lp->n) +
-                       ", nnz = " + std::to_string(lp->nnz) + "); use bo_pdhg_ref_solve");
-        return BO_INVALID_ARGUMENT;
-    }
-    bo_status st = ensure_device_public();
-    if (st != BO_SUCCESS) return st;
-    std::lock_guard<std::mutex> hold(g_noalloc_lock);
-    void* base = nullptr;
-    if ((st = scratch_address(&base)) != BO_SUCCESS) return st;
-    size_t used = 0;
-    const RefDev d = carve(static_cast<unsigned char*>(base), lp->m, lp->n, lp->nnz, &used);
-    if (used > kNoAllocScratchBytes) {  // cannot happen within the limits above
-        set_last_error("bo_pdhg_ref_solve_noalloc: scratch layout exceeds its size");
-        return BO_EXECUTION_FAILED;
-    }
-    const auto t0 = std::chrono::high_resolution_clock::now();
-    ref_host_prep(lp, g_h_cp, g_h_ri, g_h_cv, g_h_pos, g_h_x0);
-    return ref_run(lp, d, g_h_cp, g_h_ri, g_h_cv, g_h_x0, o, x_out, y_out, res, t0);
+    return host_guard([&]() -> bo_status {
+        if (!o || !res || !valid_csr_lp(lp) || (lp->n > 0 && !x_out) || (lp->m > 0 && !y_out))
+            return BO_INVALID_ARGUMENT;
+        if (static_cast<long long>(lp->m) + lp->n > kNoAllocMaxEntries || lp->nnz > kNoAllocMaxNnz) {
+            set_last_error("bo_pdhg_ref_solve_noalloc: the LP exceeds the fixed scratch (needs m + n <= " +
+                           std::to_string(kNoAllocMaxEntries) + " and nnz <= " + std::to_string(kNoAllocMaxNnz) +
+                           "; got m + n = " + std::to_string(static_cast<long long>(lp->m) + lp->n) +
+                           ", nnz = " + std::to_string(lp->nnz) + "); use bo_pdhg_ref_solve");
+            return BO_INVALID_ARGUMENT;
+        }
+        bo_status st = ensure_device_public();
+        if (st != BO_SUCCESS) return st;
+        std::lock_guard<std::mutex> hold(g_noalloc_lock);
+        void* base = nullptr;
+        if ((st = scratch_address(&base)) != BO_SUCCESS) return st;
+        size_t used = 0;
+        const RefDev d = carve(static_cast<unsigned char*>(base), lp->m, lp->n, lp->nnz, &used);
+        if (used > kNoAllocScratchBytes) {  // cannot happen within the limits above
+            set_last_error("bo_pdhg_ref_solve_noalloc: scratch layout exceeds its size");
+            return BO_EXECUTION_FAILED;
+        }
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        ref_host_prep(lp, g_h_cp, g_h_ri, g_h_cv, g_h_pos, g_h_x0);
+        return ref_run(lp, d, g_h_cp, g_h_ri, g_h_cv, g_h_x0, o, x_out, y_out, res, t0);
+    });
 }

@@ -1130,7 +1130,7 @@ struct Engine final : EngineBase {
         // every way out of this function (they read the host arrays above).
         HostTiles ht[2];
         bool ht_built[2] = {false, false};
-        std::atomic<bool> ht_oom{false};
+        std::exception_ptr ht_err[2];  // rethrown here once the thread is joined
         std::thread tile_thread[2];
         struct Joiner {
             std::thread* t;
@@ -1141,11 +1141,11 @@ struct Engine final : EngineBase {
         } joiner{tile_thread};
         auto start_tiles = [&](int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx) {
             if (k <= 0) return;
-            tile_thread[pass] = std::thread([this, pass, k, &ptr, &gidx, &ht, &ht_built, &ht_oom] {
+            tile_thread[pass] = std::thread([this, pass, k, &ptr, &gidx, &ht, &ht_built, &ht_err] {
                 try {
                     ht_built[pass] = host_tiles(pass, k, ptr, gidx, ht[pass]);
-                } catch (const std::bad_alloc&) {
-                    ht_oom = true;
+                } catch (...) {
+                    ht_err[pass] = std::current_exception();
                 }
             });
         };
@@ -1297,10 +1297,7 @@ struct Engine final : EngineBase {
         phase("stream-kernel blocks", tph);
         for (int pass = 0; pass < 2; ++pass) {
             if (tile_thread[pass].joinable()) tile_thread[pass].join();
-            if (ht_oom) {
-                set_last_error("out of host memory while cutting the tiled layout");
-                return BO_OUT_OF_MEMORY;
-            }
+            if (ht_err[pass]) std::rethrow_exception(ht_err[pass]);  // to the entry point's guard
             if (ht_built[pass]) BO_TRY(upload_tile_pass(pass, pass == 0 ? tiles_rows : tiles_cols, ht[pass]));
             ht[pass] = HostTiles{};
             phase(pass == 0 ? "tiled layout, CSR pass (wait, upload)" : "tiled layout, CSC pass (wait, upload)", tph);
@@ -2119,51 +2116,61 @@ void bo_pdlp_default_params(bo_pdlp_params* p) {
 }
 
 bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp** out) {
-    if (!out) return BO_INVALID_ARGUMENT;
-    *out = nullptr;
-    if (!valid_csr_lp(lp)) return BO_INVALID_ARGUMENT;
-    if (lp->nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;  // int32 offsets with unrolled strides
-    bo_status s = ensure_device_public();
-    if (s != BO_SUCCESS) return s;
-    bo_pdlp_params p;
-    if (params) p = *params;
-    else bo_pdlp_default_params(&p);
-    auto h = std::make_unique<bo_pdlp>();
-    bo_status st;
-    if (p.fp64_iterates) {
-        auto e = std::make_unique<Engine<double>>();
-        st = e->create(lp, &p);
-        h->eng = std::move(e);
-    } else {
-        auto e = std::make_unique<Engine<float>>();
-        st = e->create(lp, &p);
-        h->eng = std::move(e);
-    }
-    if (st != BO_SUCCESS) return st;
-    *out = h.release();
-    return BO_SUCCESS;
+    return host_guard([&]() -> bo_status {
+        if (!out) return BO_INVALID_ARGUMENT;
+        *out = nullptr;
+        if (!valid_csr_lp(lp)) return BO_INVALID_ARGUMENT;
+        if (lp->nnz >= (1LL << 31) - 4096) return BO_INVALID_ARGUMENT;  // int32 offsets with unrolled strides
+        bo_status s = ensure_device_public();
+        if (s != BO_SUCCESS) return s;
+        bo_pdlp_params p;
+        if (params) p = *params;
+        else bo_pdlp_default_params(&p);
+        auto h = std::make_unique<bo_pdlp>();
+        bo_status st;
+        if (p.fp64_iterates) {
+            auto e = std::make_unique<Engine<double>>();
+            st = e->create(lp, &p);
+            h->eng = std::move(e);
+        } else {
+            auto e = std::make_unique<Engine<float>>();
+            st = e->create(lp, &p);
+            h->eng = std::move(e);
+        }
+        if (st != BO_SUCCESS) return st;
+        *out = h.release();
+        return BO_SUCCESS;
+    });
 }
 
 bo_status bo_pdlp_set_warm_start(bo_pdlp* h, const double* x0, const double* y0) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->set_warm_start(x0, y0);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->set_warm_start(x0, y0);
+    });
 }
 
 bo_status bo_pdlp_solve(bo_pdlp* h, bo_pdlp_stats* stats) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->solve(stats);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->solve(stats);
+    });
 }
 
 bo_status bo_pdlp_get_solution(bo_pdlp* h, double* x, double* y) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->get_solution(x, y);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->get_solution(x, y);
+    });
 }
 
 void bo_pdlp_destroy(bo_pdlp* h) { delete h; }
 
 bo_status bo_pdlp_time_kernels(bo_pdlp* h, int reps, double* ms4) {
-    if (!h || !h->eng || !ms4) return BO_INVALID_ARGUMENT;
-    return h->eng->time_kernels(reps, ms4);
+    return host_guard([&]() -> bo_status {
+        if (!h || !h->eng || !ms4) return BO_INVALID_ARGUMENT;
+        return h->eng->time_kernels(reps, ms4);
+    });
 }
 
 bo_status bo_pdlp_get_pass_info(bo_pdlp* h, int pass, bo_pdlp_pass_info* out) {
@@ -2172,8 +2179,10 @@ bo_status bo_pdlp_get_pass_info(bo_pdlp* h, int pass, bo_pdlp_pass_info* out) {
 }
 
 bo_status bo_pdlp_time_spmv(bo_pdlp* h, int reps, double* csr_ms, double* csc_ms) {
-    if (!h) return BO_INVALID_ARGUMENT;
-    return h->eng->time_spmv(reps, csr_ms, csc_ms);
+    return host_guard([&]() -> bo_status {
+        if (!h) return BO_INVALID_ARGUMENT;
+        return h->eng->time_spmv(reps, csr_ms, csc_ms);
+    });
 }
 
 }  // extern "C"
