@@ -178,7 +178,7 @@ typedef struct bo_pdlp_params {
                                    more batch time) */
     int fp64_polish;           /* batched and family engines: finish every STALLED LP with FP64 iterates from
                                   its FP32 point, within its remaining iteration budget (default 1) */
-    int compact_tail;          /* batched engine: once at most 1/8 of a batch's LPs still run, continue
+    int compact_tail;          /* batched engine: once at most 1/4 of a batch's LPs still run, continue
                                   them in a compact batch (segment kernels), from their points, primal
                                   weights and remaining budgets, repeatedly (default 1) */
     int single_workgroup;      /* single-LP engine: 1 run the whole solve in one workgroup (no kernel

@@ -1319,6 +1319,13 @@ struct TailLp {
     bo_pdlp_stats last;  // its stats at the end of its last round
 };
 
+// A round returns once at most 1/kTailShare of its LPs still run; the rest
+// go on in a compact batch.  Swept on the 512-LP ragged batch (MI355X):
+// 1/2 49.9, 1/3 61.3, 1/4 62.6, 1/5 62.5, 1/6 61.9, 1/8 60.0 LPs/s -- a
+// smaller share keeps more LPs on a layout whose tiles mostly idle, a larger
+// one moves too many onto the segment kernels.
+constexpr int kTailShare = 4;
+
 // Cut by a round's stop rule with iterations left: status 1 only (an LP the
 // time limit stopped has status 3 and is final).
 bool tail_from_round(const bo_pdlp_stats& s, long long lim) { return s.status == 1 && s.iterations < lim; }
@@ -1367,7 +1374,7 @@ bo_status tail_round(bo_pdlp_batch* h, const Deadline& dl, std::vector<TailLp>& 
     bo_status s = e.create(views.data(), P, &p);
     if (s != BO_SUCCESS) return s;
     e.set_iteration_limits(lim.data());
-    e.set_stop_running(h->compacting() && P >= 16 ? P / 8 : 0);
+    e.set_stop_running(h->compacting() && P >= 16 ? P / kTailShare : 0);
     e.set_deadline(dl);
     if ((s = e.set_primal_weights(w.data())) != BO_SUCCESS) return s;
     if ((s = e.set_warm_start(wx.data(), wy.data())) != BO_SUCCESS) return s;
@@ -1407,9 +1414,9 @@ bo_status tail_round(bo_pdlp_batch* h, const Deadline& dl, std::vector<TailLp>& 
 extern "C" {
 
 // The main batch, then the tail: with compact_tail the main batch returns
-// once at most 1/8 of its LPs still run, and those continue in compact
-// batches (segment kernels, no tile layout) that again hand over their last
-// eighth, until every LP has stopped; with fp64_polish an LP whose FP32
+// once at most 1/kTailShare of its LPs still run, and those continue in
+// compact batches (segment kernels, no tile layout) that again hand over
+// their last share, until every LP has stopped; with fp64_polish an LP whose FP32
 // iterates stall continues in an FP64 batch.  Every LP carries over its
 // point, its primal weight and the iterations it has used.
 bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_batch_stats* total) {
@@ -1422,7 +1429,7 @@ bo_status bo_pdlp_batch_solve(bo_pdlp_batch* h, bo_pdlp_stats* per_lp, bo_pdlp_b
         std::fill(h->in_tail.begin(), h->in_tail.end(), 0);
         const Deadline dl = Deadline::after(h->prm.time_limit_s);
         h->eng->set_deadline(dl);
-        h->eng->set_stop_running(h->compacting() && K >= 16 ? K / 8 : 0);
+        h->eng->set_stop_running(h->compacting() && K >= 16 ? K / kTailShare : 0);
         bo_status s = h->eng->solve(per.data(), &tot);
         if (s != BO_SUCCESS) return s;
         std::vector<TailLp> same, p64;
