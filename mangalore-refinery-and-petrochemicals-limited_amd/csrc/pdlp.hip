@@ -1606,7 +1606,11 @@ struct Engine final : EngineBase {
         } else {
             cc.push_back(plan_cols);
         }
-        const PassPlan base_r = fixed_plan(0, 4), base_c = fixed_plan(1, 4);
+        // the other pass of each timed pair: the tiled one where it is built
+        // (it is what a solve alternates with at the sizes that build it, and
+        // the cheapest to run beside each candidate), else the segment kernel
+        const PassPlan base_r = fixed_plan(0, tp[0].k > 0 ? -tp[0].k : 4);
+        const PassPlan base_c = fixed_plan(1, tp[1].k > 0 ? -tp[1].k : 4);
         double best_r = 1e30, best_c = 1e30;
         PassPlan pr = cr[0], pc = cc[0];
         for (const PassPlan& cand : cr) {
