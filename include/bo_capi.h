@@ -162,7 +162,10 @@ typedef struct bo_pdlp_params {
     int lanes_cols;            /* CSC pass schedule, same encoding; the batched engine uses the tiled
                                   layout for a pass whose code is <= 0 and the segment kernel
                                   otherwise */
-    int fp64_iterates;         /* 1: run the iteration in FP64 (error comparison run) */
+    int fp64_iterates;         /* 0 (default): FP32 iterates and scaled matrix values; 1: FP64 both (the
+                                  error comparison run); 2: FP64 iterates over FP32 matrix values, the
+                                  multi-kernel loop only (single-LP engine; the batched and family
+                                  engines take 2 as 1) */
     int use_graph;             /* 1: replay each check interval as a hipGraph */
     double time_limit_s;       /* 0 = none; else wall-clock limit of one solve call, tail rounds and FP64
                                   polish included: LPs still running when it passes stop with status 3.

@@ -602,7 +602,7 @@ struct Batch final : BatchBase {
         if (h.pairs * 4 >= (1LL << 31) - 64 || static_cast<long long>(h.slice_word.size()) * kWave * 8 >= (1LL << 31) - 64)
             return hipSuccess;
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(
-                                               pass == 0 ? &k_tile<T, kTileRows, 2> : &k_tile<T, kTileRows, 3>),
+                                               pass == 0 ? &k_tile<T, T, kTileRows, 2> : &k_tile<T, T, kTileRows, 3>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            static_cast<int>(h.W * sizeof(T)));
         if (e != hipSuccess) return e;
@@ -872,7 +872,7 @@ struct Batch final : BatchBase {
         const TilePass& tl = tp[1];
         EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, inst_c, tl.built ? live_red_c : live_chk_c};
         if (tl.built) {
-            hipLaunchKernelGGL((k_tile<T, kTileRows, 3>), dim3(tl.ntiles), dim3(kTileThreads),
+            hipLaunchKernelGGL((k_tile<T, T, kTileRows, 3>), dim3(tl.ntiles), dim3(kTileThreads),
                                static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, yg, m,
                                tl.W, tl.part, tl.active);
             launch_reduce(tl, n, e);
@@ -884,7 +884,7 @@ struct Batch final : BatchBase {
         const TilePass& tl = tp[0];
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, inst_r, tl.built ? live_red_r : live_chk_r};
         if (tl.built) {
-            hipLaunchKernelGGL((k_tile<T, kTileRows, 2>), dim3(tl.ntiles), dim3(kTileThreads),
+            hipLaunchKernelGGL((k_tile<T, T, kTileRows, 2>), dim3(tl.ntiles), dim3(kTileThreads),
                                static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, xp, n,
                                tl.W, tl.part, tl.active);
             launch_reduce(tl, m, e);
@@ -995,13 +995,13 @@ struct Batch final : BatchBase {
         for (int r = 0; r < reps; ++r) {
             const TilePass& tc = tp[1];
             const TilePass& tr = tp[0];
-            launch_ev(k_tile<T, kTileRows, 3>, dim3(tc.ntiles), dim3(kTileThreads),
+            launch_ev(k_tile<T, T, kTileRows, 3>, dim3(tc.ntiles), dim3(kTileThreads),
                       static_cast<unsigned>(tc.W * sizeof(T)), stream, &ev[0], static_cast<const TileDesc*>(tc.td),
                       static_cast<const unsigned*>(tc.words), static_cast<const T*>(tc.val),
                       static_cast<const unsigned short*>(tc.idx), static_cast<const T*>(yg), m, tc.W, tc.part,
                       static_cast<const unsigned char*>(tc.active));
             launch_reduce(tc, n, EpiPrimal<T>{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, 0, inst_c, live_red_c}, &ev[2]);
-            launch_ev(k_tile<T, kTileRows, 2>, dim3(tr.ntiles), dim3(kTileThreads),
+            launch_ev(k_tile<T, T, kTileRows, 2>, dim3(tr.ntiles), dim3(kTileThreads),
                       static_cast<unsigned>(tr.W * sizeof(T)), stream, &ev[4], static_cast<const TileDesc*>(tr.td),
                       static_cast<const unsigned*>(tr.words), static_cast<const T*>(tr.val),
                       static_cast<const unsigned short*>(tr.idx), static_cast<const T*>(xp), n, tr.W, tr.part,

@@ -968,7 +968,10 @@ struct EngineBase {
     virtual bo_status pass_info(int pass, bo_pdlp_pass_info* out) = 0;
 };
 
-template <class T>
+// T: the iterate vectors the passes gather and write (FP32 or FP64); TM:
+// the scaled matrix values the passes read (T, or FP32 under FP64 iterates:
+// 6 bytes per stored entry instead of 10, with the iterates' precision).
+template <class T, class TM = T>
 struct Engine final : EngineBase {
     int m = 0, n = 0;
     long long nnz = 0;
@@ -981,7 +984,8 @@ struct Engine final : EngineBase {
     // FP64 original (permuted) matrix in CSR and CSC; scaled working values
     int *rp = nullptr, *ci = nullptr, *cp = nullptr, *ri = nullptr;
     double *rv64 = nullptr, *cv64 = nullptr;
-    T *rv = nullptr, *cv = nullptr;
+    TM *rv = nullptr, *cv = nullptr;
+    static constexpr bool kSameTypes = std::is_same<T, TM>::value;  // the one-workgroup solve needs it
     // FP64 original vectors (permuted) and scaling
     double *b64 = nullptr, *c64 = nullptr, *lb64 = nullptr, *ub64 = nullptr;
     unsigned char* eq = nullptr;
@@ -1020,7 +1024,7 @@ struct Engine final : EngineBase {
         unsigned* words = nullptr;  // slice words
         unsigned short* idx = nullptr;
         int* src = nullptr;
-        T* val = nullptr;
+        TM* val = nullptr;
         int* seg_ptr = nullptr;     // segment -> partial slots
         unsigned* slot = nullptr;
         double* part = nullptr;     // one row of 64 partials per slice
@@ -1331,7 +1335,9 @@ struct Engine final : EngineBase {
         prm.lanes_rows = plan_code(plan_rows);
         prm.lanes_cols = plan_code(plan_cols);
         prm.hot_columns = plan_rows.hot;
-        if (prm.single_workgroup > 0 && prm.time_limit_s <= 0.0) {
+        if (!kSameTypes) {
+            // FP32 values under FP64 iterates: the multi-kernel loop only
+        } else if (prm.single_workgroup > 0 && prm.time_limit_s <= 0.0) {
             // (with a time limit the multi-kernel loop runs: it reads the
             // clock between check intervals, a one-workgroup solve cannot)
             one_wg = true;
@@ -1440,22 +1446,22 @@ struct Engine final : EngineBase {
     const int* done_flag() const { return &st->done; }
 
     template <class Epi>
-    void launch_stream(int pass, int cap, const int* ptr, const int* idx, const T* vals, const Epi& e,
+    void launch_stream(int pass, int cap, const int* ptr, const int* idx, const TM* vals, const Epi& e,
                        const int* done, const hipEvent_t* ev = nullptr) {
         const int c = cap_index(cap);
         const int nb = nsblk[pass][c];
         const int4* blk = sblk[pass][c];
         switch (c) {
             case 0:
-                launch_ev(k_stream<1024, T, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
+                launch_ev(k_stream<1024, TM, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
                           done);
                 break;
             case 1:
-                launch_ev(k_stream<2048, T, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
+                launch_ev(k_stream<2048, TM, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
                           done);
                 break;
             default:
-                launch_ev(k_stream<4096, T, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
+                launch_ev(k_stream<4096, TM, T, Epi>, dim3(nb), dim3(kBlock), 0, stream, ev, blk, ptr, idx, vals, e,
                           done);
                 break;
         }
@@ -1463,7 +1469,7 @@ struct Engine final : EngineBase {
 
     template <int PASS>
     static hipError_t set_tile_lds(int bytes) {
-        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<T, kTileRows, PASS>),
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<TM, T, kTileRows, PASS>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     }
 
@@ -1476,9 +1482,9 @@ struct Engine final : EngineBase {
         const TilePass& t = tp[PASS];
         const int nseg = PASS == 0 ? m : n, ng = PASS == 0 ? n : m;
         if (t.ntiles > 0)
-            launch_ev(k_tile<T, kTileRows, PASS>, dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream, ev4,
+            launch_ev(k_tile<TM, T, kTileRows, PASS>, dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream, ev4,
                       static_cast<const TileDesc*>(t.td), static_cast<const unsigned*>(t.words),
-                      static_cast<const T*>(t.val), static_cast<const unsigned short*>(t.idx), gvec, ng, t.W, t.part,
+                      static_cast<const TM*>(t.val), static_cast<const unsigned short*>(t.idx), gvec, ng, t.W, t.part,
                       static_cast<const unsigned char*>(nullptr));
         if (mid) hipEventRecord(mid, stream);
         launch_tile_reduce(stream, t.max_pairs, nseg, t.seg_ptr, t.slot, static_cast<int>(t.pairs), t.part,
@@ -1508,9 +1514,9 @@ struct Engine final : EngineBase {
         if (p.cap > 0) {
             launch_stream(0, p.cap, rp, ci, rv, e, done, ev4);
         } else if (p.hot > 0) {
-            launch_ev(k_segments_hot<kSumProd, T, T, EpiDual<T>>, dim3(std::min(p.hot_grid, p.bt.total_blocks)),
+            launch_ev(k_segments_hot<kSumProd, TM, T, EpiDual<T>>, dim3(std::min(p.hot_grid, p.bt.total_blocks)),
                       dim3(kBlock), static_cast<unsigned>(p.hot * sizeof(T)), stream, ev4, p.bt,
-                      static_cast<const int*>(rp), static_cast<const int*>(ci), static_cast<const T*>(rv), e, p.hot,
+                      static_cast<const int*>(rp), static_cast<const int*>(ci), static_cast<const TM*>(rv), e, p.hot,
                       done);
         } else {
             launch_rows<kSumProd>(p.bt, rv, e, nullptr, done, ev4);
@@ -1522,7 +1528,7 @@ struct Engine final : EngineBase {
     int grid_for_hot(int hot) {
         int per_cu = 0;
         hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_segments_hot<kSumProd, T, T, EpiDual<T>>), kBlock,
+            &per_cu, reinterpret_cast<const void*>(&k_segments_hot<kSumProd, TM, T, EpiDual<T>>), kBlock,
             hot * sizeof(T));
         if (err != hipSuccess || per_cu < 1) return 0;
         return per_cu * num_cus;
@@ -1729,51 +1735,53 @@ struct Engine final : EngineBase {
     }
 
     void launch_one_wg() {
-        if (const int per = small_resident_per_thread()) {
-            SmallArgs<T> a = small_args();
-            a.power = power_pending ? 1 : 0;
-            a.scal = scal;
-            const size_t bytes = small_resident_bytes<T>(m, n, nnz);
-            // lanes per row: as many as the workgroup has threads for
-            const int lr = per > 1 ? 1 : (4LL * m <= kSmallThreads ? 4 : (2LL * m <= kSmallThreads ? 2 : 1));
-            const void* fn = per > 1     ? reinterpret_cast<const void*>(&k_small_lds<T, 2, 1>)
-                             : lr == 4 ? reinterpret_cast<const void*>(&k_small_lds<T, 1, 4>)
-                             : lr == 2 ? reinterpret_cast<const void*>(&k_small_lds<T, 1, 2>)
-                                       : reinterpret_cast<const void*>(&k_small_lds<T, 1, 1>);
-            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) ==
-                hipSuccess) {
-                if (per > 1)
-                    hipLaunchKernelGGL((k_small_lds<T, 2, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
-                else if (lr == 4)
-                    hipLaunchKernelGGL((k_small_lds<T, 1, 4>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
-                else if (lr == 2)
-                    hipLaunchKernelGGL((k_small_lds<T, 1, 2>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
-                else
-                    hipLaunchKernelGGL((k_small_lds<T, 1, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
-                power_pending = false;
-                return;
+        if constexpr (kSameTypes) {  // (never chosen under FP32 values with FP64 iterates)
+            if (const int per = small_resident_per_thread()) {
+                SmallArgs<T> a = small_args();
+                a.power = power_pending ? 1 : 0;
+                a.scal = scal;
+                const size_t bytes = small_resident_bytes<T>(m, n, nnz);
+                // lanes per row: as many as the workgroup has threads for
+                const int lr = per > 1 ? 1 : (4LL * m <= kSmallThreads ? 4 : (2LL * m <= kSmallThreads ? 2 : 1));
+                const void* fn = per > 1     ? reinterpret_cast<const void*>(&k_small_lds<T, 2, 1>)
+                                 : lr == 4 ? reinterpret_cast<const void*>(&k_small_lds<T, 1, 4>)
+                                 : lr == 2 ? reinterpret_cast<const void*>(&k_small_lds<T, 1, 2>)
+                                           : reinterpret_cast<const void*>(&k_small_lds<T, 1, 1>);
+                if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) ==
+                    hipSuccess) {
+                    if (per > 1)
+                        hipLaunchKernelGGL((k_small_lds<T, 2, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                    else if (lr == 4)
+                        hipLaunchKernelGGL((k_small_lds<T, 1, 4>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                    else if (lr == 2)
+                        hipLaunchKernelGGL((k_small_lds<T, 1, 2>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                    else
+                        hipLaunchKernelGGL((k_small_lds<T, 1, 1>), dim3(1), dim3(kSmallThreads), bytes, stream, a);
+                    power_pending = false;
+                    return;
+                }
             }
-        }
-        if (power_pending) {  // the LDS-resident kernel could not be set up: the launches do it
-            const BinTable br = make_bins(row_len_sorted, 4);
-            launch_power(br);
-            launch_start(br);
-            power_pending = false;
-        }
-        SmallArgs<T> a = small_args();
-        const unsigned lds = small_lds_bytes();
-        a.lds = lds > 0 ? 1 : 0;
-        if (lds > 0 && !small_lds_set) {
-            if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kSmallMaxLds)) ==
-                hipSuccess)
-                small_lds_set = true;
-        }
-        if (!small_lds_set) {
-            a.lds = 0;
-            hipLaunchKernelGGL(k_small<T>, dim3(1), dim3(kSmallThreads), 0, stream, a);
-        } else {
-            hipLaunchKernelGGL(k_small<T>, dim3(1), dim3(kSmallThreads), a.lds ? lds : 0u, stream, a);
+            if (power_pending) {  // the LDS-resident kernel could not be set up: the launches do it
+                const BinTable br = make_bins(row_len_sorted, 4);
+                launch_power(br);
+                launch_start(br);
+                power_pending = false;
+            }
+            SmallArgs<T> a = small_args();
+            const unsigned lds = small_lds_bytes();
+            a.lds = lds > 0 ? 1 : 0;
+            if (lds > 0 && !small_lds_set) {
+                if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kSmallMaxLds)) ==
+                    hipSuccess)
+                    small_lds_set = true;
+            }
+            if (!small_lds_set) {
+                a.lds = 0;
+                hipLaunchKernelGGL(k_small<T>, dim3(1), dim3(kSmallThreads), 0, stream, a);
+            } else {
+                hipLaunchKernelGGL(k_small<T>, dim3(1), dim3(kSmallThreads), a.lds ? lds : 0u, stream, a);
+            }
         }
     }
 
@@ -1853,14 +1861,14 @@ struct Engine final : EngineBase {
             hipLaunchKernelGGL(k_apply_norm, dim3(gm), dim3(kBlock), 0, stream, rs, rnorm, static_cast<size_t>(m));
             hipLaunchKernelGGL(k_apply_norm, dim3(gn), dim3(kBlock), 0, stream, cs, cnorm, static_cast<size_t>(n));
         }
-        EpiWrite<T> wr{rs, cs, rv};
+        EpiWrite<TM> wr{rs, cs, rv};
         launch_rows<kWrite>(br, rv64, wr, nullptr, nullptr);
-        EpiWrite<T> wc{cs, rs, cv};
+        EpiWrite<TM> wc{cs, rs, cv};
         launch_cols<kWrite>(bc, cv64, wc, nullptr, nullptr);
         for (int pass = 0; pass < 2; ++pass) {
             const TilePass& t = tp[pass];
             if (t.k > 0 && t.padded > 0)
-                hipLaunchKernelGGL(k_tile_values<T>, dim3(grid_for(t.padded)), dim3(kBlock), 0, stream, t.src, t.padded,
+                hipLaunchKernelGGL(k_tile_values<TM>, dim3(grid_for(t.padded)), dim3(kBlock), 0, stream, t.src, t.padded,
                                    pass == 0 ? rv : cv, t.val);
         }
         hipLaunchKernelGGL(k_scale_cols<T>, dim3(gn), dim3(kBlock), 0, stream, n, cs, c64, lb64, ub64, wx,
@@ -2059,7 +2067,7 @@ struct Engine final : EngineBase {
     bo_status pass_info(int pass, bo_pdlp_pass_info* out) override {
         const PassPlan& p = pass == 0 ? plan_rows : plan_cols;
         const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
-        const double tb = sizeof(T);
+        const double tb = sizeof(T), tm = sizeof(TM);  // iterate and matrix value bytes
         const double epi = pass == 0 ? 49.0 + 3.0 * tb : 24.0 + 4.0 * tb;
         std::memset(out, 0, sizeof(*out));
         out->schedule = plan_code(p);
@@ -2072,7 +2080,7 @@ struct Engine final : EngineBase {
             out->slices = t.nslices;
             out->pairs = t.pairs;
             out->padded = static_cast<long long>(t.padded);
-            out->gather_bytes = (tb + 2.0) * nnz + tb * ng + 4.0 * t.nslices +
+            out->gather_bytes = (tm + 2.0) * nnz + tb * ng + 4.0 * t.nslices +
                                 static_cast<double>(sizeof(TileDesc)) * t.ntiles + 8.0 * t.pairs;
             out->reduce_bytes = 4.0 * (nseg + 1) + 12.0 * t.pairs + epi * nseg;
             int cover = 1;
@@ -2080,7 +2088,7 @@ struct Engine final : EngineBase {
             out->reduce_loads = t.reduce_u > 0 ? t.reduce_u : cover;
         } else {
             out->kernels = 1;
-            out->gather_bytes = (tb + 4.0) * nnz + 4.0 * (nseg + 1) + tb * ng + epi * nseg;
+            out->gather_bytes = (tm + 4.0) * nnz + 4.0 * (nseg + 1) + tb * ng + epi * nseg;
         }
         return BO_SUCCESS;
     }
@@ -2132,7 +2140,11 @@ bo_status bo_pdlp_create(const bo_lp* lp, const bo_pdlp_params* params, bo_pdlp*
         else bo_pdlp_default_params(&p);
         auto h = std::make_unique<bo_pdlp>();
         bo_status st;
-        if (p.fp64_iterates) {
+        if (p.fp64_iterates == 2) {  // FP64 iterates over FP32 matrix values
+            auto e = std::make_unique<Engine<double, float>>();
+            st = e->create(lp, &p);
+            h->eng = std::move(e);
+        } else if (p.fp64_iterates) {
             auto e = std::make_unique<Engine<double>>();
             st = e->create(lp, &p);
             h->eng = std::move(e);

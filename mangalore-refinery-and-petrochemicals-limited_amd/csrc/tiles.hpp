@@ -462,8 +462,8 @@ __device__ inline unsigned run_word(const unsigned* __restrict__ slice_word, int
     return q < nslice ? slice_word[first + q] : 0u;
 }
 
-template <class T, int CH, bool EARLY>
-__device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* lds, RunState& st,
+template <class T, class TX, int CH, bool EARLY>
+__device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const TX* lds, RunState& st,
                                     const WaveDesc& wd, const unsigned* __restrict__ slice_word,
                                     double* __restrict__ part, int lane) {
     // EARLY: the chunk's LDS gathers are all issued before the first is
@@ -473,14 +473,14 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
     // lanes (slices of 11 rows on average instead of 5.6), 1.6 us (27.9
     // against 29.5 us); with the short slices it had cost the column pass
     // 1.3 us.
-    T g[CH];
+    TX g[CH];
     if constexpr (EARLY) {
 #pragma unroll
         for (int u = 0; u < CH; ++u) g[u] = lds[c.g[u]];
     }
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-        const T gv = EARLY ? g[u] : lds[c.g[u]];
+        const TX gv = EARLY ? g[u] : lds[c.g[u]];
         st.acc += static_cast<double>(c.a[u]) * static_cast<double>(gv);
         if (r + u + 1 == st.bound) {  // last row of the current slice (uniform)
             double* row = part + static_cast<size_t>(wd.slice0 + st.sl) * kWave;
@@ -507,20 +507,21 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const T* ld
     }
 }
 
-// One tile per workgroup.  Dynamic LDS: W entries of T.  CH: rows of the
-// wavefront's run kept in flight per buffer.  PASS names the instantiation
-// (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of the batched
-// engine), so that profiles tell the passes apart; every pass issues a
-// chunk's LDS gathers early (chunk_finish).  active
-// (optional): per-tile flag; a tile whose flag is 0 does nothing.
-template <class T, int CH, int PASS>
+// One tile per workgroup.  T: matrix values; TX: the gathered vector, W
+// entries of it in dynamic LDS (FP32 values may meet an FP64 vector).  CH:
+// rows of the wavefront's run kept in flight per buffer.  PASS names the
+// instantiation (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of
+// the batched engine), so that profiles tell the passes apart; every pass
+// issues a chunk's LDS gathers early (chunk_finish).  active (optional):
+// per-tile flag; a tile whose flag is 0 does nothing.
+template <class T, class TX, int CH, int PASS>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_word, const T* __restrict__ val,
-       const unsigned short* __restrict__ idx, const T* __restrict__ gvec, int ng, int W,
+       const unsigned short* __restrict__ idx, const TX* __restrict__ gvec, int ng, int W,
        double* __restrict__ part, const unsigned char* __restrict__ active) {
     if (active && !active[blockIdx.x]) return;  // every LP of the tile has finished (batched engine)
     extern __shared__ __align__(16) unsigned char tile_lds[];
-    T* lds = reinterpret_cast<T*>(tile_lds);
+    TX* lds = reinterpret_cast<TX*>(tile_lds);
     const int tid = threadIdx.x;
     // wave-uniform values live in scalar registers, so nothing below
     // branches per lane
@@ -532,7 +533,7 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     // so the LDS fill overlaps the first matrix fetch.
     const int g0 = block * W;
     const int gn = min(W, ng - g0);
-    constexpr int V = 16 / sizeof(T);
+    constexpr int V = 16 / sizeof(TX);
     const int nv = gn / V;  // whole 16-byte groups of the block (g0 is a multiple of 64)
     const __amdgpu_buffer_rsrc_t rx = slice_rsrc(gvec + g0, nv * 16);
     uint4 f[kFillTrips];
@@ -571,9 +572,9 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
     int r = 0;
     for (int pairs = (nrows + 2 * CH - 1) / (2 * CH); pairs > 0; --pairs) {
         chunk_load(Bc, rv, ri, r + CH, lane);
-        chunk_finish<T, CH, true>(A, r, lds, st, wd, slice_word, part, lane);
+        chunk_finish<T, TX, CH, true>(A, r, lds, st, wd, slice_word, part, lane);
         chunk_load(A, rv, ri, r + 2 * CH, lane);
-        chunk_finish<T, CH, true>(Bc, r + CH, lds, st, wd, slice_word, part, lane);
+        chunk_finish<T, TX, CH, true>(Bc, r + CH, lds, st, wd, slice_word, part, lane);
         r += 2 * CH;
     }
 }

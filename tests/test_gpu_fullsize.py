@@ -101,3 +101,19 @@ def test_family_workload_full_size_draws(pkg, po, lpdata):
     for i, k in enumerate(ids):
         check(po, draws[i], X[i], Y[i], fx[str(k)], f"c4 draw {k}")
     f.close()
+
+
+def test_zipf_lp_fp64_iterates_over_fp32_values_full_size(pkg, po, lpdata):
+    # the 200,000 x 500,000 Zipf LP with FP64 iterates over FP32 matrix
+    # values: the FP64 engine's iteration count at 6 bytes per stored entry
+    if "c1" not in FX:
+        pytest.skip("no objective fixture for c1")
+    lp = SINGLE["c1"](lpdata)
+    print("case: c1, fp64_iterates = 2", flush=True)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, max_iterations=200000, fp64_iterates=2)
+    s = h.solve()
+    x, y = h.solution()
+    h.close()
+    print("status", s["status"], "iterations", s["iterations"], "engine rel-KKT %.3e" % s["rel_kkt"], flush=True)
+    assert s["status"] == 0
+    check(po, lp, x, y, FX["c1"], "c1 fp64 iterates over fp32 values")

@@ -59,10 +59,13 @@ def test_pdlp_reaches_tolerance_and_optimum(pkg, po, instances, name):
     h.close()
 
 
-def test_pdlp_fp64_iterates(pkg, po, instances):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_pdlp_fp64_iterates(pkg, po, instances, mode):
+    # mode 1: FP64 iterates and matrix values; mode 2: FP64 iterates over
+    # FP32 matrix values (the multi-kernel loop only)
     name = "random_300x1000_s42"
     lp = instances[name]
-    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, fp64_iterates=1)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, fp64_iterates=mode)
     s = h.solve()
     x, y = h.solution()
     check_solution(po, lp, x, y, s, name)
@@ -253,7 +256,7 @@ def test_pdlp_tiled_schedule(pkg, po, instances, k):
     assert s2["iterations"] == s["iterations"] and np.array_equal(x, x2) and np.array_equal(y, y2)
 
 
-@pytest.mark.parametrize("fp64", [0, 1])
+@pytest.mark.parametrize("fp64", [0, 1, 2])
 def test_pdlp_tiled_many_blocks(pkg, po, lpdata, fp64):
     # n = 90000 gathered entries do not fit one LDS block: several blocks,
     # partials added by the reduction kernel; compared with the segment schedule
@@ -368,3 +371,30 @@ def test_pdlp_rejects_bad_csr(pkg, lpdata):
     with pytest.raises(pkg.BoError) as e:
         pkg.Pdlp(bad)
     assert e.value.status == 4
+
+
+@pytest.mark.parametrize("fname", ["adlittle.free.mps", "blend.free.mps", "p0033.free.mps"])
+def test_pdlp_fp64_iterates_over_fp32_values_on_badly_scaled_models(pkg, po, fname):
+    # Netlib models on which FP32 iterates do not converge (100,000
+    # iterations: rel-KKT 3.5e-2, 2.7e-3 and 1.7e-2): FP64 iterates over the
+    # FP32-rounded scaled matrix solve them in the FP64 engine's iteration
+    # counts.  (kb2 and share2b need FP64 matrix values as well: with FP32
+    # values they end at rel-KKT 2.7e-4 and 1.0e-6 after 100,000 iterations.)
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    gold = json.load(open(os.path.join(here, "golden", "mps_golden.json")))[fname]
+    info, fm = pkg.read_mps(text=open(os.path.join(here, "golden", "mps", fname)).read())
+    lp = pkg.lpdata.flat_to_std(fm)
+    print("case:", fname, lp.m, lp.n, lp.nnz, flush=True)
+    h = pkg.Pdlp(lp, eps_rel_kkt=TOL, eps_objective=5e-7, max_iterations=100000, fp64_iterates=2)
+    s = h.solve()
+    x, y = h.solution()
+    h.close()
+    k = po.kkt(lp, x, y)
+    obj = (-1.0 if fm.sense == 1 else 1.0) * k["primal_obj"] + fm.obj_offset
+    f = gold["highs_objective"]
+    print(fname, "status", s["status"], "iterations", s["iterations"], "rel-KKT %.2e" % k["rel_kkt"],
+          "objective error %.2e" % (abs(obj - f) / (1 + abs(f))), flush=True)
+    assert s["status"] == 0 and k["rel_kkt"] <= TOL
+    assert abs(obj - f) <= 1e-6 * (1 + abs(f))
