@@ -739,6 +739,9 @@ struct Batch final : BatchBase {
         for (T** v : {&c_hat, &l_hat, &u_hat, &xp, &pw_v}) BO_TRY(mem.alloc(v, n));
         for (double** v : {&xk, &x0, &xo_cur, &xo_cand, &wx, &ac0, &ac1}) BO_TRY(mem.alloc(v, n));
         BO_TRY(mem.alloc(&st, std::max(K, 1)));
+        // zero states (done = 0) until prepare() uploads the solve's: the
+        // reduction tuner's passes read them
+        BO_TRY(hipMemsetAsync(st, 0, sizeof(PdlpState) * std::max(K, 1), stream));
         BO_TRY(mem.alloc(&running, 1));
         BO_TRY(up(rp, h_rp.data(), sizeof(int) * (m + 1)));
         BO_TRY(up(ci, h_ci.data(), sizeof(int) * nnz));
@@ -907,7 +910,7 @@ struct Batch final : BatchBase {
     template <class Epi>
     void launch_reduce(const TilePass& tl, int nseg, const Epi& e, const hipEvent_t* ev = nullptr) {
         launch_tile_reduce(stream, tl.max_pairs, nseg, tl.seg_ptr, tl.slot, static_cast<int>(tl.pairs), tl.part,
-                           tl.nslices * kWave, e, nullptr, 0, ev);
+                           tl.nslices * kWave, e, 0, ev);
     }
 
     void launch_check() {

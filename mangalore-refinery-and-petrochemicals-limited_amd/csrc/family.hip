@@ -146,10 +146,9 @@ struct EpiFPrimal {
     }
     __device__ void operator()(int j, int lane, const double* acc, const Pre& p) const {
         const size_t e = static_cast<size_t>(j) * kL + lane;
-        const T xt = clampT(static_cast<T>(p.xo - cf->tau[lane] * (p.c + acc[0])), p.lo, p.hi);
+        const T xt = clampT(static_cast<T>(primal_move(p.xo, cf->tau[lane], p.c, acc[0])), p.lo, p.hi);
         xp[e] = xt;
-        const double w = halpern_at(cf, lane, t);
-        xk[e] = w * (2.0 * static_cast<double>(xt) - p.xo) + (1.0 - w) * p.x0;
+        xk[e] = halpern_x(halpern_at(cf, lane, t), static_cast<double>(xt), p.xo, p.x0);
     }
 };
 
@@ -182,15 +181,15 @@ struct EpiFDual {
     }
     __device__ void operator()(int i, int lane, const double* acc, const Pre& p) const {
         const size_t e = static_cast<size_t>(i) * kL + lane;
-        double yn = p.yo + cf->sigma[lane] * (2.0 * acc[0] - p.axo - p.b);
+        double yn = dual_move(p.yo, cf->sigma[lane], acc[0], p.axo, p.b);
         if (!p.eq && yn < 0.0) yn = 0.0;
         const T yt = static_cast<T>(yn);
         yp[e] = yt;
         const double w = halpern_at(cf, lane, t);
-        const double ynew = w * (2.0 * static_cast<double>(yt) - p.yo) + (1.0 - w) * p.y0;
+        const double ynew = halpern_y(w, static_cast<double>(yt), p.yo, p.y0);
         yk[e] = ynew;
         yg[e] = static_cast<T>(ynew);
-        ax[e] = w * (2.0 * acc[0] - p.axo) + (1.0 - w) * p.ax0;
+        ax[e] = halpern_ax(w, acc[0], p.axo, p.ax0);
     }
 };
 

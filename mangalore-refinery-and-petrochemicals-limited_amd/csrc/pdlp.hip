@@ -805,10 +805,10 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
                 if (j < n) {
                     const double acc = seg_dot(lcp[j], lcp[j + 1], lri, lcv, ly);
                     const double xo = xk[r];
-                    const T xt = clampT(static_cast<T>(xo - tau * (static_cast<double>(cc[r]) + acc)), lo[r], hi[r]);
+                    const T xt = clampT(static_cast<T>(primal_move(xo, tau, static_cast<double>(cc[r]), acc)), lo[r], hi[r]);
                     xpr[r] = xt;
                     lx[j] = xt;
-                    xk[r] = w * (2.0 * static_cast<double>(xt) - xo) + (1.0 - w) * x0[r];
+                    xk[r] = halpern_x(w, static_cast<double>(xt), xo, x0[r]);
                 }
             }
             __syncthreads();
@@ -825,14 +825,14 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_lds(SmallArgs<T> a) {
                         if constexpr (LR == 4) acc += __shfl_xor(acc, 2, kWave);
                     }
                     const double axo = ax[r], yo = yk[r];
-                    double yn = yo + sigma * (2.0 * acc - axo - static_cast<double>(bb[r]));
+                    double yn = dual_move(yo, sigma, acc, axo, static_cast<double>(bb[r]));
                     if (!eq[r] && yn < 0.0) yn = 0.0;
                     const T yt = static_cast<T>(yn);
                     ypr[r] = yt;
-                    const double ynew = w * (2.0 * static_cast<double>(yt) - yo) + (1.0 - w) * y0[r];
+                    const double ynew = halpern_y(w, static_cast<double>(yt), yo, y0[r]);
                     yk[r] = ynew;
                     if (LR == 1 || tid % LR == 0) ly[i] = static_cast<T>(ynew);
-                    ax[r] = w * (2.0 * acc - axo) + (1.0 - w) * ax0[r];
+                    ax[r] = halpern_ax(w, acc, axo, ax0[r]);
                 }
             }
             __syncthreads();
@@ -1259,6 +1259,9 @@ struct Engine final : EngineBase {
         BO_TRY(mem.alloc(&wx, n));
         BO_TRY(mem.alloc(&scal, 8));
         BO_TRY(mem.alloc(&st, 1));
+        // a zero state (done = 0) until prepare() uploads the solve's: the
+        // tuner's passes read it
+        BO_TRY(hipMemsetAsync(st, 0, sizeof(PdlpState), stream));
         auto up = [&](void* d, const void* h, size_t bytes) -> hipError_t {
             return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
         };
@@ -1488,7 +1491,7 @@ struct Engine final : EngineBase {
                       static_cast<const unsigned char*>(nullptr));
         if (mid) hipEventRecord(mid, stream);
         launch_tile_reduce(stream, t.max_pairs, nseg, t.seg_ptr, t.slot, static_cast<int>(t.pairs), t.part,
-                           t.nslices * kWave, e, done, t.reduce_u, ev4 ? ev4 + 2 : nullptr);
+                           t.nslices * kWave, e, t.reduce_u, ev4 ? ev4 + 2 : nullptr);
     }
 
     // ev4 (optional): start / end events of the pass's kernels (a fused pass
@@ -2023,11 +2026,28 @@ struct Engine final : EngineBase {
         return BO_SUCCESS;
     }
 
+    // Runs f with the device state's done flag cleared, so that the pass
+    // epilogues (which store nothing for a finished LP) do a solve's work
+    // after a solve, then puts the state back.
+    template <class F>
+    bo_status with_live_state(F&& f) {
+        PdlpState keep, live;
+        BO_TRY(hipMemcpyAsync(&keep, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        live = keep;
+        live.done = 0;
+        BO_TRY(hipMemcpyAsync(st, &live, sizeof(PdlpState), hipMemcpyHostToDevice, stream));
+        const bo_status r = f();
+        BO_TRY(hipMemcpyAsync(st, &keep, sizeof(PdlpState), hipMemcpyHostToDevice, stream));
+        BO_TRY(hipStreamSynchronize(stream));
+        return r;
+    }
+
     // Times the two fused iteration kernels as they alternate in a solve.
     // Leaves the iterate buffers modified: call after get_solution (solve()
     // re-initialises everything).
     bo_status time_spmv(int reps, double* csr_ms, double* csc_ms) override {
-        return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms);
+        return with_live_state([&] { return time_pair(plan_rows, plan_cols, reps < 1 ? 1 : reps, csr_ms, csc_ms); });
     }
 
     // Each iteration kernel's own duration (events bound to its dispatch,
@@ -2035,6 +2055,9 @@ struct Engine final : EngineBase {
     // reduction, CSR gather, CSR reduction (a fused pass: its kernel in the
     // gather entry, 0 for the reduction).
     bo_status time_kernels(int reps, double* ms4) override {
+        return with_live_state([&] { return time_kernels_live(reps, ms4); });
+    }
+    bo_status time_kernels_live(int reps, double* ms4) {
         reps = reps < 1 ? 1 : reps;
         std::vector<hipEvent_t> ev(8 * static_cast<size_t>(reps));
         for (auto& e : ev) BO_TRY(hipEventCreate(&e));

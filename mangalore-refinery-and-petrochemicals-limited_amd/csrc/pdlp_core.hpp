@@ -25,10 +25,39 @@ struct PdlpState {
     int done, status, restart, restart_to_cand, best, restarts, checks, pad;
 };
 
-// Halpern weight of step t of the current interval.
-__device__ inline double halpern_w(const PdlpState* st, int t) {
-    const double k = static_cast<double>(st->iter - st->iter_last_restart + t);
+// Halpern weight of step t of the current interval, from the state's
+// iteration counters.
+__device__ inline double halpern_w(long long iter, long long iter_last_restart, int t) {
+    const double k = static_cast<double>(iter - iter_last_restart + t);
     return (k + 1.0) / (k + 2.0);
+}
+__device__ inline double halpern_w(const PdlpState* st, int t) {
+    return halpern_w(st->iter, st->iter_last_restart, t);
+}
+
+// The step's updates with every fused multiply-add written out.  Left to the
+// compiler, contraction follows instruction scheduling, so a change elsewhere
+// in a kernel (where an operand is loaded, say) can change which product is
+// rounded; a solve with FP32 iterates is sensitive to one such rounding (it
+// once moved the 200,000 x 500,000 Zipf LP from 11,008 to 12,288 iterations).
+// Every kernel that takes a step uses these.
+// x - tau (c + A'y), before the projection
+__device__ inline double primal_move(double xo, double tau, double c, double aty) {
+    return fma(-tau, c + aty, xo);
+}
+// y + sigma (2 A x+ - A x - b), before the projection
+__device__ inline double dual_move(double yo, double sigma, double ax_plus, double axo, double b) {
+    return fma(sigma, (2.0 * ax_plus - axo) - b, yo);
+}
+// Halpern updates w (2 t - z) + (1 - w) z0 of x, of y and of A x
+__device__ inline double halpern_x(double w, double xt, double xo, double x0) {
+    return fma(x0, 1.0 - w, w * fma(xt, 2.0, -xo));
+}
+__device__ inline double halpern_y(double w, double yt, double yo, double y0) {
+    return fma(w, fma(yt, 2.0, -yo), (1.0 - w) * y0);
+}
+__device__ inline double halpern_ax(double w, double ax_plus, double axo, double ax0) {
+    return fma(w, 2.0 * ax_plus - axo, (1.0 - w) * ax0);
 }
 
 template <class T>
@@ -59,19 +88,28 @@ struct EpiPrimal {
     __device__ bool skip(int j) const { return inst && st[inst[j]].done; }
     __device__ bool wg_dead(int b) const { return wg_live && !wg_live[b]; }
     // Column data the step reads besides A'y, loaded by a reduction kernel
-    // before the partials so the two sets of loads overlap.
+    // before the partials so the two sets of loads overlap.  The state's
+    // step size, iteration counters and done flag are part of it, loaded raw
+    // and used only in finish: read there, or turned into the Halpern weight
+    // up front, they would put a round trip before the stores or before the
+    // partial loads.  The per-lane state pointer keeps them vector loads,
+    // whose waits are in order (a scalar-load wait waits for every scalar
+    // load in flight).  A step of a finished LP stores nothing.
     struct Pre {
-        double xo, x0;
+        double xo, x0, tau;
+        long long it, it_restart;
         T c, lb, ub;
+        int stop;  // the column's LP has finished
     };
-    __device__ Pre load(int j) const { return Pre{xk[j], x0[j], c[j], lb[j], ub[j]}; }
-    __device__ void finish(int j, double acc, const Pre& p) const {
+    __device__ Pre load(int j) const {
         const PdlpState* s = inst ? st + inst[j] : st;
-        if (inst && s->done) return;
-        const T xt = clampT(static_cast<T>(p.xo - s->tau * (static_cast<double>(p.c) + acc)), p.lb, p.ub);
+        return Pre{xk[j], x0[j], s->tau, s->iter, s->iter_last_restart, c[j], lb[j], ub[j], s->done};
+    }
+    __device__ void finish(int j, double acc, const Pre& p) const {
+        if (p.stop) return;
+        const T xt = clampT(static_cast<T>(primal_move(p.xo, p.tau, static_cast<double>(p.c), acc)), p.lb, p.ub);
         xp[j] = xt;
-        const double w = halpern_w(s, t);
-        xk[j] = w * (2.0 * static_cast<double>(xt) - p.xo) + (1.0 - w) * p.x0;
+        xk[j] = halpern_x(halpern_w(p.it, p.it_restart, t), static_cast<double>(xt), p.xo, p.x0);
     }
     __device__ void operator()(int j, const double* acc, double*) const {
         if (inst && st[inst[j]].done) return;
@@ -101,25 +139,30 @@ struct EpiDual {
     __device__ const T* vec(int) const { return xp; }
     __device__ bool skip(int i) const { return inst && st[inst[i]].done; }
     __device__ bool wg_dead(int b) const { return wg_live && !wg_live[b]; }
-    // Row data the step reads besides A x+ (see EpiPrimal::Pre).
+    // Row data the step reads besides A x+, with the state's step size and
+    // iteration counters (see EpiPrimal::Pre).
     struct Pre {
-        double axo, yo, y0, ax0;
+        double axo, yo, y0, ax0, sigma;
+        long long it, it_restart;
         T b;
         unsigned char eq;
+        int stop;  // the row's LP has finished
     };
-    __device__ Pre load(int i) const { return Pre{ax[i], yk[i], y0[i], ax0[i], b[i], eq[i]}; }
-    __device__ void finish(int i, double acc, const Pre& p) const {
+    __device__ Pre load(int i) const {
         const PdlpState* s = inst ? st + inst[i] : st;
-        if (inst && s->done) return;
-        double yn = p.yo + s->sigma * (2.0 * acc - p.axo - static_cast<double>(p.b));
+        return Pre{ax[i], yk[i], y0[i], ax0[i], s->sigma, s->iter, s->iter_last_restart, b[i], eq[i], s->done};
+    }
+    __device__ void finish(int i, double acc, const Pre& p) const {
+        if (p.stop) return;
+        double yn = dual_move(p.yo, p.sigma, acc, p.axo, static_cast<double>(p.b));
         if (!p.eq && yn < 0.0) yn = 0.0;
         const T yt = static_cast<T>(yn);
         yp[i] = yt;
-        const double w = halpern_w(s, t);
-        const double ynew = w * (2.0 * static_cast<double>(yt) - p.yo) + (1.0 - w) * p.y0;
+        const double w = halpern_w(p.it, p.it_restart, t);
+        const double ynew = halpern_y(w, static_cast<double>(yt), p.yo, p.y0);
         yk[i] = ynew;
         yg[i] = static_cast<T>(ynew);
-        ax[i] = w * (2.0 * acc - p.axo) + (1.0 - w) * p.ax0;
+        ax[i] = halpern_ax(w, acc, p.axo, p.ax0);
     }
     __device__ void operator()(int i, const double* acc, double*) const {
         if (inst && st[inst[i]].done) return;

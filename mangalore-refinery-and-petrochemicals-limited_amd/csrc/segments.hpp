@@ -60,9 +60,13 @@ constexpr int unroll() {
 
 // Optional LDS copy of the first `count` entries of the gathered vector (the
 // hottest ones: segments of the other dimension are stored longest first).
+// The pointer is typed in the LDS address space, so a gather reads either
+// LDS or memory with its own instruction (with a generic pointer the
+// compiler merged the two into one flat load through a pointer select).
 template <class TX>
 struct HotCache {
-    const TX* lds;
+    using LdsPtr = const __attribute__((address_space(3))) TX*;
+    LdsPtr lds;
     int count;
 };
 
@@ -288,7 +292,7 @@ k_segments_hot(BinTable bt, const int* __restrict__ ptr, const int* __restrict__
     const TX* src = epi.vec(0);
     for (int i = threadIdx.x; i < hot_count; i += kBlock) lds[i] = src[i];
     __syncthreads();
-    const HotCache<TX> hot{lds, hot_count};
+    const HotCache<TX> hot{(typename HotCache<TX>::LdsPtr)lds, hot_count};
     double part[1] = {0.0};
     int bin = 0;
     for (int g = blockIdx.x; g < bt.total_blocks; g += gridDim.x) {

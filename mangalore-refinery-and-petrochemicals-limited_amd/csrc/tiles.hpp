@@ -583,19 +583,23 @@ k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_wo
 // and partial loads of up to U pairs are issued at once: raw buffer loads at
 // an out-of-range offset read 0, so no load sits under a per-lane branch.
 // An epilogue may skip a segment before anything is loaded (Epi::skip).
+// The segment's bounds are loaded before its epilogue data (vector-load
+// waits are in order: the loop bounds do not wait for the epilogue data).
+// The kernel reads no done flag of its own: the pass epilogues load their
+// LP's with the epilogue data and store nothing once it is set, so a
+// reduction that runs after convergence changes nothing, at no round trip.
 template <int U, class Epi>
 __global__ void __launch_bounds__(kBlock)
 k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restrict__ slot, int nslot,
-              const double* __restrict__ part, int npart, Epi epi, const int* __restrict__ done) {
-    if (done && *done) return;
+              const double* __restrict__ part, int npart, Epi epi) {
     if (wg_dead(epi, static_cast<int>(blockIdx.x), 0)) return;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= nseg || epi.skip(s)) return;
     constexpr unsigned kOut = 0x7FFFFFF0u;  // beyond every range below (ranges are < 2^31 bytes)
     const __amdgpu_buffer_rsrc_t rs = slice_rsrc(slot, nslot * 4);
     const __amdgpu_buffer_rsrc_t rp = slice_rsrc(part, npart * 8);
-    const auto pre = epi_preload(epi, s, 0);
     const int q0 = seg_ptr[s], q1 = seg_ptr[s + 1];
+    const auto pre = epi_preload(epi, s, 0);
     double acc = 0.0;
     for (int q = q0; q < q1; q += U) {
         unsigned p[U];
@@ -619,21 +623,20 @@ k_tile_reduce(int nseg, const int* __restrict__ seg_ptr, const unsigned* __restr
 // segment has.  ev (optional): events timing the launch (launch_ev).
 template <class Epi>
 inline void launch_tile_reduce(hipStream_t stream, int max_pairs, int nseg, const int* seg_ptr, const unsigned* slot,
-                               int nslot, const double* part, int npart, const Epi& e, const int* done, int u = 0,
+                               int nslot, const double* part, int npart, const Epi& e, int u = 0,
                                const hipEvent_t* ev = nullptr) {
     const dim3 grid(grid_for(static_cast<size_t>(nseg))), block(kBlock);
     if (u > 0) max_pairs = u;
     if (max_pairs <= 1)
-        launch_ev(k_tile_reduce<1, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
+        launch_ev(k_tile_reduce<1, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e);
     else if (max_pairs <= 2)
-        launch_ev(k_tile_reduce<2, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
+        launch_ev(k_tile_reduce<2, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e);
     else if (max_pairs <= 4)
-        launch_ev(k_tile_reduce<4, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
+        launch_ev(k_tile_reduce<4, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e);
     else if (max_pairs <= 8)
-        launch_ev(k_tile_reduce<8, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e, done);
+        launch_ev(k_tile_reduce<8, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e);
     else
-        launch_ev(k_tile_reduce<16, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e,
-                  done);
+        launch_ev(k_tile_reduce<16, Epi>, grid, block, 0, stream, ev, nseg, seg_ptr, slot, nslot, part, npart, e);
 }
 
 }  // namespace bo
