@@ -876,8 +876,9 @@ struct Batch final : BatchBase {
         EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, inst_c, tl.built ? live_red_c : live_chk_c};
         if (tl.built) {
             hipLaunchKernelGGL((k_tile<T, T, kTileRows, 3>), dim3(tl.ntiles), dim3(kTileThreads),
-                               static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, yg, m,
-                               tl.W, tl.part, tl.active);
+                               static_cast<unsigned>(tl.W * sizeof(T)), stream,
+                               static_cast<const unsigned char*>(tl.active), tl.td, tl.words, tl.val, tl.idx, yg, m,
+                               tl.W, tl.part);
             launch_reduce(tl, n, e);
         } else {
             cols<kSumProd>(cv, e);
@@ -888,8 +889,9 @@ struct Batch final : BatchBase {
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, inst_r, tl.built ? live_red_r : live_chk_r};
         if (tl.built) {
             hipLaunchKernelGGL((k_tile<T, T, kTileRows, 2>), dim3(tl.ntiles), dim3(kTileThreads),
-                               static_cast<unsigned>(tl.W * sizeof(T)), stream, tl.td, tl.words, tl.val, tl.idx, xp, n,
-                               tl.W, tl.part, tl.active);
+                               static_cast<unsigned>(tl.W * sizeof(T)), stream,
+                               static_cast<const unsigned char*>(tl.active), tl.td, tl.words, tl.val, tl.idx, xp, n,
+                               tl.W, tl.part);
             launch_reduce(tl, m, e);
         } else {
             rows<kSumProd>(rv, e);
@@ -999,16 +1001,16 @@ struct Batch final : BatchBase {
             const TilePass& tc = tp[1];
             const TilePass& tr = tp[0];
             launch_ev(k_tile<T, T, kTileRows, 3>, dim3(tc.ntiles), dim3(kTileThreads),
-                      static_cast<unsigned>(tc.W * sizeof(T)), stream, &ev[0], static_cast<const TileDesc*>(tc.td),
+                      static_cast<unsigned>(tc.W * sizeof(T)), stream, &ev[0],
+                      static_cast<const unsigned char*>(tc.active), static_cast<const TileDesc*>(tc.td),
                       static_cast<const unsigned*>(tc.words), static_cast<const T*>(tc.val),
-                      static_cast<const unsigned short*>(tc.idx), static_cast<const T*>(yg), m, tc.W, tc.part,
-                      static_cast<const unsigned char*>(tc.active));
+                      static_cast<const unsigned short*>(tc.idx), static_cast<const T*>(yg), m, tc.W, tc.part);
             launch_reduce(tc, n, EpiPrimal<T>{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, 0, inst_c, live_red_c}, &ev[2]);
             launch_ev(k_tile<T, T, kTileRows, 2>, dim3(tr.ntiles), dim3(kTileThreads),
-                      static_cast<unsigned>(tr.W * sizeof(T)), stream, &ev[4], static_cast<const TileDesc*>(tr.td),
+                      static_cast<unsigned>(tr.W * sizeof(T)), stream, &ev[4],
+                      static_cast<const unsigned char*>(tr.active), static_cast<const TileDesc*>(tr.td),
                       static_cast<const unsigned*>(tr.words), static_cast<const T*>(tr.val),
-                      static_cast<const unsigned short*>(tr.idx), static_cast<const T*>(xp), n, tr.W, tr.part,
-                      static_cast<const unsigned char*>(tr.active));
+                      static_cast<const unsigned short*>(tr.idx), static_cast<const T*>(xp), n, tr.W, tr.part);
             launch_reduce(tr, m, EpiDual<T>{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, 0, inst_r, live_red_r},
                           &ev[6]);
             BO_TRY(hipStreamSynchronize(stream));

@@ -1003,7 +1003,9 @@ struct Engine final : EngineBase {
     double *part_x = nullptr, *part_r = nullptr, *part_c = nullptr, *part_dx = nullptr, *part_dy = nullptr;
     double* scal = nullptr;  // [0] ||u||^2 [1] ||w||^2 [2] ||c_hat||^2 [3] ||b_hat||^2
     PdlpState* st = nullptr;
-    PdlpState* st_host = nullptr;  // pinned, read back after each interval
+    PdlpState* st_host = nullptr;  // pinned: the state at the end of a solve
+    PdlpState* st_ring[2] = {};    // pinned: the state after each of the two queued intervals
+    hipEvent_t st_ready[2] = {};   // recorded after each ring copy
     PdlpState* st_init = nullptr;  // pinned, staged for the upload in prepare()
     PassPlan plan_rows{}, plan_cols{};  // CSR (dual) pass and CSC (primal) pass
     bool one_wg = false;                // the whole solve in one workgroup (k_small)
@@ -1041,6 +1043,10 @@ struct Engine final : EngineBase {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
         if (graph) hipGraphDestroy(graph);
         if (st_host) hipHostFree(st_host);
+        for (int q = 0; q < 2; ++q) {
+            if (st_ring[q]) hipHostFree(st_ring[q]);
+            if (st_ready[q]) hipEventDestroy(st_ready[q]);
+        }
         if (st_init) hipHostFree(st_init);
         if (stream) hipStreamDestroy(stream);
     }
@@ -1076,6 +1082,10 @@ struct Engine final : EngineBase {
         BO_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_host), sizeof(PdlpState)));
         BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_init), sizeof(PdlpState)));
+        for (int q = 0; q < 2; ++q) {
+            BO_TRY(hipHostMalloc(reinterpret_cast<void**>(&st_ring[q]), sizeof(PdlpState)));
+            BO_TRY(hipEventCreateWithFlags(&st_ready[q], hipEventDisableTiming));
+        }
         std::memset(st_host, 0, sizeof(PdlpState));
         int dev = 0;
         BO_TRY(hipGetDevice(&dev));
@@ -1477,18 +1487,18 @@ struct Engine final : EngineBase {
     }
 
     // Tiled pass: gather kernel into the partials, `mid` (optional event),
-    // reduction + epilogue kernel.  Only the reduction reads the done flag:
-    // partials computed after convergence are never used.
+    // reduction + epilogue kernel.  Once the LP's done flag is set the
+    // epilogue stores nothing (the gather runs: its partials go nowhere).
     // ev4 (optional): start / end events of the gather and of the reduction
     template <int PASS, class Epi>
-    void launch_tiled(const T* gvec, const Epi& e, const int* done, hipEvent_t mid, const hipEvent_t* ev4 = nullptr) {
+    void launch_tiled(const T* gvec, const Epi& e, hipEvent_t mid, const hipEvent_t* ev4 = nullptr) {
         const TilePass& t = tp[PASS];
         const int nseg = PASS == 0 ? m : n, ng = PASS == 0 ? n : m;
         if (t.ntiles > 0)
             launch_ev(k_tile<TM, T, kTileRows, PASS>, dim3(t.ntiles), dim3(kTileThreads), t.lds_bytes, stream, ev4,
-                      static_cast<const TileDesc*>(t.td), static_cast<const unsigned*>(t.words),
-                      static_cast<const TM*>(t.val), static_cast<const unsigned short*>(t.idx), gvec, ng, t.W, t.part,
-                      static_cast<const unsigned char*>(nullptr));
+                      static_cast<const unsigned char*>(nullptr), static_cast<const TileDesc*>(t.td),
+                      static_cast<const unsigned*>(t.words), static_cast<const TM*>(t.val),
+                      static_cast<const unsigned short*>(t.idx), gvec, ng, t.W, t.part);
         if (mid) hipEventRecord(mid, stream);
         launch_tile_reduce(stream, t.max_pairs, nseg, t.seg_ptr, t.slot, static_cast<int>(t.pairs), t.part,
                            t.nslices * kWave, e, t.reduce_u, ev4 ? ev4 + 2 : nullptr);
@@ -1500,7 +1510,7 @@ struct Engine final : EngineBase {
                        const hipEvent_t* ev4 = nullptr) {
         EpiPrimal<T> e{yg, st, c_hat, l_hat, u_hat, xk, x0, xp, t, nullptr};
         if (p.tiled > 0) {
-            launch_tiled<1>(yg, e, done, mid, ev4);
+            launch_tiled<1>(yg, e, mid, ev4);
             return;
         }
         if (p.cap > 0) launch_stream(1, p.cap, cp, ri, cv, e, done, ev4);
@@ -1511,7 +1521,7 @@ struct Engine final : EngineBase {
                      const hipEvent_t* ev4 = nullptr) {
         EpiDual<T> e{xp, st, b_hat, eq, ax, ax0, yk, y0, yp, yg, t, nullptr};
         if (p.tiled > 0) {
-            launch_tiled<0>(xp, e, done, mid, ev4);
+            launch_tiled<0>(xp, e, mid, ev4);
             return;
         }
         if (p.cap > 0) {
@@ -1945,37 +1955,64 @@ struct Engine final : EngineBase {
             BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
             BO_TRY(hipStreamSynchronize(stream));
         }
-        while (!one_wg && !st_host->done) {
-            const long long left = prm.max_iterations - iters;
-            const int k = static_cast<int>(std::min<long long>(K, left));
-            if (k <= 0) break;
-            if (prm.use_graph && k == K) {
-                if (!graph_exec || graph_k != K) {
-                    if (graph_exec) hipGraphExecDestroy(graph_exec);
-                    if (graph) hipGraphDestroy(graph);
-                    graph_exec = nullptr;
-                    graph = nullptr;
-                    BO_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                    launch_interval(K);
-                    BO_TRY(hipStreamEndCapture(stream, &graph));
-                    BO_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-                    graph_k = K;
+        // Intervals are queued two deep while the last state read is far from
+        // the stopping test (best rel-KKT above 10 eps): the next one is
+        // launched before the host reads the state the previous one left, so
+        // the device does not wait on the host between them.  Closer in they
+        // run one at a time, so that no interval runs after convergence.  One
+        // that does (a misjudged step) changes nothing: once the done flag is
+        // set its epilogues store nothing and its check and restart kernels
+        // return, so the final state is the converged one.
+        auto depth_after = [&](const PdlpState& h) {
+            return std::min(h.rel_kkt[0], h.rel_kkt[1]) > 10.0 * prm.eps_rel_kkt ? 2 : 1;
+        };
+        bool stop = !one_wg && st_host->done, timed_out = false;
+        int queued = 0, oldest = 0, depth = depth_after(*st_host);
+        while (!one_wg && !stop) {
+            while (queued < depth) {
+                const long long left = prm.max_iterations - iters;
+                const int k = static_cast<int>(std::min<long long>(K, left));
+                if (k <= 0) break;
+                if (prm.use_graph && k == K) {
+                    if (!graph_exec || graph_k != K) {
+                        if (graph_exec) hipGraphExecDestroy(graph_exec);
+                        if (graph) hipGraphDestroy(graph);
+                        graph_exec = nullptr;
+                        graph = nullptr;
+                        BO_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                        launch_interval(K);
+                        BO_TRY(hipStreamEndCapture(stream, &graph));
+                        BO_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+                        graph_k = K;
+                    }
+                    BO_TRY(hipGraphLaunch(graph_exec, stream));
+                } else {
+                    launch_interval(k);
                 }
-                BO_TRY(hipGraphLaunch(graph_exec, stream));
-            } else {
-                launch_interval(k);
+                BO_TRY(hipGetLastError());
+                iters += k;
+                const int q = (oldest + queued) % 2;
+                BO_TRY(hipMemcpyAsync(st_ring[q], st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
+                BO_TRY(hipEventRecord(st_ready[q], stream));
+                ++queued;
             }
-            BO_TRY(hipGetLastError());
-            iters += k;
+            if (queued == 0) break;  // max_iterations reached
+            BO_TRY(hipEventSynchronize(st_ready[oldest]));
+            stop = st_ring[oldest]->done != 0;
+            depth = depth_after(*st_ring[oldest]);
+            oldest ^= 1;
+            --queued;
+            if (!stop && prm.time_limit_s > 0.0 &&
+                std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count() >
+                    prm.time_limit_s) {
+                timed_out = true;
+                stop = true;
+            }
+        }
+        if (!one_wg) {  // the queue drained: the state as the last interval left it
             BO_TRY(hipMemcpyAsync(st_host, st, sizeof(PdlpState), hipMemcpyDeviceToHost, stream));
             BO_TRY(hipStreamSynchronize(stream));
-            if (prm.time_limit_s > 0.0) {
-                double el = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
-                if (el > prm.time_limit_s && !st_host->done) {
-                    st_host->status = 3;
-                    break;
-                }
-            }
+            if (timed_out && !st_host->done) st_host->status = 3;
         }
         auto t2 = std::chrono::high_resolution_clock::now();
         if (out) {

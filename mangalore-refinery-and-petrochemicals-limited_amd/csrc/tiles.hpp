@@ -513,12 +513,17 @@ __device__ inline void chunk_finish(const RowChunk<T, CH>& c, int r, const TX* l
 // instantiation (0 / 1: CSR / CSC pass of the single-LP engine, 2 / 3: of
 // the batched engine), so that profiles tell the passes apart; every pass
 // issues a chunk's LDS gathers early (chunk_finish).  active (optional):
-// per-tile flag; a tile whose flag is 0 does nothing.
+// per-tile flag; a tile whose flag is 0 does nothing.  It comes first, among
+// the arguments the dispatch preloads into SGPRs, so testing it costs no
+// argument load.  The kernel reads no done flag: its partials only reach
+// the state through the reduction, whose epilogue stores nothing once the
+// LP is done.  (A flag test here costs a scalar round trip: the compiler
+// moves the flag's load down to the test.)
 template <class T, class TX, int CH, int PASS>
 __global__ void __launch_bounds__(kTileThreads)
-k_tile(const TileDesc* __restrict__ tiles, const unsigned* __restrict__ slice_word, const T* __restrict__ val,
-       const unsigned short* __restrict__ idx, const TX* __restrict__ gvec, int ng, int W,
-       double* __restrict__ part, const unsigned char* __restrict__ active) {
+k_tile(const unsigned char* __restrict__ active, const TileDesc* __restrict__ tiles,
+       const unsigned* __restrict__ slice_word, const T* __restrict__ val, const unsigned short* __restrict__ idx,
+       const TX* __restrict__ gvec, int ng, int W, double* __restrict__ part) {
     if (active && !active[blockIdx.x]) return;  // every LP of the tile has finished (batched engine)
     extern __shared__ __align__(16) unsigned char tile_lds[];
     TX* lds = reinterpret_cast<TX*>(tile_lds);
