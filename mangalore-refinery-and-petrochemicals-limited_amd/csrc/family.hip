@@ -445,16 +445,36 @@ kf_check_cols(int n, const LaneCoef* cf, const double* ac0, const double* ac1, c
     store_lane_sums<6>(q, part);
 }
 
-// 14 wavefronts (1024 threads): wavefront q adds check quantity q for all
-// lanes; then one thread per lane runs the stopping test, stall test and
-// restart rule of the single-LP engine.
+// The check's lane sums in two levels, each in a fixed order (one
+// workgroup walking every workgroup's partials took about 0.5 ms per check).
+// kf_fold_check: workgroup g, wavefront q adds quantity q of the blocks in
+// g's contiguous range, for all lanes, into fold[g][q][lane].  kf_decide:
+// wavefront q adds the kFoldGroupsF results of quantity q; then one thread
+// per lane runs the stopping test, stall test and restart rule of the
+// single-LP engine.
+constexpr int kFoldGroupsF = 16;
+
+__device__ inline double sum_lane_range(const double* part, int nblocks, int nq, int q, int lane) {
+    const int g = static_cast<int>(blockIdx.x), ng = static_cast<int>(gridDim.x);
+    const int per = (nblocks + ng - 1) / ng, b0 = min(nblocks, g * per), b1 = min(nblocks, b0 + per);
+    return sum_lane_q(part + static_cast<size_t>(b0) * nq * kL, b1 - b0, nq, q, lane);
+}
+
 __global__ void __launch_bounds__(1024)
-kf_decide(PdlpState* st, const double* px, const double* pr, const double* pc, int ncol_blocks, int nrow_blocks) {
+kf_fold_check(const double* px, const double* pr, const double* pc, int ncol_blocks, int nrow_blocks, double* fold) {
+    const int lane = static_cast<int>(threadIdx.x) % kWave, w = static_cast<int>(threadIdx.x) / kWave;
+    double v = 0.0;
+    if (w < 2) v = sum_lane_range(px, ncol_blocks, 2, w, lane);
+    else if (w < 8) v = sum_lane_range(pr, nrow_blocks, 6, w - 2, lane);
+    else if (w < 14) v = sum_lane_range(pc, ncol_blocks, 6, w - 8, lane);
+    if (w < 14) fold[(static_cast<size_t>(blockIdx.x) * 14 + w) * kL + lane] = v;
+}
+
+__global__ void __launch_bounds__(1024)
+kf_decide(PdlpState* st, const double* fold, int groups) {
     __shared__ double sh[14][kL];
     const int lane = static_cast<int>(threadIdx.x) % kWave, w = static_cast<int>(threadIdx.x) / kWave;
-    if (w < 2) sh[w][lane] = sum_lane_q(px, ncol_blocks, 2, w, lane);
-    else if (w < 8) sh[w][lane] = sum_lane_q(pr, nrow_blocks, 6, w - 2, lane);
-    else if (w < 14) sh[w][lane] = sum_lane_q(pc, ncol_blocks, 6, w - 8, lane);
+    if (w < 14) sh[w][lane] = sum_lane_q(fold, groups, 14, w, lane);
     __syncthreads();
     if (w != 0 || st[lane].done) return;
     double a[14];
@@ -508,12 +528,20 @@ kf_restart_y(int m, const PdlpState* st, double* yk, T* y0, const T* yp, T* yg, 
     store_lane_sums<1>(q, part);
 }
 
-// Two wavefronts: ||dx||^2 and ||dy||^2 of every lane, then the primal-weight update.
-__global__ void kf_restart_finish(PdlpState* st, const double* pdx, int ncol_blocks, const double* pdy,
-                                  int nrow_blocks) {
+// ||dx||^2 and ||dy||^2 of every lane in the same two levels: kf_fold_restart
+// (wavefront 0: dx, 1: dy) into fold[g][2][lane], then kf_restart_finish
+// adds the groups and runs the primal-weight update.
+__global__ void kf_fold_restart(const double* pdx, int ncol_blocks, const double* pdy, int nrow_blocks,
+                                double* fold) {
+    const int lane = static_cast<int>(threadIdx.x) % kWave, w = static_cast<int>(threadIdx.x) / kWave;
+    const double v = w == 0 ? sum_lane_range(pdx, ncol_blocks, 1, 0, lane) : sum_lane_range(pdy, nrow_blocks, 1, 0, lane);
+    fold[(static_cast<size_t>(blockIdx.x) * 2 + w) * kL + lane] = v;
+}
+
+__global__ void kf_restart_finish(PdlpState* st, const double* fold, int groups) {
     __shared__ double sh[2][kL];
     const int lane = static_cast<int>(threadIdx.x) % kWave, w = static_cast<int>(threadIdx.x) / kWave;
-    sh[w][lane] = w == 0 ? sum_lane_q(pdx, ncol_blocks, 1, 0, lane) : sum_lane_q(pdy, nrow_blocks, 1, 0, lane);
+    sh[w][lane] = sum_lane_q(fold, groups, 2, w, lane);
     __syncthreads();
     if (w != 0 || st[lane].done || !st[lane].restart) return;
     restart_finish_core(st + lane, sh[0][lane], sh[1][lane]);
@@ -667,6 +695,7 @@ struct Family final : FamilyBase {
     double *wx = nullptr, *wy = nullptr, *stage = nullptr;
     int has_warm = 0;
     double *part_a = nullptr, *part_b = nullptr, *part_c = nullptr;
+    double* part_fold = nullptr;  // first level of the check's and restart's lane sums
     LaneCoef* cf = nullptr;
     PdlpState* st = nullptr;
     PdlpState* st_host = nullptr;
@@ -780,6 +809,7 @@ struct Family final : FamilyBase {
         BO_TRY(mem.alloc(&yg, ML));
         BO_TRY(mem.alloc(&stage, std::max(NL, ML)));
         BO_TRY(mem.alloc(&part_a, static_cast<size_t>(kSumBlocks) * 6 * kL));
+        BO_TRY(mem.alloc(&part_fold, static_cast<size_t>(kFoldGroupsF) * 14 * kL));
         BO_TRY(mem.alloc(&part_b, static_cast<size_t>(kSumBlocks) * 6 * kL));
         BO_TRY(mem.alloc(&part_c, static_cast<size_t>(kSumBlocks) * 6 * kL));
         BO_TRY(mem.alloc(&cf, 1));
@@ -902,13 +932,19 @@ struct Family final : FamilyBase {
                            EpiFCheck2{yo_cur, yo_cand, cf, ac0, ac1});
         hipLaunchKernelGGL(kf_check_cols, dim3(gn), dim3(256), 0, stream, n, cf, ac0, ac1, c64, lb64, ub64, cs,
                            part_c);
-        hipLaunchKernelGGL(kf_decide, dim3(1), dim3(1024), 0, stream, st, part_a, part_b, part_c, static_cast<int>(gn),
-                           static_cast<int>(gm));
+        hipLaunchKernelGGL(kf_fold_check, dim3(kFoldGroupsF), dim3(1024), 0, stream,
+                           static_cast<const double*>(part_a), static_cast<const double*>(part_b),
+                           static_cast<const double*>(part_c), static_cast<int>(gn), static_cast<int>(gm), part_fold);
+        hipLaunchKernelGGL(kf_decide, dim3(1), dim3(1024), 0, stream, st, static_cast<const double*>(part_fold),
+                           kFoldGroupsF);
         hipLaunchKernelGGL(kf_restart_x<T>, dim3(gn), dim3(256), 0, stream, n, st, xk, x0, xp, part_a);
         hipLaunchKernelGGL(kf_restart_y<T>, dim3(gm), dim3(256), 0, stream, m, st, yk, y0, yp, yg, ax, ax0, ax_cand,
                            part_b);
-        hipLaunchKernelGGL(kf_restart_finish, dim3(1), dim3(2 * kL), 0, stream, st, part_a, static_cast<int>(gn),
-                           part_b, static_cast<int>(gm));
+        hipLaunchKernelGGL(kf_fold_restart, dim3(kFoldGroupsF), dim3(2 * kL), 0, stream,
+                           static_cast<const double*>(part_a), static_cast<int>(gn), static_cast<const double*>(part_b),
+                           static_cast<int>(gm), part_fold);
+        hipLaunchKernelGGL(kf_restart_finish, dim3(1), dim3(2 * kL), 0, stream, st,
+                           static_cast<const double*>(part_fold), kFoldGroupsF);
         hipLaunchKernelGGL(kf_coef, dim3(1), dim3(kL), 0, stream, st, cf, running);
     }
 
