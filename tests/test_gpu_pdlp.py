@@ -306,6 +306,39 @@ def test_pdlp_max_iterations_status(pkg, instances):
     assert s["status"] == 1 and s["iterations"] == 128
 
 
+@pytest.mark.parametrize("cap", [100, 128, 200])
+def test_pdlp_interval_queue_stops_at_max_iterations(pkg, po, instances, cap):
+    # the multi-kernel loop keeps two check intervals queued (64 steps each)
+    # while far from the stopping test: a cap inside the second interval, at
+    # its end, or after a partial third one stops exactly there, with the
+    # state of the last step taken
+    lp = instances["zipf_2000x5000_s9"]
+    h = pkg.Pdlp(lp, eps_rel_kkt=1e-12, max_iterations=cap, single_workgroup=0)
+    s = h.solve()
+    x, y = h.solution()
+    h.close()
+    k = po.kkt(lp, x, y)
+    print("cap", cap, "iterations", s["iterations"], "engine rel-KKT %.3e oracle %.3e" % (s["rel_kkt"], k["rel_kkt"]),
+          flush=True)
+    assert s["status"] == 1 and s["iterations"] == cap
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y))
+    assert abs(k["rel_kkt"] - s["rel_kkt"]) <= 1e-9 + 1e-3 * k["rel_kkt"]  # the engine's report of the same point
+
+
+def test_pdlp_time_limit_stops_the_interval_queue(pkg, lpdata):
+    # a time limit ends the multi-kernel loop with status 3 (TIME_LIMIT) at
+    # most one queued interval late
+    import time
+    lp = lpdata.zipf_lp(20000, 50000, 1.5, 500, seed=5)
+    h = pkg.Pdlp(lp, eps_rel_kkt=1e-12, max_iterations=10**7, time_limit_s=0.05, single_workgroup=0)
+    t0 = time.perf_counter()
+    s = h.solve()
+    dt = time.perf_counter() - t0
+    h.close()
+    print("time-limited solve: status", s["status"], "iterations", s["iterations"], "%.3f s" % dt, flush=True)
+    assert s["status"] == 3 and 0 < s["iterations"] < 10**7 and dt < 5.0
+
+
 def test_dropin_pdlp_engine(pkg, po, lpdata):
     # the whole drop-in with engine PDLP on the hand LP (max 3x + 5y)
     mb = lpdata.ModelBuilder(sense=1)
