@@ -591,14 +591,14 @@ struct Batch final : BatchBase {
         return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
     }
 
-    hipError_t build_tiles_pass(int pass, const std::vector<int>& ptr, const std::vector<int>& gidx) {
+    hipError_t build_tiles_pass(int pass, const int* ptr, const int* gidx) {
         TilePass& t = tp[pass];
         const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
         if (nseg == 0 || ng == 0) return hipSuccess;
         const int wmax = std::min(65536, lds_max / static_cast<int>(sizeof(T))) / 64 * 64;
         if (wmax < 64) return hipSuccess;
         HostTiles h;
-        if (!build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, 2 * num_cus, h)) return hipSuccess;
+        if (!build_tiles(ptr, gidx, nseg, ng, wmax, 2 * num_cus, h)) return hipSuccess;
         if (h.pairs * 4 >= (1LL << 31) - 64 || static_cast<long long>(h.slice_word.size()) * kWave * 8 >= (1LL << 31) - 64)
             return hipSuccess;
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(
@@ -656,8 +656,10 @@ struct Batch final : BatchBase {
         // LP; the arrays are the same as a serial pass over the whole matrix.
         std::vector<long long> zoff(K + 1, 0);
         for (int k = 0; k < K; ++k) zoff[k + 1] = zoff[k] + lps[k].nnz;
-        std::vector<int> h_rp(m + 1, 0), h_ci(nnz), h_ir(m), h_ic(n), h_cp(n + 1, 0), h_ri(nnz);
-        std::vector<double> h_rv(nnz), h_cv(nnz), hb(m), hc(n), hl(n), hu(n);
+        std::vector<int> h_rp(m + 1, 0), h_ir(m), h_ic(n), h_cp(n + 1, 0);
+        std::vector<double> hb(m), hc(n), hl(n), hu(n);
+        HostVec<int> h_ci(nnz), h_ri(nnz);  // every entry written by its LP's thread
+        HostVec<double> h_rv(nnz), h_cv(nnz);
         std::vector<unsigned char> he(m);
         std::vector<double> bn(K), cn(K);
         parallel_for_lps(K, [&](int k) {
@@ -790,8 +792,8 @@ struct Batch final : BatchBase {
         BO_TRY(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         BO_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         // a positive schedule code keeps that pass on the segment kernel (no tile layout to build)
-        if (prm.lanes_rows <= 0) BO_TRY(build_tiles_pass(0, h_rp, h_ci));
-        if (prm.lanes_cols <= 0) BO_TRY(build_tiles_pass(1, h_cp, h_ri));
+        if (prm.lanes_rows <= 0) BO_TRY(build_tiles_pass(0, h_rp.data(), h_ci.data()));
+        if (prm.lanes_cols <= 0) BO_TRY(build_tiles_pass(1, h_cp.data(), h_ri.data()));
         return BO_SUCCESS;
     }
 

@@ -1135,9 +1135,10 @@ struct Engine final : EngineBase {
         std::vector<int> rlen(m), clen(n, 0);
         for (int i = 0; i < m; ++i) rlen[i] = lp->row_ptr[i + 1] - lp->row_ptr[i];
         {  // column lengths: per-thread counts over row ranges, then added
-            std::vector<int> cc(static_cast<size_t>(nth - 1) * n, 0);
+            HostVec<int> cc(static_cast<size_t>(nth - 1) * n);  // each thread zeroes its own row
             host_parallel(nth, [&](int t, int nt) {
                 int* c = t == 0 ? clen.data() : cc.data() + static_cast<size_t>(t - 1) * n;
+                if (t > 0) std::fill(c, c + n, 0);
                 const auto r = host_part(m, t, nt);
                 for (int k = lp->row_ptr[r.first]; k < lp->row_ptr[r.second]; ++k) c[lp->col_idx[k]]++;
             });
@@ -1156,8 +1157,9 @@ struct Engine final : EngineBase {
         for (int p2 = 0; p2 < m; ++p2) row_len_sorted[p2] = rlen[row_of[p2]];
         for (int p2 = 0; p2 < n; ++p2) col_len_sorted[p2] = clen[col_of[p2]];
         // permuted CSR, rows filled on nth host threads
-        std::vector<int> h_rp(m + 1), h_ci(nnz);
-        std::vector<double> h_rv(nnz);
+        std::vector<int> h_rp(m + 1);
+        HostVec<int> h_ci(nnz);  // these and the CSC's: every entry written on the threads
+        HostVec<double> h_rv(nnz);
         h_rp[0] = 0;
         for (int p2 = 0; p2 < m; ++p2) h_rp[p2 + 1] = h_rp[p2] + row_len_sorted[p2];
         host_parallel(nth, [&](int t, int nt) {
@@ -1171,8 +1173,9 @@ struct Engine final : EngineBase {
                 }
             }
         });
-        std::vector<int> h_cp(n + 1, 0), h_ri(nnz);  // the permuted CSC, below
-        std::vector<double> h_cv(nnz);
+        std::vector<int> h_cp(n + 1, 0);  // the permuted CSC, below
+        HostVec<int> h_ri(nnz);
+        HostVec<double> h_cv(nnz);
         // The host layouts of the tiled passes are cut on threads of their
         // own while the rest of the set-up goes on; `joiner` waits for them on
         // every way out of this function (they read the host arrays above).
@@ -1187,9 +1190,9 @@ struct Engine final : EngineBase {
                     if (t[q].joinable()) t[q].join();
             }
         } joiner{tile_thread};
-        auto start_tiles = [&](int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx) {
+        auto start_tiles = [&](int pass, int k, const int* ptr, const int* gidx) {
             if (k <= 0) return;
-            tile_thread[pass] = std::thread([this, pass, k, &ptr, &gidx, &ht, &ht_built, &ht_err] {
+            tile_thread[pass] = std::thread([this, pass, k, ptr, gidx, &ht, &ht_built, &ht_err] {
                 try {
                     ht_built[pass] = host_tiles(pass, k, ptr, gidx, ht[pass]);
                 } catch (...) {
@@ -1197,14 +1200,15 @@ struct Engine final : EngineBase {
                 }
             });
         };
-        start_tiles(0, tiles_rows, h_rp, h_ci);
+        start_tiles(0, tiles_rows, h_rp.data(), h_ci.data());
         // permuted CSC (transpose of the permuted CSR): thread t counts its
         // rows' entries per column, then writes them after those of threads
         // 0 .. t-1, so every column keeps its rows in ascending order
         {
-            std::vector<int> cnt(static_cast<size_t>(nth) * n, 0);
+            HostVec<int> cnt(static_cast<size_t>(nth) * n);  // each thread zeroes its own row
             host_parallel(nth, [&](int t, int nt) {
                 int* c = cnt.data() + static_cast<size_t>(t) * n;
+                std::fill(c, c + n, 0);
                 const auto r = host_part(m, t, nt);
                 for (int k = h_rp[r.first]; k < h_rp[r.second]; ++k) c[h_ci[k]]++;
             });
@@ -1240,7 +1244,7 @@ struct Engine final : EngineBase {
                     }
             });
         }
-        start_tiles(1, tiles_cols, h_cp, h_ri);
+        start_tiles(1, tiles_cols, h_cp.data(), h_ri.data());
         std::vector<double> hb(m), hc(n), hl(n), hu(n);
         std::vector<unsigned char> he(m);
         double b2 = 0.0, c2 = 0.0;
@@ -1405,13 +1409,13 @@ struct Engine final : EngineBase {
     // Host part of the tiled layout of one pass (no device call: it runs on a
     // host thread of its own); false when the LDS is too small or the padded
     // layout would not fit 32-bit offsets (the pass then stays unbuilt).
-    bool host_tiles(int pass, int k, const std::vector<int>& ptr, const std::vector<int>& gidx, HostTiles& h) const {
+    bool host_tiles(int pass, int k, const int* ptr, const int* gidx, HostTiles& h) const {
         const int nseg = pass == 0 ? m : n, ng = pass == 0 ? n : m;
         if (nseg == 0) return false;
         // LDS holds the vector block: the largest block that fits
         const int wmax = std::min(65536, lds_max / static_cast<int>(sizeof(T))) / 64 * 64;
         if (wmax < 64) return false;
-        return build_tiles(ptr.data(), gidx.data(), nseg, ng, wmax, k * num_cus, h);
+        return build_tiles(ptr, gidx, nseg, ng, wmax, k * num_cus, h);
     }
 
     // Uploads the host layout of one pass (cut for k tiles per CU).

@@ -64,12 +64,12 @@ struct TileDesc {
 struct HostTiles {
     int B = 0;  // blocks
     int W = 0;  // entries per block
-    std::vector<TileDesc> tiles;
-    std::vector<unsigned> slice_word;   // per slice: rows | log2(lanes per pair) << kLgShift
-    std::vector<unsigned short> idx;    // padded entries: block-local gathered index
-    std::vector<int> src;               // padded entries: source nonzero, -1 for padding
+    HostVec<TileDesc> tiles;
+    HostVec<unsigned> slice_word;       // per slice: rows | log2(lanes per pair) << kLgShift
+    HostVec<unsigned short> idx;        // padded entries: block-local gathered index
+    HostVec<int> src;                   // padded entries: source nonzero, -1 for padding
     std::vector<int> seg_ptr;           // per segment: range of its partial slots in `slot`
-    std::vector<unsigned> slot;         // partial slots (64 q + lane), block order within a segment
+    HostVec<unsigned> slot;             // partial slots (64 q + lane), block order within a segment
     size_t padded = 0;
     long long pairs = 0;
     int max_wave_slices = 0;
@@ -142,7 +142,7 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
         bstart[c + 1] = zs;
         pairs[c].resize(static_cast<size_t>(ps));
     }
-    std::vector<int> ent(static_cast<size_t>(nnz));
+    HostVec<int> ent(static_cast<size_t>(nnz));  // every entry written below
     host_parallel(nth, [&](int t, int) {
         const auto r = seg_range(t);
         long long* zpos = zc.data() + static_cast<size_t>(t) * B;
@@ -333,8 +333,8 @@ inline bool build_tiles(const int* ptr, const int* gidx, int nseg, int ng, int w
     h.slice_word.resize(soff[nspan]);
     h.idx.resize(h.padded);
     h.src.resize(h.padded);
-    std::vector<int> pair_seg(qoff[nspan]);
-    std::vector<unsigned> pair_slot(qoff[nspan]);
+    HostVec<int> pair_seg(qoff[nspan]);  // these and h's arrays: every entry written below
+    HostVec<unsigned> pair_slot(qoff[nspan]);
     host_parallel(host_thread_count(nspan), [&](int t, int nt) {
         const auto r = host_part(nspan, t, nt);
         for (long long si = r.first; si < r.second; ++si) {
