@@ -76,10 +76,9 @@ def test_family_reuse_warm_start_and_count(pkg, po, family):
     per3, tot3 = f.solve()
     X3, Y3 = f.solutions()
     assert len(per3) == 8
-    for i, d in enumerate(d2):
-        if per3[i]["status"] == 0:
-            _check(po, d, X3[i], Y3[i], per3[i], "count8_%d" % i)
-    assert tot3["solved"] >= 7
+    for i, d in enumerate(d2):  # every draw solved (the FP64 polish takes any that stall)
+        _check(po, d, X3[i], Y3[i], per3[i], "count8_%d" % i)
+    assert tot3["solved"] == 8
     f.close()
 
 
